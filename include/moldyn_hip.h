@@ -1,0 +1,220 @@
+/*
+ * moldyn_hip.h -- C ABI of the MI355X (gfx950) force / neighbour / integrator
+ * path of moldyn (VishnuR23/MolecularDynamics).
+ *
+ * Plain pointers and sizes only: no C++ or torch types cross this boundary.
+ * All host arrays are in the caller's atom order ("original" indices, the
+ * order atoms were uploaded in); the device keeps atoms in a cell-sorted
+ * order internally and translates at the boundary. Every function returns
+ * an md_status; md_last_error(ctx) describes the last failure. No
+ * exception crosses the ABI: the C++ wrappers map MD_ERR_INVALID_ARGUMENT to
+ * std::invalid_argument and MD_ERR_RUNTIME / MD_ERR_CAPACITY to
+ * std::runtime_error, as the reference throws.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line of VishnuR23/MolecularDynamics (paths under src/moldyn/).
+ *
+ * Threading: one context per host thread; calls on a context are ordered on
+ * its HIP stream and are not thread-safe.
+ */
+#ifndef MOLDYN_HIP_H
+#define MOLDYN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum md_status {
+    MD_OK = 0,
+    MD_ERR_INVALID_ARGUMENT = 1, /* maps to std::invalid_argument */
+    MD_ERR_RUNTIME = 2,          /* maps to std::runtime_error */
+    MD_ERR_HIP = 3,              /* a HIP runtime call failed */
+    MD_ERR_CAPACITY = 4,         /* a neighbour row exceeded its capacity */
+    MD_ERR_STATE = 5             /* call order violated (e.g. no system uploaded) */
+} md_status;
+
+/* Pair sources of LennardJones::computeForces overloads
+ * (forcefield/lennard_jones.hpp:73-85). */
+typedef enum md_pair_source {
+    MD_PAIRS_BRUTE = 0,  /* every i<j pair: computeForces(System&) */
+    MD_PAIRS_CELLS = 1,  /* linked cells at the LJ cutoff: computeForces(System&, const CellList&) */
+    MD_PAIRS_VERLET = 2  /* the context's Verlet list: computeForces(System&, const VerletList&) */
+} md_pair_source;
+
+/* Arithmetic of the pair kernel. MD_FP64 reproduces the reference's FP64
+ * results to 1e-12 relative. MD_MIXED evaluates the pair terms in FP32 and
+ * accumulates per-atom forces, energy and virial in deterministic 64-bit
+ * fixed point; integration stays FP64. */
+typedef enum md_precision { MD_FP64 = 0, MD_MIXED = 1 } md_precision;
+
+typedef enum md_integrator_kind {
+    MD_VELOCITY_VERLET = 0, /* integrate/velocity_verlet.hpp:39-80 */
+    MD_LANGEVIN_BAOAB = 1,  /* integrate/langevin.hpp:122-190 */
+    MD_NOSE_HOOVER = 2      /* integrate/nose_hoover.hpp:26-67 */
+} md_integrator_kind;
+
+typedef struct md_ctx md_ctx;
+
+/* Lennard-Jones parameters: LennardJones(sigma, epsilon, cutoff, truncation)
+ * (forcefield/lennard_jones.hpp:63; truncation enum :48). */
+typedef struct md_lj_params {
+    double sigma;
+    double epsilon;
+    double cutoff;
+    int linear_force_shift; /* 0: Truncated, 1: LinearForceShift */
+} md_lj_params;
+
+/* Integrator parameters. VelocityVerlet(dt) (velocity_verlet.hpp:41);
+ * LangevinBAOAB(dt, friction, temperature, seed) (langevin.hpp:124);
+ * NoseHooverChain(dt, temperature, tau, chainLength, dof) (nose_hoover.hpp:31). */
+typedef struct md_integrator {
+    int kind;          /* md_integrator_kind */
+    double dt;
+    double friction;   /* BAOAB gamma */
+    double temperature;
+    double tau;        /* NHC period */
+    int chain_length;  /* NHC M >= 1 */
+    long dof;          /* NHC degrees of freedom */
+    uint64_t seed;     /* BAOAB noise seed (counter-based stream, see md_run) */
+} md_integrator;
+
+/* Per-step thermodynamic record written by md_run. */
+typedef struct md_thermo {
+    double potential; /* pair energy of the step's force evaluation (no LRC) */
+    double virial;    /* raw sum of w(r), not divided by 3 */
+    double kinetic;   /* sum 0.5 m v^2 at the end of the step */
+    double conserved; /* NHC conservedQuantityContribution(), 0 otherwise */
+    double pairs;     /* unordered pairs inside the cutoff in the step's force call */
+} md_thermo;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on the current HIP device. `hip_stream` is a hipStream_t
+ * to order work on, or NULL for a context-owned stream (default priority). */
+md_status md_ctx_create(void* hip_stream, md_ctx** out);
+md_status md_ctx_destroy(md_ctx* ctx);
+const char* md_last_error(const md_ctx* ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+md_status md_ctx_synchronize(md_ctx* ctx);
+
+/* ---- system state (core/system.hpp:11-60) -------------------------------- */
+
+/* Uploads n atoms in a periodic orthorhombic box of lengths L[3]: positions
+ * x,y,z, velocities vx,vy,vz (NULL = zero) and masses m (NULL = 1). Forces
+ * start at zero. Replaces System(Box) + addAtom (system.hpp:13-21). */
+md_status md_system_upload(md_ctx* ctx, long n, const double* L, const double* x,
+                           const double* y, const double* z, const double* vx,
+                           const double* vy, const double* vz, const double* m);
+/* Copies state back in upload order; any pointer may be NULL. Replaces
+ * System::position/velocity/force (system.hpp:26-28). */
+md_status md_system_download(md_ctx* ctx, double* x, double* y, double* z, double* vx,
+                             double* vy, double* vz, double* fx, double* fy, double* fz);
+/* System::setPosition / setVelocity for every atom (system.hpp:33-34). */
+md_status md_system_set_positions(md_ctx* ctx, const double* x, const double* y,
+                                  const double* z);
+md_status md_system_set_velocities(md_ctx* ctx, const double* vx, const double* vy,
+                                   const double* vz);
+/* System::kineticEnergy (system.cpp:124-131), FP64 deterministic reduction. */
+md_status md_system_kinetic_energy(md_ctx* ctx, double* ke);
+/* System::removeCenterOfMassMotion (system.cpp:147-164). */
+md_status md_system_remove_com_motion(md_ctx* ctx);
+md_status md_system_size(const md_ctx* ctx, long* n);
+
+/* ---- Lennard-Jones (forcefield/lennard_jones.{hpp:61-135,cpp:32-151}) ---- */
+
+/* Sets the pair potential; precomputes u(rc) and du/dr(rc)
+ * (lennard_jones.cpp:32-48). */
+md_status md_lj_set(md_ctx* ctx, const md_lj_params* p);
+/* LennardJones::computeForces overloads (lennard_jones.cpp:94-119): zero the
+ * forces, evaluate every offered pair with r2 < rc2, write forces, return
+ * E (pair energy, no tail) and W (raw virial sum). MD_PAIRS_CELLS fails with
+ * MD_ERR_INVALID_ARGUMENT when fewer than 3 cells per side fit
+ * (lennard_jones.cpp:99-111). MD_PAIRS_VERLET uses the list as last built. */
+md_status md_lj_forces(md_ctx* ctx, int pair_source, int precision, double* energy,
+                       double* virial);
+/* LennardJones::computeEnergyVirial (lennard_jones.cpp:90-92): all i<j
+ * pairs, forces untouched. */
+md_status md_lj_energy_virial(md_ctx* ctx, double* energy, double* virial);
+/* LennardJones::longRangeCorrection / longRangeCorrectionPressure
+ * (lennard_jones.cpp:121-151); host arithmetic, no device needed. */
+md_status md_lj_tail(const md_lj_params* p, long n, const double* L, double* u_lrc,
+                     double* p_lrc);
+
+/* ---- neighbour lists (neighbor/{cell_list,verlet_list}.{hpp,cpp}) ------- */
+
+/* CellList(box, cutoff).usable() (cell_list.cpp:8-14, :44-46): 1 when
+ * floor(L_d/cutoff) >= 3 on every axis. Host arithmetic. */
+int md_cells_usable(const double* L, double cutoff);
+/* VerletList(box, cutoff, skin) (verlet_list.hpp:26): configures the list. */
+md_status md_neigh_config(md_ctx* ctx, double cutoff, double skin);
+/* VerletList::build (verlet_list.cpp:8-42): pairs with
+ * |minimum image|^2 < (rc+skin)^2 from the cell grid (brute force when it
+ * is unusable), reference positions recorded, build counter incremented.
+ * Also re-sorts the device atom order by cell. */
+md_status md_neigh_build(md_ctx* ctx);
+/* VerletList::needsRebuild (verlet_list.cpp:44-62): sum of the two largest
+ * minimum-image displacements since the build exceeds the skin. */
+md_status md_neigh_needs_rebuild(md_ctx* ctx, int* flag);
+/* VerletList::rebuildIfNeeded (verlet_list.cpp:64-68). */
+md_status md_neigh_rebuild_if_needed(md_ctx* ctx, int* rebuilt);
+/* VerletList::rebuildCount (verlet_list.cpp:70-72). */
+md_status md_neigh_rebuild_count(const md_ctx* ctx, long* count);
+/* VerletList::forEachPair (verlet_list.hpp:52-57) as data: the unordered
+ * pair set (i<j, upload indices) sorted by (i, j), 2*npairs uint32 values.
+ * With cap too small, *npairs receives the count and MD_ERR_CAPACITY is
+ * returned. */
+md_status md_neigh_export_pairs(md_ctx* ctx, uint32_t* ij, long cap, long* npairs);
+/* Neighbour row capacity (entries per atom) of the device list; 0 = auto. */
+md_status md_neigh_set_capacity(md_ctx* ctx, int per_atom);
+
+/* ---- integrators ---------------------------------------------------------- */
+
+/* Selects the integrator for md_run; resets thermostat state.
+ * NoseHooverChain throws for chainLength < 1 (nose_hoover.cpp:36-38):
+ * MD_ERR_INVALID_ARGUMENT. */
+md_status md_integrator_set(md_ctx* ctx, const md_integrator* it);
+/* Runs nsteps of the selected integrator entirely on the device, with the
+ * force callback of the reference driver (apps/moldyn_run.cpp:581-587):
+ * rebuild-if-needed then Verlet-list LJ forces at `precision`. Forces must be
+ * current on entry (call md_lj_forces once first, as the reference primes
+ * them, velocity_verlet.hpp:34-38). When `thermo` is non-NULL it receives
+ * nsteps records. BAOAB noise comes from a counter-based generator keyed by
+ * (seed, step, upload index), so it is independent of the device atom order;
+ * it is not the reference's sequential mt19937_64 stream. */
+md_status md_run(md_ctx* ctx, long nsteps, int precision, md_thermo* thermo);
+/* NoseHooverChain::conservedQuantityContribution (nose_hoover.cpp:52-61)
+ * and the chain arrays (length chain_length each; NULL to skip). */
+md_status md_nhc_state(md_ctx* ctx, double* conserved, double* xi, double* p_xi);
+
+/* ---- seeded inputs (util/lattice.cpp:15-64, util/random.hpp:37-76) ------- */
+
+/* fccLattice(cells, density, temperature, seed): N = 4 cells^3 atoms, cubic
+ * box L = cbrt(N/density) written to L[3], positions folded into
+ * [-L/2, L/2), Maxwell-Boltzmann velocities from mt19937_64 + Marsaglia's
+ * polar method drawn x,y,z per atom in (ix, iy, iz, basis) order, mass 1,
+ * centre-of-mass motion removed. Bit-identical to the reference on the same
+ * C++ standard library. Host arithmetic; arrays hold 4*cells^3 values. */
+md_status md_fcc_lattice(int cells, double density, double temperature, uint64_t seed,
+                         double* L, double* x, double* y, double* z, double* vx, double* vy,
+                         double* vz);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* When enabled, every pair-kernel launch is bracketed by HIP events on the
+ * context's stream. md_timing_read returns the summed device time (ms) and
+ * launch count since the last reset, for kernel id 0 = LJ pair kernel,
+ * 1 = neighbour build, 2 = integrator kernels. */
+md_status md_timing_enable(md_ctx* ctx, int on);
+md_status md_timing_read(md_ctx* ctx, int kernel_id, double* total_ms, long* launches);
+md_status md_timing_reset(md_ctx* ctx);
+/* Candidate-pair statistics of the last force evaluation: pairs evaluated
+ * inside the cutoff (each unordered pair once). */
+md_status md_last_pair_count(md_ctx* ctx, long* pairs_in_cutoff);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOLDYN_HIP_H */

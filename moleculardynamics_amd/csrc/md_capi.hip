@@ -1,0 +1,903 @@
+// The C ABI (include/moldyn_hip.h): context, device state, and the host
+// orchestration of the kernels in md_neigh.hip, md_force.hip and
+// md_integrate.hip. No exception crosses this boundary; failures become
+// md_status codes with a message in md_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/moldyn_hip.h"
+#include "md_common.hpp"
+#include "md_kernels.hpp"
+
+struct md_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+
+    long n = 0;
+    double L[3] = {0, 0, 0};
+    mdk::Box box{};
+
+    double4 *pos = nullptr, *vel = nullptr, *frc = nullptr, *ref = nullptr;
+    double4 *pos2 = nullptr, *vel2 = nullptr, *frc2 = nullptr;
+    int *orig = nullptr, *slot = nullptr, *orig2 = nullptr, *cell2 = nullptr, *cell_of = nullptr;
+    double* stage = nullptr;
+
+    // Verlet list
+    bool neigh_configured = false;
+    bool list_built = false;
+    double rc = 0.0, skin = 0.0;
+    mdk::Grid grid{};
+    long ncell = 0, ncell_alloc = 0;
+    int *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *order = nullptr;
+    uint32_t* nbr = nullptr;
+    int* ncnt = nullptr;
+    long npad = 0;
+    int cap = 0, cap_user = 0;
+    long nbr_alloc = 0;
+
+    // cells at the LJ cutoff (MD_PAIRS_CELLS)
+    int *ci_cell_of = nullptr, *ci_count = nullptr, *ci_start = nullptr, *ci_fill = nullptr,
+        *ci_atoms = nullptr;
+    long ci_ncell_alloc = 0;
+
+    // reductions and device scalars
+    double* part_d = nullptr;
+    long long* part_l = nullptr;
+    mdk::Scalars* sc = nullptr;
+    mdk::Chain* chain = nullptr;
+    double* com4 = nullptr;
+
+    bool lj_set = false;
+    md_lj_params lj{};
+    mdk::LJConst ljc{};
+
+    bool integ_set = false;
+    md_integrator integ{};
+    uint64_t step_counter = 0;
+    bool chain_primed = false;
+
+    md_thermo* thermo_dev = nullptr;
+    long thermo_alloc = 0;
+
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
+    double ms[3] = {0, 0, 0};
+    long launches[3] = {0, 0, 0};
+};
+
+namespace {
+
+md_status fail(md_ctx* c, md_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+#define MD_HIP(ctx, call)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, MD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+hipError_t alloc(T*& p, long count) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    if (count <= 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(&p), static_cast<size_t>(count) * sizeof(T));
+}
+
+template <class T>
+void release(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+long blocks_of(long n) { return (n + mdk::kBlock - 1) / mdk::kBlock; }
+
+void set_lj_const(md_ctx* c) {
+    const md_lj_params& p = c->lj;
+    mdk::LJConst& k = c->ljc;
+    // u(rc) and du/dr(rc) as lennard_jones.cpp:32-48 computes them
+    const double s2 = (p.sigma * p.sigma) / (p.cutoff * p.cutoff);
+    const double s6 = s2 * s2 * s2;
+    const double u_rc = 4.0 * p.epsilon * (s6 * s6 - s6);
+    const double w_rc = 24.0 * p.epsilon * (2.0 * s6 * s6 - s6);
+    k.sig2 = p.sigma * p.sigma;
+    k.eps = p.epsilon;
+    k.rc = p.cutoff;
+    k.rc2 = p.cutoff * p.cutoff;
+    k.u_rc = u_rc;
+    k.dudr_rc = -w_rc / p.cutoff;
+    k.sig2f = static_cast<float>(k.sig2);
+    k.epsf = static_cast<float>(k.eps);
+    k.rc2f = static_cast<float>(k.rc2);
+    k.rcf = static_cast<float>(k.rc);
+    k.u_rcf = static_cast<float>(k.u_rc);
+    k.dudr_rcf = static_cast<float>(k.dudr_rc);
+    k.lfs = p.linear_force_shift ? 1 : 0;
+}
+
+mdk::Grid make_grid(const double L[3], double cutoff) {
+    mdk::Grid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = L[d];
+        g.nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / cutoff)));
+        g.w[d] = L[d] / static_cast<double>(g.nc[d]);
+    }
+    g.usable = (g.nc[0] >= 3 && g.nc[1] >= 3 && g.nc[2] >= 3) ? 1 : 0;
+    return g;
+}
+
+md_status need_system(md_ctx* c) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (c->n <= 0) return fail(c, MD_ERR_STATE, "no system uploaded");
+    return MD_OK;
+}
+
+md_status read_scalars(md_ctx* c, mdk::Scalars* out) {
+    MD_HIP(c, hipMemcpyAsync(out, c->sc, sizeof(mdk::Scalars), hipMemcpyDeviceToHost, c->stream));
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+void ev_begin(md_ctx* c, int id, hipEvent_t* a) {
+    if (!c->timing) return;
+    (void)hipEventCreate(a);
+    (void)hipEventRecord(*a, c->stream);
+    (void)id;
+}
+void ev_end(md_ctx* c, int id, hipEvent_t a) {
+    if (!c->timing) return;
+    hipEvent_t b;
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(b, c->stream);
+    c->events[id].emplace_back(a, b);
+}
+
+md_status resolve_events(md_ctx* c) {
+    for (int id = 0; id < 3; ++id) {
+        for (auto& e : c->events[id]) {
+            MD_HIP(c, hipEventSynchronize(e.second));
+            float ms = 0.0f;
+            MD_HIP(c, hipEventElapsedTime(&ms, e.first, e.second));
+            c->ms[id] += ms;
+            c->launches[id] += 1;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        c->events[id].clear();
+    }
+    return MD_OK;
+}
+
+// Neighbour row capacity: twice the mean full-list row at the system's
+// density plus 32, capped at n - 1.
+int auto_capacity(const md_ctx* c) {
+    const double rl = c->rc + c->skin;
+    const double rho = static_cast<double>(c->n) / (c->L[0] * c->L[1] * c->L[2]);
+    const double mean = 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho;
+    long cap = static_cast<long>(std::ceil(2.0 * mean + 32.0));
+    cap = (cap + 7) / 8 * 8;
+    cap = std::min(cap, std::max(1L, c->n - 1));
+    return static_cast<int>(cap);
+}
+
+md_status alloc_neigh(md_ctx* c) {
+    c->grid = make_grid(c->L, c->rc + c->skin);
+    c->ncell = c->grid.usable ? static_cast<long>(c->grid.nc[0]) * c->grid.nc[1] * c->grid.nc[2] : 1;
+    if (c->ncell > c->ncell_alloc) {
+        MD_HIP(c, alloc(c->cell_count, c->ncell));
+        MD_HIP(c, alloc(c->cell_start, c->ncell + 1));
+        MD_HIP(c, alloc(c->cell_fill, c->ncell));
+        c->ncell_alloc = c->ncell;
+    }
+    c->npad = (c->n + 63) / 64 * 64;
+    c->cap = c->cap_user > 0 ? c->cap_user : auto_capacity(c);
+    const long need = static_cast<long>(c->cap) * c->npad;
+    if (need > c->nbr_alloc) {
+        MD_HIP(c, alloc(c->nbr, need));
+        c->nbr_alloc = need;
+    }
+    return MD_OK;
+}
+
+mdk::NeighArgs neigh_args(md_ctx* c) {
+    mdk::NeighArgs a{};
+    a.n = c->n;
+    a.grid = c->grid;
+    a.box = c->box;
+    a.rl2 = (c->rc + c->skin) * (c->rc + c->skin);
+    a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.ref = c->ref;
+    a.orig = c->orig; a.slot = c->slot;
+    a.pos2 = c->pos2; a.vel2 = c->vel2; a.frc2 = c->frc2;
+    a.orig2 = c->orig2; a.cell2 = c->cell2;
+    a.cell_of = c->cell_of; a.cell_count = c->cell_count; a.cell_start = c->cell_start;
+    a.cell_fill = c->cell_fill; a.order = c->order;
+    a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
+    a.sc = c->sc;
+    a.flag = &c->sc->rebuild_flag;
+    return a;
+}
+
+mdk::ForceArgs force_args(md_ctx* c, int precision) {
+    mdk::ForceArgs a{};
+    a.n = c->n;
+    a.box = c->box;
+    a.lj = c->ljc;
+    a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.orig = c->orig;
+    a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
+    a.mixed = precision == MD_MIXED ? 1 : 0;
+    a.part_d = c->part_d; a.part_l = c->part_l;
+    a.sc = c->sc;
+    return a;
+}
+
+mdk::StepArgs step_args(md_ctx* c) {
+    mdk::StepArgs a{};
+    a.n = c->n;
+    a.box = c->box;
+    a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.ref = c->ref; a.orig = c->orig;
+    a.kind = c->integ.kind;
+    a.dt = c->integ.dt;
+    a.c1 = std::exp(-c->integ.friction * c->integ.dt);
+    a.c2 = std::sqrt(1.0 - a.c1 * a.c1);
+    a.temperature = c->integ.temperature;
+    a.seed = c->integ.seed;
+    a.skin = c->skin;
+    a.check = 1;
+    a.part_d = c->part_d;
+    a.sc = c->sc;
+    return a;
+}
+
+// Builds the list now (flag forced to 1), growing the row capacity until no
+// row overflows.
+md_status build_now(md_ctx* c) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        mdk::launch_set_flag(&c->sc->rebuild_flag, 1, c->stream);
+        hipEvent_t e0{};
+        ev_begin(c, 1, &e0);
+        mdk::launch_rebuild(neigh_args(c), c->stream);
+        ev_end(c, 1, e0);
+        MD_HIP(c, hipGetLastError());
+        mdk::Scalars s{};
+        md_status st = read_scalars(c, &s);
+        if (st != MD_OK) return st;
+        if (!s.overflow) {
+            c->list_built = true;
+            return MD_OK;
+        }
+        // grow and retry (the build counter is restored so one build counts once)
+        int zero = 0;
+        long long builds = s.builds - 1;
+        MD_HIP(c, hipMemcpyAsync(&c->sc->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        MD_HIP(c, hipMemcpyAsync(&c->sc->builds, &builds, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+        c->cap_user = std::min<long>(2L * c->cap, std::max(1L, c->n - 1));
+        st = alloc_neigh(c);
+        if (st != MD_OK) return st;
+    }
+    return fail(c, MD_ERR_CAPACITY, "neighbour row capacity could not be satisfied");
+}
+
+}  // namespace
+
+extern "C" {
+
+md_status md_ctx_create(void* hip_stream, md_ctx** out) {
+    if (out == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    md_ctx* c = new (std::nothrow) md_ctx();
+    if (c == nullptr) return MD_ERR_RUNTIME;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return MD_ERR_HIP;
+    }
+    if (hip_stream != nullptr) {
+        c->stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return MD_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&c->sc), sizeof(mdk::Scalars)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&c->chain), sizeof(mdk::Chain)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&c->com4), 8 * sizeof(double)) != hipSuccess) {
+        md_ctx_destroy(c);
+        return MD_ERR_HIP;
+    }
+    mdk::Scalars s{};
+    s.vscale = 1.0;
+    double zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpy(c->sc, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->com4, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess) {
+        md_ctx_destroy(c);
+        return MD_ERR_HIP;
+    }
+    *out = c;
+    return MD_OK;
+}
+
+md_status md_ctx_destroy(md_ctx* c) {
+    if (c == nullptr) return MD_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& v : c->events)
+        for (auto& e : v) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    release(c->pos); release(c->vel); release(c->frc); release(c->ref);
+    release(c->pos2); release(c->vel2); release(c->frc2);
+    release(c->orig); release(c->slot); release(c->orig2); release(c->cell2); release(c->cell_of);
+    release(c->stage);
+    release(c->cell_count); release(c->cell_start); release(c->cell_fill); release(c->order);
+    release(c->nbr); release(c->ncnt);
+    release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
+    release(c->ci_atoms);
+    release(c->part_d); release(c->part_l); release(c->sc); release(c->chain); release(c->com4);
+    release(c->thermo_dev);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MD_OK;
+}
+
+const char* md_last_error(const md_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+md_status md_ctx_synchronize(md_ctx* c) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy, const double* vz,
+                           const double* m) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (n <= 0 || n >= 0x7fffffffL) return fail(c, MD_ERR_INVALID_ARGUMENT, "atom count out of range");
+    if (L == nullptr || x == nullptr || y == nullptr || z == nullptr)
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "box and positions are required");
+    for (int d = 0; d < 3; ++d)
+        if (!(L[d] > 0.0)) return fail(c, MD_ERR_INVALID_ARGUMENT, "box lengths must be positive");
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    for (int d = 0; d < 3; ++d) c->L[d] = L[d];
+    c->box = mdk::make_box(L);
+    MD_HIP(c, alloc(c->pos, n)); MD_HIP(c, alloc(c->vel, n)); MD_HIP(c, alloc(c->frc, n));
+    MD_HIP(c, alloc(c->ref, n)); MD_HIP(c, alloc(c->pos2, n)); MD_HIP(c, alloc(c->vel2, n));
+    MD_HIP(c, alloc(c->frc2, n)); MD_HIP(c, alloc(c->orig, n)); MD_HIP(c, alloc(c->slot, n));
+    MD_HIP(c, alloc(c->orig2, n)); MD_HIP(c, alloc(c->cell2, n)); MD_HIP(c, alloc(c->cell_of, n));
+    MD_HIP(c, alloc(c->order, n)); MD_HIP(c, alloc(c->ncnt, n));
+    MD_HIP(c, alloc(c->stage, 9 * n));
+    const long nb = blocks_of(n);
+    MD_HIP(c, alloc(c->part_d, 4 * nb + 8));
+    MD_HIP(c, alloc(c->part_l, 3 * nb + 8));
+
+    std::vector<double4> p(static_cast<size_t>(n)), v(static_cast<size_t>(n)), f(static_cast<size_t>(n));
+    std::vector<int> id(static_cast<size_t>(n));
+    for (long i = 0; i < n; ++i) {
+        p[i] = make_double4(x[i], y[i], z[i], 0.0);
+        v[i] = make_double4(vx ? vx[i] : 0.0, vy ? vy[i] : 0.0, vz ? vz[i] : 0.0, m ? m[i] : 1.0);
+        f[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        id[i] = static_cast<int>(i);
+    }
+    MD_HIP(c, hipMemcpy(c->pos, p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    MD_HIP(c, hipMemcpy(c->ref, p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    MD_HIP(c, hipMemcpy(c->vel, v.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    MD_HIP(c, hipMemcpy(c->frc, f.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    MD_HIP(c, hipMemcpy(c->orig, id.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    MD_HIP(c, hipMemcpy(c->slot, id.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    mdk::Scalars s{};
+    s.vscale = 1.0;
+    MD_HIP(c, hipMemcpy(c->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+    c->list_built = false;
+    c->chain_primed = false;
+    if (c->neigh_configured) {
+        md_status st = alloc_neigh(c);
+        if (st != MD_OK) return st;
+    }
+    return MD_OK;
+}
+
+md_status md_system_download(md_ctx* c, double* x, double* y, double* z, double* vx, double* vy,
+                             double* vz, double* fx, double* fy, double* fz) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    const long n = c->n;
+    double* s = c->stage;
+    mdk::launch_unpermute(n, c->orig, c->pos, s, s + n, s + 2 * n, c->stream);
+    mdk::launch_unpermute(n, c->orig, c->vel, s + 3 * n, s + 4 * n, s + 5 * n, c->stream);
+    mdk::launch_unpermute(n, c->orig, c->frc, s + 6 * n, s + 7 * n, s + 8 * n, c->stream);
+    MD_HIP(c, hipGetLastError());
+    double* dst[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
+    for (int k = 0; k < 9; ++k)
+        if (dst[k])
+            MD_HIP(c, hipMemcpyAsync(dst[k], s + k * n, n * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+md_status md_system_set_positions(md_ctx* c, const double* x, const double* y, const double* z) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    const long n = c->n;
+    double* s = c->stage;
+    MD_HIP(c, hipMemcpyAsync(s, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + n, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + 2 * n, z, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->pos, 0, c->stream);
+    MD_HIP(c, hipGetLastError());
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+md_status md_system_set_velocities(md_ctx* c, const double* vx, const double* vy, const double* vz) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    const long n = c->n;
+    double* s = c->stage;
+    MD_HIP(c, hipMemcpyAsync(s, vx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + n, vy, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + 2 * n, vz, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->vel, 1, c->stream);
+    MD_HIP(c, hipGetLastError());
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+md_status md_system_kinetic_energy(md_ctx* c, double* ke) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_kinetic(c->n, c->vel, c->part_d, c->sc, c->stream);
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (ke) *ke = s.kinetic;
+    return MD_OK;
+}
+
+md_status md_system_remove_com_motion(md_ctx* c) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_remove_com(c->n, c->vel, c->part_d, c->com4, c->stream);
+    MD_HIP(c, hipGetLastError());
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
+md_status md_system_size(const md_ctx* c, long* n) {
+    if (c == nullptr || n == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    *n = c->n;
+    return MD_OK;
+}
+
+md_status md_lj_set(md_ctx* c, const md_lj_params* p) {
+    if (c == nullptr || p == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (!(p->cutoff > 0.0) || !(p->sigma > 0.0))
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "sigma and cutoff must be positive");
+    c->lj = *p;
+    set_lj_const(c);
+    c->lj_set = true;
+    return MD_OK;
+}
+
+md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, double* virial) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->lj_set) return fail(c, MD_ERR_STATE, "md_lj_set has not been called");
+    if (source == MD_PAIRS_VERLET) {
+        if (!c->list_built) return fail(c, MD_ERR_STATE, "the Verlet list has not been built");
+        mdk::ForceArgs a = force_args(c, precision);
+        hipEvent_t e0{};
+        ev_begin(c, 0, &e0);
+        mdk::launch_lj_force(a, c->stream);
+        ev_end(c, 0, e0);
+    } else if (source == MD_PAIRS_BRUTE || source == MD_PAIRS_CELLS) {
+        mdk::DirectArgs a{};
+        a.n = c->n;
+        a.box = c->box;
+        a.lj = c->ljc;
+        a.pos = c->pos;
+        a.frc = c->frc;
+        a.orig = c->orig;
+        a.part_d = c->part_d;
+        a.part_l = c->part_l;
+        a.sc = c->sc;
+        if (source == MD_PAIRS_CELLS) {
+            const mdk::Grid g = make_grid(c->L, c->lj.cutoff);
+            if (!g.usable)
+                return fail(c, MD_ERR_INVALID_ARGUMENT,
+                            "CellList is not usable (fewer than 3 cells per side); fall back to "
+                            "the brute-force overload");
+            const long m = static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2];
+            if (m > c->ci_ncell_alloc) {
+                MD_HIP(c, alloc(c->ci_count, m));
+                MD_HIP(c, alloc(c->ci_start, m + 1));
+                MD_HIP(c, alloc(c->ci_fill, m));
+                c->ci_ncell_alloc = m;
+            }
+            MD_HIP(c, alloc(c->ci_cell_of, c->n));
+            MD_HIP(c, alloc(c->ci_atoms, c->n));
+            mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start,
+                                  c->ci_fill, c->ci_atoms, c->stream);
+            a.grid = g;
+            a.cell_of = c->ci_cell_of;
+            a.cell_start = c->ci_start;
+            a.cell_atoms = c->ci_atoms;
+        }
+        mdk::launch_lj_direct(a, c->stream);
+    } else {
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "unknown pair source");
+    }
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (s.fix_saturated)
+        return fail(c, MD_ERR_RUNTIME, "fixed-point force accumulator saturated (|f| > 2^19)");
+    if (energy) *energy = s.energy;
+    if (virial) *virial = s.virial;
+    return MD_OK;
+}
+
+md_status md_lj_energy_virial(md_ctx* c, double* energy, double* virial) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->lj_set) return fail(c, MD_ERR_STATE, "md_lj_set has not been called");
+    mdk::DirectArgs a{};
+    a.n = c->n;
+    a.box = c->box;
+    a.lj = c->ljc;
+    a.pos = c->pos;
+    a.frc = nullptr;
+    a.orig = c->orig;
+    a.part_d = c->part_d;
+    a.part_l = c->part_l;
+    a.sc = c->sc;
+    mdk::launch_lj_direct(a, c->stream);
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (energy) *energy = s.energy;
+    if (virial) *virial = s.virial;
+    return MD_OK;
+}
+
+md_status md_lj_tail(const md_lj_params* p, long n, const double* L, double* u_lrc, double* p_lrc) {
+    if (p == nullptr || L == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    double u = 0.0, pr = 0.0;
+    if (!p->linear_force_shift) {
+        const double kPi = 3.14159265358979323846;
+        const double nn = static_cast<double>(n);
+        const double rho = nn / (L[0] * L[1] * L[2]);
+        const double t = p->sigma / p->cutoff;
+        const double t3 = t * t * t;
+        const double t9 = t3 * t3 * t3;
+        const double sig3 = p->sigma * p->sigma * p->sigma;
+        u = (8.0 / 3.0) * kPi * nn * rho * p->epsilon * sig3 * ((1.0 / 3.0) * t9 - t3);
+        pr = (16.0 / 3.0) * kPi * rho * rho * p->epsilon * sig3 * ((2.0 / 3.0) * t9 - t3);
+    }
+    if (u_lrc) *u_lrc = u;
+    if (p_lrc) *p_lrc = pr;
+    return MD_OK;
+}
+
+int md_cells_usable(const double* L, double cutoff) {
+    if (L == nullptr || !(cutoff > 0.0)) return 0;
+    return make_grid(L, cutoff).usable;
+}
+
+md_status md_neigh_config(md_ctx* c, double cutoff, double skin) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (!(cutoff > 0.0) || !(skin >= 0.0))
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "cutoff must be positive and skin non-negative");
+    c->rc = cutoff;
+    c->skin = skin;
+    c->neigh_configured = true;
+    c->list_built = false;
+    if (c->n > 0) return alloc_neigh(c);
+    return MD_OK;
+}
+
+md_status md_neigh_build(md_ctx* c) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->neigh_configured) return fail(c, MD_ERR_STATE, "md_neigh_config has not been called");
+    return build_now(c);
+}
+
+md_status md_neigh_needs_rebuild(md_ctx* c, int* flag) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->list_built) return fail(c, MD_ERR_STATE, "the Verlet list has not been built");
+    mdk::StepArgs a = step_args(c);
+    mdk::launch_check(a, c->stream);
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (flag) *flag = s.rebuild_flag;
+    return MD_OK;
+}
+
+md_status md_neigh_rebuild_if_needed(md_ctx* c, int* rebuilt) {
+    int flag = 0;
+    md_status st = md_neigh_needs_rebuild(c, &flag);
+    if (st != MD_OK) return st;
+    if (flag) st = build_now(c);
+    if (rebuilt) *rebuilt = flag;
+    return st;
+}
+
+md_status md_neigh_rebuild_count(const md_ctx* c, long* count) {
+    if (c == nullptr || count == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::Scalars s{};
+    md_ctx* m = const_cast<md_ctx*>(c);
+    md_status st = read_scalars(m, &s);
+    if (st != MD_OK) return st;
+    *count = static_cast<long>(s.builds);
+    return MD_OK;
+}
+
+md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->list_built) return fail(c, MD_ERR_STATE, "the Verlet list has not been built");
+    const long n = c->n;
+    std::vector<int> cnt(static_cast<size_t>(n)), orig(static_cast<size_t>(n));
+    std::vector<uint32_t> nb(static_cast<size_t>(c->cap) * c->npad);
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    MD_HIP(c, hipMemcpy(cnt.data(), c->ncnt, n * sizeof(int), hipMemcpyDeviceToHost));
+    MD_HIP(c, hipMemcpy(orig.data(), c->orig, n * sizeof(int), hipMemcpyDeviceToHost));
+    MD_HIP(c, hipMemcpy(nb.data(), c->nbr, nb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> keys;
+    keys.reserve(static_cast<size_t>(n) * 40);
+    for (long t = 0; t < n; ++t) {
+        const int k = std::min(cnt[t], c->cap);
+        for (int q = 0; q < k; ++q) {
+            const uint32_t e = nb[static_cast<size_t>(q) * c->npad + t];
+            const uint32_t j = e & mdk::kIndexMask;
+            const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
+            if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    if (npairs) *npairs = static_cast<long>(keys.size());
+    if (static_cast<long>(keys.size()) > cap)
+        return fail(c, MD_ERR_CAPACITY, "pair buffer too small");
+    for (size_t k = 0; k < keys.size(); ++k) {
+        ij[2 * k] = static_cast<uint32_t>(keys[k] >> 32);
+        ij[2 * k + 1] = static_cast<uint32_t>(keys[k] & 0xffffffffu);
+    }
+    return MD_OK;
+}
+
+md_status md_neigh_set_capacity(md_ctx* c, int per_atom) {
+    if (c == nullptr || per_atom < 0) return MD_ERR_INVALID_ARGUMENT;
+    c->cap_user = per_atom;
+    c->list_built = false;
+    if (c->n > 0 && c->neigh_configured) return alloc_neigh(c);
+    return MD_OK;
+}
+
+md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
+    if (c == nullptr || it == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (!(it->dt > 0.0)) return fail(c, MD_ERR_INVALID_ARGUMENT, "dt must be positive");
+    if (it->kind == MD_NOSE_HOOVER) {
+        if (it->chain_length < 1)
+            return fail(c, MD_ERR_INVALID_ARGUMENT, "NoseHooverChain: chainLength must be >= 1");
+        if (it->chain_length > mdk::kMaxChain)
+            return fail(c, MD_ERR_INVALID_ARGUMENT, "chain length above the supported maximum (16)");
+        mdk::Chain ch{};
+        ch.M = it->chain_length;
+        ch.dof = static_cast<double>(it->dof);
+        ch.T = it->temperature;
+        // Q_1 = dof T tau^2, Q_k = T tau^2 (nose_hoover.cpp:45-49)
+        for (int k = 0; k < ch.M; ++k) ch.Q[k] = it->temperature * it->tau * it->tau;
+        ch.Q[0] = static_cast<double>(it->dof) * it->temperature * it->tau * it->tau;
+        MD_HIP(c, hipMemcpy(c->chain, &ch, sizeof(ch), hipMemcpyHostToDevice));
+    } else if (it->kind != MD_VELOCITY_VERLET && it->kind != MD_LANGEVIN_BAOAB) {
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "unknown integrator kind");
+    }
+    c->integ = *it;
+    c->integ_set = true;
+    c->step_counter = 0;
+    return MD_OK;
+}
+
+md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->lj_set || !c->integ_set || !c->list_built)
+        return fail(c, MD_ERR_STATE, "md_run needs md_lj_set, md_integrator_set and a built list");
+    if (nsteps <= 0) return MD_OK;
+    if (thermo && nsteps > c->thermo_alloc) {
+        MD_HIP(c, alloc(c->thermo_dev, nsteps));
+        c->thermo_alloc = nsteps;
+    }
+    const bool nhc = c->integ.kind == MD_NOSE_HOOVER;
+    mdk::StepArgs sa = step_args(c);
+    mdk::NeighArgs na = neigh_args(c);
+    mdk::ForceArgs fa = force_args(c, precision);
+    fa.kick = 1;
+    fa.half_dt = 0.5 * c->integ.dt;
+    fa.chain = nhc ? c->chain : nullptr;
+    fa.chain_delta = 0.5 * c->integ.dt;
+    if (nhc) {
+        // the first half-chain of the run: kinetic energy of the current
+        // velocities, one chain propagation of dt/2, rescale left pending in
+        // vscale for the first pre-step pass
+        mdk::launch_kinetic(c->n, c->vel, c->part_d, c->sc, c->stream);
+        mdk::launch_chain_first(c->chain, c->sc, 0.5 * c->integ.dt, c->stream);
+    }
+    for (long k = 0; k < nsteps; ++k) {
+        sa.step = c->step_counter++;
+        hipEvent_t e2{};
+        ev_begin(c, 2, &e2);
+        mdk::launch_pre_step(sa, c->stream);
+        ev_end(c, 2, e2);
+        hipEvent_t e1{};
+        ev_begin(c, 1, &e1);
+        mdk::launch_rebuild(na, c->stream);
+        ev_end(c, 1, e1);
+        fa.thermo = thermo ? static_cast<void*>(c->thermo_dev + k) : nullptr;
+        fa.chain_twice = (nhc && k + 1 < nsteps) ? 1 : 0;
+        hipEvent_t e0{};
+        ev_begin(c, 0, &e0);
+        mdk::launch_lj_force(fa, c->stream);
+        ev_end(c, 0, e0);
+    }
+    if (nhc) mdk::launch_apply_scale(c->n, c->vel, c->sc, c->stream);
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (s.overflow) return fail(c, MD_ERR_CAPACITY, "neighbour row capacity exceeded during md_run");
+    if (s.fix_saturated)
+        return fail(c, MD_ERR_RUNTIME, "fixed-point force accumulator saturated (|f| > 2^19)");
+    if (thermo)
+        MD_HIP(c, hipMemcpy(thermo, c->thermo_dev, nsteps * sizeof(md_thermo), hipMemcpyDeviceToHost));
+    return MD_OK;
+}
+
+md_status md_nhc_state(md_ctx* c, double* conserved, double* xi, double* p_xi) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::Chain ch{};
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    MD_HIP(c, hipMemcpy(&ch, c->chain, sizeof(ch), hipMemcpyDeviceToHost));
+    if (c->integ.kind != MD_NOSE_HOOVER) {
+        if (conserved) *conserved = 0.0;
+        return MD_OK;
+    }
+    double acc = ch.dof * ch.T * ch.xi[0];
+    for (int k = 1; k < ch.M; ++k) acc += ch.T * ch.xi[k];
+    for (int k = 0; k < ch.M; ++k) acc += 0.5 * ch.p[k] * ch.p[k] / ch.Q[k];
+    if (conserved) *conserved = acc;
+    for (int k = 0; k < ch.M; ++k) {
+        if (xi) xi[k] = ch.xi[k];
+        if (p_xi) p_xi[k] = ch.p[k];
+    }
+    return MD_OK;
+}
+
+md_status md_timing_enable(md_ctx* c, int on) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    c->timing = on != 0;
+    return MD_OK;
+}
+
+md_status md_timing_read(md_ctx* c, int kernel_id, double* total_ms, long* launches) {
+    if (c == nullptr || kernel_id < 0 || kernel_id > 2) return MD_ERR_INVALID_ARGUMENT;
+    md_status st = resolve_events(c);
+    if (st != MD_OK) return st;
+    if (total_ms) *total_ms = c->ms[kernel_id];
+    if (launches) *launches = c->launches[kernel_id];
+    return MD_OK;
+}
+
+md_status md_timing_reset(md_ctx* c) {
+    if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    md_status st = resolve_events(c);
+    for (int k = 0; k < 3; ++k) {
+        c->ms[k] = 0.0;
+        c->launches[k] = 0;
+    }
+    return st;
+}
+
+// Seeded fcc lattice (util/lattice.cpp:15-64) with the reference's random
+// source (util/random.hpp:37-76): mt19937_64, uniform u = 1 - U[0,1) in
+// (0,1], Marsaglia polar normals with a cached spare.
+md_status md_fcc_lattice(int cells, double density, double temperature, uint64_t seed, double* L,
+                         double* x, double* y, double* z, double* vx, double* vy, double* vz) {
+#pragma clang fp contract(off)
+    if (cells < 1 || !(density > 0.0) || L == nullptr || x == nullptr || y == nullptr ||
+        z == nullptr || vx == nullptr || vy == nullptr || vz == nullptr)
+        return MD_ERR_INVALID_ARGUMENT;
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> unif(0.0, 1.0);
+    bool have_spare = false;
+    double spare = 0.0;
+    auto uniform = [&]() { return 1.0 - unif(eng); };
+    auto normal = [&]() {
+        if (have_spare) {
+            have_spare = false;
+            return spare;
+        }
+        double a, b, s;
+        do {
+            a = 2.0 * uniform() - 1.0;
+            b = 2.0 * uniform() - 1.0;
+            s = a * a + b * b;
+        } while (s >= 1.0 || s == 0.0);
+        const double k = std::sqrt(-2.0 * std::log(s) / s);
+        spare = b * k;
+        have_spare = true;
+        return a * k;
+    };
+    const long n = 4L * cells * cells * cells;
+    const double len = std::cbrt(static_cast<double>(n) / density);
+    const double lat = len / static_cast<double>(cells);
+    const double sv = std::sqrt(temperature / 1.0);
+    const double basis[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.5, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}};
+    auto wrap = [len](double v) { return v - len * std::floor(v / len + 0.5); };
+    long k = 0;
+    for (int ix = 0; ix < cells; ++ix)
+        for (int iy = 0; iy < cells; ++iy)
+            for (int iz = 0; iz < cells; ++iz)
+                for (const auto& b : basis) {
+                    x[k] = wrap(lat * (static_cast<double>(ix) + b[0]) - 0.5 * len);
+                    y[k] = wrap(lat * (static_cast<double>(iy) + b[1]) - 0.5 * len);
+                    z[k] = wrap(lat * (static_cast<double>(iz) + b[2]) - 0.5 * len);
+                    const double g0 = normal();
+                    const double g1 = normal();
+                    const double g2 = normal();
+                    vx[k] = sv * g0;
+                    vy[k] = sv * g1;
+                    vz[k] = sv * g2;
+                    ++k;
+                }
+    // centre-of-mass removal with unit masses (core/system.cpp:147-164)
+    double mass = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (long i = 0; i < n; ++i) mass += 1.0;
+    for (long i = 0; i < n; ++i) {
+        px += 1.0 * vx[i];
+        py += 1.0 * vy[i];
+        pz += 1.0 * vz[i];
+    }
+    const double cx = px / mass, cy = py / mass, cz = pz / mass;
+    for (long i = 0; i < n; ++i) {
+        vx[i] -= cx;
+        vy[i] -= cy;
+        vz[i] -= cz;
+    }
+    L[0] = L[1] = L[2] = len;
+    return MD_OK;
+}
+
+md_status md_last_pair_count(md_ctx* c, long* pairs) {
+    if (c == nullptr || pairs == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::Scalars s{};
+    md_status st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    *pairs = static_cast<long>(s.pairs_in_cut);
+    return MD_OK;
+}
+
+}  // extern "C"

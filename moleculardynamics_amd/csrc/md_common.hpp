@@ -1,0 +1,199 @@
+// Shared device helpers and the device-side state layout of the MI355X
+// moldyn path.
+//
+// Layout in HBM (all arrays in cell-sorted slot order t = 0..n-1):
+//   pos  double4  x, y, z, (w unused)            32 B/atom, gathered by the pair kernel
+//   vel  double4  vx, vy, vz, mass               32 B/atom
+//   frc  double4  fx, fy, fz, (w unused)         32 B/atom
+//   ref  double4  positions at the last list build (displacement check)
+//   orig int32    upload index of slot t; slot int32 inverse map
+//   nbr  uint32   full Verlet list, column-major: nbr[k*npad + t], bit 31 set
+//                 when the upload index of t is larger than that of the
+//                 neighbour (the pair is oriented as the reference's i<j)
+//   ncnt int32    row length of t
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mdk {
+
+constexpr int kBlock = 256;     // 4 wave64s per workgroup
+constexpr int kWave = 64;
+constexpr uint32_t kFlipBit = 0x80000000u;
+constexpr uint32_t kIndexMask = 0x7fffffffu;
+constexpr double kFixScale = 4294967296.0;          // 2^32: fixed-point resolution 2.3e-10
+constexpr double kFixInv = 1.0 / 4294967296.0;
+constexpr double kMagic = 6755399441055744.0;        // 1.5 * 2^52
+
+// Device scalars shared by the step kernels (one instance per context).
+struct Scalars {
+    double energy, virial, kinetic;  // last reductions (FP64)
+    double top1, top2;               // displacement top-2 of the last check
+    double vscale;                   // pending velocity scale (NHC), 1 otherwise
+    long long pairs_in_cut;          // unordered pairs inside the cutoff, last force call
+    long long builds;                // list builds performed on the device
+    int rebuild_flag;                // 1 when the next rebuild kernels must run
+    int overflow;                    // neighbour row overflow seen
+    int fix_saturated;               // fixed-point conversion saturated
+    int pad;
+    unsigned int counter[8];         // last-workgroup tickets, one per kernel family
+};
+
+// Nose-Hoover chain state (MTK96), held on the device so a whole run stays
+// there. Chains longer than kMaxChain are refused at configuration.
+constexpr int kMaxChain = 16;
+struct Chain {
+    int M;
+    double dof, T;
+    double Q[kMaxChain], xi[kMaxChain], p[kMaxChain];
+};
+
+struct Box {
+    double L[3];
+    // thresholds for the exact fast fold (see fold())
+    double lo[3], hi1[3], hi2[3];
+};
+
+inline Box make_box(const double L[3]) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.L[d] = L[d];
+        b.lo[d] = 0.4999999 * L[d];
+        b.hi1[d] = 0.5000001 * L[d];
+        b.hi2[d] = 1.4999999 * L[d];
+    }
+    return b;
+}
+
+// v - L*floor(v/L + 0.5) with IEEE division and no contraction: the
+// reference's periodic fold (core/box.cpp:12-14) bit for bit.
+__device__ __forceinline__ double fold_ref(double v, double len) {
+#pragma clang fp contract(off)
+    const double k = floor(v / len + 0.5);
+    const double s = len * k;
+    return v - s;
+}
+
+// The same function, exact, without the division on the common paths:
+// |v| <= 0.4999999 L gives k = 0 (returns v); 0.5000001 L <= |v| <= 1.4999999 L
+// gives k = sign(v) (returns v - sign(v) L, one correctly rounded subtraction,
+// as L*k is exact); the narrow bands between fall back to fold_ref.
+__device__ __forceinline__ double fold(double v, double len, double lo, double hi1, double hi2) {
+#pragma clang fp contract(off)
+    const double a = fabs(v);
+    if (a <= lo) return v;
+    if (a >= hi1 && a <= hi2) return v - copysign(len, v);
+    return fold_ref(v, len);
+}
+
+__device__ __forceinline__ void min_image(const Box& b, double& dx, double& dy, double& dz) {
+    dx = fold(dx, b.L[0], b.lo[0], b.hi1[0], b.hi2[0]);
+    dy = fold(dy, b.L[1], b.lo[1], b.hi1[1], b.hi2[1]);
+    dz = fold(dz, b.L[2], b.lo[2], b.hi1[2], b.hi2[2]);
+}
+
+// (x*x + y*y) + z*z with separate roundings (core/vec3.hpp:59-65).
+__device__ __forceinline__ double norm2_ref(double x, double y, double z) {
+#pragma clang fp contract(off)
+    double s = x * x;
+    s = s + y * y;
+    s = s + z * z;
+    return s;
+}
+
+// Oriented minimum-image difference of the unordered pair (a, b): the
+// reference evaluates r_i - r_j with i < j by upload index
+// (forcefield/lennard_jones.cpp:71, neighbor/verlet_list.cpp:19). `flip`
+// says a is the larger index, so the difference is r_b - r_a.
+__device__ __forceinline__ void pair_delta(const Box& b, const double4& pa, const double4& pb,
+                                           bool flip, double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+    if (flip) {
+        dx = pb.x - pa.x;
+        dy = pb.y - pa.y;
+        dz = pb.z - pa.z;
+    } else {
+        dx = pa.x - pb.x;
+        dy = pa.y - pb.y;
+        dz = pa.z - pb.z;
+    }
+    min_image(b, dx, dy, dz);
+}
+
+// ---- wave64 / workgroup reductions (fixed tree: deterministic) -------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over the workgroup of kBlock threads; result valid in thread 0.
+template <class T, class WaveFn>
+__device__ __forceinline__ T block_reduce(T v, T* smem, WaveFn wf) {
+    v = wf(v);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    if (lane == 0) smem[wid] = v;
+    __syncthreads();
+    T r = T(0);
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kBlock / kWave; ++w) r += smem[w];
+    }
+    __syncthreads();
+    return r;
+}
+
+// Top-2 (largest, second largest, with multiplicity) merge.
+__device__ __forceinline__ void top2_merge(double& a1, double& a2, double b1, double b2) {
+    if (b1 > a1) {
+        a2 = fmax(a1, b2);
+        a1 = b1;
+    } else {
+        a2 = fmax(a2, b1);
+    }
+}
+
+__device__ __forceinline__ void wave_top2(double& t1, double& t2) {
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o1 = __shfl_xor(t1, off, kWave);
+        const double o2 = __shfl_xor(t2, off, kWave);
+        top2_merge(t1, t2, o1, o2);
+    }
+}
+
+// Last-workgroup election: every workgroup publishes its partials, then the
+// one that takes the final ticket reduces them in a fixed order. Returns
+// true in all threads of the last workgroup (and resets the ticket).
+__device__ __forceinline__ bool last_block(unsigned int* ticket) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int t = atomicAdd(ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+        if (is_last) *ticket = 0u;
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+
+// Float to 64-bit fixed point (scale 2^32), round to nearest even, via the
+// 1.5*2^52 bias; values beyond +-2^19 saturate and raise *sat.
+__device__ __forceinline__ long long to_fix(float v, int* sat) {
+    const double d = static_cast<double>(v) * kFixScale;
+    if (fabs(d) < 2251799813685248.0) {  // 2^51
+        const double t = d + kMagic;
+        return __double_as_longlong(t) - __double_as_longlong(kMagic);
+    }
+    atomicOr(sat, 1);
+    return d > 0 ? (1LL << 62) : -(1LL << 62);
+}
+
+}  // namespace mdk

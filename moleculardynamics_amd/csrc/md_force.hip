@@ -1,0 +1,470 @@
+// Lennard-Jones pair kernels.
+//
+// k_lj: one thread per atom over its row of the full Verlet list (each
+// unordered pair is evaluated from both atoms, so every thread owns its
+// atom's force outright: no atomics, a fixed per-thread summation order, and
+// a deterministic result). Pair arithmetic follows
+// forcefield/lennard_jones.cpp:50-88:
+//   d = minimumImage(r_i - r_j) (i<j by upload index), r2 = |d|^2,
+//   skip unless r2 < rc2, s2 = sig^2/r2, s6 = s2^3,
+//   u = 4 eps (s6^2 - s6), w = 24 eps (2 s6^2 - s6),
+//   LinearForceShift: u -= u(rc) + (r - rc) u'(rc), w += r u'(rc),
+//   f_i = +(w/r2) d, f_j = -(w/r2) d, E = sum u, W = sum w.
+// FP64: the minimum image and r2 are evaluated exactly as the reference does
+// (same cutoff decisions); s2 and w/r2 use one reciprocal (a last-bit
+// difference from the reference's two divisions, far inside 1e-12).
+// MIXED: d is formed in FP64 and rounded to FP32; r2, s2, s6, u, w and the
+// force components are FP32; per-atom force, energy and virial accumulate
+// in 64-bit fixed point (scale 2^32), so sums are exact integer additions.
+//
+// E and W come out of a fixed reduction tree: wave shuffles, a 4-wave LDS
+// step, per-workgroup partials, and the last workgroup summing the partials
+// in index order. The last workgroup also records the step's thermo row and
+// advances the Nose-Hoover chain when one is attached.
+#include "md_common.hpp"
+#include "md_kernels.hpp"
+
+#include "../../include/moldyn_hip.h"
+
+namespace mdk {
+
+namespace {
+
+// One MTK96 chain propagation of length delta with Suzuki-Yoshida weights
+// (integrate/nose_hoover.cpp:63-124). The chain rescales every velocity by a
+// common factor, so the kinetic energy after a rescale is KE * f^2; it is
+// tracked that way instead of re-summing the velocities. Returns the product
+// of the three rescale factors and updates ke.
+__device__ double chain_half(Chain& c, double& ke, double delta) {
+    const double w1 = 1.0 / (2.0 - cbrt(2.0));
+    const double ws[3] = {w1, 1.0 - 2.0 * w1, w1};
+    const int M = c.M;
+    double scale = 1.0;
+    for (int s = 0; s < 3; ++s) {
+        const double d = ws[s] * delta;
+        const double d2 = 0.5 * d, d4 = 0.25 * d;
+        auto link = [&](int j) {
+            const bool next = j + 1 < M;
+            if (next) c.p[j] *= exp(-d4 * c.p[j + 1] / c.Q[j + 1]);
+            const double g = (j == 0) ? 2.0 * ke - c.dof * c.T
+                                      : c.p[j - 1] * c.p[j - 1] / c.Q[j - 1] - c.T;
+            c.p[j] += d2 * g;
+            if (next) c.p[j] *= exp(-d4 * c.p[j + 1] / c.Q[j + 1]);
+        };
+        for (int j = M - 1; j >= 0; --j) link(j);
+        for (int k = 0; k < M; ++k) c.xi[k] += d * c.p[k] / c.Q[k];
+        const double f = exp(-d * c.p[0] / c.Q[0]);
+        scale *= f;
+        ke *= f * f;
+        for (int j = 0; j < M; ++j) link(j);
+    }
+    return scale;
+}
+
+__device__ double chain_conserved(const Chain& c) {
+    double acc = c.dof * c.T * c.xi[0];
+    for (int k = 1; k < c.M; ++k) acc += c.T * c.xi[k];
+    for (int k = 0; k < c.M; ++k) acc += 0.5 * c.p[k] * c.p[k] / c.Q[k];
+    return acc;
+}
+
+// Final reduction by the last workgroup: partial sums in index order.
+__device__ void finish_step(const ForceArgs& a, bool mixed) {
+    __shared__ double red_d[3][kBlock];
+    __shared__ long long red_l[3][kBlock];
+    const int nb = gridDim.x;
+    for (int q = 0; q < 3; ++q) {
+        double sd = 0.0;
+        long long sl = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) {
+            sd += a.part_d[3 * b + q];
+            sl += a.part_l[3 * b + q];
+        }
+        red_d[q][threadIdx.x] = sd;
+        red_l[q][threadIdx.x] = sl;
+    }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int q = 0; q < 3; ++q) {
+                red_d[q][threadIdx.x] += red_d[q][threadIdx.x + off];
+                red_l[q][threadIdx.x] += red_l[q][threadIdx.x + off];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double E, W;
+    if (mixed) {
+        E = 0.5 * (static_cast<double>(red_l[0][0]) * kFixInv);
+        W = 0.5 * (static_cast<double>(red_l[1][0]) * kFixInv);
+    } else {
+        E = 0.5 * red_d[0][0];
+        W = 0.5 * red_d[1][0];
+    }
+    double ke = red_d[2][0];
+    Scalars* sc = a.sc;
+    sc->energy = E;
+    sc->virial = W;
+    sc->pairs_in_cut = red_l[2][0];
+    double conserved = 0.0;
+    if (a.kick) {
+        if (a.chain != nullptr) {
+            Chain& c = *a.chain;
+            double s = chain_half(c, ke, a.chain_delta);
+            conserved = chain_conserved(c);
+            if (a.chain_twice) {
+                double ke2 = ke;
+                s *= chain_half(c, ke2, a.chain_delta);
+            }
+            sc->vscale = s;
+        }
+        sc->kinetic = ke;
+    }
+    if (a.thermo != nullptr) {
+        md_thermo* th = static_cast<md_thermo*>(a.thermo);
+        th->potential = E;
+        th->virial = W;
+        th->kinetic = ke;
+        th->conserved = conserved;
+        th->pairs = static_cast<double>(red_l[2][0]);
+    }
+}
+
+template <bool LFS, bool MIXED, bool KICK>
+__global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
+    __shared__ double sm_d[kBlock / kWave];
+    __shared__ long long sm_l[kBlock / kWave];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double e = 0.0, w = 0.0, ke = 0.0;
+    long long ef = 0, wf = 0, np = 0;
+    if (t < a.n) {
+        const double4 pi = a.pos[t];
+        const int cnt = min(a.ncnt[t], a.cap);
+        const uint32_t* row = a.nbr + t;
+        double fx, fy, fz;
+        if (!MIXED) {
+            fx = fy = fz = 0.0;
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t jj = row[static_cast<long>(k) * a.npad];
+                const bool flip = (jj & kFlipBit) != 0;
+                const double4 pj = a.pos[jj & kIndexMask];
+                double dx, dy, dz;
+                pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+                const double r2 = norm2_ref(dx, dy, dz);
+                if (r2 < a.lj.rc2) {
+                    const double inv = 1.0 / r2;
+                    const double s2 = a.lj.sig2 * inv;
+                    const double s6 = s2 * s2 * s2;
+                    double u = 4.0 * a.lj.eps * (s6 * s6 - s6);
+                    double ww = 24.0 * a.lj.eps * (2.0 * s6 * s6 - s6);
+                    if (LFS) {
+                        const double r = sqrt(r2);
+                        u = u - a.lj.u_rc - (r - a.lj.rc) * a.lj.dudr_rc;
+                        ww = ww + r * a.lj.dudr_rc;
+                    }
+                    e += u;
+                    w += ww;
+                    np += flip ? 0 : 1;
+                    const double c = flip ? -(ww * inv) : ww * inv;
+                    fx += c * dx;
+                    fy += c * dy;
+                    fz += c * dz;
+                }
+            }
+        } else {
+            long long ax = 0, ay = 0, az = 0;
+            int sat = 0;
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t jj = row[static_cast<long>(k) * a.npad];
+                const bool flip = (jj & kFlipBit) != 0;
+                const double4 pj = a.pos[jj & kIndexMask];
+                double dx, dy, dz;
+                pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+                const float xf = static_cast<float>(dx);
+                const float yf = static_cast<float>(dy);
+                const float zf = static_cast<float>(dz);
+                const float r2 = xf * xf + yf * yf + zf * zf;
+                if (r2 < a.lj.rc2f) {
+                    const float inv = 1.0f / r2;
+                    const float s2 = a.lj.sig2f * inv;
+                    const float s6 = s2 * s2 * s2;
+                    float u = 4.0f * a.lj.epsf * (s6 * s6 - s6);
+                    float ww = 24.0f * a.lj.epsf * (2.0f * s6 * s6 - s6);
+                    if (LFS) {
+                        const float r = sqrtf(r2);
+                        u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
+                        ww = ww + r * a.lj.dudr_rcf;
+                    }
+                    ef += to_fix(u, &sat);
+                    wf += to_fix(ww, &sat);
+                    np += flip ? 0 : 1;
+                    const float c = flip ? -(ww * inv) : ww * inv;
+                    ax += to_fix(c * xf, &sat);
+                    ay += to_fix(c * yf, &sat);
+                    az += to_fix(c * zf, &sat);
+                }
+            }
+            if (sat) atomicOr(&a.sc->fix_saturated, 1);
+            fx = static_cast<double>(ax) * kFixInv;
+            fy = static_cast<double>(ay) * kFixInv;
+            fz = static_cast<double>(az) * kFixInv;
+        }
+        a.frc[t] = make_double4(fx, fy, fz, 0.0);
+        if (KICK) {
+            double4 v = a.vel[t];
+            const double ch = a.half_dt * (1.0 / v.w);
+            v.x = fma(ch, fx, v.x);
+            v.y = fma(ch, fy, v.y);
+            v.z = fma(ch, fz, v.z);
+            a.vel[t] = v;
+            ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+        }
+    }
+    // workgroup partials
+    const double be = block_reduce(e, sm_d, wave_sum);
+    const double bw = block_reduce(w, sm_d, wave_sum);
+    const double bk = block_reduce(ke, sm_d, wave_sum);
+    const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
+    const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
+    const long long bnp = block_reduce(np, sm_l, wave_sum_ll);
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.x;
+        a.part_d[3 * b + 0] = be;
+        a.part_d[3 * b + 1] = bw;
+        a.part_d[3 * b + 2] = bk;
+        a.part_l[3 * b + 0] = bef;
+        a.part_l[3 * b + 1] = bwf;
+        a.part_l[3 * b + 2] = bnp;
+    }
+    if (last_block(&a.sc->counter[0])) finish_step(a, MIXED);
+}
+
+template <bool LFS, bool MIXED>
+void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
+    if (a.kick)
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, true>), g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, false>), g, dim3(kBlock), 0, s, a);
+}
+
+// ---- direct (no list) evaluations: brute force and cells at the cutoff ----
+
+__device__ void finish_direct(const DirectArgs& a) {
+    __shared__ double red[2][kBlock];
+    __shared__ long long redl[kBlock];
+    const int nb = gridDim.x;
+    double s0 = 0, s1 = 0;
+    long long s2 = 0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        s0 += a.part_d[3 * b];
+        s1 += a.part_d[3 * b + 1];
+        s2 += a.part_l[3 * b];
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    redl[threadIdx.x] = s2;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+            redl[threadIdx.x] += redl[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.sc->energy = 0.5 * red[0][0];
+        a.sc->virial = 0.5 * red[1][0];
+        a.sc->pairs_in_cut = redl[0];
+    }
+}
+
+template <bool LFS>
+__device__ __forceinline__ void direct_pair(const DirectArgs& a, const double4& pi, int oi, int j,
+                                            double& e, double& w, double& fx, double& fy,
+                                            double& fz, long long& np) {
+    const bool flip = oi > a.orig[j];
+    double dx, dy, dz;
+    pair_delta(a.box, pi, a.pos[j], flip, dx, dy, dz);
+    const double r2 = norm2_ref(dx, dy, dz);
+    if (!(r2 < a.lj.rc2)) return;
+    // the reference's two divisions, for the standalone calls
+    const double s2 = a.lj.sig2 / r2;
+    const double s6 = s2 * s2 * s2;
+    double u = 4.0 * a.lj.eps * (s6 * s6 - s6);
+    double ww = 24.0 * a.lj.eps * (2.0 * s6 * s6 - s6);
+    if (LFS) {
+        const double r = sqrt(r2);
+        u = u - a.lj.u_rc - (r - a.lj.rc) * a.lj.dudr_rc;
+        ww = ww + r * a.lj.dudr_rc;
+    }
+    e += u;
+    w += ww;
+    np += flip ? 0 : 1;
+    const double c = flip ? -(ww / r2) : ww / r2;
+    fx += c * dx;
+    fy += c * dy;
+    fz += c * dz;
+}
+
+template <bool LFS>
+__global__ __launch_bounds__(kBlock) void k_lj_direct(DirectArgs a) {
+    __shared__ double sm_d[kBlock / kWave];
+    __shared__ long long sm_l[kBlock / kWave];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double e = 0, w = 0, fx = 0, fy = 0, fz = 0;
+    long long np = 0;
+    if (t < a.n) {
+        const double4 pi = a.pos[t];
+        const int oi = a.orig[t];
+        if (a.cell_atoms == nullptr) {
+            for (int j = 0; j < static_cast<int>(a.n); ++j)
+                if (j != t) direct_pair<LFS>(a, pi, oi, j, e, w, fx, fy, fz, np);
+        } else {
+            const Grid& g = a.grid;
+            const int c = a.cell_of[t];
+            const int cx = c % g.nc[0];
+            const int cy = (c / g.nc[0]) % g.nc[1];
+            const int cz = c / (g.nc[0] * g.nc[1]);
+            for (int oz = -1; oz <= 1; ++oz)
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int nb = (cx + ox + g.nc[0]) % g.nc[0] +
+                                       g.nc[0] * ((cy + oy + g.nc[1]) % g.nc[1] +
+                                                  g.nc[1] * ((cz + oz + g.nc[2]) % g.nc[2]));
+                        for (int k = a.cell_start[nb]; k < a.cell_start[nb + 1]; ++k) {
+                            const int j = a.cell_atoms[k];
+                            if (j != t) direct_pair<LFS>(a, pi, oi, j, e, w, fx, fy, fz, np);
+                        }
+                    }
+        }
+        if (a.frc != nullptr) a.frc[t] = make_double4(fx, fy, fz, 0.0);
+    }
+    const double be = block_reduce(e, sm_d, wave_sum);
+    const double bw = block_reduce(w, sm_d, wave_sum);
+    const long long bn = block_reduce(np, sm_l, wave_sum_ll);
+    if (threadIdx.x == 0) {
+        a.part_d[3 * blockIdx.x] = be;
+        a.part_d[3 * blockIdx.x + 1] = bw;
+        a.part_l[3 * blockIdx.x] = bn;
+    }
+    if (last_block(&a.sc->counter[1])) finish_direct(a);
+}
+
+__global__ void k_bin_index(const double4* __restrict__ pos, long n, Grid g,
+                            int* __restrict__ cell_of, int* __restrict__ cell_count) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double4 p = pos[t];
+    int c[3];
+    const double r[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d) {
+        const double f = fold_ref(r[d], g.L[d]);
+        int k = static_cast<int>(floor((f + 0.5 * g.L[d]) / g.w[d]));
+        c[d] = k < 0 ? 0 : (k >= g.nc[d] ? g.nc[d] - 1 : k);
+    }
+    const int cell = c[0] + g.nc[0] * (c[1] + g.nc[1] * c[2]);
+    cell_of[t] = cell;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_zero_int(int* a, long n) {
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_plain(const int* __restrict__ cnt, long m,
+                                                     int* __restrict__ start,
+                                                     int* __restrict__ fill) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const long chunk = (m + 1023) / 1024;
+    const long b = t * chunk;
+    const long e = b + chunk < m ? b + chunk : m;
+    int s = 0;
+    for (long c = b; c < e; ++c) s += cnt[c];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (long c = b; c < e; ++c) {
+        start[c] = run;
+        fill[c] = run;
+        run += cnt[c];
+    }
+    if (t == 1023) start[m] = part[1023];
+}
+
+__global__ void k_place_index(const int* __restrict__ cell_of, long n, int* __restrict__ fill,
+                              int* __restrict__ cell_atoms) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cell_atoms[atomicAdd(&fill[cell_of[t]], 1)] = static_cast<int>(t);
+}
+
+__global__ void k_sort_index(const int* __restrict__ start, long m, int* __restrict__ cell_atoms) {
+    const long c = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    for (int i = start[c] + 1; i < start[c + 1]; ++i) {
+        const int v = cell_atoms[i];
+        int j = i - 1;
+        while (j >= start[c] && cell_atoms[j] > v) {
+            cell_atoms[j + 1] = cell_atoms[j];
+            --j;
+        }
+        cell_atoms[j + 1] = v;
+    }
+}
+
+__global__ void k_chain_first(Chain* chain, Scalars* sc, double delta) {
+    if (threadIdx.x != 0) return;
+    double ke = sc->kinetic;
+    sc->vscale = chain_half(*chain, ke, delta);
+}
+
+inline unsigned grid_of(long n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) {
+    hipLaunchKernelGGL(k_chain_first, dim3(1), dim3(64), 0, s, chain, sc, delta);
+}
+
+void launch_lj_force(const ForceArgs& a, hipStream_t s) {
+    const dim3 g(grid_of(a.n));
+    if (a.lj.lfs) {
+        if (a.mixed) dispatch_kick<true, true>(a, g, s);
+        else dispatch_kick<true, false>(a, g, s);
+    } else {
+        if (a.mixed) dispatch_kick<false, true>(a, g, s);
+        else dispatch_kick<false, false>(a, g, s);
+    }
+}
+
+void launch_lj_direct(const DirectArgs& a, hipStream_t s) {
+    const dim3 g(grid_of(a.n));
+    if (a.lj.lfs)
+        hipLaunchKernelGGL(k_lj_direct<true>, g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_lj_direct<false>, g, dim3(kBlock), 0, s, a);
+}
+
+void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,
+                      int* cell_start, int* cell_fill, int* cell_atoms, hipStream_t s) {
+    const long m = static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2];
+    hipLaunchKernelGGL(k_zero_int, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_count, m);
+    hipLaunchKernelGGL(k_bin_index, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, g, cell_of,
+                       cell_count);
+    hipLaunchKernelGGL(k_scan_plain, dim3(1), dim3(1024), 0, s, cell_count, m, cell_start, cell_fill);
+    hipLaunchKernelGGL(k_place_index, dim3(grid_of(n)), dim3(kBlock), 0, s, cell_of, n, cell_fill,
+                       cell_atoms);
+    hipLaunchKernelGGL(k_sort_index, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_start, m, cell_atoms);
+}
+
+}  // namespace mdk

@@ -1,0 +1,358 @@
+// Integrator passes and per-atom utility kernels.
+//
+// k_pre: everything an integrator does to an atom between two force calls
+// that does not need the new force, fused into one pass over pos/vel/frc:
+//   velocity Verlet (integrate/velocity_verlet.hpp:48-64):
+//     v = fma(h/m, f, v); r = wrap(fma(dt, v, r))
+//   Nose-Hoover chain (integrate/nose_hoover.hpp:37-42): the pending chain
+//     rescale of the previous half-chains is applied first, then as above
+//   BAOAB (integrate/langevin.hpp:36-71):
+//     B: v = fma(h/m, f, v); A: r = wrap(fma(h, v, r));
+//     O: v = c1 v + (c2 sqrt(T/m)) xi; A: r = wrap(fma(h, v, r))
+// followed by the Verlet-list displacement check (verlet_list.cpp:44-62):
+// the top-2 of |minimumImage(r - r_ref)| reduced by a fixed tree, and the
+// rebuild flag set on the device when their sum exceeds the skin. The
+// second half-kick runs inside the force kernel.
+//
+// BAOAB noise: Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC11), key = seed, counter = (upload index,
+// step, stream), two 32-bit words per 53-bit uniform in (0,1), and the
+// Box-Muller transform. The stream is a function of the atom's upload
+// index, so it does not depend on the device sort order.
+#include "md_common.hpp"
+#include "md_kernels.hpp"
+
+namespace mdk {
+
+namespace {
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+    const uint64_t v = (static_cast<uint64_t>(a) << 21) ^ static_cast<uint64_t>(b >> 11);
+    return (static_cast<double>(v & ((1ULL << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void normals3(uint64_t seed, uint64_t step, uint32_t id, double& g0,
+                                         double& g1, double& g2) {
+    const uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+    const U4 a = philox10(U4{id, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32), 0u}, k0, k1);
+    const U4 b = philox10(U4{id, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32), 1u}, k0, k1);
+    const double u1 = u53(a.x, a.y), u2 = u53(a.z, a.w);
+    const double u3 = u53(b.x, b.y), u4 = u53(b.z, b.w);
+    const double r1 = sqrt(-2.0 * log(u1));
+    const double r2 = sqrt(-2.0 * log(u3));
+    double s1, c1, s2, c2;
+    sincospi(2.0 * u2, &s1, &c1);
+    sincospi(2.0 * u4, &s2, &c2);
+    g0 = r1 * c1;
+    g1 = r1 * s1;
+    g2 = r2 * c2;
+}
+
+__device__ __forceinline__ double4 drift(const Box& b, double4 p, const double4& v, double h) {
+    p.x = fold(fma(h, v.x, p.x), b.L[0], b.lo[0], b.hi1[0], b.hi2[0]);
+    p.y = fold(fma(h, v.y, p.y), b.L[1], b.lo[1], b.hi1[1], b.hi2[1]);
+    p.z = fold(fma(h, v.z, p.z), b.L[2], b.lo[2], b.hi1[2], b.hi2[2]);
+    return p;
+}
+
+__device__ __forceinline__ double displacement(const Box& b, const double4& p, const double4& r) {
+#pragma clang fp contract(off)
+    double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+    min_image(b, dx, dy, dz);
+    return sqrt(norm2_ref(dx, dy, dz));
+}
+
+__device__ void finish_check(const StepArgs& a) {
+    __shared__ double t1s[kBlock], t2s[kBlock];
+    double t1 = 0.0, t2 = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock)
+        top2_merge(t1, t2, a.part_d[2 * b], a.part_d[2 * b + 1]);
+    t1s[threadIdx.x] = t1;
+    t2s[threadIdx.x] = t2;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double x1 = t1s[threadIdx.x], x2 = t2s[threadIdx.x];
+            top2_merge(x1, x2, t1s[threadIdx.x + off], t2s[threadIdx.x + off]);
+            t1s[threadIdx.x] = x1;
+            t2s[threadIdx.x] = x2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.sc->top1 = t1s[0];
+        a.sc->top2 = t2s[0];
+        a.sc->rebuild_flag = (t1s[0] + t2s[0]) > a.skin ? 1 : 0;
+        a.sc->vscale = 1.0;
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
+    __shared__ double sm1[kBlock / kWave], sm2[kBlock / kWave];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double d = 0.0;
+    if (t < a.n) {
+        double4 p = a.pos[t];
+        double4 v = a.vel[t];
+        const double4 f = a.frc[t];
+        if (KIND == 2) {
+            const double s = a.sc->vscale;
+            v.x = v.x * s;
+            v.y = v.y * s;
+            v.z = v.z * s;
+        }
+        const double c = (0.5 * a.dt) * (1.0 / v.w);
+        v.x = fma(c, f.x, v.x);
+        v.y = fma(c, f.y, v.y);
+        v.z = fma(c, f.z, v.z);
+        if (KIND == 1) {
+            const double h = 0.5 * a.dt;
+            p = drift(a.box, p, v, h);
+            double g0, g1, g2;
+            normals3(a.seed, a.step, static_cast<uint32_t>(a.orig[t]), g0, g1, g2);
+            const double k = a.c2 * sqrt(a.temperature / v.w);
+            v.x = a.c1 * v.x + g0 * k;
+            v.y = a.c1 * v.y + g1 * k;
+            v.z = a.c1 * v.z + g2 * k;
+            p = drift(a.box, p, v, h);
+        } else {
+            p = drift(a.box, p, v, a.dt);
+        }
+        a.pos[t] = p;
+        a.vel[t] = v;
+        if (a.check) d = displacement(a.box, p, a.ref[t]);
+    }
+    double t1 = d, t2 = 0.0;
+    wave_top2(t1, t2);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        sm1[wid] = t1;
+        sm2[wid] = t2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b1 = 0.0, b2 = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) top2_merge(b1, b2, sm1[w], sm2[w]);
+        a.part_d[2 * blockIdx.x] = b1;
+        a.part_d[2 * blockIdx.x + 1] = b2;
+    }
+    if (last_block(&a.sc->counter[2])) {
+        if (a.check) {
+            finish_check(a);
+        } else if (threadIdx.x == 0) {
+            a.sc->vscale = 1.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_check(StepArgs a) {
+    __shared__ double sm1[kBlock / kWave], sm2[kBlock / kWave];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double d = 0.0;
+    if (t < a.n) d = displacement(a.box, a.pos[t], a.ref[t]);
+    double t1 = d, t2 = 0.0;
+    wave_top2(t1, t2);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        sm1[wid] = t1;
+        sm2[wid] = t2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b1 = 0.0, b2 = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) top2_merge(b1, b2, sm1[w], sm2[w]);
+        a.part_d[2 * blockIdx.x] = b1;
+        a.part_d[2 * blockIdx.x + 1] = b2;
+    }
+    if (last_block(&a.sc->counter[3])) {
+        const double keep = a.sc->vscale;
+        finish_check(a);
+        if (threadIdx.x == 0) a.sc->vscale = keep;
+    }
+}
+
+__global__ void k_apply_scale(long n, double4* vel, Scalars* sc) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double s = sc->vscale;
+    if (t < n) {
+        double4 v = vel[t];
+        v.x = v.x * s;
+        v.y = v.y * s;
+        v.z = v.z * s;
+        vel[t] = v;
+    }
+}
+
+__global__ void k_reset_scale(Scalars* sc) {
+    if (threadIdx.x == 0) sc->vscale = 1.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_kinetic(long n, const double4* vel, double* part,
+                                                    Scalars* sc) {
+    __shared__ double sm[kBlock / kWave];
+    __shared__ double red[kBlock];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double ke = 0.0;
+    if (t < n) {
+        const double4 v = vel[t];
+        ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+    }
+    const double b = block_reduce(ke, sm, wave_sum);
+    if (threadIdx.x == 0) part[blockIdx.x] = b;
+    if (last_block(&sc->counter[4])) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < static_cast<int>(gridDim.x); k += kBlock) s += part[k];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = kBlock / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) sc->kinetic = red[0];
+    }
+}
+
+// Mass and momentum sums (core/system.cpp:137-164) into out4 = {M, Px, Py, Pz}.
+__global__ __launch_bounds__(kBlock) void k_momentum(long n, const double4* vel, double* part,
+                                                     double* out4, unsigned int* ticket) {
+    __shared__ double sm[kBlock / kWave];
+    __shared__ double red[4][kBlock];
+    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double q[4] = {0, 0, 0, 0};
+    if (t < n) {
+        const double4 v = vel[t];
+        q[0] = v.w;
+        q[1] = v.w * v.x;
+        q[2] = v.w * v.y;
+        q[3] = v.w * v.z;
+    }
+    for (int k = 0; k < 4; ++k) {
+        const double b = block_reduce(q[k], sm, wave_sum);
+        if (threadIdx.x == 0) part[4 * blockIdx.x + k] = b;
+    }
+    if (last_block(ticket)) {
+        for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock) s += part[4 * b + k];
+            red[k][threadIdx.x] = s;
+        }
+        __syncthreads();
+        for (int off = kBlock / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off)
+                for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) out4[k] = red[k][0];
+    }
+}
+
+__global__ void k_sub_com(long n, double4* vel, const double* out4) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double M = out4[0];
+    if (t >= n || M == 0.0) return;
+    const double cx = out4[1] / M, cy = out4[2] / M, cz = out4[3] / M;
+    double4 v = vel[t];
+    v.x -= cx;
+    v.y -= cy;
+    v.z -= cz;
+    vel[t] = v;
+}
+
+__global__ void k_unpermute(long n, const int* __restrict__ orig, const double4* __restrict__ a,
+                            double* __restrict__ x, double* __restrict__ y, double* __restrict__ z) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double4 v = a[t];
+    const int o = orig[t];
+    x[o] = v.x;
+    y[o] = v.y;
+    z[o] = v.z;
+}
+
+__global__ void k_permute_in(long n, const int* __restrict__ orig, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             double4* __restrict__ a, int keep_w) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int o = orig[t];
+    double4 v = a[t];
+    v.x = x[o];
+    v.y = y[o];
+    v.z = z[o];
+    if (!keep_w) v.w = 0.0;
+    a[t] = v;
+}
+
+__global__ void k_set_flag(int* flag, int v) {
+    if (threadIdx.x == 0) *flag = v;
+}
+
+inline unsigned grid_of(long n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_pre_step(const StepArgs& a, hipStream_t s) {
+    const dim3 g(grid_of(a.n));
+    if (a.kind == 1)
+        hipLaunchKernelGGL(k_pre<1>, g, dim3(kBlock), 0, s, a);
+    else if (a.kind == 2)
+        hipLaunchKernelGGL(k_pre<2>, g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_pre<0>, g, dim3(kBlock), 0, s, a);
+}
+
+void launch_check(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_check, dim3(grid_of(a.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_apply_scale(long n, double4* vel, Scalars* sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_apply_scale, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, sc);
+    hipLaunchKernelGGL(k_reset_scale, dim3(1), dim3(64), 0, s, sc);
+}
+
+void launch_kinetic(long n, const double4* vel, double* part, Scalars* sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_kinetic, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, part, sc);
+}
+
+void launch_remove_com(long n, double4* vel, double* part, double* out4, hipStream_t s) {
+    // out4[4..] holds the ticket of the momentum reduction
+    unsigned int* ticket = reinterpret_cast<unsigned int*>(out4 + 4);
+    hipLaunchKernelGGL(k_momentum, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, part, out4, ticket);
+    hipLaunchKernelGGL(k_sub_com, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, out4);
+}
+
+void launch_unpermute(long n, const int* orig, const double4* a, double* x, double* y, double* z,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_unpermute, dim3(grid_of(n)), dim3(kBlock), 0, s, n, orig, a, x, y, z);
+}
+
+void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
+                       double4* a, int keep_w, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_in, dim3(grid_of(n)), dim3(kBlock), 0, s, n, orig, x, y, z, a, keep_w);
+}
+
+void launch_set_flag(int* flag, int v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, flag, v);
+}
+
+}  // namespace mdk

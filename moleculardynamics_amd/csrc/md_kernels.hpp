@@ -1,0 +1,140 @@
+// Host-side launch interfaces of the device kernels (one struct of
+// arguments per kernel family).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "md_common.hpp"
+
+namespace mdk {
+
+// Cell grid of the list build: nc = max(1, floor(L/cutoff)) per axis
+// (neighbor/cell_list.cpp:8-14), width w = L/nc; usable when nc >= 3 on
+// every axis (cell_list.cpp:44-46).
+struct Grid {
+    int nc[3];
+    double w[3];
+    double L[3];
+    int usable;
+};
+
+struct NeighArgs {
+    long n;
+    Grid grid;
+    Box box;
+    double rl2;  // (rc + skin)^2 as the reference computes it
+    double4 *pos, *vel, *frc, *ref;
+    int *orig, *slot;
+    double4 *pos2, *vel2, *frc2;
+    int *orig2, *cell2;
+    int *cell_of, *cell_count, *cell_start, *cell_fill, *order;
+    uint32_t* nbr;
+    int* ncnt;
+    long npad;
+    int cap;
+    Scalars* sc;
+    const int* flag;
+};
+void launch_rebuild(const NeighArgs& a, hipStream_t s);
+
+// LJ pair kernel over the full Verlet list.
+struct LJConst {
+    double sig2, eps, rc, rc2, u_rc, dudr_rc;
+    float sig2f, epsf, rc2f, rcf, u_rcf, dudr_rcf;
+    int lfs;
+};
+
+struct ForceArgs {
+    long n;
+    Box box;
+    LJConst lj;
+    const double4* pos;
+    double4* vel;  // kicked when kick != 0
+    double4* frc;
+    const int* orig;
+    const uint32_t* nbr;
+    const int* ncnt;
+    long npad;
+    int cap;         // row capacity: rows longer than this overflowed and are clamped
+    double half_dt;  // kick coefficient h (v += h/m f) when kick != 0
+    int kick;
+    int mixed;
+    // reductions
+    double* part_d;     // per-workgroup FP64 partials, 3 per group (E, W, KE)
+    long long* part_l;  // per-workgroup int64 partials, 3 per group (E, W, pairs)
+    Scalars* sc;
+    Chain* chain;       // NHC: chain update in the last workgroup (null otherwise)
+    double chain_delta; // dt/2 of the chain half-steps
+    int chain_twice;    // 1: also propagate the next step's first half-chain
+    void* thermo;       // md_thermo* record for this step, or null
+};
+void launch_lj_force(const ForceArgs& a, hipStream_t s);
+// One chain propagation of length delta from sc->kinetic; the velocity
+// rescale is left pending in sc->vscale.
+void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s);
+
+// Brute-force (all pairs) and cell-list (at the LJ cutoff, through an index
+// array) force / energy evaluations for the standalone API calls.
+struct DirectArgs {
+    long n;
+    Box box;
+    LJConst lj;
+    const double4* pos;
+    double4* frc;  // null: energy/virial only
+    const int* orig;
+    // cells (null cell_atoms: brute force)
+    Grid grid;
+    const int* cell_of;
+    const int* cell_start;
+    const int* cell_atoms;
+    double* part_d;
+    long long* part_l;
+    Scalars* sc;
+};
+void launch_lj_direct(const DirectArgs& a, hipStream_t s);
+// Bins atoms into cells without permuting them: cell_atoms holds slot
+// indices grouped by cell (ordered by slot within a cell).
+void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,
+                      int* cell_start, int* cell_fill, int* cell_atoms, hipStream_t s);
+
+// Integrator per-atom passes before the force call, with the displacement
+// check folded in.
+struct StepArgs {
+    long n;
+    Box box;
+    double4* pos;
+    double4* vel;
+    const double4* frc;
+    const double4* ref;
+    const int* orig;
+    int kind;         // md_integrator_kind
+    double dt;
+    double c1, c2, temperature;  // BAOAB
+    uint64_t seed;
+    uint64_t step;    // global step counter (BAOAB noise counter)
+    double skin;
+    int check;        // compute the displacement check and set the rebuild flag
+    double* part_d;   // 2 per group (top1, top2)
+    Scalars* sc;
+};
+void launch_pre_step(const StepArgs& a, hipStream_t s);
+// Displacement check alone (md_neigh_needs_rebuild).
+void launch_check(const StepArgs& a, hipStream_t s);
+
+// Applies sc->vscale to every velocity and resets it to 1.
+void launch_apply_scale(long n, double4* vel, Scalars* sc, hipStream_t s);
+// KE reduction into sc->kinetic.
+void launch_kinetic(long n, const double4* vel, double* part_d, Scalars* sc, hipStream_t s);
+// Momentum reduction and centre-of-mass removal (System::removeCenterOfMassMotion).
+void launch_remove_com(long n, double4* vel, double* part_d, double* out4, hipStream_t s);
+// Upload-order scatter for download: out[orig[t]] = component.
+void launch_unpermute(long n, const int* orig, const double4* a, double* x, double* y, double* z,
+                      hipStream_t s);
+// Upload-order gather for set_positions / set_velocities.
+void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
+                       double4* a, int keep_w, hipStream_t s);
+void launch_set_flag(int* flag, int v, hipStream_t s);
+
+}  // namespace mdk

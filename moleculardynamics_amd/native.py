@@ -1,0 +1,285 @@
+"""ctypes binding of the C ABI in include/moldyn_hip.h (libmoldyn_hip.so).
+
+This is plumbing for tests and the benchmark: it calls the HIP library and
+nothing else. There is no CPU fallback: a missing library raises at load.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmoldyn_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "moldyn_hip.h")
+
+MD_OK, MD_ERR_INVALID_ARGUMENT, MD_ERR_RUNTIME, MD_ERR_HIP, MD_ERR_CAPACITY, MD_ERR_STATE = range(6)
+BRUTE, CELLS, VERLET = 0, 1, 2
+FP64, MIXED = 0, 1
+VV, BAOAB, NHC = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class LJParams(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("epsilon", C.c_double), ("cutoff", C.c_double),
+                ("linear_force_shift", C.c_int)]
+
+
+class Integrator(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dt", C.c_double), ("friction", C.c_double),
+                ("temperature", C.c_double), ("tau", C.c_double), ("chain_length", C.c_int),
+                ("dof", C.c_long), ("seed", C.c_uint64)]
+
+
+class Thermo(C.Structure):
+    _fields_ = [("potential", C.c_double), ("virial", C.c_double), ("kinetic", C.c_double),
+                ("conserved", C.c_double), ("pairs", C.c_double)]
+
+
+class MDError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[status {status}] {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libmoldyn_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m moleculardynamics_amd.build`")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.md_ctx_create.argtypes = [vp, C.POINTER(vp)]
+    lib.md_ctx_destroy.argtypes = [vp]
+    lib.md_last_error.argtypes = [vp]
+    lib.md_last_error.restype = C.c_char_p
+    lib.md_ctx_synchronize.argtypes = [vp]
+    lib.md_system_upload.argtypes = [vp, C.c_long, _dp] + [_dp] * 7
+    lib.md_system_download.argtypes = [vp] + [_dp] * 9
+    lib.md_system_set_positions.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_system_set_velocities.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_system_kinetic_energy.argtypes = [vp, _dp]
+    lib.md_system_remove_com_motion.argtypes = [vp]
+    lib.md_system_size.argtypes = [vp, C.POINTER(C.c_long)]
+    lib.md_lj_set.argtypes = [vp, C.POINTER(LJParams)]
+    lib.md_lj_forces.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+    lib.md_lj_energy_virial.argtypes = [vp, _dp, _dp]
+    lib.md_lj_tail.argtypes = [C.POINTER(LJParams), C.c_long, _dp, _dp, _dp]
+    lib.md_cells_usable.argtypes = [_dp, C.c_double]
+    lib.md_cells_usable.restype = C.c_int
+    lib.md_neigh_config.argtypes = [vp, C.c_double, C.c_double]
+    lib.md_neigh_build.argtypes = [vp]
+    lib.md_neigh_needs_rebuild.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.md_neigh_rebuild_if_needed.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.md_neigh_rebuild_count.argtypes = [vp, C.POINTER(C.c_long)]
+    lib.md_neigh_export_pairs.argtypes = [vp, _u32p, C.c_long, C.POINTER(C.c_long)]
+    lib.md_neigh_set_capacity.argtypes = [vp, C.c_int]
+    lib.md_integrator_set.argtypes = [vp, C.POINTER(Integrator)]
+    lib.md_run.argtypes = [vp, C.c_long, C.c_int, C.POINTER(Thermo)]
+    lib.md_nhc_state.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_timing_enable.argtypes = [vp, C.c_int]
+    lib.md_timing_read.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_long)]
+    lib.md_timing_reset.argtypes = [vp]
+    lib.md_last_pair_count.argtypes = [vp, C.POINTER(C.c_long)]
+    lib.md_fcc_lattice.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64] + [_dp] * 7
+    _lib = lib
+    return lib
+
+
+def declared_symbols(header: str = HEADER) -> list[str]:
+    """Function names declared in include/moldyn_hip.h."""
+    import re
+
+    txt = open(header).read()
+    return sorted(set(re.findall(r"\b(md_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _col(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One md_ctx: a device-resident system plus its list and integrator."""
+
+    def __init__(self, stream=None):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.md_ctx_create(stream, C.byref(h))
+        if st != MD_OK:
+            raise MDError(st, "md_ctx_create failed (is a GPU visible?)")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.md_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != MD_OK:
+            raise MDError(st, self.lib.md_last_error(self.h).decode())
+
+    # ---- state ------------------------------------------------------------
+    def upload(self, L, pos, vel=None, mass=None):
+        pos = np.asarray(pos, dtype=np.float64)
+        n = pos.shape[0]
+        cols = [_col(pos[:, d]) for d in range(3)]
+        vcols = [None] * 3 if vel is None else [_col(np.asarray(vel)[:, d]) for d in range(3)]
+        m = None if mass is None else _col(mass)
+        Lb = _col(L)
+        self._keep = (cols, vcols, m, Lb)
+        self._chk(self.lib.md_system_upload(self.h, n, _p(Lb), *[_p(c) for c in cols],
+                                            *[_p(c) for c in vcols], _p(m)))
+        self.n = n
+
+    def download(self):
+        a = [np.zeros(self.n) for _ in range(9)]
+        self._chk(self.lib.md_system_download(self.h, *[_p(v) for v in a]))
+        return (np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1), np.stack(a[6:9], axis=1))
+
+    def set_positions(self, pos):
+        cols = [_col(np.asarray(pos)[:, d]) for d in range(3)]
+        self._chk(self.lib.md_system_set_positions(self.h, *[_p(c) for c in cols]))
+
+    def set_velocities(self, vel):
+        cols = [_col(np.asarray(vel)[:, d]) for d in range(3)]
+        self._chk(self.lib.md_system_set_velocities(self.h, *[_p(c) for c in cols]))
+
+    def kinetic_energy(self) -> float:
+        ke = C.c_double()
+        self._chk(self.lib.md_system_kinetic_energy(self.h, C.byref(ke)))
+        return ke.value
+
+    def remove_com_motion(self):
+        self._chk(self.lib.md_system_remove_com_motion(self.h))
+
+    # ---- LJ ---------------------------------------------------------------
+    def set_lj(self, sigma=1.0, epsilon=1.0, cutoff=2.5, lfs=False):
+        self.ljp = LJParams(sigma, epsilon, cutoff, int(lfs))
+        self._chk(self.lib.md_lj_set(self.h, C.byref(self.ljp)))
+
+    def forces(self, source=VERLET, precision=FP64):
+        e, w = C.c_double(), C.c_double()
+        self._chk(self.lib.md_lj_forces(self.h, source, precision, C.byref(e), C.byref(w)))
+        return e.value, w.value
+
+    def energy_virial(self):
+        e, w = C.c_double(), C.c_double()
+        self._chk(self.lib.md_lj_energy_virial(self.h, C.byref(e), C.byref(w)))
+        return e.value, w.value
+
+    # ---- neighbour list ----------------------------------------------------
+    def config_list(self, cutoff, skin):
+        self._chk(self.lib.md_neigh_config(self.h, cutoff, skin))
+
+    def build_list(self):
+        self._chk(self.lib.md_neigh_build(self.h))
+
+    def needs_rebuild(self) -> bool:
+        f = C.c_int()
+        self._chk(self.lib.md_neigh_needs_rebuild(self.h, C.byref(f)))
+        return bool(f.value)
+
+    def rebuild_if_needed(self) -> bool:
+        f = C.c_int()
+        self._chk(self.lib.md_neigh_rebuild_if_needed(self.h, C.byref(f)))
+        return bool(f.value)
+
+    def rebuild_count(self) -> int:
+        c = C.c_long()
+        self._chk(self.lib.md_neigh_rebuild_count(self.h, C.byref(c)))
+        return c.value
+
+    def set_capacity(self, per_atom: int):
+        self._chk(self.lib.md_neigh_set_capacity(self.h, per_atom))
+
+    def pairs(self) -> np.ndarray:
+        npairs = C.c_long()
+        st = self.lib.md_neigh_export_pairs(self.h, None, 0, C.byref(npairs))
+        if st not in (MD_OK, MD_ERR_CAPACITY):
+            self._chk(st)
+        out = np.zeros(2 * max(1, npairs.value), dtype=np.uint32)
+        self._chk(self.lib.md_neigh_export_pairs(self.h, out.ctypes.data_as(_u32p), npairs.value,
+                                                 C.byref(npairs)))
+        return out[: 2 * npairs.value].reshape(-1, 2)
+
+    # ---- integration -------------------------------------------------------
+    def set_integrator(self, kind=VV, dt=0.005, friction=1.0, temperature=1.0, tau=0.5,
+                       chain_length=3, dof=0, seed=1):
+        self.itg = Integrator(kind, dt, friction, temperature, tau, chain_length, dof, seed)
+        self._chk(self.lib.md_integrator_set(self.h, C.byref(self.itg)))
+
+    def run(self, nsteps, precision=FP64, thermo=True):
+        th = (Thermo * nsteps)() if thermo else None
+        self._chk(self.lib.md_run(self.h, nsteps, precision, th))
+        if not thermo:
+            return None
+        a = np.frombuffer(th, dtype=np.float64).reshape(nsteps, 5)
+        return dict(pe=a[:, 0].copy(), virial=a[:, 1].copy(), ke=a[:, 2].copy(),
+                    conserved=a[:, 3].copy(), pairs=a[:, 4].copy())
+
+    def nhc_state(self, M=3):
+        cq = C.c_double()
+        xi, p = np.zeros(M), np.zeros(M)
+        self._chk(self.lib.md_nhc_state(self.h, C.byref(cq), _p(xi), _p(p)))
+        return cq.value, xi, p
+
+    # ---- measurement -------------------------------------------------------
+    def timing(self, on=True):
+        self._chk(self.lib.md_timing_enable(self.h, int(on)))
+
+    def timing_read(self, kernel_id=0):
+        ms, k = C.c_double(), C.c_long()
+        self._chk(self.lib.md_timing_read(self.h, kernel_id, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
+
+    def timing_reset(self):
+        self._chk(self.lib.md_timing_reset(self.h))
+
+    def pair_count(self) -> int:
+        c = C.c_long()
+        self._chk(self.lib.md_last_pair_count(self.h, C.byref(c)))
+        return c.value
+
+    def synchronize(self):
+        self._chk(self.lib.md_ctx_synchronize(self.h))
+
+
+def cells_usable(L, cutoff) -> bool:
+    return bool(load().md_cells_usable(_p(_col(L)), cutoff))
+
+
+def fcc_lattice(cells: int, density: float, temperature: float, seed: int):
+    """fccLattice through the C ABI: returns (L[3], positions (N,3), velocities (N,3))."""
+    n = 4 * cells ** 3
+    L = np.zeros(3)
+    a = [np.zeros(n) for _ in range(6)]
+    st = load().md_fcc_lattice(cells, density, temperature, seed, _p(L), *[_p(v) for v in a])
+    if st != MD_OK:
+        raise MDError(st, "md_fcc_lattice failed")
+    return L, np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1)
+
+
+def lj_tail(n, L, sigma=1.0, epsilon=1.0, cutoff=2.5, lfs=False):
+    p = LJParams(sigma, epsilon, cutoff, int(lfs))
+    u, pr = C.c_double(), C.c_double()
+    st = load().md_lj_tail(C.byref(p), n, _p(_col(L)), C.byref(u), C.byref(pr))
+    if st != MD_OK:
+        raise MDError(st, "md_lj_tail failed")
+    return u.value, pr.value

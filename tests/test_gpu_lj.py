@@ -1,0 +1,288 @@
+"""GPU parity of the Lennard-Jones force path through the C ABI, against the
+CPU oracle on identical inputs (and NIST where NIST publishes values).
+
+Tolerances: E, W and F to 1e-12 relative (the reference's own list-vs-brute
+bound, tests/test_neighbor.cpp:37-61 of the reference); the Verlet pair set
+bit-exact as sorted (i, j) pairs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, nist_lj
+from oracle.pyoracle import load_nist
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def _pos(fr):
+    return np.stack([fr.x, fr.y, fr.z], axis=1)
+
+
+def _vel(fr):
+    return np.stack([fr.vx, fr.vy, fr.vz], axis=1)
+
+
+def _close(a, b, rel=REL):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = np.maximum(np.abs(b), 1.0)
+    return np.max(np.abs(a - b) / scale)
+
+
+def _force_err(f, g):
+    """max_i |f_i - g_i| / max(|g|_max, 1)"""
+    return float(np.max(np.abs(f - g)) / max(np.max(np.abs(g)), 1.0))
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4])
+@pytest.mark.parametrize("rc,lfs", [(3.0, False), (4.0, False), (3.0, True)])
+def test_nist_energy_virial_brute(ctx, orc, cfg, rc, lfs):
+    fr = load_nist(nist_lj(cfg))
+    ctx.upload(fr.L, _pos(fr))
+    ctx.set_lj(1.0, 1.0, rc, lfs)
+    e, w = ctx.energy_virial()
+    e0, w0 = orc.lj_energy_virial(fr, rc=rc, lfs=lfs)
+    assert abs(e - e0) <= REL * abs(e0)
+    assert abs(w - w0) <= REL * abs(w0)
+    nist = json.load(open(os.path.join(GOLDEN, "nist_values.json")))["lj"]
+    row = [r for r in nist if r[0] == cfg and r[2] == rc and (r[1] == "LFS") == lfs][0]
+    half = lambda v: 0.5 * 10 ** (np.floor(np.log10(abs(v))) - 4)  # noqa: E731
+    assert abs(e - row[3]) <= half(row[3])
+    assert abs(w - row[4]) <= half(row[4])
+
+
+@pytest.mark.parametrize("mode", ["brute", "cells", "verlet"])
+@pytest.mark.parametrize("lfs", [False, True])
+def test_nist1_forces_all_sources(ctx, orc, mode, lfs):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_lj(1))  # N=800, L=10, rc=3 -> 3 cells per side (tight case)
+    ctx.upload(fr.L, _pos(fr))
+    ctx.set_lj(1.0, 1.0, 3.0, lfs)
+    src = {"brute": native.BRUTE, "cells": native.CELLS, "verlet": native.VERLET}[mode]
+    if mode == "verlet":
+        ctx.config_list(3.0, 0.3)
+        ctx.build_list()
+    e, w = ctx.forces(src)
+    _, _, f = ctx.download()
+    f0, e0, w0 = orc.lj_forces(fr, rc=3.0, lfs=lfs, mode="brute")
+    assert abs(e - e0) <= REL * abs(e0)
+    assert abs(w - w0) <= REL * abs(w0)
+    assert _force_err(f, f0) <= REL
+
+
+def test_unusable_cell_list_is_refused(ctx):
+    from moleculardynamics_amd import native
+
+    L = np.array([5.0, 5.0, 5.0])
+    ctx.upload(L, np.array([[0.0, 0.0, 0.0], [1.1, 0.0, 0.0]]))
+    ctx.set_lj(1.0, 1.0, 3.0, False)
+    with pytest.raises(native.MDError) as ei:
+        ctx.forces(native.CELLS)
+    assert ei.value.status == native.MD_ERR_INVALID_ARGUMENT
+    # the brute-force source still answers
+    e, _ = ctx.forces(native.BRUTE)
+    assert np.isfinite(e)
+
+
+@pytest.mark.parametrize("case", ["nist1", "nist3", "fcc864", "fcc108_small_box"])
+def test_pair_set_bit_exact(ctx, orc, case):
+    if case.startswith("nist"):
+        fr = load_nist(nist_lj(int(case[-1])))
+        rc, skin = 3.0, 0.3
+    elif case == "fcc864":
+        fr = orc.fcc_lattice(6, 0.8442, 0.722, 12345)
+        rc, skin = 2.5, 0.3
+    else:  # L = 4.76 < 2*(rc+skin): brute-force build, single nearest image
+        fr = orc.fcc_lattice(3, 1.0, 1.0, 7)
+        rc, skin = 2.5, 0.3
+    ctx.upload(fr.L, _pos(fr))
+    ctx.config_list(rc, skin)
+    ctx.build_list()
+    got = ctx.pairs()
+    want = orc.verlet_pairs(fr, rc, skin)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+def test_pair_set_after_dynamics(ctx, orc):
+    """Re-sorted device order, moved atoms: the set still matches on the same snapshot."""
+    fr = orc.fcc_lattice(6, 0.8442, 0.722, 4242)
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, False)
+    ctx.config_list(2.5, 0.3)
+    ctx.build_list()
+    ctx.forces()
+    ctx.set_integrator(kind=0, dt=0.005)
+    ctx.run(150, thermo=False)
+    assert ctx.rebuild_count() >= 2
+    ctx.build_list()
+    x, _, _ = ctx.download()
+    snap = fr.copy()
+    snap.x, snap.y, snap.z = x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy()
+    want = orc.verlet_pairs(snap, 2.5, 0.3)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert np.array_equal(ctx.pairs(), want)
+
+
+def test_rebuild_trigger_two_largest(ctx, orc):
+    fr = load_nist(nist_lj(1))
+    skin = 0.3
+    for moves, expect in [([(0, (0.3 * skin, 0, 0))], False),
+                          ([(0, (0.6 * skin, 0, 0)), (1, (0, 0.6 * skin, 0))], True)]:
+        ctx.upload(fr.L, _pos(fr))
+        ctx.config_list(3.0, skin)
+        ctx.build_list()
+        p = _pos(fr).copy()
+        for i, d in moves:
+            p[i] += np.array(d)
+        ctx.set_positions(p)
+        assert ctx.needs_rebuild() is expect
+        cur = fr.copy()
+        cur.x, cur.y, cur.z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+        assert orc.needs_rebuild(fr, cur, 3.0, skin) is expect
+
+
+def test_exp02_trajectory_matches_golden_and_oracle(ctx, orc):
+    """fcc(3), rho 0.80, T 1, seed 12345, LFS rc 2.5, skin 0.4, NVE dt 0.001,
+    1000 steps: the reference's committed thermo CSV (10 significant digits,
+    produced on another platform) and the oracle on this one."""
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 12345)
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, True)
+    ctx.config_list(2.5, 0.4)
+    ctx.build_list()
+    ctx.forces()
+    ctx.set_integrator(kind=0, dt=0.001)
+    th = ctx.run(1000)
+    sim = orc.sim(fr, kind="vv", dt=0.001, rc=2.5, lfs=True, skin=0.4)
+    ot = sim.run(1000)
+    assert np.max(np.abs(th["pe"] - ot["pe"]) / np.abs(ot["pe"])) < 1e-10
+    assert np.max(np.abs(th["ke"] - ot["ke"]) / np.abs(ot["ke"])) < 1e-10
+    rows = [ln.strip().split(",") for ln in open(os.path.join(GOLDEN, "results", "exp02_thermo_dt0.001.csv"))
+            if ln[0].isdigit()]
+    pe_csv = np.array([float(r[3]) for r in rows[1:]])
+    ke_csv = np.array([float(r[4]) for r in rows[1:]])
+    assert np.max(np.abs(th["pe"] - pe_csv) / np.abs(pe_csv)) < 2e-9
+    assert np.max(np.abs(th["ke"] - ke_csv) / np.abs(ke_csv)) < 2e-9
+    x, v, _ = ctx.download()
+    ox, ov, _ = sim.state()
+    assert np.max(np.abs(x - ox)) < 1e-10
+    assert ctx.rebuild_count() == sim.rebuilds
+
+
+def test_energy_error_scales_dt2(ctx, orc):
+    """Fitted log-log slope of max |E(t) - E0|/|E0| against dt in (1.85, 2.15)
+    (the reference's gate, tests/test_integrator.cpp:10-50)."""
+    fr0 = orc.fcc_lattice(3, 0.80, 1.0, 12345)
+    logs = []
+    for dt in [0.001, 0.002, 0.004, 0.008]:
+        ctx.upload(fr0.L, _pos(fr0), _vel(fr0))
+        ctx.set_lj(1.0, 1.0, 2.5, True)
+        ctx.config_list(2.5, 0.4)
+        ctx.build_list()
+        e0 = ctx.forces()[0] + ctx.kinetic_energy()
+        ctx.set_integrator(kind=0, dt=dt)
+        th = ctx.run(int(round(1.0 / dt)))
+        dev = np.max(np.abs(th["pe"] + th["ke"] - e0))
+        logs.append((np.log(dt), np.log(dev / abs(e0))))
+    x, y = np.array(logs).T
+    slope = np.polyfit(x, y, 1)[0]
+    assert 1.85 < slope < 2.15, slope
+
+
+def test_time_reversibility(ctx, orc):
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 999)
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, True)
+    ctx.config_list(2.5, 0.3)
+    ctx.build_list()
+    ctx.forces()
+    ctx.set_integrator(kind=0, dt=0.002)
+    ctx.run(200, thermo=False)
+    x, v, _ = ctx.download()
+    ctx.set_velocities(-v)
+    ctx.run(200, thermo=False)
+    x2, _, _ = ctx.download()
+    d = x2 - _pos(fr)
+    d -= fr.L * np.floor(d / fr.L + 0.5)
+    assert np.max(np.linalg.norm(d, axis=1)) < 1e-8
+
+
+def test_nhc_short_trajectory_matches_oracle(ctx, orc):
+    fr = orc.fcc_lattice(4, 0.8442, 0.722, 20260001)
+    n = fr.n
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, False)
+    ctx.config_list(2.5, 0.3)
+    ctx.build_list()
+    ctx.forces()
+    ctx.set_integrator(kind=2, dt=0.005, temperature=0.722, tau=0.5, chain_length=3, dof=3 * n - 3)
+    th = ctx.run(50)
+    sim = orc.sim(fr, kind="nhc", dt=0.005, rc=2.5, lfs=False, skin=0.3, T=0.722, tau=0.5, M=3,
+                  dof=3 * n - 3)
+    ot = sim.run(50)
+    assert np.max(np.abs(th["pe"] - ot["pe"]) / np.abs(ot["pe"])) < 1e-9
+    assert np.max(np.abs(th["ke"] - ot["ke"]) / np.abs(ot["ke"])) < 1e-9
+    assert abs(th["conserved"][-1] - ot["conserved"][-1]) < 1e-9 * abs(ot["pe"][-1])
+    x, v, _ = ctx.download()
+    ox, ov, _ = sim.state()
+    assert np.max(np.abs(v - ov)) < 1e-9
+
+
+def test_nhc_invalid_chain_length(ctx):
+    from moleculardynamics_amd import native
+
+    with pytest.raises(native.MDError) as ei:
+        ctx.set_integrator(kind=2, dt=0.005, chain_length=0, dof=3)
+    assert ei.value.status == native.MD_ERR_INVALID_ARGUMENT
+
+
+def test_nhc_conserves_extended_hamiltonian(ctx, orc):
+    """H' = PE + KE + chain contribution drifts < 1e-3 over 20k steps
+    (tests/test_thermostat.cpp:224-263 of the reference)."""
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 56)
+    n = fr.n
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, True)
+    ctx.config_list(2.5, 0.3)
+    ctx.build_list()
+    pe0 = ctx.forces()[0]
+    h0 = pe0 + ctx.kinetic_energy()
+    ctx.set_integrator(kind=2, dt=0.002, temperature=1.0, tau=0.5, chain_length=3, dof=3 * n - 3)
+    th = ctx.run(20000)
+    h = th["pe"] + th["ke"] + th["conserved"]
+    assert np.max(np.abs(h - h0)) / abs(h0) < 1e-3
+
+
+def test_mixed_precision_force_error(ctx, orc):
+    """FP32 pair arithmetic with fixed-point accumulation against FP64 on the
+    same snapshot: RMS relative force error reported, bounded at 1e-5 here
+    (the target 1e-6 is reported by the benchmark, not asserted)."""
+    fr = orc.fcc_lattice(8, 0.8442, 0.722, 20260001)
+    ctx.upload(fr.L, _pos(fr), _vel(fr))
+    ctx.set_lj(1.0, 1.0, 2.5, False)
+    ctx.config_list(2.5, 0.3)
+    ctx.build_list()
+    ctx.forces()
+    ctx.set_integrator(kind=0, dt=0.005)
+    ctx.run(200, thermo=False)  # leave the lattice: a liquid-like snapshot
+    from moleculardynamics_amd import native
+
+    e64, w64 = ctx.forces(native.VERLET, native.FP64)
+    _, _, f64 = ctx.download()
+    e32, w32 = ctx.forces(native.VERLET, native.MIXED)
+    _, _, f32 = ctx.download()
+    rms = np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2))
+    assert rms < 1e-5, rms
+    assert abs(e32 - e64) / abs(e64) < 1e-5
+    assert abs(w32 - w64) / abs(w64) < 1e-4
+    # determinism: a second evaluation is bit-identical
+    ctx.forces(native.VERLET, native.MIXED)
+    _, _, f32b = ctx.download()
+    assert np.array_equal(f32, f32b)

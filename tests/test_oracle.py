@@ -1,0 +1,423 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Every oracle function is checked against a reference-held golden value
+(NIST tables, the reference's committed exp02 trajectory and summary), a
+closed form, an invariant, brute force, or -- where built -- the reference
+compiled from its own sources (oracle/_ref), bit for bit.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, nist_lj, nist_spce
+from oracle.pyoracle import load_nist
+
+NIST = json.load(open(os.path.join(GOLDEN, "nist_values.json")))
+
+
+def half_width(v, sig_figs):
+    """Half a unit in the last published significant figure."""
+    if v == 0.0:
+        return 0.0
+    return 0.5 * 10 ** (np.floor(np.log10(abs(v))) - (sig_figs - 1))
+
+
+# ---- NIST Lennard-Jones ------------------------------------------------------
+
+@pytest.mark.parametrize("row", NIST["lj"], ids=lambda r: f"cfg{r[0]}-{r[1]}-rc{r[2]}")
+def test_nist_lj_energy_virial_lrc(orc, row):
+    cfg, scheme, rc, upair, wpair, ulrc = row
+    fr = load_nist(nist_lj(cfg))
+    lfs = scheme == "LFS"
+    e, w = orc.lj_energy_virial(fr, rc=rc, lfs=lfs)
+    assert abs(e - upair) <= half_width(upair, 5)
+    assert abs(w - wpair) <= half_width(wpair, 5)
+    u, _ = orc.lj_lrc(fr.n, fr.L, rc=rc, lfs=lfs)
+    if ulrc == 0.0:
+        assert abs(u) <= 1e-12
+    else:
+        assert abs(u - ulrc) <= half_width(ulrc, 5)
+
+
+def test_half_width_helper():
+    assert half_width(-4.3515e3, 5) == pytest.approx(0.05)
+    assert half_width(-5.4517e-1, 5) == pytest.approx(5e-6)
+    assert half_width(9.95387e4, 6) == pytest.approx(0.05)
+    assert half_width(-2.84469e6, 6) == pytest.approx(5.0)
+
+
+@pytest.mark.parametrize("rho,rc,u_n,p", NIST["lrc_hand"])
+def test_lrc_hand_values(orc, rho, rc, u_n, p):
+    n = 500
+    side = (n / rho) ** (1.0 / 3.0)
+    u, pr = orc.lj_lrc(n, [side] * 3, rc=rc)
+    assert u / n == pytest.approx(u_n, rel=2e-3)
+    assert pr == pytest.approx(p, rel=2e-3)
+
+
+def test_lrc_intensive_extensive_and_lfs_zero(orc):
+    s1, s2 = (500 / 0.5) ** (1 / 3), (1000 / 0.5) ** (1 / 3)
+    u1, p1 = orc.lj_lrc(500, [s1] * 3)
+    u2, p2 = orc.lj_lrc(1000, [s2] * 3)
+    assert p2 == pytest.approx(p1, rel=1e-12)
+    assert u2 == pytest.approx(2 * u1, rel=1e-12)
+    assert orc.lj_lrc(500, [s1] * 3, lfs=True) == (0.0, 0.0)
+
+
+def test_lj_cutoff_closed_form(orc):
+    """Two atoms: u and w by hand, LFS shift vanishing at rc."""
+    from oracle.pyoracle import Frame
+
+    for r in [0.95, 1.1224620483093730, 1.7, 2.4999]:
+        fr = Frame(np.array([10.0, 10.0, 10.0]), np.array([0.0, r]), np.zeros(2), np.zeros(2))
+        e, w = orc.lj_energy_virial(fr, rc=2.5)
+        s6 = r ** -6
+        assert e == pytest.approx(4 * (s6 * s6 - s6), rel=1e-14, abs=1e-15)
+        assert w == pytest.approx(24 * (2 * s6 * s6 - s6), rel=1e-13, abs=1e-13)
+        f, _, _ = orc.lj_forces(fr, rc=2.5)
+        # f_i = (w/r^2) (r_i - r_j): atom 0 sits at -r along x relative to atom 1
+        assert f[0, 0] == pytest.approx(-w / r, rel=1e-13, abs=1e-13)
+        assert f[1, 0] == -f[0, 0]
+    # u(2.5) of the truncated potential (shift used for E_shift)
+    s6 = 2.5 ** -6
+    assert 4 * (s6 * s6 - s6) == pytest.approx(-0.016316891, rel=1e-7)
+    fr = Frame(np.array([10.0, 10.0, 10.0]), np.array([0.0, 2.5 - 1e-9]), np.zeros(2), np.zeros(2))
+    e, w = orc.lj_energy_virial(fr, rc=2.5, lfs=True)
+    assert abs(e) < 1e-9 and abs(w) < 1e-8
+
+
+# ---- forces: gradient, Newton 3, list vs brute ------------------------------
+
+@pytest.mark.parametrize("lfs", [False, True])
+def test_force_is_minus_gradient(orc, lfs):
+    fr = load_nist(nist_lj(4))
+    f, _, _ = orc.lj_forces(fr, rc=3.0, lfs=lfs)
+    h = 1e-5
+    for i in (0, 7, 29):
+        for d, arr in enumerate((fr.x, fr.y, fr.z)):
+            orig = arr[i]
+            arr[i] = orig + h
+            up = orc.lj_energy_virial(fr, rc=3.0, lfs=lfs)[0]
+            arr[i] = orig - h
+            um = orc.lj_energy_virial(fr, rc=3.0, lfs=lfs)[0]
+            arr[i] = orig
+            num = -(up - um) / (2 * h)
+            assert f[i, d] == pytest.approx(num, rel=1e-8, abs=1e-8)
+
+
+def test_newton_third_law_and_energy_consistency(orc):
+    fr = load_nist(nist_lj(4))
+    f, e, w = orc.lj_forces(fr, rc=3.0)
+    assert np.all(np.abs(f.sum(axis=0)) < 1e-10)
+    fr2 = load_nist(nist_lj(2))
+    e1, w1 = orc.lj_energy_virial(fr2, rc=3.0)
+    _, e2, w2 = orc.lj_forces(fr2, rc=3.0)
+    assert e2 == pytest.approx(e1, rel=1e-12)
+    assert w2 == pytest.approx(w1, rel=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["cells", "verlet"])
+def test_lists_reproduce_brute_force(orc, mode):
+    fr = load_nist(nist_lj(1))
+    fb, eb, wb = orc.lj_forces(fr, rc=3.0, mode="brute")
+    fl, el, wl = orc.lj_forces(fr, rc=3.0, mode=mode, skin=0.3)
+    assert el == pytest.approx(eb, rel=1e-12)
+    assert wl == pytest.approx(wb, rel=1e-12)
+    assert np.max(np.abs(fl - fb)) <= 1e-12 * np.max(np.abs(fb))
+
+
+def test_unusable_cells_refused(orc):
+    from oracle.pyoracle import Frame
+
+    assert not orc.cells_usable([5.0, 5.0, 5.0], 3.0)
+    fr = Frame(np.array([5.0, 5.0, 5.0]), np.array([0.0, 1.1]), np.zeros(2), np.zeros(2))
+    with pytest.raises(ValueError):
+        orc.lj_forces(fr, rc=3.0, mode="cells")
+
+
+def test_verlet_pair_set_is_brute_force_set(orc):
+    """The list is exactly {i<j : |minimum image|^2 < (rc+skin)^2}."""
+    fr = load_nist(nist_lj(3))
+    got = orc.verlet_pairs(fr, 3.0, 0.3)
+    got = set(map(tuple, got.tolist()))
+    p = np.stack([fr.x, fr.y, fr.z], axis=1)
+    want = set()
+    for i in range(fr.n):
+        d = p[i] - p[i + 1:]
+        d -= fr.L * np.floor(d / fr.L + 0.5)
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        for j in np.nonzero(r2 < 3.3 * 3.3)[0]:
+            want.add((i, i + 1 + int(j)))
+    assert got == want
+
+
+def test_fcc_shell_counts(orc):
+    """fcc at rho 0.8442: 54 neighbours within 2.5 and 78 within 2.8 per atom."""
+    fr = orc.fcc_lattice(6, 0.8442, 0.722, 12345)
+    assert orc.verlet_pairs(fr, 2.5, 0.0).shape[0] == 864 * 54 // 2
+    assert orc.verlet_pairs(fr, 2.5, 0.3).shape[0] == 864 * 78 // 2
+
+
+def test_rebuild_trigger(orc):
+    fr = load_nist(nist_lj(1))
+    skin = 0.3
+    cur = fr.copy()
+    cur.x[0] += 0.3 * skin
+    assert not orc.needs_rebuild(fr, cur, 3.0, skin)
+    cur = fr.copy()
+    cur.x[0] += 0.6 * skin
+    cur.y[1] += 0.6 * skin
+    assert orc.needs_rebuild(fr, cur, 3.0, skin)
+
+
+# ---- seeded inputs and dynamics: the reference's committed exp02 outputs -----
+
+def _exp02_rows(dt):
+    path = os.path.join(GOLDEN, "results", f"exp02_thermo_dt{dt}.csv")
+    rows = [ln.strip().split(",") for ln in open(path) if ln[:1].isdigit()]
+    return np.array([[float(r[3]), float(r[4]), float(r[6]), float(r[7])] for r in rows])
+
+
+@pytest.mark.parametrize("dt", ["0.001", "0.008"])
+def test_exp02_trajectory(orc, dt):
+    """fccLattice + Rng + LJ(LFS) + Verlet list + VV against the reference's
+    committed 108-atom NVE trajectory (10 significant digits, another
+    platform's libm: compared at 2e-9)."""
+    gold = _exp02_rows(dt)
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 12345)
+    n, V = fr.n, float(np.prod(fr.L))
+    sim = orc.sim(fr, kind="vv", dt=float(dt), rc=2.5, lfs=True, skin=0.4)
+    _, v0, _ = sim.state()
+    ke0 = 0.5 * np.sum(v0 * v0)
+    assert ke0 == pytest.approx(gold[0, 1], rel=2e-9)
+    th = sim.run(gold.shape[0] - 1)
+    T = 2 * th["ke"] / (3 * n - 3)
+    P = n / V * T + th["virial"] / (3 * V)
+    for col, got in [(0, th["pe"]), (1, th["ke"]), (2, T), (3, P)]:
+        assert np.max(np.abs(got - gold[1:, col]) / np.abs(gold[1:, col])) < 2e-9
+
+
+def test_exp02_summary_and_dt2_slope(orc):
+    """e0, per-dt relative errors (rounded to 4 significant digits in the
+    reference CSV) and the fitted slope 2.003996 +- 0.15."""
+    rows = [ln.strip().split(",") for ln in open(os.path.join(GOLDEN, "results", "exp02_energy_conservation.csv"))
+            if ln[:1].isdigit()]
+    xs, ys = [], []
+    for r in rows:
+        dt, e0_ref, rel_ref = float(r[0]), float(r[1]), float(r[3])
+        fr = orc.fcc_lattice(3, 0.80, 1.0, 12345)
+        sim = orc.sim(fr, kind="vv", dt=dt, rc=2.5, lfs=True, skin=0.4)
+        _, v0, _ = sim.state()
+        pe0 = sim.run(0)["pe"] if False else None  # noqa: F841
+        th = sim.run(int(round(1.0 / dt)))
+        e = th["pe"] + th["ke"]
+        e0 = e0_ref
+        rel = np.max(np.abs(e - e0)) / abs(e0)
+        assert rel == pytest.approx(rel_ref, rel=2e-3)
+        xs.append(np.log(dt))
+        ys.append(np.log(rel))
+    slope = np.polyfit(xs, ys, 1)[0]
+    assert 1.85 < slope < 2.15
+    assert slope == pytest.approx(2.003996, abs=5e-3)
+
+
+def test_time_reversible(orc):
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 999)
+    sim = orc.sim(fr, kind="vv", dt=0.002, rc=2.5, lfs=True)
+    sim.run(200)
+    x, v, _ = sim.state()
+    sim.set_velocities(-v)
+    sim.run(200)
+    x2, _, _ = sim.state()
+    d = x2 - np.stack([fr.x, fr.y, fr.z], axis=1)
+    d -= fr.L * np.floor(d / fr.L + 0.5)
+    assert np.max(np.linalg.norm(d, axis=1)) < 1e-8
+
+
+def test_nhc_conserved_quantity(orc):
+    """H' drift under the chain (shortened from the reference's 20k steps)."""
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 56)
+    n = fr.n
+    sim = orc.sim(fr, kind="nhc", dt=0.002, rc=2.5, lfs=True, T=1.0, tau=0.5, M=3, dof=3 * n - 3)
+    _, v0, _ = sim.state()
+    pe0 = orc.lj_energy_virial(fr, rc=2.5, lfs=True)[0]
+    h0 = pe0 + 0.5 * np.sum(v0 * v0)
+    th = sim.run(4000)
+    h = th["pe"] + th["ke"] + th["conserved"]
+    assert np.max(np.abs(h - h0)) / abs(h0) < 1e-3
+
+
+def test_nhc_rejects_empty_chain(orc):
+    fr = orc.fcc_lattice(2, 0.8, 1.0, 1)
+    with pytest.raises(ValueError):
+        orc.sim(fr, kind="nhc", M=0)
+
+
+def test_baoab_equipartition(orc):
+    """Mean T over a window within 2% of the target, dof = 3N (shortened)."""
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 42)
+    n = fr.n
+    sim = orc.sim(fr, kind="baoab", dt=0.005, rc=2.5, lfs=True, gamma=1.0, T=1.0, seed=777)
+    sim.run(1000)
+    th = sim.run(4000)
+    T = np.mean(2 * th["ke"] / (3 * n))
+    assert 0.98 < T < 1.02
+
+
+# ---- reference build: bit-for-bit -------------------------------------------
+
+def test_oracle_matches_reference_build(orc, ref):
+    assert ref.is_reference and not orc.is_reference
+    for cfg in (1, 4):
+        fr = load_nist(nist_lj(cfg))
+        for lfs in (False, True):
+            assert orc.lj_energy_virial(fr, rc=3.0, lfs=lfs) == ref.lj_energy_virial(fr, rc=3.0, lfs=lfs)
+            for mode in ("brute", "cells", "verlet"):
+                a, b = orc.lj_forces(fr, rc=3.0, lfs=lfs, mode=mode), ref.lj_forces(fr, rc=3.0, lfs=lfs, mode=mode)
+                assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+    fr = load_nist(nist_lj(1))
+    assert np.array_equal(orc.verlet_pairs(fr, 3.0, 0.3), ref.verlet_pairs(fr, 3.0, 0.3))
+    f1, f2 = orc.fcc_lattice(5, 0.8442, 0.722, 3), ref.fcc_lattice(5, 0.8442, 0.722, 3)
+    for k in ("L", "x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(f1, k), getattr(f2, k))
+
+
+@pytest.mark.parametrize("kind", ["vv", "baoab", "nhc"])
+def test_oracle_dynamics_match_reference_build(orc, ref, kind):
+    fr = orc.fcc_lattice(3, 0.80, 1.0, 12345)
+    kw = dict(kind=kind, dt=0.002, rc=2.5, lfs=True, skin=0.3, T=1.0, seed=11)
+    a, b = orc.sim(fr, **kw), ref.sim(fr, **kw)
+    ta, tb = a.run(300), b.run(300)
+    for k in ta:
+        assert np.array_equal(ta[k], tb[k]), k
+    for p, q in zip(a.state(), b.state()):
+        assert np.array_equal(p, q)
+    assert a.rebuilds == b.rebuilds
+
+
+# ---- electrostatics ----------------------------------------------------------
+
+def test_coulomb_prefactor(orc):
+    assert orc.coulomb_prefactor() == pytest.approx(NIST["coulomb_prefactor_K_A"], rel=1e-9)
+
+
+def _water(cfg):
+    fr = load_nist(nist_spce(cfg))
+    n = fr.n
+    q = np.array([-0.8476 if i % 3 == 0 else 0.4238 for i in range(n)])
+    return fr, q, np.arange(n) // 3
+
+
+@pytest.mark.parametrize("row", NIST["spce"], ids=lambda r: f"spce{r[0]}")
+def test_nist_spce_components(orc, row):
+    cfg, edisp, elrc, ereal, efour, eself, eintra, etot = row
+    fr, q, mol = _water(cfg)
+    ox = fr.copy()
+    sel = np.arange(0, fr.n, 3)
+    ox.x, ox.y, ox.z = fr.x[sel], fr.y[sel], fr.z[sel]
+    d, _ = orc.lj_energy_virial(ox, sig=3.16555789, eps=78.19743111, rc=10.0)
+    lrc, _ = orc.lj_lrc(ox.n, ox.L, sig=3.16555789, eps=78.19743111, rc=10.0)
+    t, _ = orc.ewald(fr, q, mol, rc=10.0, kmax=5)
+    comps = [(d, edisp), (lrc, elrc), (t["real"], ereal), (t["recip"], efour), (t["self"], eself),
+             (t["intra"], eintra)]
+    for got, want in comps:
+        assert abs(got - want) <= half_width(want, 6)
+    total = sum(c[0] for c in comps)
+    bound = sum(half_width(c[1], 6) for c in comps) / abs(etot)
+    assert abs(total - etot) / abs(etot) <= bound
+
+
+def test_ewald_force_is_minus_gradient_and_newton3(orc):
+    fr, q, mol = _water(1)
+    _, f = orc.ewald(fr, q, mol, forces=True)
+    h = 1e-5
+    for i in (0, 1, 2, 150, 299):
+        for d, arr in enumerate((fr.x, fr.y, fr.z)):
+            o = arr[i]
+            arr[i] = o + h
+            up = sum(orc.ewald(fr, q, mol)[0].values())
+            arr[i] = o - h
+            um = sum(orc.ewald(fr, q, mol)[0].values())
+            arr[i] = o
+            assert f[i, d] == pytest.approx(-(up - um) / (2 * h), rel=1e-7, abs=1e-4)
+    fr2, q2, mol2 = _water(2)
+    _, f2 = orc.ewald(fr2, q2, mol2, forces=True)
+    assert np.all(np.abs(f2.sum(axis=0)) < 1e-8 * np.max(np.abs(f2)) * fr2.n)
+
+
+def test_ewald_matches_reference_build(orc, ref):
+    fr, q, mol = _water(1)
+    a, fa = orc.ewald(fr, q, mol, forces=True)
+    b, fb = ref.ewald(fr, q, mol, forces=True)
+    for k in ("real", "recip", "self"):
+        assert a[k] == b[k]
+    # the reference sums the intramolecular term over an unordered_map
+    assert a["intra"] == pytest.approx(b["intra"], rel=1e-13)
+    assert np.max(np.abs(fa - fb)) < 1e-9 * np.max(np.abs(fb))
+
+
+# ---- rigid water ---------------------------------------------------------------
+
+def _water_masses(n):
+    return np.array([15.9994 if i % 3 == 0 else 1.008 for i in range(n)])
+
+
+def _geometry_error(p, L):
+    worst = 0.0
+    hh = 2.0 * np.sin(0.5 * np.deg2rad(109.47))
+    for a, b, t in ((0, 1, 1.0), (0, 2, 1.0), (1, 2, hh)):
+        d = p[a::3] - p[b::3]
+        d -= L * np.floor(d / L + 0.5)
+        worst = max(worst, np.max(np.abs(np.linalg.norm(d, axis=1) - t) / t))
+    return worst
+
+
+def test_rigid_water_shake_invariants(orc):
+    fr, _, _ = _water(1)
+    m = _water_masses(fr.n)
+    p0 = np.stack([fr.x, fr.y, fr.z], axis=1)
+    assert _geometry_error(p0, fr.L) < 1e-6
+    it, p1 = orc.rigid_shake(fr, m, fr)
+    assert np.max(np.abs(p1 - p0)) < 1e-9  # idempotent on a rigid input
+    moved = fr.copy()
+    s = 0.01 * (np.arange(fr.n) % 7) - 0.03
+    moved.x = fr.x + s
+    moved.y = fr.y - 0.5 * s
+    moved.z = fr.z + 0.25 * s
+    it, p2 = orc.rigid_shake(moved, m, fr)
+    pm = np.stack([moved.x, moved.y, moved.z], axis=1)
+    com = lambda p: (p.reshape(-1, 3, 3) * m.reshape(-1, 3, 1)).sum(1) / m.reshape(-1, 3).sum(1)[:, None]  # noqa: E731
+    assert np.max(np.abs(com(p2) - com(pm))) < 1e-10
+    assert _geometry_error(p2, fr.L) < 1e-10
+    assert orc.rigid_shake(moved, m, fr, maxit=1)  # one sweep is allowed to finish
+    with pytest.raises(RuntimeError):
+        orc.rigid_shake(moved, m, fr, tol=1e-30, maxit=2)
+
+
+def test_rigid_water_bad_arguments(orc):
+    fr, _, _ = _water(1)
+    m = _water_masses(fr.n)
+    with pytest.raises(ValueError):
+        orc.rigid_shake(fr, m, fr, oh=-1.0)
+    with pytest.raises(ValueError):
+        orc.rigid_shake(fr, m, fr, angle=180.0)
+
+
+def test_spce_dynamics_matches_reference_build(orc, ref):
+    fr, _, _ = _water(1)
+    rng = np.random.default_rng(3)
+    m = _water_masses(fr.n)
+    v = rng.normal(size=(fr.n, 3)) * np.sqrt(298.0 / m)[:, None]
+    v -= (v * m[:, None]).sum(0) / m.sum()
+    fr.vx, fr.vy, fr.vz = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    _, vr = orc.rigid_rattle(fr, m)
+    fr.vx, fr.vy, fr.vz = vr[:, 0].copy(), vr[:, 1].copy(), vr[:, 2].copy()
+    wa, pa, ka = orc.water_run(fr, nsteps=5)
+    wb, pb, kb = ref.water_run(fr, nsteps=5)
+    assert np.max(np.abs(pa - pb) / np.abs(pb)) < 1e-12
+    assert np.max(np.abs(ka - kb) / np.abs(kb)) < 1e-11
+    pa_, pb_ = np.stack([wa.x, wa.y, wa.z], 1), np.stack([wb.x, wb.y, wb.z], 1)
+    assert np.max(np.abs(pa_ - pb_)) < 1e-10
+    assert _geometry_error(pa_, fr.L) < 1e-10
