@@ -116,6 +116,9 @@ md_status md_system_set_positions(md_ctx* ctx, const double* x, const double* y,
                                   const double* z);
 md_status md_system_set_velocities(md_ctx* ctx, const double* vx, const double* vy,
                                    const double* vz);
+/* System::setForce for every atom (system.hpp:35): forces an integrator
+ * pass will use when no force call precedes it. */
+md_status md_system_set_forces(md_ctx* ctx, const double* fx, const double* fy, const double* fz);
 /* System::kineticEnergy (system.cpp:124-131), FP64 deterministic reduction. */
 md_status md_system_kinetic_energy(md_ctx* ctx, double* ke);
 /* System::removeCenterOfMassMotion (system.cpp:147-164). */
@@ -184,6 +187,25 @@ md_status md_integrator_set(md_ctx* ctx, const md_integrator* it);
  * (seed, step, upload index), so it is independent of the device atom order;
  * it is not the reference's sequential mt19937_64 stream. */
 md_status md_run(md_ctx* ctx, long nsteps, int precision, md_thermo* thermo);
+/* Single per-atom integrator passes, for a host-driven step with an
+ * arbitrary force callback between them (the reference's
+ * `template <class ForceFn> step(System&, ForceFn&&)`,
+ * velocity_verlet.hpp:48-74, langevin.hpp:36-88, nose_hoover.hpp:37-42):
+ *   md_half_kick:   v = fma((dt/2) * (1/m), f, v)      (velocity_verlet.hpp:55-58)
+ *   md_drift:       r = wrap(fma(h, v, r))             (velocity_verlet.hpp:61-64)
+ *   md_langevin_o:  v = c1 v + (c2 sqrt(T/m)) xi, xi from the counter-based
+ *                   stream (seed, counter, upload index) (langevin.hpp:58-64)
+ *   md_scale_velocities: v = v * s                      (nose_hoover.cpp:119-124) */
+md_status md_half_kick(md_ctx* ctx, double dt);
+md_status md_drift(md_ctx* ctx, double h);
+md_status md_langevin_o(md_ctx* ctx, double c1, double c2, double temperature, uint64_t seed,
+                        uint64_t counter);
+md_status md_scale_velocities(md_ctx* ctx, double s);
+/* LennardJones::computeForces(System&, const CellList&) with the CellList's
+ * own cutoff (lennard_jones.cpp:99-114, cell_list.cpp:8-14): candidates are
+ * the 27 cells around each atom on a grid of floor(L/cell_cutoff) cells per
+ * side; MD_ERR_INVALID_ARGUMENT when fewer than 3 fit. */
+md_status md_lj_forces_cells(md_ctx* ctx, double cell_cutoff, double* energy, double* virial);
 /* NoseHooverChain::conservedQuantityContribution (nose_hoover.cpp:52-61)
  * and the chain arrays (length chain_length each; NULL to skip). */
 md_status md_nhc_state(md_ctx* ctx, double* conserved, double* xi, double* p_xi);
@@ -199,6 +221,114 @@ md_status md_nhc_state(md_ctx* ctx, double* conserved, double* xi, double* p_xi)
 md_status md_fcc_lattice(int cells, double density, double temperature, uint64_t seed,
                          double* L, double* x, double* y, double* z, double* vx, double* vy,
                          double* vz);
+
+/* ---- batched independent replicas ------------------------------------------
+ * No reference counterpart as one object: the reference runs one System per
+ * process (apps/moldyn_run.cpp:570-777). A batch is many independent LJ
+ * systems, each with its own box, Verlet list (verlet_list.cpp:8-69) and
+ * LangevinBAOAB thermostat (langevin.hpp:36-88), advanced together: one
+ * kernel launch per step for all replicas, multi-step segments captured in a
+ * HIP graph. Forces use MD_MIXED or MD_FP64 pair arithmetic; LJ Truncated. */
+typedef struct md_batch md_batch;
+
+/* The replica recipe: from Rng(master_seed) (util/random.hpp:37-48), per
+ * replica in order: fcc cells uniform in {3..10} (N = 4 cells^3 in
+ * {108, ..., 4000}), density uniform in [0.5, 0.9), temperature uniform in
+ * [0.7, 1.5); lattice seed master_seed + r. Host arithmetic. */
+md_status md_batch_draw(int count, uint64_t master_seed, int* cells, double* rho, double* T,
+                        uint64_t* seeds);
+/* Flat inputs: replica r owns atoms [sum n[<r], sum n[<=r]) of x..vz (mass 1);
+ * L holds 3 lengths per replica; T and seeds one per replica. */
+md_status md_batch_create(void* hip_stream, int nrep, const long* n, const double* L,
+                          const double* x, const double* y, const double* z, const double* vx,
+                          const double* vy, const double* vz, const double* T,
+                          const uint64_t* seeds, const md_lj_params* lj, double skin, double dt,
+                          double friction, int precision, md_batch** out);
+md_status md_batch_destroy(md_batch* b);
+const char* md_batch_last_error(const md_batch* b);
+/* Builds every replica's rows and primes the forces (the reference primes
+ * its force callback once before the first step, velocity_verlet.hpp:34-38). */
+md_status md_batch_prime(md_batch* b);
+/* Advances every replica nsteps BAOAB steps; with use_graph, `segment`
+ * launches are captured once in a HIP graph and replayed. thermo_out (or
+ * NULL) receives nsteps x nrep rows of (E, W, KE, step). */
+md_status md_batch_run(md_batch* b, long nsteps, long segment, int use_graph, double* thermo_out);
+/* Lanes per atom of each replica's force rows (1, 2, 4, 8, 16) and the
+ * workgroup size (256, 512, 1024). */
+md_status md_batch_set_tpa(md_batch* b, const int* tpa, int block);
+/* Times candidate (workgroup size, per-size-class lanes-per-atom) choices on
+ * the batch itself from a saved state, applies the fastest, restores the
+ * state. */
+md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* block_out,
+                            double* best_ms_per_step);
+md_status md_batch_download(md_batch* b, double* x, double* y, double* z, double* vx, double* vy,
+                            double* vz, double* fx, double* fy, double* fz);
+/* Per-replica list build counts (VerletList::rebuildCount). */
+md_status md_batch_rebuild_counts(md_batch* b, long* counts);
+
+/* ---- rigid SPC/E water with Ewald -------------------------------------------
+ * SpceForceField (forcefield/spce_forcefield.{hpp:23-51,cpp:8-48}) = oxygen LJ
+ * (Truncated + tail) + Ewald (forcefield/ewald.{hpp:49-76,cpp:33-188}) under
+ * RigidWater SHAKE/RATTLE (constrain/rigid_water.{hpp:28-66,cpp:30-161}) and
+ * ConstrainedVelocityVerlet (integrate/constrained_velocity_verlet.hpp:22-68).
+ * Units: A, amu, e, energies in K (E/kB), time in units of 1.0966875353 ps
+ * (core/units.hpp:33-40). Sites are O, H, H per molecule. */
+typedef struct md_water md_water;
+
+typedef struct md_water_params {
+    double qH;          /* 0.42380 e; qO = -2 qH (spce.hpp:17) */
+    double sigma_o;     /* 3.16555789 A */
+    double epsilon_o;   /* 78.19743111 K */
+    double r_oh;        /* 1.0 A */
+    double angle_hoh;   /* 109.47 deg */
+    double mass_o;      /* 15.9994 amu */
+    double mass_h;      /* 1.008 amu */
+    double cutoff;      /* real-space and LJ cutoff (SpceForceField default 10 A) */
+    int kmax;           /* Ewald(cutoff, kmax) (ewald.hpp:51) */
+    int n2max;          /* |n|^2 < n2max; <= 0 selects kmax^2 + 2 (ewald.cpp:119) */
+    double alpha;       /* <= 0 selects the reference's 5.6 / min(L) (ewald.cpp:40) */
+    double skin;        /* Verlet skin of the real-space list */
+    double tolerance;   /* RigidWater tolerance (1e-12) */
+    int max_iterations; /* RigidWater maxIterations (500) */
+} md_water_params;
+
+/* Ewald parameters for a target RMS force accuracy relative to the force
+ * between two unit charges 1 A apart, from the Kolafa-Perram estimates (no
+ * reference counterpart: the reference fixes alpha = 5.6/L, ewald.cpp:40). */
+md_status md_ewald_parameters(long n, const double* L, double sum_q2, double cutoff,
+                              double accuracy, double* alpha, int* kmax);
+/* Config-3 input builder (no reference counterpart; the reference rescales
+ * a NIST configuration, apps/water_rdf.cpp:53-83): m^3 molecules on a cubic
+ * lattice, uniform random orientations (Shoemake quaternions) and
+ * Maxwell-Boltzmann velocities from a seeded mt19937_64. */
+md_status md_water_lattice(int molecules_per_side, double box_length, uint64_t seed,
+                           double temperature, const md_water_params* p, double* x, double* y,
+                           double* z, double* vx, double* vy, double* vz);
+/* buildSpceSystem (forcefield/spce.cpp:8-40): charges and masses by site. */
+md_status md_water_create(void* hip_stream, long n, const double* L, const double* x,
+                          const double* y, const double* z, const double* vx, const double* vy,
+                          const double* vz, const md_water_params* p, md_water** out);
+md_status md_water_destroy(md_water* w);
+const char* md_water_last_error(const md_water* w);
+/* SpceForceField::computeForces: forces zeroed then dispersion + Ewald;
+ * terms[6] = {Edisp, ELRC, Ereal, Erecip, Eself, Eintra} (K), *pe their sum.
+ * MD_MIXED evaluates real-space erfc in FP32 (k-space stays FP64). */
+md_status md_water_forces(md_water* w, int precision, double* terms, double* pe);
+md_status md_water_alpha(const md_water* w, double* alpha, int* nk_half);
+md_status md_water_build_list(md_water* w);
+/* RigidWater::constrainPositions against reference positions (returns the
+ * sweep count; MD_ERR_RUNTIME on non-convergence or singularity). */
+md_status md_water_shake(md_water* w, const double* rx, const double* ry, const double* rz,
+                         int* iterations);
+/* RigidWater::constrainVelocities. */
+md_status md_water_rattle(md_water* w, int* iterations);
+/* nsteps of ConstrainedVelocityVerlet::step with the SPC/E force field;
+ * pe/ke (or NULL) receive per-step values in K. */
+md_status md_water_run(md_water* w, long nsteps, double dt, int precision, double* pe, double* ke);
+md_status md_water_download(md_water* w, double* x, double* y, double* z, double* vx, double* vy,
+                            double* vz, double* fx, double* fy, double* fz);
+md_status md_water_set_state(md_water* w, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz);
 
 /* ---- measurement ---------------------------------------------------------- */
 

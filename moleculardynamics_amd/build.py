@@ -20,7 +20,8 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libmoldyn_hip.so")
 
-SOURCES = ["md_capi.hip", "md_neigh.hip", "md_force.hip", "md_integrate.hip"]
+SOURCES = ["md_capi.hip", "md_neigh.hip", "md_force.hip", "md_integrate.hip", "md_batch.hip",
+           "md_batch_capi.hip", "md_water.hip", "md_water_capi.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [
@@ -60,6 +61,25 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_mirror_test(verbose: bool = True) -> str:
+    """Compiles tests/cpp/test_mirror.cpp (the reference's test cases against
+    the C++ drop-in classes of host/moldyn_gpu.hpp), linked to the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_mirror.cpp")
+    outdir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    exe = os.path.join(outdir, "test_mirror")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", exe, src, "-L" + LIBDIR, "-lmoldyn_hip",
+           "-Wl,-rpath,$ORIGIN/../../../moleculardynamics_amd/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"test_mirror build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_mirror_test(verbose=True)
     sys.exit(0)

@@ -200,6 +200,8 @@ md_status alloc_neigh(md_ctx* c) {
     c->ncell = c->grid.usable ? static_cast<long>(c->grid.nc[0]) * c->grid.nc[1] * c->grid.nc[2] : 1;
     if (c->ncell > c->ncell_alloc) {
         MD_HIP(c, alloc(c->cell_count, c->ncell));
+        // the rebuild kernels keep cell_count all-zero between builds
+        MD_HIP(c, hipMemset(c->cell_count, 0, c->ncell * sizeof(int)));
         MD_HIP(c, alloc(c->cell_start, c->ncell + 1));
         MD_HIP(c, alloc(c->cell_fill, c->ncell));
         c->ncell_alloc = c->ncell;
@@ -459,6 +461,20 @@ md_status md_system_set_velocities(md_ctx* c, const double* vx, const double* vy
     return MD_OK;
 }
 
+md_status md_system_set_forces(md_ctx* c, const double* fx, const double* fy, const double* fz) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    const long n = c->n;
+    double* s = c->stage;
+    MD_HIP(c, hipMemcpyAsync(s, fx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + n, fy, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MD_HIP(c, hipMemcpyAsync(s + 2 * n, fz, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->frc, 0, c->stream);
+    MD_HIP(c, hipGetLastError());
+    MD_HIP(c, hipStreamSynchronize(c->stream));
+    return MD_OK;
+}
+
 md_status md_system_kinetic_energy(md_ctx* c, double* ke) {
     md_status st = need_system(c);
     if (st != MD_OK) return st;
@@ -671,7 +687,7 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
     for (long t = 0; t < n; ++t) {
         const int k = std::min(cnt[t], c->cap);
         for (int q = 0; q < k; ++q) {
-            const uint32_t e = nb[static_cast<size_t>(q) * c->npad + t];
+            const uint32_t e = nb[static_cast<size_t>(mdk::nbr_index(t, q, c->cap))];
             const uint32_t j = e & mdk::kIndexMask;
             const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
             if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
@@ -773,6 +789,84 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
         return fail(c, MD_ERR_RUNTIME, "fixed-point force accumulator saturated (|f| > 2^19)");
     if (thermo)
         MD_HIP(c, hipMemcpy(thermo, c->thermo_dev, nsteps * sizeof(md_thermo), hipMemcpyDeviceToHost));
+    return MD_OK;
+}
+
+md_status md_half_kick(md_ctx* c, double dt) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_half_kick(c->n, c->vel, c->frc, 0.5 * dt, c->stream);
+    MD_HIP(c, hipGetLastError());
+    return MD_OK;
+}
+
+md_status md_drift(md_ctx* c, double h) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_drift(c->n, c->box, c->pos, c->vel, h, c->stream);
+    MD_HIP(c, hipGetLastError());
+    return MD_OK;
+}
+
+md_status md_langevin_o(md_ctx* c, double c1, double c2, double temperature, uint64_t seed,
+                        uint64_t counter) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_langevin_o(c->n, c->vel, c->orig, c1, c2, temperature, seed, counter, c->stream);
+    MD_HIP(c, hipGetLastError());
+    return MD_OK;
+}
+
+md_status md_scale_velocities(md_ctx* c, double s) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    mdk::launch_scale(c->n, c->vel, s, c->stream);
+    MD_HIP(c, hipGetLastError());
+    return MD_OK;
+}
+
+md_status md_lj_forces_cells(md_ctx* c, double cell_cutoff, double* energy, double* virial) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->lj_set) return fail(c, MD_ERR_STATE, "md_lj_set has not been called");
+    if (!(cell_cutoff > 0.0)) return fail(c, MD_ERR_INVALID_ARGUMENT, "cell cutoff must be positive");
+    const mdk::Grid g = make_grid(c->L, cell_cutoff);
+    if (!g.usable)
+        return fail(c, MD_ERR_INVALID_ARGUMENT,
+                    "CellList is not usable (fewer than 3 cells per side); fall back to the "
+                    "brute-force overload");
+    const long m = static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2];
+    if (m > c->ci_ncell_alloc) {
+        MD_HIP(c, alloc(c->ci_count, m));
+        MD_HIP(c, alloc(c->ci_start, m + 1));
+        MD_HIP(c, alloc(c->ci_fill, m));
+        c->ci_ncell_alloc = m;
+    }
+    MD_HIP(c, alloc(c->ci_cell_of, c->n));
+    MD_HIP(c, alloc(c->ci_atoms, c->n));
+    mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start, c->ci_fill,
+                          c->ci_atoms, c->stream);
+    mdk::DirectArgs a{};
+    a.n = c->n;
+    a.box = c->box;
+    a.lj = c->ljc;
+    a.pos = c->pos;
+    a.frc = c->frc;
+    a.orig = c->orig;
+    a.part_d = c->part_d;
+    a.part_l = c->part_l;
+    a.sc = c->sc;
+    a.grid = g;
+    a.cell_of = c->ci_cell_of;
+    a.cell_start = c->ci_start;
+    a.cell_atoms = c->ci_atoms;
+    mdk::launch_lj_direct(a, c->stream);
+    MD_HIP(c, hipGetLastError());
+    mdk::Scalars s{};
+    st = read_scalars(c, &s);
+    if (st != MD_OK) return st;
+    if (energy) *energy = s.energy;
+    if (virial) *virial = s.virial;
     return MD_OK;
 }
 

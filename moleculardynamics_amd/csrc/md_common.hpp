@@ -103,6 +103,16 @@ __device__ __forceinline__ double norm2_ref(double x, double y, double z) {
     return s;
 }
 
+// (ax*bx + ay*by) + az*bz with separate roundings (core/vec3.hpp:59-61).
+__device__ __forceinline__ double norm2_dot(double ax, double ay, double az, double bx, double by,
+                                            double bz) {
+#pragma clang fp contract(off)
+    double s = ax * bx;
+    s = s + ay * by;
+    s = s + az * bz;
+    return s;
+}
+
 // Oriented minimum-image difference of the unordered pair (a, b): the
 // reference evaluates r_i - r_j with i < j by upload index
 // (forcefield/lennard_jones.cpp:71, neighbor/verlet_list.cpp:19). `flip`
@@ -120,6 +130,35 @@ __device__ __forceinline__ void pair_delta(const Box& b, const double4& pa, cons
         dz = pa.z - pb.z;
     }
     min_image(b, dx, dy, dz);
+}
+
+// Slot k of atom t in the wave-blocked neighbour layout: every group of 64
+// consecutive atoms owns one contiguous [cap][64] tile, so a wave reading
+// slot k of its 64 atoms touches one 256-byte line, and all its slots sit in
+// one cap*256-byte span.
+__host__ __device__ __forceinline__ long nbr_index(long t, int k, int cap) {
+    return ((t >> 6) * static_cast<long>(cap) + k) * 64 + (t & 63);
+}
+
+// FP32 LJ pair terms of the mixed path (Truncated), written once so every
+// kernel that uses them evaluates the same operations in the same order:
+// r2 = (x*x + y*y) + z*z, s2 = sig^2 * rcp(r2) (hardware reciprocal, 1 ulp),
+// s6 = s2^3, u = 4 eps (s6^2 - s6), w = 24 eps (2 s6^2 - s6), c = w * rcp(r2).
+// Returns false outside the cutoff.
+__device__ __forceinline__ bool lj_terms_f32(float x, float y, float z, float sig2, float eps,
+                                             float rc2, float& r2, float& inv, float& u, float& w) {
+#pragma clang fp contract(off)
+    r2 = x * x;
+    r2 = r2 + y * y;
+    r2 = r2 + z * z;
+    if (!(r2 < rc2)) return false;
+    inv = __builtin_amdgcn_rcpf(r2);
+    const float s2 = sig2 * inv;
+    const float s6 = (s2 * s2) * s2;
+    const float s12 = s6 * s6;
+    u = (4.0f * eps) * (s12 - s6);
+    w = (24.0f * eps) * ((2.0f * s12) - s6);
+    return true;
 }
 
 // ---- wave64 / workgroup reductions (fixed tree: deterministic) -------------

@@ -131,6 +131,98 @@ __device__ void finish_step(const ForceArgs& a, bool mixed) {
     }
 }
 
+struct Acc64 {
+    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
+    long long np = 0;
+};
+
+template <bool LFS>
+__device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, uint32_t jj,
+                                       const double4& pj, Acc64& q) {
+    const bool flip = (jj & kFlipBit) != 0;
+    double dx, dy, dz;
+    pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+    const double r2 = norm2_ref(dx, dy, dz);
+    if (r2 < a.lj.rc2) {
+        const double inv = 1.0 / r2;
+        const double s2 = a.lj.sig2 * inv;
+        const double s6 = s2 * s2 * s2;
+        double u = 4.0 * a.lj.eps * (s6 * s6 - s6);
+        double ww = 24.0 * a.lj.eps * (2.0 * s6 * s6 - s6);
+        if (LFS) {
+            const double r = sqrt(r2);
+            u = u - a.lj.u_rc - (r - a.lj.rc) * a.lj.dudr_rc;
+            ww = ww + r * a.lj.dudr_rc;
+        }
+        q.e += u;
+        q.w += ww;
+        q.np += flip ? 0 : 1;
+        const double c = flip ? -(ww * inv) : ww * inv;
+        q.fx += c * dx;
+        q.fy += c * dy;
+        q.fz += c * dz;
+    }
+}
+
+struct AccFix {
+    long long fx = 0, fy = 0, fz = 0, e = 0, w = 0, np = 0;
+    int sat = 0;
+};
+
+template <bool LFS>
+__device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, uint32_t jj,
+                                       const double4& pj, AccFix& q) {
+    const bool flip = (jj & kFlipBit) != 0;
+    double dx, dy, dz;
+    pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+    const float xf = static_cast<float>(dx);
+    const float yf = static_cast<float>(dy);
+    const float zf = static_cast<float>(dz);
+    float r2, inv, u, ww;
+    if (lj_terms_f32(xf, yf, zf, a.lj.sig2f, a.lj.epsf, a.lj.rc2f, r2, inv, u, ww)) {
+        if (LFS) {
+            const float r = sqrtf(r2);
+            u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
+            ww = ww + r * a.lj.dudr_rcf;
+        }
+        q.e += to_fix(u, &q.sat);
+        q.w += to_fix(ww, &q.sat);
+        q.np += flip ? 0 : 1;
+        const float c = flip ? -(ww * inv) : ww * inv;
+        q.fx += to_fix(c * xf, &q.sat);
+        q.fy += to_fix(c * yf, &q.sat);
+        q.fz += to_fix(c * zf, &q.sat);
+    }
+}
+
+// Row walk with four gathers in flight per thread: the four indices of a
+// group are loaded together, then the four partner positions, then the
+// four pair terms are evaluated in row order.
+template <class Acc, class PairFn>
+__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, const double4& pi,
+                                         Acc& q, PairFn pf) {
+    const uint32_t* row = a.nbr + nbr_index(t, 0, a.cap);
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const uint32_t j0 = row[(k + 0) * 64];
+        const uint32_t j1 = row[(k + 1) * 64];
+        const uint32_t j2 = row[(k + 2) * 64];
+        const uint32_t j3 = row[(k + 3) * 64];
+        const double4 p0 = a.pos[j0 & kIndexMask];
+        const double4 p1 = a.pos[j1 & kIndexMask];
+        const double4 p2 = a.pos[j2 & kIndexMask];
+        const double4 p3 = a.pos[j3 & kIndexMask];
+        pf(a, pi, j0, p0, q);
+        pf(a, pi, j1, p1, q);
+        pf(a, pi, j2, p2, q);
+        pf(a, pi, j3, p3, q);
+    }
+    for (; k < cnt; ++k) {
+        const uint32_t j0 = row[k * 64];
+        pf(a, pi, j0, a.pos[j0 & kIndexMask], q);
+    }
+}
+
 template <bool LFS, bool MIXED, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
@@ -141,74 +233,26 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     if (t < a.n) {
         const double4 pi = a.pos[t];
         const int cnt = min(a.ncnt[t], a.cap);
-        const uint32_t* row = a.nbr + t;
         double fx, fy, fz;
         if (!MIXED) {
-            fx = fy = fz = 0.0;
-            for (int k = 0; k < cnt; ++k) {
-                const uint32_t jj = row[static_cast<long>(k) * a.npad];
-                const bool flip = (jj & kFlipBit) != 0;
-                const double4 pj = a.pos[jj & kIndexMask];
-                double dx, dy, dz;
-                pair_delta(a.box, pi, pj, flip, dx, dy, dz);
-                const double r2 = norm2_ref(dx, dy, dz);
-                if (r2 < a.lj.rc2) {
-                    const double inv = 1.0 / r2;
-                    const double s2 = a.lj.sig2 * inv;
-                    const double s6 = s2 * s2 * s2;
-                    double u = 4.0 * a.lj.eps * (s6 * s6 - s6);
-                    double ww = 24.0 * a.lj.eps * (2.0 * s6 * s6 - s6);
-                    if (LFS) {
-                        const double r = sqrt(r2);
-                        u = u - a.lj.u_rc - (r - a.lj.rc) * a.lj.dudr_rc;
-                        ww = ww + r * a.lj.dudr_rc;
-                    }
-                    e += u;
-                    w += ww;
-                    np += flip ? 0 : 1;
-                    const double c = flip ? -(ww * inv) : ww * inv;
-                    fx += c * dx;
-                    fy += c * dy;
-                    fz += c * dz;
-                }
-            }
+            Acc64 q;
+            walk_row(a, t, cnt, pi, q, pair64<LFS>);
+            fx = q.fx;
+            fy = q.fy;
+            fz = q.fz;
+            e = q.e;
+            w = q.w;
+            np = q.np;
         } else {
-            long long ax = 0, ay = 0, az = 0;
-            int sat = 0;
-            for (int k = 0; k < cnt; ++k) {
-                const uint32_t jj = row[static_cast<long>(k) * a.npad];
-                const bool flip = (jj & kFlipBit) != 0;
-                const double4 pj = a.pos[jj & kIndexMask];
-                double dx, dy, dz;
-                pair_delta(a.box, pi, pj, flip, dx, dy, dz);
-                const float xf = static_cast<float>(dx);
-                const float yf = static_cast<float>(dy);
-                const float zf = static_cast<float>(dz);
-                const float r2 = xf * xf + yf * yf + zf * zf;
-                if (r2 < a.lj.rc2f) {
-                    const float inv = 1.0f / r2;
-                    const float s2 = a.lj.sig2f * inv;
-                    const float s6 = s2 * s2 * s2;
-                    float u = 4.0f * a.lj.epsf * (s6 * s6 - s6);
-                    float ww = 24.0f * a.lj.epsf * (2.0f * s6 * s6 - s6);
-                    if (LFS) {
-                        const float r = sqrtf(r2);
-                        u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
-                        ww = ww + r * a.lj.dudr_rcf;
-                    }
-                    ef += to_fix(u, &sat);
-                    wf += to_fix(ww, &sat);
-                    np += flip ? 0 : 1;
-                    const float c = flip ? -(ww * inv) : ww * inv;
-                    ax += to_fix(c * xf, &sat);
-                    ay += to_fix(c * yf, &sat);
-                    az += to_fix(c * zf, &sat);
-                }
-            }
-            if (sat) atomicOr(&a.sc->fix_saturated, 1);
-            fx = static_cast<double>(ax) * kFixInv;
-            fy = static_cast<double>(ay) * kFixInv;
-            fz = static_cast<double>(az) * kFixInv;
+            AccFix q;
+            walk_row(a, t, cnt, pi, q, pair32<LFS>);
+            if (q.sat) atomicOr(&a.sc->fix_saturated, 1);
+            fx = static_cast<double>(q.fx) * kFixInv;
+            fy = static_cast<double>(q.fy) * kFixInv;
+            fz = static_cast<double>(q.fz) * kFixInv;
+            ef = q.e;
+            wf = q.w;
+            np = q.np;
         }
         a.frc[t] = make_double4(fx, fy, fz, 0.0);
         if (KICK) {

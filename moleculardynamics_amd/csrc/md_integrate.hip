@@ -303,6 +303,53 @@ __global__ void k_permute_in(long n, const int* __restrict__ orig, const double*
     a[t] = v;
 }
 
+// Single integrator passes, for host-driven steps with an arbitrary force
+// callback between them (the reference's step(System&, ForceFn&&)).
+__global__ void k_half_kick(long n, double4* __restrict__ vel, const double4* __restrict__ frc,
+                            double half_dt) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double4 v = vel[t];
+    const double4 f = frc[t];
+    const double c = half_dt * (1.0 / v.w);
+    v.x = fma(c, f.x, v.x);
+    v.y = fma(c, f.y, v.y);
+    v.z = fma(c, f.z, v.z);
+    vel[t] = v;
+}
+
+__global__ void k_drift(long n, Box box, double4* __restrict__ pos, const double4* __restrict__ vel,
+                        double h) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    pos[t] = drift(box, pos[t], vel[t], h);
+}
+
+__global__ void k_langevin_o(long n, double4* __restrict__ vel, const int* __restrict__ orig,
+                             double c1, double c2, double temperature, uint64_t seed,
+                             uint64_t counter) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double4 v = vel[t];
+    double g0, g1, g2;
+    normals3(seed, counter, static_cast<uint32_t>(orig[t]), g0, g1, g2);
+    const double k = c2 * sqrt(temperature / v.w);
+    v.x = c1 * v.x + g0 * k;
+    v.y = c1 * v.y + g1 * k;
+    v.z = c1 * v.z + g2 * k;
+    vel[t] = v;
+}
+
+__global__ void k_scale(long n, double4* __restrict__ vel, double s) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double4 v = vel[t];
+    v.x = v.x * s;
+    v.y = v.y * s;
+    v.z = v.z * s;
+    vel[t] = v;
+}
+
 __global__ void k_set_flag(int* flag, int v) {
     if (threadIdx.x == 0) *flag = v;
 }
@@ -349,6 +396,24 @@ void launch_unpermute(long n, const int* orig, const double4* a, double* x, doub
 void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
                        double4* a, int keep_w, hipStream_t s) {
     hipLaunchKernelGGL(k_permute_in, dim3(grid_of(n)), dim3(kBlock), 0, s, n, orig, x, y, z, a, keep_w);
+}
+
+void launch_half_kick(long n, double4* vel, const double4* frc, double half_dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_half_kick, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, frc, half_dt);
+}
+
+void launch_drift(long n, const Box& box, double4* pos, const double4* vel, double h, hipStream_t s) {
+    hipLaunchKernelGGL(k_drift, dim3(grid_of(n)), dim3(kBlock), 0, s, n, box, pos, vel, h);
+}
+
+void launch_langevin_o(long n, double4* vel, const int* orig, double c1, double c2, double T,
+                       uint64_t seed, uint64_t counter, hipStream_t s) {
+    hipLaunchKernelGGL(k_langevin_o, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, orig, c1, c2, T,
+                       seed, counter);
+}
+
+void launch_scale(long n, double4* vel, double sfac, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, sfac);
 }
 
 void launch_set_flag(int* flag, int v, hipStream_t s) {

@@ -136,5 +136,11 @@ void launch_unpermute(long n, const int* orig, const double4* a, double* x, doub
 void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
                        double4* a, int keep_w, hipStream_t s);
 void launch_set_flag(int* flag, int v, hipStream_t s);
+// Single integrator passes for host-driven steps.
+void launch_half_kick(long n, double4* vel, const double4* frc, double half_dt, hipStream_t s);
+void launch_drift(long n, const Box& box, double4* pos, const double4* vel, double h, hipStream_t s);
+void launch_langevin_o(long n, double4* vel, const int* orig, double c1, double c2, double T,
+                       uint64_t seed, uint64_t counter, hipStream_t s);
+void launch_scale(long n, double4* vel, double sfac, hipStream_t s);
 
 }  // namespace mdk

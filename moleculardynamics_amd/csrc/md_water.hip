@@ -1,0 +1,531 @@
+// Rigid SPC/E water with Ewald electrostatics on the device.
+//
+// Reference semantics (forcefield/ewald.cpp:33-175, spce_forcefield.cpp:26-41,
+// constrain/rigid_water.cpp:61-161, integrate/constrained_velocity_verlet.hpp:27-57):
+//  real   C qq erfc(a r)/r over intermolecular pairs with r < rc (minimum
+//         image), w = C qq (2a/sqrt(pi) e^{-a^2 r^2} + erfc(a r)/r), f = (w/r^2) d;
+//  LJ     oxygen-oxygen, Truncated at rc (plus the analytic tail, host side);
+//  self   -C a/sqrt(pi) sum q^2 (host side);
+//  intra  -C qq erf(a r)/r over the three pairs of each molecule (minimum image),
+//         w = C qq (2a/sqrt(pi) e^{-a^2 r^2} - erf(a r)/r);
+//  recip  sum over 0 < |n|^2 < n2max of C (2 pi/V) e^{-k^2/4a^2}/k^2 |S(k)|^2,
+//         S(k) = sum_i q_i e^{i k.r_i}, f_i = 2 C (2 pi/V) e^{..}/k^2 q_i (Re S sin - Im S cos) k.
+//
+// Layout: sites in molecule order (O, H, H) as double4 (x, y, z, charge),
+// velocities double4 (vx, vy, vz, mass). Real space and LJ run over a full
+// Verlet list of intermolecular pairs (same wave-blocked rows as the LJ
+// path). Reciprocal space uses per-site tables e^{i n k0 x} (n = 0..kmax per
+// axis; negative n by conjugation) and only one of each +-k pair (S(-k) is
+// the conjugate of S(k), so the +k and -k terms are equal):
+//  k_sk     one thread per k-vector, sites staged in LDS tiles, partial S(k)
+//           per site tile, then k_sk_reduce sums the tiles in order;
+//  k_recip  one thread per site, loops over the k-vectors of the half space.
+// The mixed path evaluates erfc in FP32 with Abramowitz & Stegun 7.1.26
+// (|error| <= 1.5e-7); the k-space sums stay FP64.
+#include "md_common.hpp"
+#include "md_water.hpp"
+
+namespace mdk {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kSiteTile = 16;  // sites per LDS tile of the S(k) kernel
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// erfc(x) for x >= 0, Abramowitz & Stegun 7.1.26: 1 - (a1 t + ... + a5 t^5) e^{-x^2},
+// t = 1/(1 + p x); the e^{-x^2} factor is shared with the force term.
+__device__ __forceinline__ float erfc_as(float x, float ex2) {
+    const float p = 0.3275911f;
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
+                a5 = 1.061405429f;
+    const float t = 1.0f / (1.0f + p * x);
+    const float poly = t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5))));
+    return poly * ex2;
+}
+
+#define GRID_STRIDE(i, n)                                                              \
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+
+// Full list of intermolecular partners within rc + skin: cells (index array,
+// no re-sorting) when at least 3 per side fit, all sites otherwise.
+__global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int* __restrict__ cell_of,
+                                                  const int* __restrict__ cell_start,
+                                                  const int* __restrict__ cell_atoms) {
+    GRID_STRIDE(i, a.n) {
+        const double4 pi = a.pos[i];
+        const long mi = i / 3;
+        int cnt = 0;
+        auto visit = [&](int j) {
+            if (j / 3 == mi) return;
+            const bool flip = i > j;
+            double dx, dy, dz;
+            pair_delta(a.box, pi, a.pos[j], flip, dx, dy, dz);
+            if (norm2_ref(dx, dy, dz) < a.rl2) {
+                if (cnt < a.cap) a.nbr[nbr_index(i, cnt, a.cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+                ++cnt;
+            }
+        };
+        if (g.usable) {
+            const int c = cell_of[i];
+            const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+            for (int oz = -1; oz <= 1; ++oz)
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int nb = (cx + ox + g.nc[0]) % g.nc[0] +
+                                       g.nc[0] * ((cy + oy + g.nc[1]) % g.nc[1] +
+                                                  g.nc[1] * ((cz + oz + g.nc[2]) % g.nc[2]));
+                        for (int k = cell_start[nb]; k < cell_start[nb + 1]; ++k) visit(cell_atoms[k]);
+                    }
+        } else {
+            for (int j = 0; j < static_cast<int>(a.n); ++j) visit(j);
+        }
+        a.ncnt[i] = cnt;
+        if (cnt > a.cap) atomicOr(a.overflow, 1);
+        a.ref[i] = pi;
+    }
+}
+
+// Real-space Coulomb + oxygen LJ, intramolecular correction; forces are
+// written (not accumulated), the reciprocal kernel adds its part.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
+    __shared__ double sm[kBlock / kWave];
+    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double ereal = 0.0, elj = 0.0, eintra = 0.0, wsum = 0.0;
+    if (i < a.n) {
+        const double4 pi = a.pos[i];
+        const bool oi = (i % 3) == 0;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const int cnt = min(a.ncnt[i], a.cap);
+        const uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t jj = row[k * 64];
+            const int j = static_cast<int>(jj & kIndexMask);
+            const bool flip = (jj & kFlipBit) != 0;
+            const double4 pj = a.pos[j];
+            double dx, dy, dz;
+            pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+            const double r2 = norm2_ref(dx, dy, dz);
+            if (!(r2 < a.rc2)) continue;
+            double w = 0.0;
+            const double qq = pi.w * pj.w;
+            if (MIXED) {
+                const float rf = sqrtf(static_cast<float>(r2));
+                const float ar = static_cast<float>(a.alpha) * rf;
+                const float ex2 = __expf(-ar * ar);
+                const float ec = erfc_as(ar, ex2);
+                const float cq = static_cast<float>(a.coulomb * qq);
+                const float inv_r = 1.0f / rf;
+                ereal += static_cast<double>(cq * ec * inv_r);
+                w += static_cast<double>(cq * (static_cast<float>(a.c2a) * ex2 + ec * inv_r));
+            } else {
+                const double r = sqrt(r2);
+                const double ec = erfc(a.alpha * r);
+                ereal += a.coulomb * qq * ec / r;
+                w += a.coulomb * qq * (a.c2a * exp(-a.alpha * a.alpha * r2) + ec / r);
+            }
+            if (oi && (j % 3) == 0) {
+                const double s2 = a.sig2 / r2;
+                const double s6 = s2 * s2 * s2;
+                elj += 4.0 * a.eps * (s6 * s6 - s6);
+                w += 24.0 * a.eps * (2.0 * s6 * s6 - s6);
+            }
+            wsum += w;
+            const double c = flip ? -(w / r2) : w / r2;
+            fx += c * dx;
+            fy += c * dy;
+            fz += c * dz;
+        }
+        // intramolecular correction: the two partners of this site
+        const long m0 = (i / 3) * 3;
+        for (long j = m0; j < m0 + 3; ++j) {
+            if (j == i) continue;
+            const double4 pj = a.pos[j];
+            const bool flip = i > j;
+            double dx, dy, dz;
+            pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+            const double r2 = norm2_ref(dx, dy, dz);
+            const double r = sqrt(r2);
+            const double qq = pi.w * pj.w;
+            const double ef = erf(a.alpha * r);
+            eintra += -a.coulomb * qq * ef / r;
+            const double w = a.coulomb * qq * (a.c2a * exp(-a.alpha * a.alpha * r2) - ef / r);
+            const double c = flip ? -(w / r2) : w / r2;
+            fx += c * dx;
+            fy += c * dy;
+            fz += c * dz;
+        }
+        a.frc[i] = make_double4(fx, fy, fz, 0.0);
+    }
+    const double b0 = block_reduce(ereal, sm, wave_sum);
+    const double b1 = block_reduce(elj, sm, wave_sum);
+    const double b2 = block_reduce(eintra, sm, wave_sum);
+    const double b3 = block_reduce(wsum, sm, wave_sum);
+    if (threadIdx.x == 0) {
+        a.part[4 * blockIdx.x + 0] = b0;
+        a.part[4 * blockIdx.x + 1] = b1;
+        a.part[4 * blockIdx.x + 2] = b2;
+        a.part[4 * blockIdx.x + 3] = b3;
+    }
+}
+
+// Per-site phase tables: tab[(d * (kmax+1) + n) * nsite + i] = e^{i n k0_d r_d}
+// by repeated multiplication (n = 0..kmax).
+__global__ void k_weik(WaterArgs a) {
+    GRID_STRIDE(i, a.n) {
+        const double4 p = a.pos[i];
+        const double r[3] = {p.x, p.y, p.z};
+        for (int d = 0; d < 3; ++d) {
+            double s, c;
+            sincos(a.k0[d] * r[d], &s, &c);
+            const double2 e1 = make_double2(c, s);
+            double2 e = make_double2(1.0, 0.0);
+            for (int nn = 0; nn <= a.kmax; ++nn) {
+                a.eik[(static_cast<long>(d) * (a.kmax + 1) + nn) * a.n + i] = e;
+                e = cmul(e, e1);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double2 phase(const double2* t, int nn) {
+    // e^{i n theta} for n of either sign from the table of n >= 0
+    const double2 v = t[nn < 0 ? -nn : nn];
+    return nn < 0 ? make_double2(v.x, -v.y) : v;
+}
+
+// Partial structure factors: block (k-chunk, site-tile-group). Each thread
+// owns one k-vector; sites are staged in LDS (their q and the three
+// per-axis tables) kSiteTile at a time.
+__global__ __launch_bounds__(kBlock) void k_sk(WaterArgs a, int sites_per_group) {
+    __shared__ double2 tab[kSiteTile][3][kMaxK + 1];
+    __shared__ double qs[kSiteTile];
+    const int kv = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = kv < a.nk;
+    int3 nv = make_int3(0, 0, 0);
+    if (valid) nv = a.kvec[kv];
+    const long s0 = static_cast<long>(blockIdx.y) * sites_per_group;
+    const long s1 = min(s0 + sites_per_group, a.n);
+    double re = 0.0, im = 0.0;
+    for (long t0 = s0; t0 < s1; t0 += kSiteTile) {
+        const int cnt = static_cast<int>(min(static_cast<long>(kSiteTile), s1 - t0));
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * 3 * (a.kmax + 1); e += kBlock) {
+            const int s = e / (3 * (a.kmax + 1));
+            const int rem = e % (3 * (a.kmax + 1));
+            const int d = rem / (a.kmax + 1), nn = rem % (a.kmax + 1);
+            tab[s][d][nn] = a.eik[(static_cast<long>(d) * (a.kmax + 1) + nn) * a.n + t0 + s];
+        }
+        for (int s = threadIdx.x; s < cnt; s += kBlock) qs[s] = a.pos[t0 + s].w;
+        __syncthreads();
+        if (valid) {
+            for (int s = 0; s < cnt; ++s) {
+                const double2 e = cmul(cmul(phase(tab[s][0], nv.x), phase(tab[s][1], nv.y)),
+                                       phase(tab[s][2], nv.z));
+                re += qs[s] * e.x;
+                im += qs[s] * e.y;
+            }
+        }
+    }
+    if (valid) a.skpart[static_cast<long>(blockIdx.y) * a.nk + kv] = make_double2(re, im);
+}
+
+// S(k) = sum of the site-group partials in order; E_k term and the force
+// coefficient G(k) = 2 factor(k) (factor(k) = C (2 pi/V) e^{-k^2/4a^2}/k^2;
+// both signs of k counted once each).
+__global__ void k_sk_reduce(WaterArgs a, int groups) {
+    __shared__ double sm[kBlock / kWave];
+    const int kv = blockIdx.x * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (kv < a.nk) {
+        double re = 0.0, im = 0.0;
+        for (int g = 0; g < groups; ++g) {
+            const double2 p = a.skpart[static_cast<long>(g) * a.nk + kv];
+            re += p.x;
+            im += p.y;
+        }
+        a.sk[kv] = make_double2(re, im);
+        e = 2.0 * a.kfac[kv] * (re * re + im * im);
+    }
+    const double b = block_reduce(e, sm, wave_sum);
+    if (threadIdx.x == 0) a.part2[blockIdx.x] = b;
+}
+
+// Reciprocal forces: f_i += sum_k 2 * [2 factor q_i (Re S sin - Im S cos)] k
+// over the half space (the -k term equals the +k term).
+__global__ __launch_bounds__(kBlock) void k_recip(WaterArgs a) {
+    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const double q = a.pos[i].w;
+    const double2* tx = a.eik + (0L * (a.kmax + 1)) * a.n + i;
+    const double2* ty = a.eik + (1L * (a.kmax + 1)) * a.n + i;
+    const double2* tz = a.eik + (2L * (a.kmax + 1)) * a.n + i;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int kv = 0; kv < a.nk; ++kv) {
+        const int3 nv = a.kvec[kv];
+        const double2 ex = tx[static_cast<long>(nv.x < 0 ? -nv.x : nv.x) * a.n];
+        const double2 ey = ty[static_cast<long>(nv.y < 0 ? -nv.y : nv.y) * a.n];
+        const double2 ez = tz[static_cast<long>(nv.z < 0 ? -nv.z : nv.z) * a.n];
+        const double2 e = cmul(cmul(nv.x < 0 ? make_double2(ex.x, -ex.y) : ex,
+                                    nv.y < 0 ? make_double2(ey.x, -ey.y) : ey),
+                               nv.z < 0 ? make_double2(ez.x, -ez.y) : ez);
+        const double2 S = a.sk[kv];
+        const double coeff = 2.0 * (a.kfac[kv] * 2.0 * q * (S.x * e.y - S.y * e.x));
+        fx += coeff * (a.k0[0] * nv.x);
+        fy += coeff * (a.k0[1] * nv.y);
+        fz += coeff * (a.k0[2] * nv.z);
+    }
+    double4 f = a.frc[i];
+    f.x += fx;
+    f.y += fy;
+    f.z += fz;
+    a.frc[i] = f;
+}
+
+// ---- rigid-body constraints: one thread per molecule ------------------------
+// Gauss-Seidel SHAKE on (O,H1), (O,H2), (H1,H2) along the reference bonds until
+// every relative violation is within tol; a molecule that converged stays
+// unchanged, so iterating molecules independently is the reference's global
+// sweep (rigid_water.cpp:61-116). iters[m] receives the sweep count or -1
+// (no convergence) / -2 (singular).
+__global__ void k_shake(WaterArgs a, const double4* __restrict__ refp, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+        const long b = 3 * m;
+        double4 r[3] = {a.pos[b], a.pos[b + 1], a.pos[b + 2]};
+        const double4 rr[3] = {refp[b], refp[b + 1], refp[b + 2]};
+        const double mass[3] = {a.vel[b].w, a.vel[b + 1].w, a.vel[b + 2].w};
+        const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
+        const double tgt[3] = {a.oh, a.oh, a.hh};
+        int it = 0, res = -1;
+        for (; it < a.max_iter; ++it) {
+            double worst = 0.0;
+            for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+                const int p = ci[c], q = cj[c];
+                const double d2 = tgt[c] * tgt[c];
+                double dx = r[p].x - r[q].x, dy = r[p].y - r[q].y, dz = r[p].z - r[q].z;
+                min_image(a.box, dx, dy, dz);
+                const double s = norm2_ref(dx, dy, dz) - d2;
+                const double viol = fabs(s) / d2;
+                worst = fmax(worst, viol);
+                if (viol <= a.tol) continue;
+                double ex = rr[p].x - rr[q].x, ey = rr[p].y - rr[q].y, ez = rr[p].z - rr[q].z;
+                min_image(a.box, ex, ey, ez);
+                const double inv = 1.0 / mass[p] + 1.0 / mass[q];
+                const double den = 2.0 * inv * norm2_dot(dx, dy, dz, ex, ey, ez);
+                if (fabs(den) < 1e-30) {
+                    res = -2;
+                    it = a.max_iter;
+                    break;
+                }
+                const double gg = s / den;
+                const double gp = gg / mass[p], gq = gg / mass[q];
+                r[p].x = r[p].x - ex * gp; r[p].y = r[p].y - ey * gp; r[p].z = r[p].z - ez * gp;
+                r[q].x = r[q].x + ex * gq; r[q].y = r[q].y + ey * gq; r[q].z = r[q].z + ez * gq;
+            }
+            if (res == -2) break;
+            if (worst <= a.tol) {
+                res = it + 1;
+                break;
+            }
+        }
+        a.pos[b] = r[0];
+        a.pos[b + 1] = r[1];
+        a.pos[b + 2] = r[2];
+        status[m] = res;
+    }
+}
+
+// RATTLE: remove the relative velocity along each bond (rigid_water.cpp:118-161).
+__global__ void k_rattle(WaterArgs a, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+        const long b = 3 * m;
+        const double4 r[3] = {a.pos[b], a.pos[b + 1], a.pos[b + 2]};
+        double4 v[3] = {a.vel[b], a.vel[b + 1], a.vel[b + 2]};
+        const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
+        const double tgt[3] = {a.oh, a.oh, a.hh};
+        int res = -1;
+        for (int it = 0; it < a.max_iter; ++it) {
+            double worst = 0.0;
+            for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+                const int p = ci[c], q = cj[c];
+                const double d2 = tgt[c] * tgt[c];
+                double dx = r[p].x - r[q].x, dy = r[p].y - r[q].y, dz = r[p].z - r[q].z;
+                min_image(a.box, dx, dy, dz);
+                const double vx = v[p].x - v[q].x, vy = v[p].y - v[q].y, vz = v[p].z - v[q].z;
+                const double rv = norm2_dot(dx, dy, dz, vx, vy, vz);
+                const double sc = fabs(rv) / d2;
+                worst = fmax(worst, sc);
+                if (sc <= a.tol) continue;
+                const double inv = 1.0 / v[p].w + 1.0 / v[q].w;
+                const double kk = rv / (inv * norm2_ref(dx, dy, dz));
+                const double kp = kk / v[p].w, kq = kk / v[q].w;
+                v[p].x = v[p].x - dx * kp; v[p].y = v[p].y - dy * kp; v[p].z = v[p].z - dz * kp;
+                v[q].x = v[q].x + dx * kq; v[q].y = v[q].y + dy * kq; v[q].z = v[q].z + dz * kq;
+            }
+            if (worst <= a.tol) {
+                res = it + 1;
+                break;
+            }
+        }
+        a.vel[b] = v[0];
+        a.vel[b + 1] = v[1];
+        a.vel[b + 2] = v[2];
+        status[m] = res;
+    }
+}
+
+// Constrained velocity Verlet, first half: save the start positions, kick
+// v + f (dt/2/m) (plain multiply and add), drift without wrap, keep the
+// unconstrained positions (constrained_velocity_verlet.hpp:33-42).
+__global__ void k_cvv_first(WaterArgs a, double dt, double4* __restrict__ start,
+                            double4* __restrict__ free_pos) {
+    GRID_STRIDE(i, a.n) {
+#pragma clang fp contract(off)
+        const double4 p = a.pos[i];
+        double4 v = a.vel[i];
+        const double4 f = a.frc[i];
+        start[i] = p;
+        const double c = 0.5 * dt / v.w;
+        v.x = v.x + f.x * c;
+        v.y = v.y + f.y * c;
+        v.z = v.z + f.z * c;
+        const double4 np = make_double4(p.x + v.x * dt, p.y + v.y * dt, p.z + v.z * dt, p.w);
+        a.pos[i] = np;
+        a.vel[i] = v;
+        free_pos[i] = np;
+    }
+}
+
+// v += minimumImage(r_constrained - r_free) / dt (constrained_velocity_verlet.hpp:44-47).
+__global__ void k_cvv_correct(WaterArgs a, double dt, const double4* __restrict__ free_pos) {
+    GRID_STRIDE(i, a.n) {
+#pragma clang fp contract(off)
+        const double4 p = a.pos[i], u = free_pos[i];
+        double dx = p.x - u.x, dy = p.y - u.y, dz = p.z - u.z;
+        min_image(a.box, dx, dy, dz);
+        const double inv = 1.0 / dt;
+        double4 v = a.vel[i];
+        v.x = v.x + dx * inv;
+        v.y = v.y + dy * inv;
+        v.z = v.z + dz * inv;
+        a.vel[i] = v;
+    }
+}
+
+__global__ void k_cvv_kick(WaterArgs a, double dt) {
+    GRID_STRIDE(i, a.n) {
+#pragma clang fp contract(off)
+        double4 v = a.vel[i];
+        const double4 f = a.frc[i];
+        const double c = 0.5 * dt / v.w;
+        v.x = v.x + f.x * c;
+        v.y = v.y + f.y * c;
+        v.z = v.z + f.z * c;
+        a.vel[i] = v;
+    }
+}
+
+// Top-2 displacement from the list's reference positions (one workgroup).
+__global__ __launch_bounds__(1024) void k_wcheck(WaterArgs a, int* flag) {
+    __shared__ double s1[1024], s2[1024];
+    double t1 = 0.0, t2 = 0.0;
+    for (long i = threadIdx.x; i < a.n; i += 1024) {
+        const double4 p = a.pos[i], r = a.ref[i];
+        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        min_image(a.box, dx, dy, dz);
+        top2_merge(t1, t2, sqrt(norm2_ref(dx, dy, dz)), 0.0);
+    }
+    s1[threadIdx.x] = t1;
+    s2[threadIdx.x] = t2;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double x1 = s1[threadIdx.x], x2 = s2[threadIdx.x];
+            top2_merge(x1, x2, s1[threadIdx.x + off], s2[threadIdx.x + off]);
+            s1[threadIdx.x] = x1;
+            s2[threadIdx.x] = x2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *flag = (s1[0] + s2[0]) > a.skin ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wkinetic(WaterArgs a) {
+    __shared__ double sm[kBlock / kWave];
+    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    double ke = 0.0;
+    if (i < a.n) {
+        const double4 v = a.vel[i];
+        ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+    }
+    const double b = block_reduce(ke, sm, wave_sum);
+    if (threadIdx.x == 0) a.part2[blockIdx.x] = b;
+}
+
+inline unsigned gridn(long n) {
+    const long g = (n + kBlock - 1) / kBlock;
+    return static_cast<unsigned>(g > 0 ? g : 1);
+}
+
+}  // namespace
+
+void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
+                       const int* cell_atoms, hipStream_t s) {
+    hipLaunchKernelGGL(k_wlist, dim3(gridn(a.n)), dim3(kBlock), 0, s, a, g, cell_of, cell_start, cell_atoms);
+}
+
+void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(k_wreal<true>, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_wreal<false>, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_weik, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+    const int groups = water_site_groups(a.n);
+    const int spg = static_cast<int>((a.n + groups - 1) / groups);
+    hipLaunchKernelGGL(k_sk, dim3((a.nk + kBlock - 1) / kBlock, groups), dim3(kBlock), 0, s, a, spg);
+    hipLaunchKernelGGL(k_sk_reduce, dim3((a.nk + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, groups);
+    hipLaunchKernelGGL(k_recip, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+}
+
+void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_shake, dim3(gridn(a.n / 3)), dim3(kBlock), 0, s, a, refp, status);
+}
+
+void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_rattle, dim3(gridn(a.n / 3)), dim3(kBlock), 0, s, a, status);
+}
+
+void water_launch_cvv_first(const WaterArgs& a, double dt, double4* start, double4* free_pos,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_cvv_first, dim3(gridn(a.n)), dim3(kBlock), 0, s, a, dt, start, free_pos);
+}
+
+void water_launch_cvv_correct(const WaterArgs& a, double dt, const double4* free_pos, hipStream_t s) {
+    hipLaunchKernelGGL(k_cvv_correct, dim3(gridn(a.n)), dim3(kBlock), 0, s, a, dt, free_pos);
+}
+
+void water_launch_cvv_kick(const WaterArgs& a, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_cvv_kick, dim3(gridn(a.n)), dim3(kBlock), 0, s, a, dt);
+}
+
+void water_launch_check(const WaterArgs& a, int* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_wcheck, dim3(1), dim3(1024), 0, s, a, flag);
+}
+
+void water_launch_kinetic(const WaterArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_wkinetic, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+}
+
+int water_site_groups(long n) {
+    const long g = (n + 511) / 512;
+    return static_cast<int>(g < 1 ? 1 : (g > 64 ? 64 : g));
+}
+
+}  // namespace mdk

@@ -1,0 +1,55 @@
+// Launch interface of the SPC/E water kernels (md_water.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "md_common.hpp"
+#include "md_kernels.hpp"
+
+namespace mdk {
+
+constexpr int kMaxK = 24;  // largest supported kmax per axis
+
+struct WaterArgs {
+    long n;  // sites (3 per molecule, O H H)
+    Box box;
+    double4* pos;  // x, y, z, charge
+    double4* vel;  // vx, vy, vz, mass
+    double4* frc;
+    double4* ref;  // positions at the last list build
+    uint32_t* nbr;
+    int* ncnt;
+    int cap;
+    double rl2, rc2, skin;
+    double alpha, c2a, coulomb;  // c2a = 2 alpha / sqrt(pi)
+    double sig2, eps;            // oxygen LJ
+    double* part;                // 4 per workgroup: real, lj, intra, virial
+    double* part2;               // 1 per workgroup: k-space energy / kinetic energy
+    int kmax, nk;                // half-space k-vector count
+    const int3* kvec;
+    const double* kfac;          // C (2 pi / V) e^{-k^2/4a^2} / k^2
+    double k0[3];
+    double2* eik;                // [3][kmax+1][n]
+    double2* skpart;             // [groups][nk]
+    double2* sk;                 // [nk]
+    int* overflow;
+    double oh, hh, tol;
+    int max_iter;
+};
+
+void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
+                       const int* cell_atoms, hipStream_t s);
+void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s);
+void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hipStream_t s);
+void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s);
+void water_launch_cvv_first(const WaterArgs& a, double dt, double4* start, double4* free_pos,
+                            hipStream_t s);
+void water_launch_cvv_correct(const WaterArgs& a, double dt, const double4* free_pos, hipStream_t s);
+void water_launch_cvv_kick(const WaterArgs& a, double dt, hipStream_t s);
+void water_launch_check(const WaterArgs& a, int* flag, hipStream_t s);
+void water_launch_kinetic(const WaterArgs& a, hipStream_t s);
+int water_site_groups(long n);
+
+}  // namespace mdk

@@ -1,0 +1,528 @@
+// C ABI of the rigid SPC/E water path (include/moldyn_hip.h, md_water_*):
+// the SpceForceField (forcefield/spce_forcefield.cpp:8-48) and Ewald
+// (forcefield/ewald.cpp:17-188) forces, RigidWater SHAKE/RATTLE
+// (constrain/rigid_water.cpp:30-161) and the constrained velocity Verlet
+// (integrate/constrained_velocity_verlet.hpp:27-57) on the device, plus the
+// config-3 input builder and the Ewald parameter estimate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/moldyn_hip.h"
+#include "md_water.hpp"
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+// e^2/(4 pi eps0 kB) * 1e10 in K*A from the 2010 CODATA constants
+// (forcefield/ewald.hpp:10-13, ewald.cpp:17-21).
+double coulomb_prefactor() {
+    const double e = 1.602176565e-19, eps0 = 8.854187817e-12, kb = 1.3806488e-23;
+    return (e * e) / (4.0 * kPi * eps0) * 1e10 / kb;
+}
+}  // namespace
+
+struct md_water {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    long n = 0;
+    double L[3] = {0, 0, 0};
+    md_water_params p{};
+    double alpha = 0.0;
+    mdk::WaterArgs a{};
+    mdk::Grid grid{};
+    // device buffers
+    double4 *pos = nullptr, *vel = nullptr, *frc = nullptr, *ref = nullptr, *start = nullptr,
+            *freep = nullptr;
+    uint32_t* nbr = nullptr;
+    int *ncnt = nullptr, *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr,
+        *cell_fill = nullptr, *cell_atoms = nullptr, *status = nullptr, *flags = nullptr;
+    double *part = nullptr, *part2 = nullptr, *kfac = nullptr;
+    int3* kvec = nullptr;
+    double2 *eik = nullptr, *skpart = nullptr, *sk = nullptr;
+    long builds = 0;
+    bool list_built = false;
+};
+
+namespace {
+
+md_status wfail(md_water* w, md_status s, const std::string& m) {
+    if (w) w->err = m;
+    return s;
+}
+
+#define WHIP(w, call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return wfail(w, MD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+hipError_t walloc(T*& p, long count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    if (count <= 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(&p), static_cast<size_t>(count) * sizeof(T));
+}
+
+long wblocks(long n) { return (n + mdk::kBlock - 1) / mdk::kBlock; }
+
+md_status build_list(md_water* w) {
+    const mdk::Grid& g = w->grid;
+    if (g.usable) {
+        mdk::launch_bin_index(w->n, w->pos, g, w->cell_of, w->cell_count, w->cell_start,
+                              w->cell_fill, w->cell_atoms, w->stream);
+    }
+    mdk::water_launch_list(w->a, g, w->cell_of, w->cell_start, w->cell_atoms, w->stream);
+    WHIP(w, hipGetLastError());
+    int ov = 0;
+    WHIP(w, hipStreamSynchronize(w->stream));
+    WHIP(w, hipMemcpy(&ov, w->flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (ov) return wfail(w, MD_ERR_CAPACITY, "water neighbour row capacity exceeded");
+    w->builds += 1;
+    w->list_built = true;
+    return MD_OK;
+}
+
+// Forces on the current positions; terms = {disp, lrc, real, recip, self, intra}.
+md_status compute(md_water* w, bool mixed, double* terms) {
+    if (!w->list_built) {
+        md_status st = build_list(w);
+        if (st != MD_OK) return st;
+    }
+    mdk::water_launch_forces(w->a, mixed, w->stream);
+    WHIP(w, hipGetLastError());
+    const long nb = wblocks(w->n);
+    const long nbk = (w->a.nk + mdk::kBlock - 1) / mdk::kBlock;
+    std::vector<double> part(static_cast<size_t>(4 * nb)), part2(static_cast<size_t>(nbk));
+    WHIP(w, hipStreamSynchronize(w->stream));
+    WHIP(w, hipMemcpy(part.data(), w->part, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    WHIP(w, hipMemcpy(part2.data(), w->part2, part2.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double real = 0, lj = 0, intra = 0, vir = 0, recip = 0;
+    for (long b = 0; b < nb; ++b) {
+        real += part[4 * b];
+        lj += part[4 * b + 1];
+        intra += part[4 * b + 2];
+        vir += part[4 * b + 3];
+    }
+    for (long b = 0; b < nbk; ++b) recip += part2[b];
+    // every intermolecular and intramolecular pair was seen from both sites
+    real *= 0.5;
+    lj *= 0.5;
+    intra *= 0.5;
+    // self term and the oxygen LJ tail (lennard_jones.cpp:121-135 on the oxygens)
+    const double C = coulomb_prefactor();
+    const double qO = -2.0 * w->p.qH, qH = w->p.qH;
+    const long nm = w->n / 3;
+    const double q2 = static_cast<double>(nm) * (qO * qO + 2.0 * qH * qH);
+    const double self = -C * w->alpha / std::sqrt(kPi) * q2;
+    const double V = w->L[0] * w->L[1] * w->L[2];
+    const double no = static_cast<double>(nm);
+    const double rho = no / V;
+    const double t = w->p.sigma_o / w->p.cutoff;
+    const double t3 = t * t * t, t9 = t3 * t3 * t3;
+    const double sig3 = w->p.sigma_o * w->p.sigma_o * w->p.sigma_o;
+    const double lrc = (8.0 / 3.0) * kPi * no * rho * w->p.epsilon_o * sig3 * ((1.0 / 3.0) * t9 - t3);
+    if (terms) {
+        terms[0] = lj; terms[1] = lrc; terms[2] = real; terms[3] = recip; terms[4] = self; terms[5] = intra;
+    }
+    return MD_OK;
+}
+
+md_status status_check(md_water* w, const char* what, int* iters) {
+    const long nm = w->n / 3;
+    std::vector<int> st(static_cast<size_t>(nm));
+    WHIP(w, hipStreamSynchronize(w->stream));
+    WHIP(w, hipMemcpy(st.data(), w->status, nm * sizeof(int), hipMemcpyDeviceToHost));
+    int worst = 0;
+    for (long m = 0; m < nm; ++m) {
+        if (st[m] == -2) return wfail(w, MD_ERR_RUNTIME, std::string(what) + ": constraint became singular");
+        if (st[m] < 0)
+            return wfail(w, MD_ERR_RUNTIME, std::string(what) + " failed to converge in " +
+                                                std::to_string(w->p.max_iterations) + " iterations");
+        worst = std::max(worst, st[m]);
+    }
+    if (iters) *iters = worst;
+    return MD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Kolafa & Perram (Mol. Simul. 9, 351 (1992)) RMS force-error estimates:
+//   real  dF_r = 2 Q2 exp(-a^2 rc^2) / sqrt(N rc V)
+//   recip dF_k = 2 Q2 a / (pi L) * sqrt(1/(pi kmax N)) * exp(-(pi kmax/(a L))^2)
+// with Q2 = sum q^2; both set to accuracy * C (the force between two unit
+// charges 1 A apart): a from the real-space estimate, then the smallest kmax
+// meeting the reciprocal one.
+md_status md_ewald_parameters(long n, const double* L, double sum_q2, double cutoff,
+                              double accuracy, double* alpha, int* kmax) {
+    if (n <= 0 || L == nullptr || !(cutoff > 0.0) || !(accuracy > 0.0) || alpha == nullptr ||
+        kmax == nullptr)
+        return MD_ERR_INVALID_ARGUMENT;
+    const double C = coulomb_prefactor();
+    const double target = accuracy * C;
+    const double V = L[0] * L[1] * L[2];
+    const double Lmin = std::min({L[0], L[1], L[2]});
+    const double scale = 2.0 * sum_q2 * C / std::sqrt(static_cast<double>(n) * cutoff * V);
+    // solve scale * exp(-a^2 rc^2) = target for a
+    double a = std::sqrt(std::max(1e-12, std::log(scale / target))) / cutoff;
+    int km = 1;
+    for (; km <= mdk::kMaxK; ++km) {
+        const double est = 2.0 * sum_q2 * C * a / (kPi * Lmin) *
+                           std::sqrt(1.0 / (kPi * km * static_cast<double>(n))) *
+                           std::exp(-std::pow(kPi * km / (a * Lmin), 2.0));
+        if (est <= target) break;
+    }
+    if (km > mdk::kMaxK) return MD_ERR_INVALID_ARGUMENT;
+    *alpha = a;
+    *kmax = km;
+    return MD_OK;
+}
+
+// molecules_per_side^3 SPC/E molecules on a cubic lattice of spacing
+// L/molecules_per_side (oxygen on the lattice point), each rotated by a
+// uniform random rotation (Shoemake's quaternion from three uniforms of the
+// mt19937_64 stream, uniform in (0,1] as util/random.hpp:43-48), hydrogens at
+// r_OH = rOH with the H-O-H angle angleHOH; Maxwell-Boltzmann velocities at
+// temperature (Marsaglia polar normals of the same stream), centre-of-mass
+// motion removed. Sites are O, H, H per molecule; x..vz hold 3 m^3 values.
+md_status md_water_lattice(int molecules_per_side, double box_length, uint64_t seed,
+                           double temperature, const md_water_params* p, double* x, double* y,
+                           double* z, double* vx, double* vy, double* vz) {
+    if (molecules_per_side < 1 || !(box_length > 0.0) || p == nullptr || x == nullptr)
+        return MD_ERR_INVALID_ARGUMENT;
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> unif(0.0, 1.0);
+    auto uniform = [&]() { return 1.0 - unif(eng); };
+    bool have = false;
+    double spare = 0.0;
+    auto normal = [&]() {
+        if (have) {
+            have = false;
+            return spare;
+        }
+        double a, b, s;
+        do {
+            a = 2.0 * uniform() - 1.0;
+            b = 2.0 * uniform() - 1.0;
+            s = a * a + b * b;
+        } while (s >= 1.0 || s == 0.0);
+        const double k = std::sqrt(-2.0 * std::log(s) / s);
+        spare = b * k;
+        have = true;
+        return a * k;
+    };
+    const int m = molecules_per_side;
+    const double a0 = box_length / m;
+    const double half = 0.5 * p->angle_hoh * kPi / 180.0;
+    // body frame: O at the origin, H's in the xz plane
+    const double hb[2][3] = {{p->r_oh * std::sin(half), 0.0, p->r_oh * std::cos(half)},
+                             {-p->r_oh * std::sin(half), 0.0, p->r_oh * std::cos(half)}};
+    long s = 0;
+    for (int ix = 0; ix < m; ++ix)
+        for (int iy = 0; iy < m; ++iy)
+            for (int iz = 0; iz < m; ++iz) {
+                const double o[3] = {a0 * (ix + 0.5) - 0.5 * box_length,
+                                     a0 * (iy + 0.5) - 0.5 * box_length,
+                                     a0 * (iz + 0.5) - 0.5 * box_length};
+                const double u1 = uniform(), u2 = uniform(), u3 = uniform();
+                const double qx = std::sqrt(1.0 - u1) * std::sin(2.0 * kPi * u2);
+                const double qy = std::sqrt(1.0 - u1) * std::cos(2.0 * kPi * u2);
+                const double qz = std::sqrt(u1) * std::sin(2.0 * kPi * u3);
+                const double qw = std::sqrt(u1) * std::cos(2.0 * kPi * u3);
+                const double R[3][3] = {
+                    {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)},
+                    {2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)},
+                    {2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)}};
+                x[s] = o[0]; y[s] = o[1]; z[s] = o[2];
+                for (int h = 0; h < 2; ++h) {
+                    double r[3];
+                    for (int d = 0; d < 3; ++d)
+                        r[d] = o[d] + R[d][0] * hb[h][0] + R[d][1] * hb[h][1] + R[d][2] * hb[h][2];
+                    x[s + 1 + h] = r[0]; y[s + 1 + h] = r[1]; z[s + 1 + h] = r[2];
+                }
+                s += 3;
+            }
+    const long n = s;
+    double M = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (long i = 0; i < n; ++i) {
+        const double mi = (i % 3 == 0) ? p->mass_o : p->mass_h;
+        const double sv = std::sqrt(temperature / mi);
+        const double g0 = normal(), g1 = normal(), g2 = normal();
+        vx[i] = sv * g0; vy[i] = sv * g1; vz[i] = sv * g2;
+        M += mi;
+        px += mi * vx[i]; py += mi * vy[i]; pz += mi * vz[i];
+    }
+    for (long i = 0; i < n; ++i) {
+        vx[i] -= px / M;
+        vy[i] -= py / M;
+        vz[i] -= pz / M;
+    }
+    return MD_OK;
+}
+
+md_status md_water_create(void* hip_stream, long n, const double* L, const double* x,
+                          const double* y, const double* z, const double* vx, const double* vy,
+                          const double* vz, const md_water_params* p, md_water** out) {
+    if (out == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (p == nullptr || L == nullptr || x == nullptr || y == nullptr || z == nullptr)
+        return MD_ERR_INVALID_ARGUMENT;
+    md_water* w = new (std::nothrow) md_water();
+    if (w == nullptr) return MD_ERR_RUNTIME;
+    *out = w;
+    if (n <= 0 || n % 3 != 0)
+        return wfail(w, MD_ERR_INVALID_ARGUMENT, "atom count must be a positive multiple of 3 (O, H, H)");
+    if (!(p->r_oh > 0.0)) return wfail(w, MD_ERR_INVALID_ARGUMENT, "RigidWater: ohDistance must be positive");
+    if (!(p->angle_hoh > 0.0) || !(p->angle_hoh < 180.0))
+        return wfail(w, MD_ERR_INVALID_ARGUMENT, "RigidWater: hohAngleDegrees must lie in (0, 180)");
+    if (!(p->tolerance > 0.0)) return wfail(w, MD_ERR_INVALID_ARGUMENT, "RigidWater: tolerance must be positive");
+    if (p->max_iterations < 1) return wfail(w, MD_ERR_INVALID_ARGUMENT, "RigidWater: maxIterations must be at least 1");
+    if (p->kmax < 1 || p->kmax > mdk::kMaxK)
+        return wfail(w, MD_ERR_INVALID_ARGUMENT, "kmax out of range (1..24)");
+    if (hip_stream) {
+        w->stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+        WHIP(w, hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+        w->own_stream = true;
+    }
+    w->n = n;
+    w->p = *p;
+    for (int d = 0; d < 3; ++d) w->L[d] = L[d];
+    const double minL = std::min({L[0], L[1], L[2]});
+    w->alpha = p->alpha > 0.0 ? p->alpha : 5.6 / minL;
+    const double rl = p->cutoff + p->skin;
+    w->grid = mdk::Grid{};
+    for (int d = 0; d < 3; ++d) {
+        w->grid.L[d] = L[d];
+        w->grid.nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / rl)));
+        w->grid.w[d] = L[d] / w->grid.nc[d];
+    }
+    w->grid.usable = (w->grid.nc[0] >= 3 && w->grid.nc[1] >= 3 && w->grid.nc[2] >= 3) ? 1 : 0;
+    const long ncell = static_cast<long>(w->grid.nc[0]) * w->grid.nc[1] * w->grid.nc[2];
+    const double rho = n / (L[0] * L[1] * L[2]);
+    long cap = static_cast<long>(std::ceil(1.5 * 4.0 / 3.0 * kPi * rl * rl * rl * rho + 64.0));
+    cap = std::min<long>((cap + 7) / 8 * 8, n - 1);
+    // k-vectors of the half space with 0 < |n|^2 < n2max
+    const int km = p->kmax;
+    const int n2max = p->n2max > 0 ? p->n2max : km * km + 2;
+    std::vector<int3> kv;
+    std::vector<double> kf;
+    const double V = L[0] * L[1] * L[2];
+    const double C = coulomb_prefactor();
+    const double k0[3] = {2.0 * kPi / L[0], 2.0 * kPi / L[1], 2.0 * kPi / L[2]};
+    for (int nx = 0; nx <= km; ++nx)
+        for (int ny = -km; ny <= km; ++ny)
+            for (int nz = -km; nz <= km; ++nz) {
+                const int n2 = nx * nx + ny * ny + nz * nz;
+                if (n2 == 0 || n2 >= n2max) continue;
+                if (nx == 0 && (ny < 0 || (ny == 0 && nz < 0))) continue;
+                const double kx = k0[0] * nx, ky = k0[1] * ny, kz = k0[2] * nz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                kv.push_back(make_int3(nx, ny, nz));
+                kf.push_back(C * (2.0 * kPi / V) * std::exp(-k2 / (4.0 * w->alpha * w->alpha)) / k2);
+            }
+    const int nk = static_cast<int>(kv.size());
+    const int groups = mdk::water_site_groups(n);
+    WHIP(w, walloc(w->pos, n)); WHIP(w, walloc(w->vel, n)); WHIP(w, walloc(w->frc, n));
+    WHIP(w, walloc(w->ref, n)); WHIP(w, walloc(w->start, n)); WHIP(w, walloc(w->freep, n));
+    const long npad = (n + 63) / 64 * 64;
+    WHIP(w, walloc(w->nbr, cap * npad)); WHIP(w, walloc(w->ncnt, n));
+    WHIP(w, walloc(w->cell_of, n)); WHIP(w, walloc(w->cell_count, ncell));
+    WHIP(w, walloc(w->cell_start, ncell + 1)); WHIP(w, walloc(w->cell_fill, ncell));
+    WHIP(w, walloc(w->cell_atoms, n)); WHIP(w, walloc(w->status, n / 3)); WHIP(w, walloc(w->flags, 4));
+    WHIP(w, walloc(w->part, 4 * wblocks(n) + 8)); WHIP(w, walloc(w->part2, wblocks(std::max<long>(n, nk)) + 8));
+    WHIP(w, walloc(w->kvec, std::max(1, nk))); WHIP(w, walloc(w->kfac, std::max(1, nk)));
+    WHIP(w, walloc(w->eik, 3L * (km + 1) * n));
+    WHIP(w, walloc(w->skpart, static_cast<long>(groups) * std::max(1, nk)));
+    WHIP(w, walloc(w->sk, std::max(1, nk)));
+    WHIP(w, hipMemset(w->flags, 0, 4 * sizeof(int)));
+    if (nk > 0) {
+        WHIP(w, hipMemcpy(w->kvec, kv.data(), nk * sizeof(int3), hipMemcpyHostToDevice));
+        WHIP(w, hipMemcpy(w->kfac, kf.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::vector<double4> hp(static_cast<size_t>(n)), hv(static_cast<size_t>(n)),
+        hf(static_cast<size_t>(n), make_double4(0, 0, 0, 0));
+    for (long i = 0; i < n; ++i) {
+        const bool oxy = (i % 3) == 0;
+        hp[i] = make_double4(x[i], y[i], z[i], oxy ? -2.0 * p->qH : p->qH);
+        hv[i] = make_double4(vx ? vx[i] : 0.0, vy ? vy[i] : 0.0, vz ? vz[i] : 0.0,
+                             oxy ? p->mass_o : p->mass_h);
+    }
+    WHIP(w, hipMemcpy(w->pos, hp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    WHIP(w, hipMemcpy(w->vel, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    WHIP(w, hipMemcpy(w->frc, hf.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    mdk::WaterArgs& a = w->a;
+    a.n = n;
+    a.box = mdk::make_box(L);
+    a.pos = w->pos; a.vel = w->vel; a.frc = w->frc; a.ref = w->ref;
+    a.nbr = w->nbr; a.ncnt = w->ncnt; a.cap = static_cast<int>(cap);
+    a.rl2 = rl * rl;
+    a.rc2 = p->cutoff * p->cutoff;
+    a.skin = p->skin;
+    a.alpha = w->alpha;
+    a.c2a = 2.0 * w->alpha / std::sqrt(kPi);
+    a.coulomb = C;
+    a.sig2 = p->sigma_o * p->sigma_o;
+    a.eps = p->epsilon_o;
+    a.part = w->part; a.part2 = w->part2;
+    a.kmax = km; a.nk = nk; a.kvec = w->kvec; a.kfac = w->kfac;
+    for (int d = 0; d < 3; ++d) a.k0[d] = k0[d];
+    a.eik = w->eik; a.skpart = w->skpart; a.sk = w->sk;
+    a.overflow = w->flags;
+    a.oh = p->r_oh;
+    a.hh = 2.0 * p->r_oh * std::sin(0.5 * p->angle_hoh * kPi / 180.0);
+    a.tol = p->tolerance;
+    a.max_iter = p->max_iterations;
+    return MD_OK;
+}
+
+md_status md_water_destroy(md_water* w) {
+    if (w == nullptr) return MD_OK;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (void* q : {static_cast<void*>(w->pos), static_cast<void*>(w->vel), static_cast<void*>(w->frc),
+                    static_cast<void*>(w->ref), static_cast<void*>(w->start), static_cast<void*>(w->freep),
+                    static_cast<void*>(w->nbr), static_cast<void*>(w->ncnt), static_cast<void*>(w->cell_of),
+                    static_cast<void*>(w->cell_count), static_cast<void*>(w->cell_start),
+                    static_cast<void*>(w->cell_fill), static_cast<void*>(w->cell_atoms),
+                    static_cast<void*>(w->status), static_cast<void*>(w->flags), static_cast<void*>(w->part),
+                    static_cast<void*>(w->part2), static_cast<void*>(w->kvec), static_cast<void*>(w->kfac),
+                    static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk)})
+        if (q) (void)hipFree(q);
+    if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
+    delete w;
+    return MD_OK;
+}
+
+const char* md_water_last_error(const md_water* w) { return w ? w->err.c_str() : "null water"; }
+
+md_status md_water_forces(md_water* w, int precision, double* terms, double* pe) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    double t[6];
+    md_status st = compute(w, precision == MD_MIXED, t);
+    if (st != MD_OK) return st;
+    if (terms)
+        for (int k = 0; k < 6; ++k) terms[k] = t[k];
+    if (pe) *pe = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
+    return MD_OK;
+}
+
+md_status md_water_alpha(const md_water* w, double* alpha, int* nk) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (alpha) *alpha = w->alpha;
+    if (nk) *nk = w->a.nk;
+    return MD_OK;
+}
+
+md_status md_water_build_list(md_water* w) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    return build_list(w);
+}
+
+md_status md_water_shake(md_water* w, const double* rx, const double* ry, const double* rz,
+                         int* iterations) {
+    if (w == nullptr || rx == nullptr || ry == nullptr || rz == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    std::vector<double4> r(static_cast<size_t>(w->n));
+    for (long i = 0; i < w->n; ++i) r[i] = make_double4(rx[i], ry[i], rz[i], 0.0);
+    WHIP(w, hipMemcpy(w->start, r.data(), w->n * sizeof(double4), hipMemcpyHostToDevice));
+    mdk::water_launch_shake(w->a, w->start, w->status, w->stream);
+    WHIP(w, hipGetLastError());
+    return status_check(w, "RigidWater: SHAKE", iterations);
+}
+
+md_status md_water_rattle(md_water* w, int* iterations) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::water_launch_rattle(w->a, w->status, w->stream);
+    WHIP(w, hipGetLastError());
+    return status_check(w, "RigidWater: RATTLE", iterations);
+}
+
+// nsteps of constrained velocity Verlet with the SPC/E force field; forces
+// must be current on entry (md_water_forces once). The Verlet list is
+// rebuilt when the two largest displacements exceed the skin.
+md_status md_water_run(md_water* w, long nsteps, double dt, int precision, double* pe, double* ke) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    for (long k = 0; k < nsteps; ++k) {
+        mdk::water_launch_cvv_first(w->a, dt, w->start, w->freep, w->stream);
+        mdk::water_launch_shake(w->a, w->start, w->status, w->stream);
+        md_status st = status_check(w, "RigidWater: SHAKE", nullptr);
+        if (st != MD_OK) return st;
+        mdk::water_launch_cvv_correct(w->a, dt, w->freep, w->stream);
+        mdk::water_launch_check(w->a, w->flags + 1, w->stream);
+        int flag = 0;
+        WHIP(w, hipMemcpyAsync(&flag, w->flags + 1, sizeof(int), hipMemcpyDeviceToHost, w->stream));
+        WHIP(w, hipStreamSynchronize(w->stream));
+        if (flag) {
+            st = build_list(w);
+            if (st != MD_OK) return st;
+        }
+        double t[6];
+        st = compute(w, precision == MD_MIXED, t);
+        if (st != MD_OK) return st;
+        mdk::water_launch_cvv_kick(w->a, dt, w->stream);
+        mdk::water_launch_rattle(w->a, w->status, w->stream);
+        st = status_check(w, "RigidWater: RATTLE", nullptr);
+        if (st != MD_OK) return st;
+        if (pe) pe[k] = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
+        if (ke) {
+            mdk::water_launch_kinetic(w->a, w->stream);
+            const long nb = wblocks(w->n);
+            std::vector<double> part(static_cast<size_t>(nb));
+            WHIP(w, hipStreamSynchronize(w->stream));
+            WHIP(w, hipMemcpy(part.data(), w->part2, nb * sizeof(double), hipMemcpyDeviceToHost));
+            double s = 0.0;
+            for (double v : part) s += v;
+            ke[k] = s;
+        }
+    }
+    return MD_OK;
+}
+
+md_status md_water_download(md_water* w, double* x, double* y, double* z, double* vx, double* vy,
+                            double* vz, double* fx, double* fy, double* fz) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    std::vector<double4> t(static_cast<size_t>(w->n));
+    double* outs[3][3] = {{x, y, z}, {vx, vy, vz}, {fx, fy, fz}};
+    double4* src[3] = {w->pos, w->vel, w->frc};
+    WHIP(w, hipStreamSynchronize(w->stream));
+    for (int k = 0; k < 3; ++k) {
+        if (!outs[k][0]) continue;
+        WHIP(w, hipMemcpy(t.data(), src[k], w->n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (long i = 0; i < w->n; ++i) {
+            outs[k][0][i] = t[i].x;
+            outs[k][1][i] = t[i].y;
+            outs[k][2][i] = t[i].z;
+        }
+    }
+    return MD_OK;
+}
+
+md_status md_water_set_state(md_water* w, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    std::vector<double4> t(static_cast<size_t>(w->n));
+    WHIP(w, hipStreamSynchronize(w->stream));
+    if (x) {
+        WHIP(w, hipMemcpy(t.data(), w->pos, w->n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (long i = 0; i < w->n; ++i) { t[i].x = x[i]; t[i].y = y[i]; t[i].z = z[i]; }
+        WHIP(w, hipMemcpy(w->pos, t.data(), w->n * sizeof(double4), hipMemcpyHostToDevice));
+        w->list_built = false;
+    }
+    if (vx) {
+        WHIP(w, hipMemcpy(t.data(), w->vel, w->n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (long i = 0; i < w->n; ++i) { t[i].x = vx[i]; t[i].y = vy[i]; t[i].z = vz[i]; }
+        WHIP(w, hipMemcpy(w->vel, t.data(), w->n * sizeof(double4), hipMemcpyHostToDevice));
+    }
+    return MD_OK;
+}
+
+}  // extern "C"

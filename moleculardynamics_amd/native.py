@@ -39,6 +39,21 @@ class Thermo(C.Structure):
                 ("conserved", C.c_double), ("pairs", C.c_double)]
 
 
+class WaterParams(C.Structure):
+    _fields_ = [("qH", C.c_double), ("sigma_o", C.c_double), ("epsilon_o", C.c_double),
+                ("r_oh", C.c_double), ("angle_hoh", C.c_double), ("mass_o", C.c_double),
+                ("mass_h", C.c_double), ("cutoff", C.c_double), ("kmax", C.c_int),
+                ("n2max", C.c_int), ("alpha", C.c_double), ("skin", C.c_double),
+                ("tolerance", C.c_double), ("max_iterations", C.c_int)]
+
+
+def water_params(cutoff=10.0, kmax=5, alpha=0.0, skin=1.0, n2max=0, tolerance=1e-12,
+                 max_iterations=500, r_oh=1.0, angle_hoh=109.47):
+    """SPC/E parameters of the reference (forcefield/spce.hpp:16-24)."""
+    return WaterParams(0.42380, 3.16555789, 78.19743111, r_oh, angle_hoh, 15.9994, 1.008, cutoff,
+                       kmax, n2max, alpha, skin, tolerance, max_iterations)
+
+
 class MDError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"[status {status}] {msg}")
@@ -91,6 +106,43 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_timing_reset.argtypes = [vp]
     lib.md_last_pair_count.argtypes = [vp, C.POINTER(C.c_long)]
     lib.md_fcc_lattice.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64] + [_dp] * 7
+    lib.md_system_set_forces.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_half_kick.argtypes = [vp, C.c_double]
+    lib.md_drift.argtypes = [vp, C.c_double]
+    lib.md_langevin_o.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64]
+    lib.md_scale_velocities.argtypes = [vp, C.c_double]
+    lib.md_lj_forces_cells.argtypes = [vp, C.c_double, _dp, _dp]
+    _ip = C.POINTER(C.c_int)
+    _lp = C.POINTER(C.c_long)
+    _u64p = C.POINTER(C.c_uint64)
+    lib.md_batch_draw.argtypes = [C.c_int, C.c_uint64, _ip, _dp, _dp, _u64p]
+    lib.md_batch_create.argtypes = ([vp, C.c_int, _lp, _dp] + [_dp] * 6 +
+                                    [_dp, _u64p, C.POINTER(LJParams), C.c_double, C.c_double,
+                                     C.c_double, C.c_int, C.POINTER(vp)])
+    lib.md_batch_destroy.argtypes = [vp]
+    lib.md_batch_last_error.argtypes = [vp]
+    lib.md_batch_last_error.restype = C.c_char_p
+    lib.md_batch_prime.argtypes = [vp]
+    lib.md_batch_run.argtypes = [vp, C.c_long, C.c_long, C.c_int, _dp]
+    lib.md_batch_set_tpa.argtypes = [vp, _ip, C.c_int]
+    lib.md_batch_autotune.argtypes = [vp, C.c_int, _ip, _ip, _dp]
+    lib.md_batch_download.argtypes = [vp] + [_dp] * 9
+    lib.md_batch_rebuild_counts.argtypes = [vp, _lp]
+    wp = C.POINTER(WaterParams)
+    lib.md_ewald_parameters.argtypes = [C.c_long, _dp, C.c_double, C.c_double, C.c_double, _dp, _ip]
+    lib.md_water_lattice.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_double, wp] + [_dp] * 6
+    lib.md_water_create.argtypes = [vp, C.c_long, _dp] + [_dp] * 6 + [wp, C.POINTER(vp)]
+    lib.md_water_destroy.argtypes = [vp]
+    lib.md_water_last_error.argtypes = [vp]
+    lib.md_water_last_error.restype = C.c_char_p
+    lib.md_water_forces.argtypes = [vp, C.c_int, _dp, _dp]
+    lib.md_water_alpha.argtypes = [vp, _dp, _ip]
+    lib.md_water_build_list.argtypes = [vp]
+    lib.md_water_shake.argtypes = [vp, _dp, _dp, _dp, _ip]
+    lib.md_water_rattle.argtypes = [vp, _ip]
+    lib.md_water_run.argtypes = [vp, C.c_long, C.c_double, C.c_int, _dp, _dp]
+    lib.md_water_download.argtypes = [vp] + [_dp] * 9
+    lib.md_water_set_state.argtypes = [vp] + [_dp] * 6
     _lib = lib
     return lib
 
@@ -263,6 +315,181 @@ class Context:
 
 def cells_usable(L, cutoff) -> bool:
     return bool(load().md_cells_usable(_p(_col(L)), cutoff))
+
+
+def batch_draw(count: int, master_seed: int):
+    """The replica recipe through the C ABI: (cells, rho, T, seeds) per replica."""
+    cells = np.zeros(count, dtype=np.int32)
+    rho, T = np.zeros(count), np.zeros(count)
+    seeds = np.zeros(count, dtype=np.uint64)
+    st = load().md_batch_draw(count, master_seed, cells.ctypes.data_as(C.POINTER(C.c_int)),
+                              _p(rho), _p(T), seeds.ctypes.data_as(C.POINTER(C.c_uint64)))
+    if st != MD_OK:
+        raise MDError(st, "md_batch_draw failed")
+    return cells, rho, T, seeds
+
+
+class Batch:
+    """md_batch: many independent replicas advanced by one launch per step."""
+
+    def __init__(self, systems, temperatures, seeds, *, sigma=1.0, epsilon=1.0, cutoff=2.5,
+                 skin=0.3, dt=0.005, friction=1.0, precision=MIXED):
+        """systems: list of (L[3], pos (n,3), vel (n,3))."""
+        self.lib = load()
+        self.sizes = np.array([s[1].shape[0] for s in systems], dtype=np.int64)
+        self.nrep = len(systems)
+        L = np.concatenate([np.asarray(s[0], dtype=float) for s in systems])
+        pos = np.concatenate([s[1] for s in systems])
+        vel = np.concatenate([s[2] for s in systems])
+        cols = [_col(pos[:, d]) for d in range(3)] + [_col(vel[:, d]) for d in range(3)]
+        T = _col(temperatures)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        self.ljp = LJParams(sigma, epsilon, cutoff, 0)
+        h = C.c_void_p()
+        st = self.lib.md_batch_create(None, self.nrep, self.sizes.ctypes.data_as(C.POINTER(C.c_long)),
+                                      _p(L), *[_p(c) for c in cols], _p(T),
+                                      sd.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(self.ljp),
+                                      skin, dt, friction, precision, C.byref(h))
+        self.h = h
+        if st != MD_OK:
+            msg = self.lib.md_batch_last_error(h).decode() if h else "md_batch_create failed"
+            self.close()
+            raise MDError(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.md_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != MD_OK:
+            raise MDError(st, self.lib.md_batch_last_error(self.h).decode())
+
+    def prime(self):
+        self._chk(self.lib.md_batch_prime(self.h))
+
+    def run(self, nsteps, segment=100, graph=True, thermo=False):
+        out = np.zeros(nsteps * self.nrep * 4) if thermo else None
+        self._chk(self.lib.md_batch_run(self.h, nsteps, segment, int(graph), _p(out)))
+        return None if out is None else out.reshape(nsteps, self.nrep, 4)
+
+    def set_tpa(self, tpa, block=512):
+        t = np.ascontiguousarray(tpa, dtype=np.int32)
+        self._chk(self.lib.md_batch_set_tpa(self.h, t.ctypes.data_as(C.POINTER(C.c_int)), block))
+
+    def autotune(self, trial_steps=5):
+        t = np.zeros(self.nrep, dtype=np.int32)
+        blk = C.c_int()
+        ms = C.c_double()
+        self._chk(self.lib.md_batch_autotune(self.h, trial_steps, t.ctypes.data_as(C.POINTER(C.c_int)),
+                                             C.byref(blk), C.byref(ms)))
+        return t, blk.value, ms.value
+
+    def download(self):
+        n = int(self.sizes.sum())
+        a = [np.zeros(n) for _ in range(9)]
+        self._chk(self.lib.md_batch_download(self.h, *[_p(v) for v in a]))
+        split = np.cumsum(self.sizes)[:-1]
+        out = []
+        for k in range(3):
+            arr = np.stack(a[3 * k:3 * k + 3], axis=1)
+            out.append(np.split(arr, split))
+        return out  # [positions per replica], [velocities], [forces]
+
+    def rebuild_counts(self):
+        c = np.zeros(self.nrep, dtype=np.int64)
+        self._chk(self.lib.md_batch_rebuild_counts(self.h, c.ctypes.data_as(C.POINTER(C.c_long))))
+        return c
+
+
+def ewald_parameters(n, L, sum_q2, cutoff, accuracy):
+    a, k = C.c_double(), C.c_int()
+    st = load().md_ewald_parameters(n, _p(_col(L)), sum_q2, cutoff, accuracy, C.byref(a), C.byref(k))
+    if st != MD_OK:
+        raise MDError(st, "md_ewald_parameters failed")
+    return a.value, k.value
+
+
+def water_lattice(m, L, seed, temperature, params=None):
+    params = params or water_params()
+    n = 3 * m ** 3
+    a = [np.zeros(n) for _ in range(6)]
+    st = load().md_water_lattice(m, L, seed, temperature, C.byref(params), *[_p(v) for v in a])
+    if st != MD_OK:
+        raise MDError(st, "md_water_lattice failed")
+    return np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1)
+
+
+class Water:
+    """md_water: rigid SPC/E sites with Ewald on the device."""
+
+    def __init__(self, L, pos, vel=None, params=None):
+        self.lib = load()
+        self.params = params or water_params()
+        pos = np.asarray(pos, dtype=float)
+        self.n = pos.shape[0]
+        cols = [_col(pos[:, d]) for d in range(3)]
+        vcols = [None] * 3 if vel is None else [_col(np.asarray(vel)[:, d]) for d in range(3)]
+        h = C.c_void_p()
+        st = self.lib.md_water_create(None, self.n, _p(_col(L)), *[_p(c) for c in cols],
+                                      *[_p(c) for c in vcols], C.byref(self.params), C.byref(h))
+        self.h = h
+        if st != MD_OK:
+            msg = self.lib.md_water_last_error(h).decode() if h else "md_water_create failed"
+            self.close()
+            raise MDError(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.md_water_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != MD_OK:
+            raise MDError(st, self.lib.md_water_last_error(self.h).decode())
+
+    def forces(self, precision=FP64):
+        t = np.zeros(6)
+        pe = C.c_double()
+        self._chk(self.lib.md_water_forces(self.h, precision, _p(t), C.byref(pe)))
+        return dict(disp=t[0], lrc=t[1], real=t[2], recip=t[3], self=t[4], intra=t[5]), pe.value
+
+    def alpha(self):
+        a, k = C.c_double(), C.c_int()
+        self._chk(self.lib.md_water_alpha(self.h, C.byref(a), C.byref(k)))
+        return a.value, k.value
+
+    def shake(self, ref_pos):
+        cols = [_col(np.asarray(ref_pos)[:, d]) for d in range(3)]
+        it = C.c_int()
+        self._chk(self.lib.md_water_shake(self.h, *[_p(c) for c in cols], C.byref(it)))
+        return it.value
+
+    def rattle(self):
+        it = C.c_int()
+        self._chk(self.lib.md_water_rattle(self.h, C.byref(it)))
+        return it.value
+
+    def run(self, nsteps, dt, precision=FP64):
+        pe, ke = np.zeros(nsteps), np.zeros(nsteps)
+        self._chk(self.lib.md_water_run(self.h, nsteps, dt, precision, _p(pe), _p(ke)))
+        return pe, ke
+
+    def download(self):
+        a = [np.zeros(self.n) for _ in range(9)]
+        self._chk(self.lib.md_water_download(self.h, *[_p(v) for v in a]))
+        return (np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1), np.stack(a[6:9], axis=1))
+
+    def set_state(self, pos=None, vel=None):
+        pc = [None] * 3 if pos is None else [_col(np.asarray(pos)[:, d]) for d in range(3)]
+        vc = [None] * 3 if vel is None else [_col(np.asarray(vel)[:, d]) for d in range(3)]
+        self._chk(self.lib.md_water_set_state(self.h, *[_p(c) for c in pc], *[_p(c) for c in vc]))
 
 
 def fcc_lattice(cells: int, density: float, temperature: float, seed: int):

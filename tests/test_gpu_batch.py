@@ -1,0 +1,149 @@
+"""Batched uneven replicas (one launch per step for all replicas, HIP-graph
+segments) against the oracle and against the single-system path."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(count, master=4000, max_cells=None):
+    from moleculardynamics_amd import native
+
+    cells, rho, T, seeds = native.batch_draw(count, master)
+    if max_cells is not None:
+        cells = np.minimum(cells, max_cells)
+    systems = [native.fcc_lattice(int(c), float(r), float(t), int(s))
+               for c, r, t, s in zip(cells, rho, T, seeds)]
+    return systems, T, seeds, cells, rho
+
+
+def test_recipe_draw_is_deterministic_and_in_range():
+    from moleculardynamics_amd import native
+
+    c1, r1, t1, s1 = native.batch_draw(256, 4000)
+    c2, r2, t2, s2 = native.batch_draw(256, 4000)
+    assert np.array_equal(c1, c2) and np.array_equal(r1, r2) and np.array_equal(t1, t2)
+    assert set(np.unique(c1)) <= set(range(3, 11)) and len(np.unique(c1)) == 8
+    assert r1.min() >= 0.5 and r1.max() < 0.9 and t1.min() >= 0.7 and t1.max() < 1.5
+    assert np.array_equal(s1, 4000 + np.arange(256))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_primed_forces_match_oracle(gpu_ok, orc, precision):
+    from moleculardynamics_amd import native
+    from oracle.pyoracle import Frame
+
+    systems, T, seeds, _, _ = _systems(12, master=4100, max_cells=7)
+    prec = native.FP64 if precision == "fp64" else native.MIXED
+    b = native.Batch(systems, T, seeds, precision=prec)
+    b.prime()
+    _, _, forces = b.download()
+    for (L, x, v), f in zip(systems, forces):
+        fr = Frame(L, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+        f0, _, _ = orc.lj_forces(fr, rc=2.5, mode="brute")
+        if precision == "fp64":
+            assert np.max(np.abs(f - f0)) <= 1e-12 * max(1.0, np.max(np.abs(f0)))
+        else:
+            rms = np.sqrt(np.sum((f - f0) ** 2) / np.sum(f0 ** 2))
+            assert rms < 2e-6
+    b.close()
+
+
+def test_small_box_single_image_semantics(gpu_ok, orc):
+    """N=108 at rho 0.9: L = 4.93 < 2 (rc + skin); forces follow the
+    reference's single nearest image."""
+    from moleculardynamics_amd import native
+    from oracle.pyoracle import Frame
+
+    L, x, v = native.fcc_lattice(3, 0.9, 1.0, 77)
+    b = native.Batch([(L, x, v)], [1.0], [77], precision=native.FP64)
+    b.prime()
+    b.run(40, graph=False)
+    pos, _, f = b.download()
+    fr = Frame(L, pos[0][:, 0].copy(), pos[0][:, 1].copy(), pos[0][:, 2].copy())
+    f0, _, _ = orc.lj_forces(fr, rc=2.5, mode="brute")
+    assert np.max(np.abs(f[0] - f0)) <= 1e-11 * max(1.0, np.max(np.abs(f0)))
+    b.close()
+
+
+def test_graph_replay_matches_plain_launches(gpu_ok):
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, _, _ = _systems(16, master=4200, max_cells=6)
+    outs = []
+    for graph in (False, True):
+        b = native.Batch(systems, T, seeds, precision=native.MIXED)
+        b.prime()
+        th = b.run(200, segment=100, graph=graph, thermo=True)
+        outs.append((b.download(), th))
+        b.close()
+    (p0, v0, f0), th0 = outs[0]
+    (p1, v1, f1), th1 = outs[1]
+    for a, c in zip(p0 + v0 + f0, p1 + v1 + f1):
+        assert np.array_equal(a, c)
+    assert np.array_equal(th0, th1)
+
+
+def test_batch_matches_single_system_path(gpu_ok):
+    """Same seed, same counter-based noise, fixed-point forces: a replica in
+    the batch and the same system on the plain path follow one trajectory."""
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, _, _ = _systems(4, master=4300, max_cells=5)
+    b = native.Batch(systems, T, seeds, precision=native.MIXED)
+    b.prime()
+    b.run(30, graph=False)
+    pos, vel, _ = b.download()
+    b.close()
+    for k, (L, x, v) in enumerate(systems):
+        ctx = native.Context()
+        ctx.upload(L, x, v)
+        ctx.set_lj(1.0, 1.0, 2.5, False)
+        ctx.config_list(2.5, 0.3)
+        ctx.build_list()
+        ctx.forces(native.VERLET, native.MIXED)
+        ctx.set_integrator(kind=native.BAOAB, dt=0.005, friction=1.0, temperature=float(T[k]),
+                           seed=int(seeds[k]))
+        ctx.run(30, native.MIXED, thermo=False)
+        x1, v1, _ = ctx.download()
+        ctx.close()
+        assert np.max(np.abs(x1 - pos[k])) < 1e-9
+        assert np.max(np.abs(v1 - vel[k])) < 1e-9
+
+
+def test_batch_thermostat_equipartition(gpu_ok):
+    """Every replica's mean kinetic temperature (dof = 3N) within 3% of its
+    own target after relaxation (the reference's equipartition gate is 2% on
+    a 10k-step window; this window is shorter)."""
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, cells, _ = _systems(16, master=4400, max_cells=6)
+    b = native.Batch(systems, T, seeds, precision=native.MIXED)
+    b.prime()
+    b.run(2000, segment=100)
+    th = b.run(4000, segment=100, thermo=True)
+    n = np.array([s[1].shape[0] for s in systems])
+    Tm = np.mean(2.0 * th[:, :, 2] / (3 * n[None, :]), axis=0)
+    assert np.all(np.abs(Tm / T - 1.0) < 0.03), (Tm / T)
+    assert np.all(b.rebuild_counts() > 1)
+    b.close()
+
+
+def test_autotune_keeps_trajectory(gpu_ok):
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, _, _ = _systems(24, master=4500)
+    ref = native.Batch(systems, T, seeds)
+    ref.prime()
+    ref.run(20, graph=False)
+    want = ref.download()
+    ref.close()
+    b = native.Batch(systems, T, seeds)
+    b.prime()
+    tpa, block, ms = b.autotune(trial_steps=3)
+    assert block in (256, 512, 1024) and set(np.unique(tpa)) <= {1, 2, 4, 8, 16} and ms > 0
+    b.run(20, graph=False)
+    got = b.download()
+    b.close()
+    for a, c in zip(want[0] + want[1], got[0] + got[1]):
+        assert np.array_equal(a, c)

@@ -1,0 +1,181 @@
+"""Rigid SPC/E water with Ewald on the GPU (md_water_*) against NIST and the
+oracle: the 24 NIST energy components, Ewald and SPC/E forces, SHAKE/RATTLE,
+constrained dynamics, the config-3 builder and the Ewald parameter choice."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, nist_spce
+from oracle.pyoracle import load_nist
+
+pytestmark = pytest.mark.gpu
+
+NIST = json.load(open(os.path.join(GOLDEN, "nist_values.json")))
+
+
+def _half(v):
+    return 0.5 * 10 ** (np.floor(np.log10(abs(v))) - 5)
+
+
+def _pos(fr):
+    return np.stack([fr.x, fr.y, fr.z], axis=1)
+
+
+def _q_mol(n):
+    return np.array([-0.8476 if i % 3 == 0 else 0.4238 for i in range(n)]), np.arange(n) // 3
+
+
+def _masses(n):
+    return np.array([15.9994 if i % 3 == 0 else 1.008 for i in range(n)])
+
+
+@pytest.mark.parametrize("row", NIST["spce"], ids=lambda r: f"spce{r[0]}")
+def test_nist_spce_terms(gpu_ok, orc, row):
+    from moleculardynamics_amd import native
+
+    cfg, edisp, elrc, ereal, efour, eself, eintra, etot = row
+    fr = load_nist(nist_spce(cfg))
+    w = native.Water(fr.L, _pos(fr), params=native.water_params(cutoff=10.0, kmax=5))
+    t, pe = w.forces()
+    for got, want in [(t["disp"], edisp), (t["lrc"], elrc), (t["real"], ereal), (t["recip"], efour),
+                      (t["self"], eself), (t["intra"], eintra)]:
+        assert abs(got - want) <= _half(want)
+    q, mol = _q_mol(fr.n)
+    o, _ = orc.ewald(fr, q, mol, rc=10.0, kmax=5)
+    assert t["real"] == pytest.approx(o["real"], rel=1e-12)
+    assert t["self"] == pytest.approx(o["self"], rel=1e-13)
+    assert t["intra"] == pytest.approx(o["intra"], rel=1e-12)
+    assert t["recip"] == pytest.approx(o["recip"], rel=1e-10)
+    f0, pe0 = orc.spce_forces(fr, rc=10.0, kmax=5)
+    assert pe == pytest.approx(pe0, rel=1e-11)
+    _, _, f = w.download()
+    assert np.max(np.abs(f - f0)) <= 1e-10 * np.max(np.abs(f0))
+    w.close()
+
+
+def test_ewald_forces_vs_oracle_and_newton3(gpu_ok, orc):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(2))
+    q, mol = _q_mol(fr.n)
+    w = native.Water(fr.L, _pos(fr), params=native.water_params(cutoff=10.0, kmax=5))
+    w.forces()
+    _, _, f = w.download()
+    f_all, pe = orc.spce_forces(fr, rc=10.0, kmax=5)
+    assert np.max(np.abs(f - f_all)) <= 1e-10 * np.max(np.abs(f_all))
+    assert np.all(np.abs(f.sum(axis=0)) < 1e-8 * np.max(np.abs(f)) * fr.n)
+
+
+def test_shake_rattle_match_oracle(gpu_ok, orc):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(1))
+    m = _masses(fr.n)
+    moved = fr.copy()
+    s = 0.01 * (np.arange(fr.n) % 7) - 0.03
+    moved.x, moved.y, moved.z = fr.x + s, fr.y - 0.5 * s, fr.z + 0.25 * s
+    w = native.Water(moved.L, _pos(moved), params=native.water_params(cutoff=10.0, kmax=5))
+    it = w.shake(_pos(fr))
+    p, _, _ = w.download()
+    it0, p0 = orc.rigid_shake(moved, m, fr)
+    assert it == it0
+    assert np.max(np.abs(p - p0)) < 1e-12
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(fr.n, 3)) * np.sqrt(300.0 / m)[:, None]
+    w.set_state(vel=v)
+    it = w.rattle()
+    _, vg, _ = w.download()
+    after = moved.copy()
+    after.x, after.y, after.z = p0[:, 0].copy(), p0[:, 1].copy(), p0[:, 2].copy()
+    after.vx, after.vy, after.vz = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    it1, v0 = orc.rigid_rattle(after, m)
+    assert it == it1
+    assert np.max(np.abs(vg - v0)) < 1e-11 * np.max(np.abs(v0))
+    w.close()
+
+
+def test_shake_failure_is_an_error(gpu_ok):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(1))
+    moved = fr.copy()
+    moved.x = fr.x + 0.05 * (np.arange(fr.n) % 5)
+    w = native.Water(moved.L, _pos(moved), params=native.water_params(cutoff=10.0, kmax=5, max_iterations=1))
+    with pytest.raises(native.MDError) as ei:
+        w.shake(_pos(fr))
+    assert ei.value.status == native.MD_ERR_RUNTIME
+    w.close()
+
+
+def test_constrained_dynamics_match_oracle(gpu_ok, orc):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(1))
+    m = _masses(fr.n)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(fr.n, 3)) * np.sqrt(298.0 / m)[:, None]
+    v -= (v * m[:, None]).sum(0) / m.sum()
+    fr.vx, fr.vy, fr.vz = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    _, vr = orc.rigid_rattle(fr, m)
+    fr.vx, fr.vy, fr.vz = vr[:, 0].copy(), vr[:, 1].copy(), vr[:, 2].copy()
+    dt = 0.0018236739  # 2 fs in units of 1.0966875353 ps
+    wo, pe0, ke0 = orc.water_run(fr, dt=dt, nsteps=10)
+    w = native.Water(fr.L, _pos(fr), vr, params=native.water_params(cutoff=10.0, kmax=5))
+    w.forces()
+    pe, ke = w.run(10, dt)
+    assert np.max(np.abs(pe - pe0) / np.abs(pe0)) < 1e-10
+    assert np.max(np.abs(ke - ke0) / np.abs(ke0)) < 1e-10
+    p, _, _ = w.download()
+    assert np.max(np.abs(p - np.stack([wo.x, wo.y, wo.z], 1))) < 1e-10
+    w.close()
+
+
+def test_config3_builder_and_ewald_parameters(gpu_ok):
+    from moleculardynamics_amd import native
+
+    L = 49.71853
+    pos, vel = native.water_lattice(16, L, 2026, 300.0)
+    assert pos.shape == (12288, 3)
+    # rigid geometry
+    hh = 2.0 * np.sin(0.5 * np.deg2rad(109.47))
+    for a, b, t in ((0, 1, 1.0), (0, 2, 1.0), (1, 2, hh)):
+        d = np.linalg.norm(pos[a::3] - pos[b::3], axis=1)
+        assert np.max(np.abs(d - t)) < 1e-12
+    m = _masses(pos.shape[0])
+    assert np.all(np.abs((vel * m[:, None]).sum(0)) < 1e-9)
+    q2 = 4096 * (0.8476 ** 2 + 2 * 0.4238 ** 2)
+    alpha, kmax = native.ewald_parameters(12288, [L] * 3, q2, 9.0, 1e-5)
+    assert 3.0 <= alpha * 9.0 <= 3.6
+    assert 15 <= kmax <= 24
+
+
+def test_alpha_independence_of_total(gpu_ok):
+    """The converged Ewald total does not depend on alpha (config 1 of NIST,
+    two well-converged (alpha, kmax) pairs)."""
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(1))
+    tot = []
+    for alpha, kmax in [(0.33, 10), (0.38, 13)]:
+        w = native.Water(fr.L, _pos(fr), params=native.water_params(cutoff=9.9, kmax=kmax, alpha=alpha))
+        t, _ = w.forces()
+        tot.append(t["real"] + t["recip"] + t["self"] + t["intra"])
+        w.close()
+    assert abs(tot[0] - tot[1]) / abs(tot[1]) < 2e-5
+
+
+def test_mixed_erfc_real_space(gpu_ok):
+    from moleculardynamics_amd import native
+
+    fr = load_nist(nist_spce(3))
+    w = native.Water(fr.L, _pos(fr), params=native.water_params(cutoff=10.0, kmax=5))
+    t64, _ = w.forces(native.FP64)
+    _, _, f64 = w.download()
+    t32, _ = w.forces(native.MIXED)
+    _, _, f32 = w.download()
+    assert abs(t32["real"] - t64["real"]) / abs(t64["real"]) < 1e-5
+    rms = np.sqrt(np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2))
+    assert rms < 1e-4
+    w.close()
