@@ -193,8 +193,8 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
                 if (MIXED) {
                     const float xf = static_cast<float>(dx), yf = static_cast<float>(dy),
                                 zf = static_cast<float>(dz);
-                    float r2, inv, u, ww;
-                    if (lj_terms_f32(xf, yf, zf, a.lj.sig2f, a.lj.epsf, a.lj.rc2f, r2, inv, u, ww)) {
+                    float inv, u, ww;
+                    if (lj_terms_f32(norm2_ref(dx, dy, dz), a.lj.rc2, a.lj.sig2f, a.lj.epsf, inv, u, ww)) {
                         ef += to_fix(u, &sat);
                         wf += to_fix(ww, &sat);
                         const float c = flip ? -(ww * inv) : ww * inv;
@@ -271,20 +271,12 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
             th[3] = static_cast<double>(step);
         }
     }
-    // the last workgroup advances the device step counter
-    if (a.mode == 1) {
-        __shared__ bool last;
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned int tk = atomicAdd(a.ticket, 1u);
-            last = (tk == gridDim.x - 1);
-            if (last) {
-                *a.ticket = 0u;
-                *a.step = step + 1;
-            }
-        }
-    }
+}
+
+// Advances the device step counter after every replica finished the step
+// (a separate one-thread launch: stream order makes it the barrier).
+__global__ void k_batch_advance(unsigned long long* step) {
+    if (threadIdx.x == 0) *step += 1;
 }
 
 }  // namespace
@@ -295,6 +287,7 @@ hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s) {
         hipLaunchKernelGGL(k_batch<true>, dim3(a.nrep), dim3(block), shmem, s, a);
     else
         hipLaunchKernelGGL(k_batch<false>, dim3(a.nrep), dim3(block), shmem, s, a);
+    if (a.mode == 1) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, a.step);
     return hipGetLastError();
 }
 

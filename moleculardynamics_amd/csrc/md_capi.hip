@@ -303,7 +303,7 @@ md_status md_ctx_create(void* hip_stream, md_ctx** out) {
     *out = nullptr;
     md_ctx* c = new (std::nothrow) md_ctx();
     if (c == nullptr) return MD_ERR_RUNTIME;
-    if (hipGetDevice(&c->device) != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess || mdk::neigh_init() != hipSuccess) {
         delete c;
         return MD_ERR_HIP;
     }
@@ -523,6 +523,7 @@ md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, dou
         ev_begin(c, 0, &e0);
         mdk::launch_lj_force(a, c->stream);
         ev_end(c, 0, e0);
+        mdk::launch_lj_finish(a, c->stream);
     } else if (source == MD_PAIRS_BRUTE || source == MD_PAIRS_CELLS) {
         mdk::DirectArgs a{};
         a.n = c->n;
@@ -778,6 +779,7 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
         ev_begin(c, 0, &e0);
         mdk::launch_lj_force(fa, c->stream);
         ev_end(c, 0, e0);
+        mdk::launch_lj_finish(fa, c->stream);
     }
     if (nhc) mdk::launch_apply_scale(c->n, c->vel, c->sc, c->stream);
     MD_HIP(c, hipGetLastError());

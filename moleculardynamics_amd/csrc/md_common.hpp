@@ -140,24 +140,26 @@ __host__ __device__ __forceinline__ long nbr_index(long t, int k, int cap) {
     return ((t >> 6) * static_cast<long>(cap) + k) * 64 + (t & 63);
 }
 
-// FP32 LJ pair terms of the mixed path (Truncated), written once so every
-// kernel that uses them evaluates the same operations in the same order:
-// r2 = (x*x + y*y) + z*z, s2 = sig^2 * rcp(r2) (hardware reciprocal, 1 ulp),
-// s6 = s2^3, u = 4 eps (s6^2 - s6), w = 24 eps (2 s6^2 - s6), c = w * rcp(r2).
+// LJ pair terms of the mixed path (Truncated), written once so every kernel
+// that uses them evaluates the same operations in the same order. The cutoff
+// test is the FP64 one of the plain path (same pairs as the reference); the
+// terms are FP32: r2 rounded to FP32, 1/r2 from the hardware reciprocal plus
+// one Newton step, s2 = sig^2/r2, s6 = s2^3, u = 4 eps (s6*s6 - s6) and
+// w = 24 eps (2 s6*s6 - s6) with the products fused into the subtraction
+// (one rounding where the two terms cancel near the potential minimum).
 // Returns false outside the cutoff.
-__device__ __forceinline__ bool lj_terms_f32(float x, float y, float z, float sig2, float eps,
-                                             float rc2, float& r2, float& inv, float& u, float& w) {
-#pragma clang fp contract(off)
-    r2 = x * x;
-    r2 = r2 + y * y;
-    r2 = r2 + z * z;
-    if (!(r2 < rc2)) return false;
-    inv = __builtin_amdgcn_rcpf(r2);
+__device__ __forceinline__ bool lj_terms_f32(double r2d, double rc2d, float sig2, float eps,
+                                             float& inv, float& u, float& w) {
+    if (!(r2d < rc2d)) return false;
+    const float r2 = static_cast<float>(r2d);
+    float y = __builtin_amdgcn_rcpf(r2);
+    const float e = __builtin_fmaf(-r2, y, 1.0f);
+    y = __builtin_fmaf(y, e, y);
+    inv = y;
     const float s2 = sig2 * inv;
     const float s6 = (s2 * s2) * s2;
-    const float s12 = s6 * s6;
-    u = (4.0f * eps) * (s12 - s6);
-    w = (24.0f * eps) * ((2.0f * s12) - s6);
+    u = (4.0f * eps) * __builtin_fmaf(s6, s6, -s6);
+    w = (24.0f * eps) * __builtin_fmaf(2.0f * s6, s6, -s6);
     return true;
 }
 

@@ -68,11 +68,11 @@ __device__ double chain_conserved(const Chain& c) {
     return acc;
 }
 
-// Final reduction by the last workgroup: partial sums in index order.
-__device__ void finish_step(const ForceArgs& a, bool mixed) {
+// Final reduction (one workgroup, launched after the pair kernel): the nb
+// per-workgroup partials summed in a fixed order.
+__device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
     __shared__ double red_d[3][kBlock];
     __shared__ long long red_l[3][kBlock];
-    const int nb = gridDim.x;
     for (int q = 0; q < 3; ++q) {
         double sd = 0.0;
         long long sl = 0;
@@ -144,7 +144,12 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
     pair_delta(a.box, pi, pj, flip, dx, dy, dz);
     const double r2 = norm2_ref(dx, dy, dz);
     if (r2 < a.lj.rc2) {
-        const double inv = 1.0 / r2;
+        // hardware reciprocal refined by two Newton steps (within an ulp of 1/r2)
+        double inv = __builtin_amdgcn_rcp(r2);
+        double e = fma(-r2, inv, 1.0);
+        inv = fma(inv, e, inv);
+        e = fma(-r2, inv, 1.0);
+        inv = fma(inv, e, inv);
         const double s2 = a.lj.sig2 * inv;
         const double s6 = s2 * s2 * s2;
         double u = 4.0 * a.lj.eps * (s6 * s6 - s6);
@@ -175,11 +180,13 @@ __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, ui
     const bool flip = (jj & kFlipBit) != 0;
     double dx, dy, dz;
     pair_delta(a.box, pi, pj, flip, dx, dy, dz);
-    const float xf = static_cast<float>(dx);
-    const float yf = static_cast<float>(dy);
-    const float zf = static_cast<float>(dz);
-    float r2, inv, u, ww;
-    if (lj_terms_f32(xf, yf, zf, a.lj.sig2f, a.lj.epsf, a.lj.rc2f, r2, inv, u, ww)) {
+    const double r2d = norm2_ref(dx, dy, dz);
+    float inv, u, ww;
+    if (lj_terms_f32(r2d, a.lj.rc2, a.lj.sig2f, a.lj.epsf, inv, u, ww)) {
+        const float xf = static_cast<float>(dx);
+        const float yf = static_cast<float>(dy);
+        const float zf = static_cast<float>(dz);
+        const float r2 = static_cast<float>(r2d);
         if (LFS) {
             const float r = sqrtf(r2);
             u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
@@ -281,7 +288,10 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         a.part_l[3 * b + 1] = bwf;
         a.part_l[3 * b + 2] = bnp;
     }
-    if (last_block(&a.sc->counter[0])) finish_step(a, MIXED);
+}
+
+__global__ __launch_bounds__(kBlock) void k_lj_finish(ForceArgs a, int nb) {
+    finish_step(a, a.mixed != 0, nb);
 }
 
 template <bool LFS, bool MIXED>
@@ -294,10 +304,9 @@ void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
 
 // ---- direct (no list) evaluations: brute force and cells at the cutoff ----
 
-__device__ void finish_direct(const DirectArgs& a) {
+__device__ void finish_direct(const DirectArgs& a, int nb) {
     __shared__ double red[2][kBlock];
     __shared__ long long redl[kBlock];
-    const int nb = gridDim.x;
     double s0 = 0, s1 = 0;
     long long s2 = 0;
     for (int b = threadIdx.x; b < nb; b += kBlock) {
@@ -393,7 +402,10 @@ __global__ __launch_bounds__(kBlock) void k_lj_direct(DirectArgs a) {
         a.part_d[3 * blockIdx.x + 1] = bw;
         a.part_l[3 * blockIdx.x] = bn;
     }
-    if (last_block(&a.sc->counter[1])) finish_direct(a);
+}
+
+__global__ __launch_bounds__(kBlock) void k_direct_finish(DirectArgs a, int nb) {
+    finish_direct(a, nb);
 }
 
 __global__ void k_bin_index(const double4* __restrict__ pos, long n, Grid g,
@@ -491,12 +503,17 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
     }
 }
 
+void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(grid_of(a.n)));
+}
+
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {
     const dim3 g(grid_of(a.n));
     if (a.lj.lfs)
         hipLaunchKernelGGL(k_lj_direct<true>, g, dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_lj_direct<false>, g, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_direct_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(g.x));
 }
 
 void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,

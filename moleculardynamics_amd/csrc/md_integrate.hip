@@ -80,10 +80,10 @@ __device__ __forceinline__ double displacement(const Box& b, const double4& p, c
     return sqrt(norm2_ref(dx, dy, dz));
 }
 
-__device__ void finish_check(const StepArgs& a) {
+__device__ void finish_check(const StepArgs& a, int nb) {
     __shared__ double t1s[kBlock], t2s[kBlock];
     double t1 = 0.0, t2 = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock)
+    for (int b = threadIdx.x; b < nb; b += kBlock)
         top2_merge(t1, t2, a.part_d[2 * b], a.part_d[2 * b + 1]);
     t1s[threadIdx.x] = t1;
     t2s[threadIdx.x] = t2;
@@ -155,12 +155,15 @@ __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
         a.part_d[2 * blockIdx.x] = b1;
         a.part_d[2 * blockIdx.x + 1] = b2;
     }
-    if (last_block(&a.sc->counter[2])) {
-        if (a.check) {
-            finish_check(a);
-        } else if (threadIdx.x == 0) {
-            a.sc->vscale = 1.0;
-        }
+}
+
+// One workgroup after k_pre: rebuild decision from the partials; the chain
+// rescale applied by k_pre is consumed.
+__global__ __launch_bounds__(kBlock) void k_pre_finish(StepArgs a, int nb) {
+    if (a.check) {
+        finish_check(a, nb);
+    } else if (threadIdx.x == 0) {
+        a.sc->vscale = 1.0;
     }
 }
 
@@ -183,11 +186,12 @@ __global__ __launch_bounds__(kBlock) void k_check(StepArgs a) {
         a.part_d[2 * blockIdx.x] = b1;
         a.part_d[2 * blockIdx.x + 1] = b2;
     }
-    if (last_block(&a.sc->counter[3])) {
-        const double keep = a.sc->vscale;
-        finish_check(a);
-        if (threadIdx.x == 0) a.sc->vscale = keep;
-    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_finish(StepArgs a, int nb) {
+    const double keep = a.sc->vscale;
+    finish_check(a, nb);
+    if (threadIdx.x == 0) a.sc->vscale = keep;
 }
 
 __global__ void k_apply_scale(long n, double4* vel, Scalars* sc) {
@@ -218,17 +222,21 @@ __global__ __launch_bounds__(kBlock) void k_kinetic(long n, const double4* vel, 
     }
     const double b = block_reduce(ke, sm, wave_sum);
     if (threadIdx.x == 0) part[blockIdx.x] = b;
-    if (last_block(&sc->counter[4])) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < static_cast<int>(gridDim.x); k += kBlock) s += part[k];
-        red[threadIdx.x] = s;
+    (void)red;
+    (void)sc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_kinetic_finish(const double* part, int nb, Scalars* sc) {
+    __shared__ double red[kBlock];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) s += part[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
-        for (int off = kBlock / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) sc->kinetic = red[0];
     }
+    if (threadIdx.x == 0) sc->kinetic = red[0];
 }
 
 // Mass and momentum sums (core/system.cpp:137-164) into out4 = {M, Px, Py, Pz}.
@@ -249,21 +257,26 @@ __global__ __launch_bounds__(kBlock) void k_momentum(long n, const double4* vel,
         const double b = block_reduce(q[k], sm, wave_sum);
         if (threadIdx.x == 0) part[4 * blockIdx.x + k] = b;
     }
-    if (last_block(ticket)) {
-        for (int k = 0; k < 4; ++k) {
-            double s = 0.0;
-            for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kBlock) s += part[4 * b + k];
-            red[k][threadIdx.x] = s;
-        }
-        __syncthreads();
-        for (int off = kBlock / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off)
-                for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 4; ++k) out4[k] = red[k][0];
+    (void)red;
+    (void)out4;
+    (void)ticket;
+}
+
+__global__ __launch_bounds__(kBlock) void k_momentum_finish(const double* part, int nb, double* out4) {
+    __shared__ double red[4][kBlock];
+    for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += part[4 * b + k];
+        red[k][threadIdx.x] = s;
     }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) out4[k] = red[k][0];
 }
 
 __global__ void k_sub_com(long n, double4* vel, const double* out4) {
@@ -366,10 +379,12 @@ void launch_pre_step(const StepArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_pre<2>, g, dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_pre<0>, g, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_pre_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(g.x));
 }
 
 void launch_check(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_check, dim3(grid_of(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_check_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(grid_of(a.n)));
 }
 
 void launch_apply_scale(long n, double4* vel, Scalars* sc, hipStream_t s) {
@@ -379,12 +394,13 @@ void launch_apply_scale(long n, double4* vel, Scalars* sc, hipStream_t s) {
 
 void launch_kinetic(long n, const double4* vel, double* part, Scalars* sc, hipStream_t s) {
     hipLaunchKernelGGL(k_kinetic, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, part, sc);
+    hipLaunchKernelGGL(k_kinetic_finish, dim3(1), dim3(kBlock), 0, s, part, static_cast<int>(grid_of(n)), sc);
 }
 
 void launch_remove_com(long n, double4* vel, double* part, double* out4, hipStream_t s) {
-    // out4[4..] holds the ticket of the momentum reduction
-    unsigned int* ticket = reinterpret_cast<unsigned int*>(out4 + 4);
-    hipLaunchKernelGGL(k_momentum, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, part, out4, ticket);
+    hipLaunchKernelGGL(k_momentum, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, part, out4,
+                       static_cast<unsigned int*>(nullptr));
+    hipLaunchKernelGGL(k_momentum_finish, dim3(1), dim3(kBlock), 0, s, part, static_cast<int>(grid_of(n)), out4);
     hipLaunchKernelGGL(k_sub_com, dim3(grid_of(n)), dim3(kBlock), 0, s, n, vel, out4);
 }
 

@@ -38,6 +38,8 @@ struct NeighArgs {
     const int* flag;
 };
 void launch_rebuild(const NeighArgs& a, hipStream_t s);
+// Raises the dynamic-LDS limit of the tiled list build (once per process).
+hipError_t neigh_init();
 
 // LJ pair kernel over the full Verlet list.
 struct LJConst {
@@ -71,6 +73,9 @@ struct ForceArgs {
     void* thermo;       // md_thermo* record for this step, or null
 };
 void launch_lj_force(const ForceArgs& a, hipStream_t s);
+// One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
+// thermo row and NHC chain; launched right after launch_lj_force.
+void launch_lj_finish(const ForceArgs& a, hipStream_t s);
 // One chain propagation of length delta from sc->kinetic; the velocity
 // rescale is left pending in sc->vscale.
 void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s);

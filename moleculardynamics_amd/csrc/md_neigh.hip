@@ -101,24 +101,41 @@ __global__ void k_place(const int* __restrict__ cell_of, long n, int* __restrict
     }
 }
 
-// Within each cell, order the slots by upload index (insertion sort; cells
-// hold a few tens of atoms). Makes the new order a pure function of the
-// positions, independent of atomic arrival order.
-__global__ void k_sort_cells(const int* __restrict__ cell_start, long ncell,
-                             const int* __restrict__ orig, int* __restrict__ order,
-                             const int* flag) {
+// Within each cell, order the slots by upload index, which makes the new
+// order a pure function of the positions (independent of atomic arrival
+// order). One wave per cell: with up to 64 entries each lane holds one and
+// its rank is the number of smaller keys (upload indices are distinct);
+// larger cells fall back to an insertion sort by one lane.
+__global__ __launch_bounds__(kBlock) void k_sort_cells(const int* __restrict__ cell_start, long ncell,
+                                                       const int* __restrict__ orig,
+                                                       int* __restrict__ order, const int* flag) {
     if (*flag == 0) return;
-    GRID_STRIDE(c, ncell) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
+    for (long c = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; c < ncell;
+         c += waves) {
         const int b = cell_start[c], e = cell_start[c + 1];
-        for (int i = b + 1; i < e; ++i) {
-            const int v = order[i];
-            const int key = orig[v];
-            int j = i - 1;
-            while (j >= b && orig[order[j]] > key) {
-                order[j + 1] = order[j];
-                --j;
+        const int cnt = e - b;
+        if (cnt <= kWave) {
+            int v = 0, key = 0x7fffffff;
+            if (lane < cnt) {
+                v = order[b + lane];
+                key = orig[v];
             }
-            order[j + 1] = v;
+            int rank = 0;
+            for (int j = 0; j < cnt; ++j) rank += (__shfl(key, j, kWave) < key) ? 1 : 0;
+            if (lane < cnt) order[b + rank] = v;
+        } else if (lane == 0) {
+            for (int i = b + 1; i < e; ++i) {
+                const int v = order[i];
+                const int key = orig[v];
+                int j = i - 1;
+                while (j >= b && orig[order[j]] > key) {
+                    order[j + 1] = order[j];
+                    --j;
+                }
+                order[j + 1] = v;
+            }
         }
     }
 }
@@ -201,6 +218,133 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     }
 }
 
+// Tiled commit + build for a usable grid. A workgroup owns a run of up to
+// kSegMax consecutive cells along x in one (y, z) row; after the sort those
+// cells' atoms are one contiguous slot range. The candidates of the whole
+// run -- cells x0-1 .. x1 in the 9 rows (y+-1, z+-1) -- are staged once in
+// LDS (x, y, z, upload index, slot), so every pair test reads LDS instead of
+// gathering from L2. The exact test and the orientation are those of
+// k_commit_build. A run whose candidates exceed the LDS capacity reads them
+// from global memory instead.
+constexpr int kSegMax = 16;
+constexpr int kTileLdsCap = 3072;                   // staged candidates per run
+constexpr int kTileLdsBytes = kTileLdsCap * 32;     // x, y, z (FP64) + upload index + slot
+
+__global__ __launch_bounds__(kBlock) void k_commit_build_tiles(
+    long n, Grid g, Box box, double rl2, int seg, int nseg, int lds_cap,
+    const double4* __restrict__ pos2, const double4* __restrict__ vel2,
+    const double4* __restrict__ frc2, const int* __restrict__ orig2, const int* __restrict__ cell2,
+    const int* __restrict__ cell_start, double4* __restrict__ pos, double4* __restrict__ vel,
+    double4* __restrict__ frc, double4* __restrict__ ref, int* __restrict__ orig,
+    int* __restrict__ slot, int* __restrict__ cell_of, uint32_t* __restrict__ nbr,
+    int* __restrict__ ncnt, int cap, Scalars* sc, const int* flag) {
+    if (*flag == 0) return;
+    extern __shared__ double lds[];
+    double* lx = lds;
+    double* ly = lx + lds_cap;
+    double* lz = ly + lds_cap;
+    int* lo = reinterpret_cast<int*>(lz + lds_cap);
+    int* lg = lo + lds_cap;
+    __shared__ int coff[9][kSegMax + 2], ccnt[9][kSegMax + 2], cbeg[9][kSegMax + 2];
+    __shared__ int total;
+
+    const int sgi = blockIdx.x % nseg;
+    const int y = (blockIdx.x / nseg) % g.nc[1];
+    const int z = blockIdx.x / (nseg * g.nc[1]);
+    const int x0 = sgi * seg;
+    const int x1 = min(x0 + seg, g.nc[0]);
+    const int ncl = x1 - x0 + 2;  // candidate cells per row
+    if (x0 >= g.nc[0]) return;
+
+    // candidate cell table (one thread), then the staging copy
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int r = 0; r < 9; ++r) {
+            const int yy = (y + (r % 3) - 1 + g.nc[1]) % g.nc[1];
+            const int zz = (z + (r / 3) - 1 + g.nc[2]) % g.nc[2];
+            for (int k = 0; k < ncl; ++k) {
+                const int xx = (x0 - 1 + k + g.nc[0]) % g.nc[0];
+                const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
+                const int b = cell_start[c], e = cell_start[c + 1];
+                coff[r][k] = off;
+                ccnt[r][k] = e - b;
+                cbeg[r][k] = b;
+                off += e - b;
+            }
+        }
+        total = off;
+    }
+    __syncthreads();
+    const bool staged = total <= lds_cap;
+    if (staged) {
+        for (int r = 0; r < 9; ++r)
+            for (int k = 0; k < ncl; ++k) {
+                const int b = cbeg[r][k], c = ccnt[r][k], o = coff[r][k];
+                for (int m = threadIdx.x; m < c; m += blockDim.x) {
+                    const double4 p = pos2[b + m];
+                    lx[o + m] = p.x;
+                    ly[o + m] = p.y;
+                    lz[o + m] = p.z;
+                    lo[o + m] = orig2[b + m];
+                    lg[o + m] = b + m;
+                }
+            }
+    }
+    __syncthreads();
+
+    const int home_cell0 = x0 + g.nc[0] * (y + g.nc[1] * z);
+    const int hb = cell_start[home_cell0];
+    const int he = cell_start[home_cell0 + (x1 - x0)];
+    for (int t = hb + threadIdx.x; t < he; t += blockDim.x) {
+        const double4 pi = pos2[t];
+        const int oi = orig2[t];
+        const int ci = cell2[t];
+        pos[t] = pi;
+        ref[t] = pi;
+        vel[t] = vel2[t];
+        frc[t] = frc2[t];
+        orig[t] = oi;
+        slot[oi] = t;
+        cell_of[t] = ci;
+        const int kx = (ci % g.nc[0]) - x0;  // home cell index in the run: 0..seg-1
+        int cnt = 0;
+        for (int r = 0; r < 9; ++r) {
+            for (int kk = 0; kk < 3; ++kk) {
+                const int k = kx + kk;
+                const int c = ccnt[r][k];
+                if (staged) {
+                    const int o = coff[r][k];
+                    for (int m = o; m < o + c; ++m) {
+                        const int j = lg[m];
+                        if (j == t) continue;
+                        const bool flip = oi > lo[m];
+                        double dx, dy, dz;
+                        pair_delta(box, pi, make_double4(lx[m], ly[m], lz[m], 0.0), flip, dx, dy, dz);
+                        if (norm2_ref(dx, dy, dz) < rl2) {
+                            if (cnt < cap) nbr[nbr_index(t, cnt, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+                            ++cnt;
+                        }
+                    }
+                } else {
+                    const int b = cbeg[r][k];
+                    for (int j = b; j < b + c; ++j) {
+                        if (j == t) continue;
+                        const bool flip = oi > orig2[j];
+                        double dx, dy, dz;
+                        pair_delta(box, pi, pos2[j], flip, dx, dy, dz);
+                        if (norm2_ref(dx, dy, dz) < rl2) {
+                            if (cnt < cap) nbr[nbr_index(t, cnt, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+        ncnt[t] = cnt;
+        if (cnt > cap) atomicOr(&sc->overflow, 1);
+    }
+}
+
 inline unsigned grid_capped(long n) {
     const long g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g < static_cast<long>(kGridCap) ? (g > 0 ? g : 1) : kGridCap);
@@ -219,9 +363,24 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_cells, gc, b, 0, s, a.cell_start, ncell, a.orig, a.order, a.flag);
     hipLaunchKernelGGL(k_permute, gn, b, 0, s, a.order, n, a.pos, a.vel, a.frc, a.orig, a.cell_of,
                        a.pos2, a.vel2, a.frc2, a.orig2, a.cell2, a.flag);
-    hipLaunchKernelGGL(k_commit_build, gn, b, 0, s, n, a.grid, a.box, a.rl2, a.pos2, a.vel2,
-                       a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref, a.orig,
-                       a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
+    if (a.grid.usable) {
+        const int nseg = (a.grid.nc[0] + 11) / 12;
+        const int seg = (a.grid.nc[0] + nseg - 1) / nseg;
+        const unsigned nwg = static_cast<unsigned>(nseg) * a.grid.nc[1] * a.grid.nc[2];
+        hipLaunchKernelGGL(k_commit_build_tiles, dim3(nwg), b, kTileLdsBytes, s, n, a.grid, a.box,
+                           a.rl2, seg, nseg, kTileLdsCap, a.pos2, a.vel2, a.frc2, a.orig2, a.cell2,
+                           a.cell_start, a.pos, a.vel, a.frc, a.ref, a.orig, a.slot, a.cell_of,
+                           a.nbr, a.ncnt, a.cap, a.sc, a.flag);
+    } else {
+        hipLaunchKernelGGL(k_commit_build, gn, b, 0, s, n, a.grid, a.box, a.rl2, a.pos2, a.vel2,
+                           a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref, a.orig,
+                           a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
+    }
+}
+
+hipError_t neigh_init() {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_commit_build_tiles),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsBytes);
 }
 
 }  // namespace mdk
