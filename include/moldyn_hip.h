@@ -330,6 +330,77 @@ md_status md_water_download(md_water* w, double* x, double* y, double* z, double
 md_status md_water_set_state(md_water* w, const double* x, const double* y, const double* z,
                              const double* vx, const double* vy, const double* vz);
 
+/* ---- per-type-pair LJ, parameter derivatives and the adjoint ---------------
+ * No reference counterpart: the reference LJ is single-type
+ * (forcefield/lennard_jones.hpp:26). The pair terms follow
+ * LennardJones::computePairSum (lennard_jones.cpp:50-88) with sigma, epsilon
+ * and the cutoff taken per unordered type pair; the list is the Verlet list
+ * with skin (neighbor/verlet_list.cpp:8-73); the trajectory is
+ * VelocityVerlet::step (integrate/velocity_verlet.hpp:48-74). Derivatives are
+ * taken at fixed cutoffs. Arrays are in upload order; up to 4 types.
+ * Tables of ntypes*ntypes entries are row-major and symmetric. Per-type-pair
+ * outputs use the canonical order of unordered pairs (a <= b), row-major:
+ * (0,0), (0,1), ..., (0,nt-1), (1,1), ... */
+typedef struct md_mix md_mix;
+
+md_status md_mix_create(void* hip_stream, md_mix** out);
+md_status md_mix_destroy(md_mix* m);
+const char* md_mix_last_error(const md_mix* m);
+/* type[i] in 0..3; mass (or NULL: 1) and velocities (or NULL: 0). */
+md_status md_mix_upload(md_mix* m, long n, const double* L, const double* x, const double* y,
+                        const double* z, const double* vx, const double* vy, const double* vz,
+                        const double* mass, const int* type);
+/* LennardJones(sigma_ab, epsilon_ab, cutoff_ab, Truncated | LinearForceShift)
+ * for every type pair (lennard_jones.cpp:32-48). */
+md_status md_mix_set_params(md_mix* m, int ntypes, const double* sigma, const double* epsilon,
+                            const double* cutoff, int linear_force_shift);
+/* Verlet skin of the list (default 0.3); the list radius is max cutoff + skin. */
+md_status md_mix_set_skin(md_mix* m, double skin);
+/* Forces into the device state; E and W (raw sum of w, as EnergyVirial).
+ * tables (or NULL) receives 6 values per type pair:
+ * {E_ab, W_ab, dE/deps_ab, dE/dsigma_ab, dW/deps_ab, dW/dsigma_ab}. */
+md_status md_mix_forces(md_mix* m, double* energy, double* virial, double* tables);
+/* Vector-Jacobian product at the current positions for a cotangent g on the
+ * forces (NULL: zero) and weights on E and W:
+ *   o = (dF/dx)^T g + w_energy dE/dx + w_virial dW/dx   ((dF/dx) = -Hessian)
+ *   dtheta (2 per type pair) = {g . dF/dp + w_energy dE/dp + w_virial dW/dp}
+ *   for p = eps_ab, sigma_ab. */
+md_status md_mix_force_vjp(md_mix* m, double w_energy, double w_virial, const double* gx,
+                           const double* gy, const double* gz, double* ox, double* oy, double* oz,
+                           double* dtheta);
+/* Steepest descent: nsteps moves x <- wrap(x + gamma F) with
+ * gamma = min(step, max_disp / max_i |F_i|), the list rebuilt when the
+ * displacement check asks for it. energies (or NULL) receives nsteps + 1
+ * values (before the first move and after each). */
+md_status md_mix_minimize(md_mix* m, int nsteps, double step, double max_disp, double* energies);
+/* nsteps velocity-Verlet steps, checkpoints every checkpoint_every steps
+ * (kept for md_mix_adjoint); energies / virials (or NULL) receive the
+ * nsteps + 1 frame values E(x_t), W(x_t), t = 0..nsteps. */
+md_status md_mix_run(md_mix* m, double dt, int nsteps, int checkpoint_every, double* energies,
+                     double* virials);
+/* Adjoint of the last md_mix_run for
+ *   L = sum_t [w_energy E(x_t) + w_virial W(x_t)] + g . x_T + gv . v_T
+ * (g, gv or NULL for zero). dtheta (2 per type pair) = {dL/deps_ab,
+ * dL/dsigma_ab}; d* (or NULL) = dL/dx_0, dL/dv_0. Frames are recomputed from
+ * the checkpoints; the final state of the run is restored afterwards. */
+md_status md_mix_adjoint(md_mix* m, double w_energy, double w_virial, const double* gx,
+                         const double* gy, const double* gz, const double* gvx, const double* gvy,
+                         const double* gvz, double* dtheta, double* dx, double* dy, double* dz,
+                         double* dvx, double* dvy, double* dvz);
+md_status md_mix_download(md_mix* m, double* x, double* y, double* z, double* vx, double* vy,
+                          double* vz, double* fx, double* fy, double* fz);
+md_status md_mix_set_state(md_mix* m, const double* x, const double* y, const double* z,
+                           const double* vx, const double* vy, const double* vz);
+md_status md_mix_list_builds(const md_mix* m, long* builds);
+/* Config-5 input builder: uniform random positions (u - 1/2) L per
+ * coordinate (the recipe of tests/test_analysis.cpp:24-28 with
+ * Rng::uniform), then Maxwell-Boltzmann velocities normal() sqrt(T/m) from
+ * the same stream and the centre-of-mass motion removed
+ * (core/system.cpp:147-164). mass may be NULL (unit masses). */
+md_status md_mix_random_state(long n, double box_length, uint64_t seed, double temperature,
+                              const double* mass, double* x, double* y, double* z, double* vx,
+                              double* vy, double* vz);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* When enabled, every pair-kernel launch is bracketed by HIP events on the

@@ -21,7 +21,8 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libmoldyn_hip.so")
 
 SOURCES = ["md_capi.hip", "md_neigh.hip", "md_force.hip", "md_integrate.hip", "md_batch.hip",
-           "md_batch_capi.hip", "md_water.hip", "md_water_capi.hip"]
+           "md_batch_capi.hip", "md_water.hip", "md_water_capi.hip", "md_mix.hip",
+           "md_mix_capi.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [

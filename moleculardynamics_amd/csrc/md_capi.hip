@@ -39,6 +39,7 @@ struct md_ctx {
     mdk::Grid grid{};
     long ncell = 0, ncell_alloc = 0;
     int *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *order = nullptr;
+    int* tile_sum = nullptr;
     uint32_t* nbr = nullptr;
     int* ncnt = nullptr;
     long npad = 0;
@@ -204,6 +205,7 @@ md_status alloc_neigh(md_ctx* c) {
         MD_HIP(c, hipMemset(c->cell_count, 0, c->ncell * sizeof(int)));
         MD_HIP(c, alloc(c->cell_start, c->ncell + 1));
         MD_HIP(c, alloc(c->cell_fill, c->ncell));
+        MD_HIP(c, alloc(c->tile_sum, mdk::scan_tiles(c->ncell)));
         c->ncell_alloc = c->ncell;
     }
     c->npad = (c->n + 63) / 64 * 64;
@@ -227,7 +229,7 @@ mdk::NeighArgs neigh_args(md_ctx* c) {
     a.pos2 = c->pos2; a.vel2 = c->vel2; a.frc2 = c->frc2;
     a.orig2 = c->orig2; a.cell2 = c->cell2;
     a.cell_of = c->cell_of; a.cell_count = c->cell_count; a.cell_start = c->cell_start;
-    a.cell_fill = c->cell_fill; a.order = c->order;
+    a.cell_fill = c->cell_fill; a.order = c->order; a.tile_sum = c->tile_sum;
     a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
     a.sc = c->sc;
     a.flag = &c->sc->rebuild_flag;
@@ -303,7 +305,7 @@ md_status md_ctx_create(void* hip_stream, md_ctx** out) {
     *out = nullptr;
     md_ctx* c = new (std::nothrow) md_ctx();
     if (c == nullptr) return MD_ERR_RUNTIME;
-    if (hipGetDevice(&c->device) != hipSuccess || mdk::neigh_init() != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess) {
         delete c;
         return MD_ERR_HIP;
     }
@@ -347,6 +349,7 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->orig); release(c->slot); release(c->orig2); release(c->cell2); release(c->cell_of);
     release(c->stage);
     release(c->cell_count); release(c->cell_start); release(c->cell_fill); release(c->order);
+    release(c->tile_sum);
     release(c->nbr); release(c->ncnt);
     release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
     release(c->ci_atoms);

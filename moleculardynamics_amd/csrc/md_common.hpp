@@ -26,6 +26,22 @@ constexpr uint32_t kIndexMask = 0x7fffffffu;
 constexpr double kFixScale = 4294967296.0;          // 2^32: fixed-point resolution 2.3e-10
 constexpr double kFixInv = 1.0 / 4294967296.0;
 constexpr double kMagic = 6755399441055744.0;        // 1.5 * 2^52
+constexpr unsigned kXcd = 8;                         // accelerator dies (each with its own L2)
+
+// XCD-aware workgroup order. The dispatcher hands workgroup b to die b % 8,
+// so consecutive workgroups (neighbouring atoms in cell order) would land on
+// eight different L2s. Launched with a grid that is a multiple of 8, logical
+// block (b % 8) * (grid / 8) + b / 8 gives every die one contiguous run of
+// atoms, whose cell neighbourhoods then share that die's L2.
+inline unsigned xcd_grid(long n) {
+    const long g = (n + kBlock - 1) / kBlock;
+    const long r = ((g > 0 ? g : 1) + kXcd - 1) / kXcd * kXcd;
+    return static_cast<unsigned>(r);
+}
+__device__ __forceinline__ long xcd_block() {
+    const unsigned per = gridDim.x / kXcd;
+    return static_cast<long>(blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
 
 // Device scalars shared by the step kernels (one instance per context).
 struct Scalars {
@@ -38,7 +54,6 @@ struct Scalars {
     int overflow;                    // neighbour row overflow seen
     int fix_saturated;               // fixed-point conversion saturated
     int pad;
-    unsigned int counter[8];         // last-workgroup tickets, one per kernel family
 };
 
 // Nose-Hoover chain state (MTK96), held on the device so a whole run stays
@@ -120,16 +135,38 @@ __device__ __forceinline__ double norm2_dot(double ax, double ay, double az, dou
 __device__ __forceinline__ void pair_delta(const Box& b, const double4& pa, const double4& pb,
                                            bool flip, double& dx, double& dy, double& dz) {
 #pragma clang fp contract(off)
-    if (flip) {
-        dx = pb.x - pa.x;
-        dy = pb.y - pa.y;
-        dz = pb.z - pa.z;
-    } else {
-        dx = pa.x - pb.x;
-        dy = pa.y - pb.y;
-        dz = pa.z - pb.z;
-    }
+    // b - a is exactly -(a - b) under round-to-nearest, so a sign select
+    // replaces the branch with the same value
+    dx = pa.x - pb.x;
+    dy = pa.y - pb.y;
+    dz = pa.z - pb.z;
+    dx = flip ? -dx : dx;
+    dy = flip ? -dy : dy;
+    dz = flip ? -dz : dz;
     min_image(b, dx, dy, dz);
+}
+
+// Row-oriented minimum image d = r_i - r_j with the bits of pair_delta's
+// result (negated when flip): when every component lies inside the fast band
+// (|v| <= 0.4999999 L) the fold is the identity, which is odd, so no
+// orientation is needed; otherwise the pair is oriented by upload index,
+// folded exactly and turned back to the row's orientation. Squares, and so
+// r2 and every cutoff decision, are the same from both rows.
+__device__ __forceinline__ void row_delta(const Box& b, const double4& pi, const double4& pj,
+                                          bool flip, double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+    if (!(fabs(dx) <= b.lo[0] && fabs(dy) <= b.lo[1] && fabs(dz) <= b.lo[2])) {
+        dx = flip ? -dx : dx;
+        dy = flip ? -dy : dy;
+        dz = flip ? -dz : dz;
+        min_image(b, dx, dy, dz);
+        dx = flip ? -dx : dx;
+        dy = flip ? -dy : dy;
+        dz = flip ? -dz : dz;
+    }
 }
 
 // Slot k of atom t in the wave-blocked neighbour layout: every group of 64
@@ -206,23 +243,6 @@ __device__ __forceinline__ void wave_top2(double& t1, double& t2) {
         const double o2 = __shfl_xor(t2, off, kWave);
         top2_merge(t1, t2, o1, o2);
     }
-}
-
-// Last-workgroup election: every workgroup publishes its partials, then the
-// one that takes the final ticket reduces them in a fixed order. Returns
-// true in all threads of the last workgroup (and resets the ticket).
-__device__ __forceinline__ bool last_block(unsigned int* ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int t = atomicAdd(ticket, 1u);
-        is_last = (t == gridDim.x - 1);
-        if (is_last) *ticket = 0u;
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
 }
 
 // Float to 64-bit fixed point (scale 2^32), round to nearest even, via the

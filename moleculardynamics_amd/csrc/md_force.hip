@@ -141,7 +141,7 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
                                        const double4& pj, Acc64& q) {
     const bool flip = (jj & kFlipBit) != 0;
     double dx, dy, dz;
-    pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+    row_delta(a.box, pi, pj, flip, dx, dy, dz);
     const double r2 = norm2_ref(dx, dy, dz);
     if (r2 < a.lj.rc2) {
         // hardware reciprocal refined by two Newton steps (within an ulp of 1/r2)
@@ -162,24 +162,44 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
         q.e += u;
         q.w += ww;
         q.np += flip ? 0 : 1;
-        const double c = flip ? -(ww * inv) : ww * inv;
+        const double c = ww * inv;
         q.fx += c * dx;
         q.fy += c * dy;
         q.fz += c * dz;
     }
 }
 
+// 64-bit fixed point (scale 2^32) by biased accumulation: every term adds
+// the bit pattern of fma(v, 2^32, 1.5 * 2^52), whose value is
+// round(v * 2^32) + 1.5 * 2^52 (ties to even) while |v| < 2^19; the bias is
+// removed once per accumulator (n terms) at the end, in 64-bit modular
+// arithmetic. One conversion, one FMA and one 64-bit add per term, no
+// branches; the largest |term| is tracked and checked against 2^19 at the end.
 struct AccFix {
-    long long fx = 0, fy = 0, fz = 0, e = 0, w = 0, np = 0;
-    int sat = 0;
+    unsigned long long fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    unsigned long long n = 0;
+    long long np = 0;
+    float big = 0.0f;
 };
+
+__device__ __forceinline__ unsigned long long fix_biased(float v) {
+    return static_cast<unsigned long long>(
+        __double_as_longlong(fma(static_cast<double>(v), kFixScale, kMagic)));
+}
+
+__device__ __forceinline__ long long fix_value(unsigned long long acc, unsigned long long n) {
+    const unsigned long long bias = static_cast<unsigned long long>(__double_as_longlong(kMagic));
+    return static_cast<long long>(acc - n * bias);
+}
+
+constexpr float kFixLimit = 524288.0f;  // 2^19
 
 template <bool LFS>
 __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, uint32_t jj,
                                        const double4& pj, AccFix& q) {
     const bool flip = (jj & kFlipBit) != 0;
     double dx, dy, dz;
-    pair_delta(a.box, pi, pj, flip, dx, dy, dz);
+    row_delta(a.box, pi, pj, flip, dx, dy, dz);
     const double r2d = norm2_ref(dx, dy, dz);
     float inv, u, ww;
     if (lj_terms_f32(r2d, a.lj.rc2, a.lj.sig2f, a.lj.epsf, inv, u, ww)) {
@@ -192,13 +212,17 @@ __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, ui
             u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
             ww = ww + r * a.lj.dudr_rcf;
         }
-        q.e += to_fix(u, &q.sat);
-        q.w += to_fix(ww, &q.sat);
+        const float c = ww * inv;
+        const float tx = c * xf, ty = c * yf, tz = c * zf;
+        q.e += fix_biased(u);
+        q.w += fix_biased(ww);
+        q.fx += fix_biased(tx);
+        q.fy += fix_biased(ty);
+        q.fz += fix_biased(tz);
+        q.n += 1;
         q.np += flip ? 0 : 1;
-        const float c = flip ? -(ww * inv) : ww * inv;
-        q.fx += to_fix(c * xf, &q.sat);
-        q.fy += to_fix(c * yf, &q.sat);
-        q.fz += to_fix(c * zf, &q.sat);
+        q.big = fmaxf(q.big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
+                                   fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
     }
 }
 
@@ -234,7 +258,9 @@ template <bool LFS, bool MIXED, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
     __shared__ long long sm_l[kBlock / kWave];
-    const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long blk = xcd_block();
+    if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
+    const long t = blk * kBlock + threadIdx.x;
     double e = 0.0, w = 0.0, ke = 0.0;
     long long ef = 0, wf = 0, np = 0;
     if (t < a.n) {
@@ -253,12 +279,12 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         } else {
             AccFix q;
             walk_row(a, t, cnt, pi, q, pair32<LFS>);
-            if (q.sat) atomicOr(&a.sc->fix_saturated, 1);
-            fx = static_cast<double>(q.fx) * kFixInv;
-            fy = static_cast<double>(q.fy) * kFixInv;
-            fz = static_cast<double>(q.fz) * kFixInv;
-            ef = q.e;
-            wf = q.w;
+            if (!(q.big < kFixLimit)) atomicOr(&a.sc->fix_saturated, 1);
+            fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
+            fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
+            fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
+            ef = fix_value(q.e, q.n);
+            wf = fix_value(q.w, q.n);
             np = q.np;
         }
         a.frc[t] = make_double4(fx, fy, fz, 0.0);
@@ -280,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
     const long long bnp = block_reduce(np, sm_l, wave_sum_ll);
     if (threadIdx.x == 0) {
-        const int b = blockIdx.x;
+        const long b = blk;
         a.part_d[3 * b + 0] = be;
         a.part_d[3 * b + 1] = bw;
         a.part_d[3 * b + 2] = bk;
@@ -493,7 +519,7 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
 }
 
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
-    const dim3 g(grid_of(a.n));
+    const dim3 g(xcd_grid(a.n));
     if (a.lj.lfs) {
         if (a.mixed) dispatch_kick<true, true>(a, g, s);
         else dispatch_kick<true, false>(a, g, s);

@@ -30,6 +30,7 @@ struct NeighArgs {
     double4 *pos2, *vel2, *frc2;
     int *orig2, *cell2;
     int *cell_of, *cell_count, *cell_start, *cell_fill, *order;
+    int* tile_sum;  // scan_tiles(cell count) entries
     uint32_t* nbr;
     int* ncnt;
     long npad;
@@ -38,8 +39,8 @@ struct NeighArgs {
     const int* flag;
 };
 void launch_rebuild(const NeighArgs& a, hipStream_t s);
-// Raises the dynamic-LDS limit of the tiled list build (once per process).
-hipError_t neigh_init();
+// Number of scan tiles (entries of NeighArgs::tile_sum) for ncell cells.
+long scan_tiles(long ncell);
 
 // LJ pair kernel over the full Verlet list.
 struct LJConst {

@@ -10,11 +10,12 @@
 // an atom's cell (cells at least rc + skin wide, at least 3 per side), or from
 // every atom when the grid is unusable (the reference's brute-force fallback).
 //
-// Six kernels: bin -> scan -> place -> sort-within-cell -> permute ->
-// commit+build. Every kernel takes the device rebuild flag and returns at
-// once when it is 0, so a multi-step run launches the rebuild
-// unconditionally and keeps the decision on the device; the kernels use
-// grid-stride loops over a capped grid so a skipped rebuild costs little.
+// Seven kernels: bin -> tile count -> scan -> place -> sort-within-cell ->
+// permute -> commit+build. Every kernel takes the device rebuild flag and
+// returns at once when it is 0, so a multi-step run launches the rebuild
+// unconditionally and keeps the decision on the device. The O(N) passes use
+// grid-stride loops over a capped grid so a skipped rebuild costs little;
+// commit+build runs one thread per slot in the XCD-aware block order.
 // Invariant: cell_count is all zero between rebuilds (k_scan clears it).
 #include "md_common.hpp"
 #include "md_kernels.hpp"
@@ -55,39 +56,67 @@ __global__ void k_bin(const double4* __restrict__ pos, long n, Grid g, int* __re
     }
 }
 
-// Exclusive scan of cell_count into cell_start (and the fill cursor), one
-// workgroup of 1024 threads each owning a contiguous chunk; clears
-// cell_count afterwards and counts the build.
-__global__ __launch_bounds__(1024) void k_scan(int* __restrict__ cell_count, long ncell,
-                                               int* __restrict__ cell_start,
-                                               int* __restrict__ cell_fill, Scalars* sc,
-                                               const int* flag) {
+// Exclusive scan of cell_count into cell_start (and the fill cursor) in two
+// passes over tiles of kScanTile cells: k_tile_count sums each tile, then
+// k_scan gives every tile its offset (the sum of the tile sums before it) and
+// scans inside the tile with wave prefix sums. k_scan clears cell_count and
+// its last tile writes the total and counts the build.
+constexpr int kScanTile = 1024;
+
+__device__ __forceinline__ int wave_prefix(int x, int lane) {
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    return x;  // inclusive
+}
+
+__global__ __launch_bounds__(kScanTile) void k_tile_count(const int* __restrict__ cell_count,
+                                                          long ncell, int* __restrict__ tile_sum,
+                                                          const int* flag) {
     if (*flag == 0) return;
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const long chunk = (ncell + 1023) / 1024;
-    const long b = t * chunk;
-    const long e = b + chunk < ncell ? b + chunk : ncell;
-    int s = 0;
-    for (long c = b; c < e; ++c) s += cell_count[c];
-    part[t] = s;
+    __shared__ int ws[kScanTile / kWave];
+    const long c = static_cast<long>(blockIdx.x) * kScanTile + threadIdx.x;
+    int v = c < ncell ? cell_count[c] : 0;
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < kScanTile / kWave; ++k) s += ws[k];
+        tile_sum[blockIdx.x] = s;
     }
-    int run = t > 0 ? part[t - 1] : 0;
-    for (long c = b; c < e; ++c) {
-        const int k = cell_count[c];
-        cell_start[c] = run;
-        cell_fill[c] = run;
+}
+
+__global__ __launch_bounds__(kScanTile) void k_scan(int* __restrict__ cell_count, long ncell,
+                                                    const int* __restrict__ tile_sum,
+                                                    int* __restrict__ cell_start,
+                                                    int* __restrict__ cell_fill, Scalars* sc,
+                                                    const int* flag) {
+    if (*flag == 0) return;
+    __shared__ int ws[kScanTile / kWave];
+    __shared__ int base_s;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    if (wv == 0) {
+        int s = 0;
+        for (int k = lane; k < static_cast<int>(blockIdx.x); k += kWave) s += tile_sum[k];
+        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+        if (lane == 0) base_s = s;
+    }
+    const long c = static_cast<long>(blockIdx.x) * kScanTile + threadIdx.x;
+    const int k = c < ncell ? cell_count[c] : 0;
+    const int x = wave_prefix(k, lane);
+    if (lane == kWave - 1) ws[wv] = x;
+    __syncthreads();
+    int before = base_s + x - k;
+    for (int w = 0; w < wv; ++w) before += ws[w];
+    if (c < ncell) {
+        cell_start[c] = before;
+        cell_fill[c] = before;
         cell_count[c] = 0;
-        run += k;
     }
-    if (t == 1023) {
-        cell_start[ncell] = part[1023];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanTile - 1) {
+        cell_start[ncell] = before + k;
         sc->builds += 1;
     }
 }
@@ -162,6 +191,13 @@ __global__ void k_permute(const int* __restrict__ order, long n, const double4* 
 // grid is unusable) and keeps every partner within the list radius. The test
 // is the reference's exact FP64 expression on the pair oriented by upload
 // index, so both rows of a pair see the same bits.
+//
+// After the sort, cells x-1, x, x+1 of one (y, z) row are one contiguous slot
+// range unless the row wraps at x, so the 27 cells are scanned as 9 ranges
+// (3 single cells each for a wrapping row), in the order (z, y, x) and slot
+// order within a cell. Candidates are read four at a time -- positions and
+// upload indices of the four issued together -- so every thread keeps four
+// gathers in flight instead of one dependent load chain per candidate.
 __global__ __launch_bounds__(kBlock) void k_commit_build(
     long n, Grid g, Box box, double rl2, const double4* __restrict__ pos2,
     const double4* __restrict__ vel2, const double4* __restrict__ frc2,
@@ -171,178 +207,68 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     int* __restrict__ cell_of, uint32_t* __restrict__ nbr, int* __restrict__ ncnt, int cap,
     Scalars* sc, const int* flag) {
     if (*flag == 0) return;
-    GRID_STRIDE(t, n) {
-        const double4 pi = pos2[t];
-        const int oi = orig2[t];
-        const int ci = cell2[t];
-        pos[t] = pi;
-        ref[t] = pi;
-        vel[t] = vel2[t];
-        frc[t] = frc2[t];
-        orig[t] = oi;
-        slot[oi] = static_cast<int>(t);
-        cell_of[t] = ci;
-        int cnt = 0;
-        auto visit = [&](int j) {
-            if (j == t) return;
-            const bool flip = oi > orig2[j];
-            double dx, dy, dz;
-            pair_delta(box, pi, pos2[j], flip, dx, dy, dz);
-            if (norm2_ref(dx, dy, dz) < rl2) {
-                if (cnt < cap)
-                    nbr[nbr_index(t, cnt, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                ++cnt;
-            }
-        };
-        if (g.usable) {
-            const int cx = ci % g.nc[0];
-            const int cy = (ci / g.nc[0]) % g.nc[1];
-            const int cz = ci / (g.nc[0] * g.nc[1]);
-            for (int oz = -1; oz <= 1; ++oz) {
-                const int zz = (cz + oz + g.nc[2]) % g.nc[2];
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                    for (int ox = -1; ox <= 1; ++ox) {
-                        const int xx = (cx + ox + g.nc[0]) % g.nc[0];
-                        const int nb = xx + g.nc[0] * (yy + g.nc[1] * zz);
-                        const int e = cell_start[nb + 1];
-                        for (int j = cell_start[nb]; j < e; ++j) visit(j);
-                    }
-                }
-            }
-        } else {
-            for (int j = 0; j < static_cast<int>(n); ++j) visit(j);
+    const long tl = xcd_block() * kBlock + threadIdx.x;
+    if (tl >= n) return;
+    const int t = static_cast<int>(tl);
+    const double4 pi = pos2[t];
+    const int oi = orig2[t];
+    const int ci = cell2[t];
+    pos[t] = pi;
+    ref[t] = pi;
+    vel[t] = vel2[t];
+    frc[t] = frc2[t];
+    orig[t] = oi;
+    slot[oi] = t;
+    cell_of[t] = ci;
+    uint32_t* row = nbr + nbr_index(t, 0, cap);
+    int cnt = 0;
+    auto visit = [&](int j, const double4& q, int oj) {
+        const bool flip = oi > oj;
+        double dx, dy, dz;
+        row_delta(box, pi, q, flip, dx, dy, dz);
+        if (norm2_ref(dx, dy, dz) < rl2 && j != t) {
+            if (cnt < cap) row[cnt * 64] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+            ++cnt;
         }
-        ncnt[t] = cnt;
-        if (cnt > cap) atomicOr(&sc->overflow, 1);
-    }
-}
-
-// Tiled commit + build for a usable grid. A workgroup owns a run of up to
-// kSegMax consecutive cells along x in one (y, z) row; after the sort those
-// cells' atoms are one contiguous slot range. The candidates of the whole
-// run -- cells x0-1 .. x1 in the 9 rows (y+-1, z+-1) -- are staged once in
-// LDS (x, y, z, upload index, slot), so every pair test reads LDS instead of
-// gathering from L2. The exact test and the orientation are those of
-// k_commit_build. A run whose candidates exceed the LDS capacity reads them
-// from global memory instead.
-constexpr int kSegMax = 16;
-constexpr int kTileLdsCap = 3072;                   // staged candidates per run
-constexpr int kTileLdsBytes = kTileLdsCap * 32;     // x, y, z (FP64) + upload index + slot
-
-__global__ __launch_bounds__(kBlock) void k_commit_build_tiles(
-    long n, Grid g, Box box, double rl2, int seg, int nseg, int lds_cap,
-    const double4* __restrict__ pos2, const double4* __restrict__ vel2,
-    const double4* __restrict__ frc2, const int* __restrict__ orig2, const int* __restrict__ cell2,
-    const int* __restrict__ cell_start, double4* __restrict__ pos, double4* __restrict__ vel,
-    double4* __restrict__ frc, double4* __restrict__ ref, int* __restrict__ orig,
-    int* __restrict__ slot, int* __restrict__ cell_of, uint32_t* __restrict__ nbr,
-    int* __restrict__ ncnt, int cap, Scalars* sc, const int* flag) {
-    if (*flag == 0) return;
-    extern __shared__ double lds[];
-    double* lx = lds;
-    double* ly = lx + lds_cap;
-    double* lz = ly + lds_cap;
-    int* lo = reinterpret_cast<int*>(lz + lds_cap);
-    int* lg = lo + lds_cap;
-    __shared__ int coff[9][kSegMax + 2], ccnt[9][kSegMax + 2], cbeg[9][kSegMax + 2];
-    __shared__ int total;
-
-    const int sgi = blockIdx.x % nseg;
-    const int y = (blockIdx.x / nseg) % g.nc[1];
-    const int z = blockIdx.x / (nseg * g.nc[1]);
-    const int x0 = sgi * seg;
-    const int x1 = min(x0 + seg, g.nc[0]);
-    const int ncl = x1 - x0 + 2;  // candidate cells per row
-    if (x0 >= g.nc[0]) return;
-
-    // candidate cell table (one thread), then the staging copy
-    if (threadIdx.x == 0) {
-        int off = 0;
-        for (int r = 0; r < 9; ++r) {
-            const int yy = (y + (r % 3) - 1 + g.nc[1]) % g.nc[1];
-            const int zz = (z + (r / 3) - 1 + g.nc[2]) % g.nc[2];
-            for (int k = 0; k < ncl; ++k) {
-                const int xx = (x0 - 1 + k + g.nc[0]) % g.nc[0];
-                const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
-                const int b = cell_start[c], e = cell_start[c + 1];
-                coff[r][k] = off;
-                ccnt[r][k] = e - b;
-                cbeg[r][k] = b;
-                off += e - b;
-            }
+    };
+    auto range = [&](int b, int e) {
+        int j = b;
+        for (; j + 4 <= e; j += 4) {
+            const double4 q0 = pos2[j], q1 = pos2[j + 1], q2 = pos2[j + 2], q3 = pos2[j + 3];
+            const int o0 = orig2[j], o1 = orig2[j + 1], o2 = orig2[j + 2], o3 = orig2[j + 3];
+            visit(j, q0, o0);
+            visit(j + 1, q1, o1);
+            visit(j + 2, q2, o2);
+            visit(j + 3, q3, o3);
         }
-        total = off;
-    }
-    __syncthreads();
-    const bool staged = total <= lds_cap;
-    if (staged) {
-        for (int r = 0; r < 9; ++r)
-            for (int k = 0; k < ncl; ++k) {
-                const int b = cbeg[r][k], c = ccnt[r][k], o = coff[r][k];
-                for (int m = threadIdx.x; m < c; m += blockDim.x) {
-                    const double4 p = pos2[b + m];
-                    lx[o + m] = p.x;
-                    ly[o + m] = p.y;
-                    lz[o + m] = p.z;
-                    lo[o + m] = orig2[b + m];
-                    lg[o + m] = b + m;
-                }
-            }
-    }
-    __syncthreads();
-
-    const int home_cell0 = x0 + g.nc[0] * (y + g.nc[1] * z);
-    const int hb = cell_start[home_cell0];
-    const int he = cell_start[home_cell0 + (x1 - x0)];
-    for (int t = hb + threadIdx.x; t < he; t += blockDim.x) {
-        const double4 pi = pos2[t];
-        const int oi = orig2[t];
-        const int ci = cell2[t];
-        pos[t] = pi;
-        ref[t] = pi;
-        vel[t] = vel2[t];
-        frc[t] = frc2[t];
-        orig[t] = oi;
-        slot[oi] = t;
-        cell_of[t] = ci;
-        const int kx = (ci % g.nc[0]) - x0;  // home cell index in the run: 0..seg-1
-        int cnt = 0;
-        for (int r = 0; r < 9; ++r) {
-            for (int kk = 0; kk < 3; ++kk) {
-                const int k = kx + kk;
-                const int c = ccnt[r][k];
-                if (staged) {
-                    const int o = coff[r][k];
-                    for (int m = o; m < o + c; ++m) {
-                        const int j = lg[m];
-                        if (j == t) continue;
-                        const bool flip = oi > lo[m];
-                        double dx, dy, dz;
-                        pair_delta(box, pi, make_double4(lx[m], ly[m], lz[m], 0.0), flip, dx, dy, dz);
-                        if (norm2_ref(dx, dy, dz) < rl2) {
-                            if (cnt < cap) nbr[nbr_index(t, cnt, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                            ++cnt;
-                        }
-                    }
+        for (; j < e; ++j) visit(j, pos2[j], orig2[j]);
+    };
+    if (g.usable) {
+        const int nx = g.nc[0];
+        const int cx = ci % nx;
+        const int cy = (ci / nx) % g.nc[1];
+        const int cz = ci / (nx * g.nc[1]);
+        const bool inner = cx >= 1 && cx + 1 < nx;
+        for (int oz = -1; oz <= 1; ++oz) {
+            const int zz = (cz + oz + g.nc[2]) % g.nc[2];
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int yy = (cy + oy + g.nc[1]) % g.nc[1];
+                const int rb = nx * (yy + g.nc[1] * zz);
+                if (inner) {
+                    range(cell_start[rb + cx - 1], cell_start[rb + cx + 2]);
                 } else {
-                    const int b = cbeg[r][k];
-                    for (int j = b; j < b + c; ++j) {
-                        if (j == t) continue;
-                        const bool flip = oi > orig2[j];
-                        double dx, dy, dz;
-                        pair_delta(box, pi, pos2[j], flip, dx, dy, dz);
-                        if (norm2_ref(dx, dy, dz) < rl2) {
-                            if (cnt < cap) nbr[nbr_index(t, cnt, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                            ++cnt;
-                        }
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int c = rb + (cx + ox + nx) % nx;
+                        range(cell_start[c], cell_start[c + 1]);
                     }
                 }
             }
         }
-        ncnt[t] = cnt;
-        if (cnt > cap) atomicOr(&sc->overflow, 1);
+    } else {
+        range(0, static_cast<int>(n));
     }
+    ncnt[t] = cnt;
+    if (cnt > cap) atomicOr(&sc->overflow, 1);
 }
 
 inline unsigned grid_capped(long n) {
@@ -356,31 +282,20 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     const long n = a.n;
     const long ncell = a.grid.usable ? static_cast<long>(a.grid.nc[0]) * a.grid.nc[1] * a.grid.nc[2] : 1;
     const dim3 gn(grid_capped(n)), gc(grid_capped(ncell)), b(kBlock);
+    const dim3 gt(scan_tiles(ncell)), bt(kScanTile);
     hipLaunchKernelGGL(k_bin, gn, b, 0, s, a.pos, n, a.grid, a.cell_of, a.cell_count, a.flag);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.cell_count, ncell, a.cell_start,
+    hipLaunchKernelGGL(k_tile_count, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.flag);
+    hipLaunchKernelGGL(k_scan, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.cell_start,
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
     hipLaunchKernelGGL(k_sort_cells, gc, b, 0, s, a.cell_start, ncell, a.orig, a.order, a.flag);
     hipLaunchKernelGGL(k_permute, gn, b, 0, s, a.order, n, a.pos, a.vel, a.frc, a.orig, a.cell_of,
                        a.pos2, a.vel2, a.frc2, a.orig2, a.cell2, a.flag);
-    if (a.grid.usable) {
-        const int nseg = (a.grid.nc[0] + 11) / 12;
-        const int seg = (a.grid.nc[0] + nseg - 1) / nseg;
-        const unsigned nwg = static_cast<unsigned>(nseg) * a.grid.nc[1] * a.grid.nc[2];
-        hipLaunchKernelGGL(k_commit_build_tiles, dim3(nwg), b, kTileLdsBytes, s, n, a.grid, a.box,
-                           a.rl2, seg, nseg, kTileLdsCap, a.pos2, a.vel2, a.frc2, a.orig2, a.cell2,
-                           a.cell_start, a.pos, a.vel, a.frc, a.ref, a.orig, a.slot, a.cell_of,
-                           a.nbr, a.ncnt, a.cap, a.sc, a.flag);
-    } else {
-        hipLaunchKernelGGL(k_commit_build, gn, b, 0, s, n, a.grid, a.box, a.rl2, a.pos2, a.vel2,
-                           a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref, a.orig,
-                           a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
-    }
+    hipLaunchKernelGGL(k_commit_build, dim3(xcd_grid(n)), b, 0, s, n, a.grid, a.box, a.rl2, a.pos2,
+                       a.vel2, a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref,
+                       a.orig, a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
 }
 
-hipError_t neigh_init() {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_commit_build_tiles),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsBytes);
-}
+long scan_tiles(long ncell) { return (ncell + kScanTile - 1) / kScanTile; }
 
 }  // namespace mdk

@@ -143,6 +143,22 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_water_run.argtypes = [vp, C.c_long, C.c_double, C.c_int, _dp, _dp]
     lib.md_water_download.argtypes = [vp] + [_dp] * 9
     lib.md_water_set_state.argtypes = [vp] + [_dp] * 6
+    lib.md_mix_create.argtypes = [vp, C.POINTER(vp)]
+    lib.md_mix_destroy.argtypes = [vp]
+    lib.md_mix_last_error.argtypes = [vp]
+    lib.md_mix_last_error.restype = C.c_char_p
+    lib.md_mix_upload.argtypes = [vp, C.c_long, _dp] + [_dp] * 7 + [_ip]
+    lib.md_mix_set_params.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.c_int]
+    lib.md_mix_set_skin.argtypes = [vp, C.c_double]
+    lib.md_mix_forces.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_mix_force_vjp.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 7
+    lib.md_mix_minimize.argtypes = [vp, C.c_int, C.c_double, C.c_double, _dp]
+    lib.md_mix_run.argtypes = [vp, C.c_double, C.c_int, C.c_int, _dp, _dp]
+    lib.md_mix_adjoint.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 13
+    lib.md_mix_download.argtypes = [vp] + [_dp] * 9
+    lib.md_mix_set_state.argtypes = [vp] + [_dp] * 6
+    lib.md_mix_list_builds.argtypes = [vp, _lp]
+    lib.md_mix_random_state.argtypes = [C.c_long, C.c_double, C.c_uint64, C.c_double] + [_dp] * 7
     _lib = lib
     return lib
 
@@ -510,3 +526,135 @@ def lj_tail(n, L, sigma=1.0, epsilon=1.0, cutoff=2.5, lfs=False):
     if st != MD_OK:
         raise MDError(st, "md_lj_tail failed")
     return u.value, pr.value
+
+
+def _vec3(a, n):
+    """(n, 3) array -> three contiguous columns (or three Nones)."""
+    if a is None:
+        return [None] * 3
+    a = np.asarray(a, dtype=np.float64).reshape(n, 3)
+    return [_col(a[:, d]) for d in range(3)]
+
+
+def type_pairs(ntypes: int):
+    """Canonical order of the unordered type pairs (a <= b), row-major."""
+    return [(a, b) for a in range(ntypes) for b in range(a, ntypes)]
+
+
+def mix_random_state(n: int, box_length: float, seed: int, temperature: float, mass=None):
+    """md_mix_random_state: (positions (n,3), velocities (n,3))."""
+    a = [np.zeros(n) for _ in range(6)]
+    m = None if mass is None else _col(mass)
+    st = load().md_mix_random_state(n, box_length, seed, temperature, _p(m), *[_p(v) for v in a])
+    if st != MD_OK:
+        raise MDError(st, "md_mix_random_state failed")
+    return np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1)
+
+
+class Mix:
+    """md_mix: per-type-pair LJ with parameter derivatives, force VJPs,
+    steepest descent, velocity-Verlet runs with checkpoints and their adjoint."""
+
+    def __init__(self, L, pos, types, sigma, epsilon, cutoff, lfs=False, vel=None, mass=None,
+                 skin=0.3, stream=None):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.md_mix_create(stream, C.byref(h))
+        self.h = h
+        if st != MD_OK:
+            msg = self.lib.md_mix_last_error(h).decode() if h else "md_mix_create failed"
+            self.close()
+            raise MDError(st, msg)
+        self.upload(L, pos, types, vel, mass)
+        self.set_params(sigma, epsilon, cutoff, lfs)
+        self._chk(self.lib.md_mix_set_skin(self.h, skin))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.md_mix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != MD_OK:
+            raise MDError(st, self.lib.md_mix_last_error(self.h).decode())
+
+    def upload(self, L, pos, types, vel=None, mass=None):
+        pos = np.asarray(pos, dtype=np.float64)
+        self.n = pos.shape[0]
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        cols, vcols = _vec3(pos, self.n), _vec3(vel, self.n)
+        m = None if mass is None else _col(mass)
+        Lb = _col(L)
+        self._chk(self.lib.md_mix_upload(self.h, self.n, _p(Lb), *[_p(c) for c in cols],
+                                         *[_p(c) for c in vcols], _p(m),
+                                         self.types.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def set_params(self, sigma, epsilon, cutoff, lfs=False):
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.ntypes = int(sg.shape[0])
+        ep = np.ascontiguousarray(epsilon, dtype=np.float64)
+        rc = np.ascontiguousarray(cutoff, dtype=np.float64)
+        self._chk(self.lib.md_mix_set_params(self.h, self.ntypes, _p(sg.reshape(-1)),
+                                             _p(ep.reshape(-1)), _p(rc.reshape(-1)), int(lfs)))
+
+    @property
+    def npairs(self):
+        return self.ntypes * (self.ntypes + 1) // 2
+
+    def forces(self, derivatives=False):
+        """(E, W) or (E, W, tables) with tables[k] = (E_ab, W_ab, dE/deps,
+        dE/dsigma, dW/deps, dW/dsigma) for the canonical type pairs."""
+        e, w = C.c_double(), C.c_double()
+        t = np.zeros(6 * self.npairs) if derivatives else None
+        self._chk(self.lib.md_mix_forces(self.h, C.byref(e), C.byref(w), _p(t)))
+        if derivatives:
+            return e.value, w.value, t.reshape(self.npairs, 6)
+        return e.value, w.value
+
+    def force_vjp(self, g=None, w_energy=0.0, w_virial=0.0):
+        """(dF/dx)^T g + wE dE/dx + wW dW/dx, and the same cotangent on
+        (eps_ab, sigma_ab) as an (npairs, 2) array."""
+        gc = _vec3(g, self.n)
+        o = [np.zeros(self.n) for _ in range(3)]
+        th = np.zeros(2 * self.npairs)
+        self._chk(self.lib.md_mix_force_vjp(self.h, w_energy, w_virial, *[_p(c) for c in gc],
+                                            *[_p(v) for v in o], _p(th)))
+        return np.stack(o, axis=1), th.reshape(self.npairs, 2)
+
+    def minimize(self, nsteps, step=1e-3, max_disp=0.05):
+        e = np.zeros(nsteps + 1)
+        self._chk(self.lib.md_mix_minimize(self.h, int(nsteps), step, max_disp, _p(e)))
+        return e
+
+    def run(self, dt, nsteps, checkpoint_every=20):
+        e, w = np.zeros(nsteps + 1), np.zeros(nsteps + 1)
+        self._chk(self.lib.md_mix_run(self.h, dt, int(nsteps), int(checkpoint_every), _p(e), _p(w)))
+        return e, w
+
+    def adjoint(self, w_energy=0.0, w_virial=0.0, gx=None, gv=None):
+        """(dL/dtheta (npairs, 2: eps, sigma), dL/dx0, dL/dv0) of the last run."""
+        gxc, gvc = _vec3(gx, self.n), _vec3(gv, self.n)
+        th = np.zeros(2 * self.npairs)
+        dx = [np.zeros(self.n) for _ in range(3)]
+        dv = [np.zeros(self.n) for _ in range(3)]
+        self._chk(self.lib.md_mix_adjoint(self.h, w_energy, w_virial, *[_p(c) for c in gxc],
+                                          *[_p(c) for c in gvc], _p(th), *[_p(v) for v in dx],
+                                          *[_p(v) for v in dv]))
+        return th.reshape(self.npairs, 2), np.stack(dx, axis=1), np.stack(dv, axis=1)
+
+    def download(self):
+        a = [np.zeros(self.n) for _ in range(9)]
+        self._chk(self.lib.md_mix_download(self.h, *[_p(v) for v in a]))
+        return (np.stack(a[0:3], axis=1), np.stack(a[3:6], axis=1), np.stack(a[6:9], axis=1))
+
+    def set_state(self, pos=None, vel=None):
+        pc, vc = _vec3(pos, self.n), _vec3(vel, self.n)
+        self._chk(self.lib.md_mix_set_state(self.h, *[_p(c) for c in pc], *[_p(c) for c in vc]))
+
+    def list_builds(self):
+        b = C.c_long()
+        self._chk(self.lib.md_mix_list_builds(self.h, C.byref(b)))
+        return b.value

@@ -362,4 +362,53 @@ int orc_mix_lj(long n, const double* L, const double* x, const double* y, const 
     return 0;
 }
 
+namespace {
+MixLJ make_mix(int ntypes, const double* sig, const double* eps, const double* rc, int lfs) {
+    MixLJ p;
+    p.ntypes = ntypes;
+    const std::size_t nt2 = static_cast<std::size_t>(ntypes * ntypes);
+    p.sigma.assign(sig, sig + nt2);
+    p.epsilon.assign(eps, eps + nt2);
+    p.cutoff.assign(rc, rc + nt2);
+    p.lfs = lfs != 0;
+    return p;
+}
+}  // namespace
+
+// Steepest descent on the mixture; positions updated in place, energies[nsteps+1].
+int orc_mix_minimize(long n, const double* L, double* x, double* y, double* z, const int* type,
+                     int ntypes, const double* sig, const double* eps, const double* rc, int lfs,
+                     int nsteps, double step, double max_disp, double* energies) {
+    State s = make_state(n, L, x, y, z);
+    const std::vector<int> t(type, type + n);
+    const std::vector<double> e = mix_minimize(make_mix(ntypes, sig, eps, rc, lfs), s, t, nsteps,
+                                               step, max_disp);
+    copy_out(s.x, x); copy_out(s.y, y); copy_out(s.z, z);
+    copy_out(e, energies);
+    return 0;
+}
+
+// Velocity Verlet on the mixture (masses m or 1); state updated in place,
+// energies / virials[nsteps+1].
+int orc_mix_vv(long n, const double* L, double* x, double* y, double* z, double* vx, double* vy,
+               double* vz, const double* m, const int* type, int ntypes, const double* sig,
+               const double* eps, const double* rc, int lfs, double dt, int nsteps,
+               double* energies, double* virials) {
+    State s = make_state(n, L, x, y, z);
+    for (long i = 0; i < n; ++i) {
+        s.vx[i] = vx[i];
+        s.vy[i] = vy[i];
+        s.vz[i] = vz[i];
+        if (m) s.m[i] = m[i];
+    }
+    const std::vector<int> t(type, type + n);
+    std::vector<double> e, w;
+    mix_vv(make_mix(ntypes, sig, eps, rc, lfs), s, t, dt, nsteps, e, w);
+    copy_out(s.x, x); copy_out(s.y, y); copy_out(s.z, z);
+    copy_out(s.vx, vx); copy_out(s.vy, vy); copy_out(s.vz, vz);
+    copy_out(e, energies);
+    copy_out(w, virials);
+    return 0;
+}
+
 }  // extern "C"

@@ -312,4 +312,40 @@ MixEV mix_lj(const MixLJ& p, const State& s, const std::vector<int>& type, bool 
     return out;
 }
 
+std::vector<double> mix_minimize(const MixLJ& p, State& s, const std::vector<int>& type,
+                                 int nsteps, double step, double max_disp) {
+    std::vector<double> e;
+    e.push_back(mix_lj(p, s, type, true, &s).energy);
+    for (int k = 0; k < nsteps; ++k) {
+        double fmax = 0.0;
+        for (std::size_t i = 0; i < s.n(); ++i) {
+            const V3 f{s.fx[i], s.fy[i], s.fz[i]};
+            fmax = std::max(fmax, std::sqrt(dot3(f, f)));
+        }
+        double g = step;
+        if (fmax > 0.0 && max_disp / fmax < g) g = max_disp / fmax;
+        for (std::size_t i = 0; i < s.n(); ++i) {
+            s.x[i] = fold(std::fma(g, s.fx[i], s.x[i]), s.L[0]);
+            s.y[i] = fold(std::fma(g, s.fy[i], s.y[i]), s.L[1]);
+            s.z[i] = fold(std::fma(g, s.fz[i], s.z[i]), s.L[2]);
+        }
+        e.push_back(mix_lj(p, s, type, true, &s).energy);
+    }
+    return e;
+}
+
+void mix_vv(const MixLJ& p, State& s, const std::vector<int>& type, double dt, int nsteps,
+            std::vector<double>& energy, std::vector<double>& virial) {
+    energy.clear();
+    virial.clear();
+    auto eval = [&](State& st) {
+        const MixEV r = mix_lj(p, st, type, true, &st);
+        energy.push_back(r.energy);
+        virial.push_back(r.virial);
+        return r.energy;
+    };
+    eval(s);
+    for (int k = 0; k < nsteps; ++k) vv_step(s, dt, eval);
+}
+
 }  // namespace oracle

@@ -241,5 +241,14 @@ struct MixEV {
 };
 MixEV mix_lj(const MixLJ& p, const State& s, const std::vector<int>& type, bool forces,
              State* sink);
+// Steepest descent (no reference symbol): nsteps moves
+// x <- wrap(fma(gamma, F, x)), gamma = min(step, max_disp / max_i |F_i|)
+// (gamma = step when every force is zero), brute-force forces. Returns the
+// energies before the first move and after each one (nsteps + 1 values).
+std::vector<double> mix_minimize(const MixLJ& p, State& s, const std::vector<int>& type,
+                                 int nsteps, double step, double max_disp);
+// nsteps of vv_step with per-type-pair forces; E_t and W_t for t = 0..nsteps.
+void mix_vv(const MixLJ& p, State& s, const std::vector<int>& type, double dt, int nsteps,
+            std::vector<double>& energy, std::vector<double>& virial);
 
 }  // namespace oracle

@@ -113,6 +113,12 @@ class Oracle:
         if hasattr(lib, "orc_mix_lj"):
             lib.orc_mix_lj.argtypes = ([C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp,
                                         C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
+            lib.orc_mix_minimize.argtypes = ([C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int, _dp,
+                                              _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                              _dp])
+            lib.orc_mix_vv.argtypes = ([C.c_long, _dp] + [_dp] * 7 + [_ip, C.c_int, _dp, _dp, _dp,
+                                                                      C.c_int, C.c_double, C.c_int,
+                                                                      _dp, _dp])
 
     @property
     def is_reference(self) -> bool:
@@ -255,6 +261,34 @@ class Oracle:
         return dict(energy=float(out[0]), virial=float(out[1]),
                     e_ab=tabs[0].reshape(nt, nt), w_ab=tabs[1].reshape(nt, nt),
                     y_ab=tabs[2].reshape(nt, nt), forces=np.stack(f, axis=1))
+
+    @staticmethod
+    def _mix_tables(types, sig, eps, rc):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        tabs = [np.ascontiguousarray(a, dtype=float).reshape(-1) for a in (sig, eps, rc)]
+        return t, int(np.asarray(sig).shape[0]), tabs
+
+    def mix_minimize(self, fr: Frame, types, sig, eps, rc, nsteps, step, max_disp, lfs=False):
+        """Steepest descent; returns (relaxed Frame, energies[nsteps+1])."""
+        out = fr.copy()
+        t, nt, (sg, ep, rcs) = self._mix_tables(types, sig, eps, rc)
+        e = np.zeros(nsteps + 1)
+        self.lib.orc_mix_minimize(out.n, _ptr(out.L), _ptr(out.x), _ptr(out.y), _ptr(out.z),
+                                  _ptr(t, _ip), nt, _ptr(sg), _ptr(ep), _ptr(rcs), int(lfs),
+                                  int(nsteps), step, max_disp, _ptr(e))
+        return out, e
+
+    def mix_vv(self, fr: Frame, types, sig, eps, rc, dt, nsteps, lfs=False, mass=None):
+        """Velocity Verlet on the mixture; returns (final Frame, E[nsteps+1], W[nsteps+1])."""
+        out = fr.copy()
+        t, nt, (sg, ep, rcs) = self._mix_tables(types, sig, eps, rc)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=float)
+        e, w = np.zeros(nsteps + 1), np.zeros(nsteps + 1)
+        self.lib.orc_mix_vv(out.n, _ptr(out.L), _ptr(out.x), _ptr(out.y), _ptr(out.z),
+                            _ptr(out.vx), _ptr(out.vy), _ptr(out.vz),
+                            None if m is None else _ptr(m), _ptr(t, _ip), nt, _ptr(sg), _ptr(ep),
+                            _ptr(rcs), int(lfs), dt, int(nsteps), _ptr(e), _ptr(w))
+        return out, e, w
 
 
 class Sim:

@@ -22,6 +22,41 @@ def nist_spce(cfg: int) -> str:
     return os.path.join(GOLDEN, "nist", "spce", f"spce_sample_config_periodic{cfg}.txt")
 
 
+# Binary mixture of the differentiated workload: 80:20 A:B, eps_AA = 1,
+# sigma_AA = 1, eps_AB = 1.5, sigma_AB = 0.8, eps_BB = 0.5, sigma_BB = 0.88,
+# cutoff 2.5 sigma_ab.
+KA_SIGMA = [[1.0, 0.8], [0.8, 0.88]]
+KA_EPS = [[1.0, 1.5], [1.5, 0.5]]
+KA_RC = [[2.5, 2.0], [2.0, 2.2]]
+
+
+def ka_types(n: int):
+    import numpy as np
+
+    t = np.zeros(n, dtype=np.int32)
+    t[int(round(0.8 * n)):] = 1
+    return t
+
+
+def ka_frame(n: int, rho: float, seed: int, jitter: float = 0.0):
+    """A mixture configuration on a jittered simple-cubic lattice (no
+    overlaps, so the pair terms stay moderate), types by ka_types."""
+    import numpy as np
+    from oracle.pyoracle import Frame
+
+    L = (n / rho) ** (1.0 / 3.0)
+    m = int(np.ceil(n ** (1.0 / 3.0)))
+    g = (np.arange(m) + 0.5) * (L / m) - 0.5 * L
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)[:n]
+    rng = np.random.default_rng(seed)
+    pts = pts + rng.uniform(-jitter, jitter, size=pts.shape)
+    pts = pts - L * np.floor(pts / L + 0.5)
+    v = rng.normal(size=(n, 3))
+    v -= v.mean(axis=0)
+    return Frame(np.array([L, L, L]), pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(),
+                 v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy())
+
+
 @pytest.fixture(scope="session")
 def orc():
     from oracle import pyoracle
