@@ -191,9 +191,8 @@ int auto_capacity(const md_ctx* c) {
     const double rho = static_cast<double>(c->n) / (c->L[0] * c->L[1] * c->L[2]);
     const double mean = 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho;
     long cap = static_cast<long>(std::ceil(2.0 * mean + 32.0));
-    cap = (cap + 7) / 8 * 8;
     cap = std::min(cap, std::max(1L, c->n - 1));
-    return static_cast<int>(cap);
+    return static_cast<int>((cap + 7) / 8 * 8);  // the octet layout needs a multiple of 8
 }
 
 md_status alloc_neigh(md_ctx* c) {
@@ -209,7 +208,7 @@ md_status alloc_neigh(md_ctx* c) {
         c->ncell_alloc = c->ncell;
     }
     c->npad = (c->n + 63) / 64 * 64;
-    c->cap = c->cap_user > 0 ? c->cap_user : auto_capacity(c);
+    c->cap = c->cap_user > 0 ? (c->cap_user + 7) / 8 * 8 : auto_capacity(c);
     const long need = static_cast<long>(c->cap) * c->npad;
     if (need > c->nbr_alloc) {
         MD_HIP(c, alloc(c->nbr, need));
@@ -691,7 +690,7 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
     for (long t = 0; t < n; ++t) {
         const int k = std::min(cnt[t], c->cap);
         for (int q = 0; q < k; ++q) {
-            const uint32_t e = nb[static_cast<size_t>(mdk::nbr_index(t, q, c->cap))];
+            const uint32_t e = nb[static_cast<size_t>(mdk::oct_index(t, q, c->cap))];
             const uint32_t j = e & mdk::kIndexMask;
             const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
             if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);

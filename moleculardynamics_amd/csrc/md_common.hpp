@@ -169,6 +169,15 @@ __device__ __forceinline__ void row_delta(const Box& b, const double4& pi, const
     }
 }
 
+// Slot k of atom t in the octet layout of the main Verlet list: every group
+// of 64 consecutive atoms owns a [cap/8][64][8] tile, so entries 8g..8g+7 of
+// one atom are 32 contiguous bytes (one whole-sector write by the build, two
+// 16-byte loads by the pair kernel) and group g of a wave is 2 KB
+// contiguous. cap is a multiple of 8.
+__host__ __device__ __forceinline__ long oct_index(long t, int k, int cap) {
+    return (((t >> 6) * static_cast<long>(cap >> 3) + (k >> 3)) * 64 + (t & 63)) * 8 + (k & 7);
+}
+
 // Slot k of atom t in the wave-blocked neighbour layout: every group of 64
 // consecutive atoms owns one contiguous [cap][64] tile, so a wave reading
 // slot k of its 64 atoms touches one 256-byte line, and all its slots sit in

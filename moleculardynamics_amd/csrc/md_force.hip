@@ -45,11 +45,13 @@ __device__ double chain_half(Chain& c, double& ke, double delta) {
         const double d2 = 0.5 * d, d4 = 0.25 * d;
         auto link = [&](int j) {
             const bool next = j + 1 < M;
-            if (next) c.p[j] *= exp(-d4 * c.p[j + 1] / c.Q[j + 1]);
+            // p[j+1] is not touched in between: one exp serves both damping factors
+            const double damp = next ? exp(-d4 * c.p[j + 1] / c.Q[j + 1]) : 1.0;
+            if (next) c.p[j] *= damp;
             const double g = (j == 0) ? 2.0 * ke - c.dof * c.T
                                       : c.p[j - 1] * c.p[j - 1] / c.Q[j - 1] - c.T;
             c.p[j] += d2 * g;
-            if (next) c.p[j] *= exp(-d4 * c.p[j + 1] / c.Q[j + 1]);
+            if (next) c.p[j] *= damp;
         };
         for (int j = M - 1; j >= 0; --j) link(j);
         for (int k = 0; k < M; ++k) c.xi[k] += d * c.p[k] / c.Q[k];
@@ -226,31 +228,56 @@ __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, ui
     }
 }
 
-// Row walk with four gathers in flight per thread: the four indices of a
-// group are loaded together, then the four partner positions, then the
-// four pair terms are evaluated in row order.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One octet of the row (two 16-byte loads). The list is read exactly once
+// per step, so the loads are non-temporal: the index stream does not evict
+// the partner positions that the gathers reuse from L2.
+__device__ __forceinline__ void load_octet(const uint32_t* p, u32x4& lo, u32x4& hi) {
+    const u32x4* v = reinterpret_cast<const u32x4*>(p);
+    lo = __builtin_nontemporal_load(v);
+    hi = __builtin_nontemporal_load(v + 1);
+}
+
+template <class Acc, class PairFn>
+__device__ __forceinline__ void quad(const ForceArgs& a, const double4& pi, const u32x4& j, Acc& q,
+                                     PairFn pf) {
+    const double4 p0 = a.pos[j.x & kIndexMask];
+    const double4 p1 = a.pos[j.y & kIndexMask];
+    const double4 p2 = a.pos[j.z & kIndexMask];
+    const double4 p3 = a.pos[j.w & kIndexMask];
+    pf(a, pi, j.x, p0, q);
+    pf(a, pi, j.y, p1, q);
+    pf(a, pi, j.z, p2, q);
+    pf(a, pi, j.w, p3, q);
+}
+
+// Row walk over the octet layout: the next octet's indices are loaded while
+// the current one is processed, and each half of an octet issues its four
+// partner gathers together before evaluating the four pair terms in row
+// order (the summation order is the row order).
 template <class Acc, class PairFn>
 __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, const double4& pi,
                                          Acc& q, PairFn pf) {
-    const uint32_t* row = a.nbr + nbr_index(t, 0, a.cap);
-    int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-        const uint32_t j0 = row[(k + 0) * 64];
-        const uint32_t j1 = row[(k + 1) * 64];
-        const uint32_t j2 = row[(k + 2) * 64];
-        const uint32_t j3 = row[(k + 3) * 64];
-        const double4 p0 = a.pos[j0 & kIndexMask];
-        const double4 p1 = a.pos[j1 & kIndexMask];
-        const double4 p2 = a.pos[j2 & kIndexMask];
-        const double4 p3 = a.pos[j3 & kIndexMask];
-        pf(a, pi, j0, p0, q);
-        pf(a, pi, j1, p1, q);
-        pf(a, pi, j2, p2, q);
-        pf(a, pi, j3, p3, q);
+    const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
+    const int full = cnt >> 3;
+    u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
+    if (full > 0) load_octet(row, lo, hi);
+    for (int g = 0; g < full; ++g) {
+        u32x4 nlo = {0u, 0u, 0u, 0u}, nhi = {0u, 0u, 0u, 0u};
+        if (g + 1 < full) load_octet(row + (g + 1) * 512, nlo, nhi);
+        quad(a, pi, lo, q, pf);
+        quad(a, pi, hi, q, pf);
+        lo = nlo;
+        hi = nhi;
     }
-    for (; k < cnt; ++k) {
-        const uint32_t j0 = row[k * 64];
-        pf(a, pi, j0, a.pos[j0 & kIndexMask], q);
+    const int rem = cnt & 7;
+    if (rem) {
+        load_octet(row + full * 512, lo, hi);
+        const uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int r = 0; r < 7; ++r)
+            if (r < rem) pf(a, pi, e[r], a.pos[e[r] & kIndexMask], q);
     }
 }
 

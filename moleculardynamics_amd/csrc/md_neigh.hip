@@ -220,14 +220,28 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     orig[t] = oi;
     slot[oi] = t;
     cell_of[t] = ci;
-    uint32_t* row = nbr + nbr_index(t, 0, cap);
+    // Entries are collected eight at a time in this thread's LDS column and
+    // written as one 32-byte octet, so the list goes out in whole sectors
+    // instead of 4-byte partial writes.
+    __shared__ uint32_t stage[8][kBlock];
+    uint32_t* row = nbr + oct_index(t, 0, cap);
+    auto flush = [&](int grp) {
+        uint4* dst = reinterpret_cast<uint4*>(row + static_cast<long>(grp) * 512);
+        dst[0] = make_uint4(stage[0][threadIdx.x], stage[1][threadIdx.x], stage[2][threadIdx.x],
+                            stage[3][threadIdx.x]);
+        dst[1] = make_uint4(stage[4][threadIdx.x], stage[5][threadIdx.x], stage[6][threadIdx.x],
+                            stage[7][threadIdx.x]);
+    };
     int cnt = 0;
     auto visit = [&](int j, const double4& q, int oj) {
         const bool flip = oi > oj;
         double dx, dy, dz;
         row_delta(box, pi, q, flip, dx, dy, dz);
         if (norm2_ref(dx, dy, dz) < rl2 && j != t) {
-            if (cnt < cap) row[cnt * 64] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+            if (cnt < cap) {
+                stage[cnt & 7][threadIdx.x] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+                if ((cnt & 7) == 7) flush(cnt >> 3);
+            }
             ++cnt;
         }
     };
@@ -267,6 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     } else {
         range(0, static_cast<int>(n));
     }
+    const int kept = cnt < cap ? cnt : cap;
+    if (kept & 7) flush(kept >> 3);  // the last, partial octet (slots past the row are not read)
     ncnt[t] = cnt;
     if (cnt > cap) atomicOr(&sc->overflow, 1);
 }
