@@ -124,6 +124,191 @@ def cpu_baseline(timed_steps: int = 10, warm_steps: int = 2):
                 s_per_step=dt / timed_steps)
 
 
+def _line(metric, value, unit, steps, warmup, ms, dtype, workload, extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)",
+           "config": {"workload": workload, "parallelism": "single GPU"}}
+    out.update(extra)
+    print(json.dumps(out))
+    return 0
+
+
+def bench_lj864(args) -> int:
+    """864-atom LJ fluid, FP64, velocity Verlet NVE, 1000 steps (the reference's own
+    CPU-runnable case): fccLattice(6, 0.8442, 0.722, 12345), rc 2.5 (Truncated), skin 0.3."""
+    from moleculardynamics_amd import native
+
+    L, pos, vel = native.fcc_lattice(6, RHO, TEMP, 12345)
+    ctx = native.Context()
+    ctx.upload(L, pos, vel)
+    ctx.set_lj(1.0, 1.0, RC, False)
+    ctx.config_list(RC, SKIN)
+    ctx.build_list()
+    ctx.forces(native.VERLET, native.FP64)
+    ctx.set_integrator(kind=native.VV, dt=DT)
+    ctx.run(args.warmup, native.FP64, thermo=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    th = ctx.run(1000, native.FP64, thermo=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    h = th["pe"] + th["ke"]
+    extra = {"pairs_per_step": float(np.mean(th["pairs"])),
+             "nve_drift": float(np.max(np.abs(h - h[0])) / abs(h[0]))}
+    if not args.no_cpu:
+        from oracle import pyoracle
+
+        path = pyoracle.REF_SO if os.path.exists(pyoracle.REF_SO) else pyoracle.ORACLE_SO
+        o = pyoracle.Oracle(path)
+        fr = o.fcc_lattice(6, RHO, TEMP, 12345)
+        sim = o.sim(fr, kind="vv", dt=DT, rc=RC, skin=SKIN)
+        c0 = time.perf_counter()
+        sim.run(1000)
+        cw = time.perf_counter() - c0
+        extra["cpu_baseline"] = {"value": 1000 / cw, "unit": "timesteps/s", "cores": 1,
+                                 "kind": "reference" if path == pyoracle.REF_SO else "port",
+                                 "sample": "all 1000 NVE steps, single thread"}
+    return _line("timesteps/s, 864-atom LJ NVE, FP64, 1 MI355X", 1000 / wall, "timesteps/s", 1000,
+                 args.warmup, 1e3 * wall / 1000, "f64",
+                 "LJ fluid N=864 (fcc 6^3x4), rho*=0.8442, T*=0.722, rc=2.5, skin=0.3, VV dt*=0.005",
+                 extra)
+
+
+def bench_batch(args) -> int:
+    """256 uneven LJ replicas (108..4000 atoms, rho* in [0.5, 0.9], T* in [0.7, 1.5]),
+    BAOAB gamma = 1, dt* = 0.005, 10,000 steps in HIP-graph segments of 100, mixed
+    precision; against the plain single-system path launched once per replica."""
+    from moleculardynamics_amd import native
+
+    cells, rho, T, seeds = native.batch_draw(256, 4000)
+    systems = [native.fcc_lattice(int(c), float(r), float(t), int(s))
+               for c, r, t, s in zip(cells, rho, T, seeds)]
+    atoms = int(sum(s[1].shape[0] for s in systems))
+    b = native.Batch(systems, T, seeds, precision=native.MIXED)
+    b.prime()
+    tpa, block, _ = b.autotune(trial_steps=5)
+    b.run(max(args.warmup, 100), segment=100)
+    steps = 10000
+    t0 = time.perf_counter()
+    b.run(steps, segment=100)
+    wall = time.perf_counter() - t0
+    b.close()
+    batch_rate = 256 * steps / wall  # replica-steps per second
+    # the plain path, one replica at a time, 100 steps each
+    plain_steps = 100
+    t0 = time.perf_counter()
+    for k, (L, x, v) in enumerate(systems):
+        ctx = native.Context()
+        ctx.upload(L, x, v)
+        ctx.set_lj(1.0, 1.0, RC, False)
+        ctx.config_list(RC, SKIN)
+        ctx.build_list()
+        ctx.forces(native.VERLET, native.MIXED)
+        ctx.set_integrator(kind=native.BAOAB, dt=DT, friction=1.0, temperature=float(T[k]),
+                           seed=int(seeds[k]))
+        ctx.run(plain_steps, native.MIXED, thermo=False)
+        ctx.synchronize()
+        ctx.close()
+    plain_wall = time.perf_counter() - t0
+    plain_rate = 256 * plain_steps / plain_wall
+    extra = {"replicas": 256, "atoms_total": atoms, "block": int(block),
+             "tpa_classes": sorted(set(int(t) for t in tpa)),
+             "plain_path_replica_steps_per_s": plain_rate,
+             "batch_vs_plain": batch_rate / plain_rate,
+             "plain_path_sample": f"256 replicas x {plain_steps} steps, one md_ctx each, "
+                                  "setup included"}
+    return _line("replica-steps/s, 256 uneven LJ replicas, BAOAB, mixed, 1 MI355X", batch_rate,
+                 "replica-steps/s", steps, 100, 1e3 * wall / steps,
+                 "f32 pair / i64 fixed-point accumulate / f64 state",
+                 "256 LJ replicas of 108-4000 atoms, rho* U[0.5,0.9], T* U[0.7,1.5], rc=2.5, "
+                 "BAOAB gamma=1 dt*=0.005, 10000 steps in 100-step graph segments", extra)
+
+
+def bench_water(args) -> int:
+    """4096 SPC/E molecules (12,288 sites), L = 49.71853 A, rc 9 A, alpha and kmax for a
+    1e-5 relative force error, 500 constrained velocity-Verlet steps of 2 fs at 300 K."""
+    from moleculardynamics_amd import native
+
+    L = 49.71853
+    pos, vel = native.water_lattice(16, L, 2026, 300.0)
+    q2 = 4096 * (0.8476 ** 2 + 2 * 0.4238 ** 2)
+    alpha, kmax = native.ewald_parameters(12288, [L] * 3, q2, 9.0, 1e-5)
+    prm = native.water_params(cutoff=9.0, kmax=kmax, alpha=alpha)
+    w = native.Water([L] * 3, pos, vel, params=prm)
+    dt = 0.0018236739  # 2 fs in units of 1.0966875353 ps
+    w.forces()
+    if args.warmup:
+        w.run(args.warmup, dt)
+    t0 = time.perf_counter()
+    pe, ke = w.run(500, dt)
+    wall = time.perf_counter() - t0
+    w.close()
+    extra = {"alpha": alpha, "kmax": kmax, "sites": 12288,
+             "energy_drift": float(np.max(np.abs((pe + ke) - (pe[0] + ke[0]))) / abs(pe[0] + ke[0]))}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        from oracle.pyoracle import Frame
+
+        # the reference's Ewald fixes alpha = 5.6/L, so the tuned alpha runs on the port
+        o = pyoracle.Oracle(pyoracle.ORACLE_SO)
+        fr = Frame(np.array([L] * 3), pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(),
+                   vel[:, 0].copy(), vel[:, 1].copy(), vel[:, 2].copy())
+        c0 = time.perf_counter()
+        o.water_run(fr, rc=9.0, kmax=kmax, alpha=alpha, dt=dt, nsteps=2)
+        cw = time.perf_counter() - c0
+        extra["cpu_baseline"] = {"value": 2 / cw, "unit": "timesteps/s", "cores": 1, "kind": "port",
+                                 "sample": "2 constrained VV steps (O(N^2) real space), single thread"}
+    return _line("timesteps/s, SPC/E Ewald 12,288 sites, 1 MI355X", 500 / wall, "timesteps/s", 500,
+                 args.warmup, 1e3 * wall / 500, "f64",
+                 "SPC/E 4096 molecules, L=49.71853 A, rc=9 A, Ewald 1e-5, CVV dt=2 fs, 300 K", extra)
+
+
+def bench_mixture(args) -> int:
+    """The differentiated binary mixture: N = 32,000 (80:20), rho* = 1.2, 500
+    steepest-descent steps, then 200 velocity-Verlet steps with checkpoints every 20 and
+    the adjoint of <E> + 0.1 <W> + KE(T) (linear force shift)."""
+    from moleculardynamics_amd import native
+
+    n = 32000
+    box = (n / 1.2) ** (1.0 / 3.0)
+    t = np.zeros(n, dtype=np.int32)
+    t[int(0.8 * n):] = 1
+    sig, eps, rc = [[1.0, 0.8], [0.8, 0.88]], [[1.0, 1.5], [1.5, 0.5]], [[2.5, 2.0], [2.0, 2.2]]
+    x, v = native.mix_random_state(n, box, 2027, 1.0)
+    m = native.Mix([box] * 3, x, t, sig, eps, rc, lfs=True, vel=v)
+    t0 = time.perf_counter()
+    m.minimize(500, 1e-3, 0.05)
+    t_sd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e, w = m.run(0.005, 200, 20)
+    t_fwd = time.perf_counter() - t0
+    _, vT, _ = m.download()
+    t0 = time.perf_counter()
+    m.adjoint(1.0 / 201, 0.1 / 201, None, vT)
+    t_adj = time.perf_counter() - t0
+    m.close()
+    extra = {"minimize_s": t_sd, "forward_s": t_fwd, "adjoint_s": t_adj,
+             "adjoint_over_forward": t_adj / t_fwd}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        from oracle.pyoracle import Frame
+
+        o = pyoracle.Oracle(pyoracle.ORACLE_SO)
+        fr = Frame(np.array([box] * 3), x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+        c0 = time.perf_counter()
+        o.mix_lj(fr, t, sig, eps, rc, lfs=True)
+        cw = time.perf_counter() - c0
+        extra["cpu_baseline"] = {"value": 1.0 / cw, "unit": "energy evaluations/s", "cores": 1,
+                                 "kind": "port",
+                                 "sample": "one brute-force mixture energy at N=32000, single thread"}
+    return _line("timesteps/s, 32k binary-mixture VV forward + adjoint, FP64, 1 MI355X",
+                 200 / (t_fwd + t_adj), "timesteps/s", 200, 0, 1e3 * (t_fwd + t_adj) / 200, "f64",
+                 "binary LJ mixture N=32000 80:20 rho*=1.2, 500 SD + 200 VV (K=20) + adjoint",
+                 extra)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +316,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--workload", default="lj1m",
+                    choices=["lj1m", "lj864", "water", "batch", "mixture"],
+                    help="lj1m (default): the 1M-atom LJ line; the others are the remaining "
+                         "configurations, each printing its own JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -150,6 +339,10 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
         return 0
+    other = {"lj864": bench_lj864, "water": bench_water, "batch": bench_batch,
+             "mixture": bench_mixture}
+    if args.workload in other:
+        return other[args.workload](args)
 
     from moleculardynamics_amd import native
 

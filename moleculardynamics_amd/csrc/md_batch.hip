@@ -18,9 +18,9 @@
 //      made by the autotuner), gathering partners from LDS; FP32 pair terms
 //      with 64-bit fixed-point accumulation (or FP64); the final B half-kick
 //      and the replica's E, W, KE.
-// The step counter lives on the device and the last workgroup advances it,
-// so a segment of identical launches can be captured in a HIP graph and
-// replayed.
+// The step counter lives on the device and a one-thread kernel after each
+// step advances it, so a segment of identical launches can be captured in a
+// HIP graph and replayed.
 #include "md_common.hpp"
 #include "md_batch.hpp"
 
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
                 if (j == i) continue;
                 const bool flip = i > j;
                 double dx, dy, dz;
-                pair_delta(R.box, pi, make_double4(xs[j], ys[j], zs[j], 0.0), flip, dx, dy, dz);
+                row_delta(R.box, pi, make_double4(xs[j], ys[j], zs[j], 0.0), flip, dx, dy, dz);
                 if (norm2_ref(dx, dy, dz) < a.rl2) {
                     if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
                     ++cnt;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
     const int sub = threadIdx.x % tpa;
     double e64 = 0.0, w64 = 0.0, ke = 0.0;
     long long ef = 0, wf = 0;
-    int sat = 0;
+    float big = 0.0f;
     for (int base = 0; base < n; base += groups) {
         const int i = base + static_cast<int>(threadIdx.x) / tpa;
         const bool valid = i < n;
@@ -183,24 +183,29 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
             const int cnt = min(a.ncnt[g], a.cap);
             const uint32_t* row = a.nbr + g * a.cap;
             const double xi = xs[i], yi = ys[i], zi = zs[i];
+            unsigned long long bx = 0, by = 0, bz = 0, be = 0, bw = 0, nt = 0;
             for (int k = sub; k < cnt; k += tpa) {
                 const uint32_t jj = row[k];
                 const int j = static_cast<int>(jj & kIndexMask);
                 const bool flip = (jj & kFlipBit) != 0;
                 double dx, dy, dz;
-                pair_delta(R.box, make_double4(xi, yi, zi, 0.0), make_double4(xs[j], ys[j], zs[j], 0.0),
-                           flip, dx, dy, dz);
+                row_delta(R.box, make_double4(xi, yi, zi, 0.0), make_double4(xs[j], ys[j], zs[j], 0.0),
+                          flip, dx, dy, dz);
                 if (MIXED) {
                     const float xf = static_cast<float>(dx), yf = static_cast<float>(dy),
                                 zf = static_cast<float>(dz);
                     float inv, u, ww;
                     if (lj_terms_f32(norm2_ref(dx, dy, dz), a.lj.rc2, a.lj.sig2f, a.lj.epsf, inv, u, ww)) {
-                        ef += to_fix(u, &sat);
-                        wf += to_fix(ww, &sat);
-                        const float c = flip ? -(ww * inv) : ww * inv;
-                        ax += to_fix(c * xf, &sat);
-                        ay += to_fix(c * yf, &sat);
-                        az += to_fix(c * zf, &sat);
+                        const float c = ww * inv;
+                        const float tx = c * xf, ty = c * yf, tz = c * zf;
+                        be += fix_biased(u);
+                        bw += fix_biased(ww);
+                        bx += fix_biased(tx);
+                        by += fix_biased(ty);
+                        bz += fix_biased(tz);
+                        nt += 1;
+                        big = fmaxf(big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
+                                               fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
                     }
                 } else {
                     const double r2 = norm2_ref(dx, dy, dz);
@@ -211,12 +216,19 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
                         e64 += 4.0 * a.lj.eps * (s6 * s6 - s6);
                         const double ww = 24.0 * a.lj.eps * (2.0 * s6 * s6 - s6);
                         w64 += ww;
-                        const double c = flip ? -(ww * inv) : ww * inv;
+                        const double c = ww * inv;
                         fx += c * dx;
                         fy += c * dy;
                         fz += c * dz;
                     }
                 }
+            }
+            if (MIXED) {
+                ax = fix_value(bx, nt);
+                ay = fix_value(by, nt);
+                az = fix_value(bz, nt);
+                ef += fix_value(be, nt);
+                wf += fix_value(bw, nt);
             }
         }
         // combine the tpa lanes of an atom (lanes of a group are adjacent)
@@ -250,7 +262,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
             }
         }
     }
-    if (sat) atomicOr(a.fix_sat, 1);
+    if (!(big < kFixLimit)) atomicOr(a.fix_sat, 1);
     if (a.thermo != nullptr) {
         double E, W;
         if (MIXED) {

@@ -254,16 +254,23 @@ __device__ __forceinline__ void wave_top2(double& t1, double& t2) {
     }
 }
 
-// Float to 64-bit fixed point (scale 2^32), round to nearest even, via the
-// 1.5*2^52 bias; values beyond +-2^19 saturate and raise *sat.
-__device__ __forceinline__ long long to_fix(float v, int* sat) {
-    const double d = static_cast<double>(v) * kFixScale;
-    if (fabs(d) < 2251799813685248.0) {  // 2^51
-        const double t = d + kMagic;
-        return __double_as_longlong(t) - __double_as_longlong(kMagic);
-    }
-    atomicOr(sat, 1);
-    return d > 0 ? (1LL << 62) : -(1LL << 62);
+// 64-bit fixed point (scale 2^32) by biased accumulation: every term adds
+// the bit pattern of fma(v, 2^32, 1.5 * 2^52), whose value is
+// round(v * 2^32) + 1.5 * 2^52 (ties to even) while |v| < 2^19 (v * 2^32 is
+// exact in FP64, so the FMA rounds once). The bias is removed once per
+// accumulator of n terms at the end, in 64-bit modular arithmetic: one
+// conversion, one FMA and one 64-bit add per term, no branches. Callers track
+// the largest |term| and check it against kFixLimit.
+constexpr float kFixLimit = 524288.0f;  // 2^19
+
+__device__ __forceinline__ unsigned long long fix_biased(float v) {
+    return static_cast<unsigned long long>(
+        __double_as_longlong(fma(static_cast<double>(v), kFixScale, kMagic)));
+}
+
+__device__ __forceinline__ long long fix_value(unsigned long long acc, unsigned long long n) {
+    const unsigned long long bias = static_cast<unsigned long long>(__double_as_longlong(kMagic));
+    return static_cast<long long>(acc - n * bias);
 }
 
 }  // namespace mdk
