@@ -44,10 +44,6 @@ struct md_ctx {
     int* ncnt = nullptr;
     long npad = 0;
     int cap = 0, cap_user = 0;
-    mdk::Tiles tiles{};
-    int tile_seg = 0;        // cells per tile segment (0: chosen from the density)
-    bool tiles_off = false;  // tiles could not hold the candidates: untiled kernels
-    long parts_alloc = 0;    // workgroup partials allocated (3 per workgroup)
     long nbr_alloc = 0;
 
     // cells at the LJ cutoff (MD_PAIRS_CELLS)
@@ -196,7 +192,7 @@ int auto_capacity(const md_ctx* c) {
     const double mean = 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho;
     long cap = static_cast<long>(std::ceil(2.0 * mean + 32.0));
     cap = std::min(cap, std::max(1L, c->n - 1));
-    return static_cast<int>((cap + 15) / 16 * 16);  // octet and hexadecet layouts
+    return static_cast<int>((cap + 7) / 8 * 8);  // the octet layout needs a multiple of 8
 }
 
 md_status alloc_neigh(md_ctx* c) {
@@ -211,31 +207,8 @@ md_status alloc_neigh(md_ctx* c) {
         MD_HIP(c, alloc(c->tile_sum, mdk::scan_tiles(c->ncell)));
         c->ncell_alloc = c->ncell;
     }
-    // Tiles: about 0.8 kTileCap staged candidates expected per tile, i.e.
-    // 9 (S + 2) cells of the mean occupancy for S cells per segment.
-    c->tiles = mdk::Tiles{};
-    if (c->grid.usable && !c->tiles_off) {
-        const int nx = c->grid.nc[0];
-        int seg = c->tile_seg;
-        if (seg <= 0) {
-            const double avg = static_cast<double>(c->n) / static_cast<double>(c->ncell);
-            seg = static_cast<int>(std::floor(0.8 * mdk::kTileCap / (9.0 * std::max(avg, 1e-3)) - 2.0));
-        }
-        seg = std::max(1, std::min({seg, nx, mdk::kTileSegMax}));
-        c->tile_seg = seg;
-        c->tiles.on = 1;
-        c->tiles.nseg = (nx + seg - 1) / seg;
-        c->tiles.count = c->tiles.nseg * c->grid.nc[1] * c->grid.nc[2];
-    }
-    const long parts = std::max<long>(blocks_of(c->n), c->tiles.count);
-    if (parts > c->parts_alloc) {
-        MD_HIP(c, alloc(c->part_d, 4 * parts + 8));
-        MD_HIP(c, alloc(c->part_l, 3 * parts + 8));
-        c->parts_alloc = parts;
-    }
     c->npad = (c->n + 63) / 64 * 64;
-    // a multiple of 16: the octet (32-bit) and hexadecet (16-bit) layouts
-    c->cap = c->cap_user > 0 ? (c->cap_user + 15) / 16 * 16 : auto_capacity(c);
+    c->cap = c->cap_user > 0 ? (c->cap_user + 7) / 8 * 8 : auto_capacity(c);
     const long need = static_cast<long>(c->cap) * c->npad;
     if (need > c->nbr_alloc) {
         MD_HIP(c, alloc(c->nbr, need));
@@ -256,7 +229,6 @@ mdk::NeighArgs neigh_args(md_ctx* c) {
     a.orig2 = c->orig2; a.cell2 = c->cell2;
     a.cell_of = c->cell_of; a.cell_count = c->cell_count; a.cell_start = c->cell_start;
     a.cell_fill = c->cell_fill; a.order = c->order; a.tile_sum = c->tile_sum;
-    a.tiles = c->tiles;
     a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
     a.sc = c->sc;
     a.flag = &c->sc->rebuild_flag;
@@ -270,8 +242,6 @@ mdk::ForceArgs force_args(md_ctx* c, int precision) {
     a.lj = c->ljc;
     a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.orig = c->orig;
     a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
-    a.grid = c->grid; a.tiles = c->tiles; a.cell_start = c->cell_start;
-    a.nparts = c->tiles.on ? c->tiles.count : static_cast<int>(blocks_of(c->n));
     a.mixed = precision == MD_MIXED ? 1 : 0;
     a.part_d = c->part_d; a.part_l = c->part_l;
     a.sc = c->sc;
@@ -297,10 +267,9 @@ mdk::StepArgs step_args(md_ctx* c) {
 }
 
 // Builds the list now (flag forced to 1), growing the row capacity until no
-// row overflows (overflow bit 1) and shortening the tiles -- down to the
-// untiled kernels -- until every tile's candidates fit in LDS (bit 2).
+// row overflows.
 md_status build_now(md_ctx* c) {
-    for (int attempt = 0; attempt < 12; ++attempt) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
         mdk::launch_set_flag(&c->sc->rebuild_flag, 1, c->stream);
         hipEvent_t e0{};
         ev_begin(c, 1, &e0);
@@ -319,11 +288,7 @@ md_status build_now(md_ctx* c) {
         long long builds = s.builds - 1;
         MD_HIP(c, hipMemcpyAsync(&c->sc->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
         MD_HIP(c, hipMemcpyAsync(&c->sc->builds, &builds, sizeof(long long), hipMemcpyHostToDevice, c->stream));
-        if (s.overflow & 2) {
-            if (c->tile_seg > 1) c->tile_seg = std::max(1, c->tile_seg * 2 / 3);
-            else c->tiles_off = true;
-        }
-        if (s.overflow & 1) c->cap_user = std::min<long>(2L * c->cap, std::max(1L, c->n - 1));
+        c->cap_user = std::min<long>(2L * c->cap, std::max(1L, c->n - 1));
         st = alloc_neigh(c);
         if (st != MD_OK) return st;
     }
@@ -339,8 +304,7 @@ md_status md_ctx_create(void* hip_stream, md_ctx** out) {
     *out = nullptr;
     md_ctx* c = new (std::nothrow) md_ctx();
     if (c == nullptr) return MD_ERR_RUNTIME;
-    if (hipGetDevice(&c->device) != hipSuccess || mdk::neigh_tiles_init() != hipSuccess ||
-        mdk::force_tiles_init() != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess) {
         delete c;
         return MD_ERR_HIP;
     }
@@ -425,9 +389,6 @@ md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, 
     const long nb = blocks_of(n);
     MD_HIP(c, alloc(c->part_d, 4 * nb + 8));
     MD_HIP(c, alloc(c->part_l, 3 * nb + 8));
-    c->parts_alloc = nb;
-    c->tile_seg = 0;
-    c->tiles_off = false;
 
     std::vector<double4> p(static_cast<size_t>(n)), v(static_cast<size_t>(n)), f(static_cast<size_t>(n));
     std::vector<int> id(static_cast<size_t>(n));
@@ -669,8 +630,6 @@ md_status md_neigh_config(md_ctx* c, double cutoff, double skin) {
     c->skin = skin;
     c->neigh_configured = true;
     c->list_built = false;
-    c->tile_seg = 0;
-    c->tiles_off = false;
     if (c->n > 0) return alloc_neigh(c);
     return MD_OK;
 }
@@ -728,46 +687,13 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
     MD_HIP(c, hipMemcpy(nb.data(), c->nbr, nb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     std::vector<uint64_t> keys;
     keys.reserve(static_cast<size_t>(n) * 40);
-    auto add = [&](long t, uint32_t j) {
-        const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
-        if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
-    };
-    if (c->tiles.on) {
-        // 16-bit entries index the tile's staged candidates: rebuild each
-        // tile's staged slot list (the order of tile_stage_table) on the host
-        const mdk::Grid& g = c->grid;
-        const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2], nseg = c->tiles.nseg;
-        std::vector<int> cs(static_cast<size_t>(c->ncell + 1));
-        MD_HIP(c, hipMemcpy(cs.data(), c->cell_start, cs.size() * sizeof(int), hipMemcpyDeviceToHost));
-        const uint16_t* nb16 = reinterpret_cast<const uint16_t*>(nb.data());
-        std::vector<int> staged;
-        for (long tile = 0; tile < c->tiles.count; ++tile) {
-            const int k = static_cast<int>(tile % nseg), y = static_cast<int>((tile / nseg) % ny);
-            const int z = static_cast<int>(tile / (static_cast<long>(nseg) * ny));
-            const int x0 = k * nx / nseg, x1 = (k + 1) * nx / nseg;
-            staged.clear();
-            for (int r = 0; r < 9; ++r) {
-                const int yy = (y + (r % 3) - 1 + ny) % ny, zz = (z + (r / 3) - 1 + nz) % nz;
-                for (int jx = 0; jx < x1 - x0 + 2; ++jx) {
-                    const int cell = (x0 - 1 + jx + nx) % nx + nx * (yy + ny * zz);
-                    for (int s = cs[cell]; s < cs[cell + 1]; ++s) staged.push_back(s);
-                }
-            }
-            const int rb = nx * (y + ny * z);
-            for (long t = cs[rb + x0]; t < cs[rb + x1]; ++t) {
-                const int kk = std::min(cnt[t], c->cap);
-                for (int q = 0; q < kk; ++q) {
-                    const uint32_t m = nb16[mdk::hex_index(t, q, c->cap)] & 0x7fffu;
-                    if (m >= staged.size()) return fail(c, MD_ERR_RUNTIME, "corrupt tiled list entry");
-                    add(t, static_cast<uint32_t>(staged[m]));
-                }
-            }
-        }
-    } else {
-        for (long t = 0; t < n; ++t) {
-            const int k = std::min(cnt[t], c->cap);
-            for (int q = 0; q < k; ++q)
-                add(t, nb[static_cast<size_t>(mdk::oct_index(t, q, c->cap))] & mdk::kIndexMask);
+    for (long t = 0; t < n; ++t) {
+        const int k = std::min(cnt[t], c->cap);
+        for (int q = 0; q < k; ++q) {
+            const uint32_t e = nb[static_cast<size_t>(mdk::oct_index(t, q, c->cap))];
+            const uint32_t j = e & mdk::kIndexMask;
+            const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
+            if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
         }
     }
     std::sort(keys.begin(), keys.end());

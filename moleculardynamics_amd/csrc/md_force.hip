@@ -1,11 +1,9 @@
 // Lennard-Jones pair kernels.
 //
-// k_lj_tiles (usable cell grids) and k_lj (otherwise): one thread per atom
-// over its row of the full Verlet list (each unordered pair is evaluated
-// from both atoms, so every thread owns its atom's force outright: no
-// atomics, a fixed per-thread summation order, and a deterministic result).
-// The tiled kernel reads partner positions from its tile's LDS stage, the
-// untiled one gathers them from global memory. Pair arithmetic follows
+// k_lj: one thread per atom over its row of the full Verlet list (each
+// unordered pair is evaluated from both atoms, so every thread owns its
+// atom's force outright: no atomics, a fixed per-thread summation order, and
+// a deterministic result). Pair arithmetic follows
 // forcefield/lennard_jones.cpp:50-88:
 //   d = minimumImage(r_i - r_j) (i<j by upload index), r2 = |d|^2,
 //   skip unless r2 < rc2, s2 = sig^2/r2, s6 = s2^3,
@@ -19,14 +17,12 @@
 // force components are FP32; per-atom force, energy and virial accumulate
 // in 64-bit fixed point (scale 2^32), so sums are exact integer additions.
 //
-// E and W come out of a fixed reduction tree: wave shuffles, an LDS step
-// over the workgroup's waves, per-workgroup partials, and a one-workgroup
-// finish kernel summing the partials in index order. The finish kernel also
-// records the step's thermo row and advances the Nose-Hoover chain when one
-// is attached.
+// E and W come out of a fixed reduction tree: wave shuffles, a 4-wave LDS
+// step, per-workgroup partials, and a one-workgroup finish kernel summing the
+// partials in index order. The finish kernel also records the step's thermo
+// row and advances the Nose-Hoover chain when one is attached.
 #include "md_common.hpp"
 #include "md_kernels.hpp"
-#include "md_tiles.hpp"
 
 #include "../../include/moldyn_hip.h"
 
@@ -269,128 +265,59 @@ __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, co
     }
 }
 
-// ---- tiled row walk: 16-bit entries index the tile's LDS-staged partners ----
-
-// Four entries (two 32-bit words, low half first) against LDS positions.
-template <class Acc, class PairFn>
-__device__ __forceinline__ void quad16(const ForceArgs& a, const double4& pi, uint32_t w0,
-                                       uint32_t w1, const double* lx, const double* ly,
-                                       const double* lz, Acc& q, PairFn pf) {
-    const uint32_t e0 = w0 & 0xffffu, e1 = w0 >> 16, e2 = w1 & 0xffffu, e3 = w1 >> 16;
-    const uint32_t m0 = e0 & 0x7fffu, m1 = e1 & 0x7fffu, m2 = e2 & 0x7fffu, m3 = e3 & 0x7fffu;
-    const double4 p0 = make_double4(lx[m0], ly[m0], lz[m0], 0.0);
-    const double4 p1 = make_double4(lx[m1], ly[m1], lz[m1], 0.0);
-    const double4 p2 = make_double4(lx[m2], ly[m2], lz[m2], 0.0);
-    const double4 p3 = make_double4(lx[m3], ly[m3], lz[m3], 0.0);
-    pf(a, pi, (e0 & kTileFlip) ? kFlipBit : 0u, p0, q);
-    pf(a, pi, (e1 & kTileFlip) ? kFlipBit : 0u, p1, q);
-    pf(a, pi, (e2 & kTileFlip) ? kFlipBit : 0u, p2, q);
-    pf(a, pi, (e3 & kTileFlip) ? kFlipBit : 0u, p3, q);
-}
-
-// Row walk over the hexadecet layout (16 entries = 32 bytes per group, the
-// next group's entries loaded while the current one is evaluated), in row
-// order.
-template <class Acc, class PairFn>
-__device__ __forceinline__ void walk_tile_row(const ForceArgs& a, long t, int cnt, const double4& pi,
-                                              const double* lx, const double* ly, const double* lz,
-                                              Acc& q, PairFn pf) {
-    const uint16_t* row = reinterpret_cast<const uint16_t*>(a.nbr) + hex_index(t, 0, a.cap);
-    const int full = cnt >> 4;
-    u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
-    if (full > 0) load_octet(reinterpret_cast<const uint32_t*>(row), lo, hi);
-    for (int g = 0; g < full; ++g) {
-        u32x4 nlo = {0u, 0u, 0u, 0u}, nhi = {0u, 0u, 0u, 0u};
-        if (g + 1 < full) load_octet(reinterpret_cast<const uint32_t*>(row + (g + 1) * 1024), nlo, nhi);
-        quad16(a, pi, lo.x, lo.y, lx, ly, lz, q, pf);
-        quad16(a, pi, lo.z, lo.w, lx, ly, lz, q, pf);
-        quad16(a, pi, hi.x, hi.y, lx, ly, lz, q, pf);
-        quad16(a, pi, hi.z, hi.w, lx, ly, lz, q, pf);
-        lo = nlo;
-        hi = nhi;
-    }
-    const int rem = cnt & 15;
-    if (rem) {
-        load_octet(reinterpret_cast<const uint32_t*>(row + full * 1024), lo, hi);
-        const uint32_t wv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-#pragma unroll
-        for (int r = 0; r < 15; ++r) {
-            if (r < rem) {
-                const uint32_t e = (wv[r >> 1] >> (16 * (r & 1))) & 0xffffu;
-                const uint32_t m = e & 0x7fffu;
-                pf(a, pi, (e & kTileFlip) ? kFlipBit : 0u,
-                   make_double4(lx[m], ly[m], lz[m], 0.0), q);
-            }
-        }
-    }
-}
-
-// ---- per-atom force, kick and sums (shared by both pair kernels) ----------
-
-struct AtomSums {
+template <bool LFS, bool MIXED, bool KICK>
+__global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
+    __shared__ double sm_d[kBlock / kWave];
+    __shared__ long long sm_l[kBlock / kWave];
+    const long blk = xcd_block();
+    if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
+    const long t = blk * kBlock + threadIdx.x;
     double e = 0.0, w = 0.0, ke = 0.0;
     long long ef = 0, wf = 0, np = 0;
-};
-
-template <bool LFS, bool MIXED, bool KICK, class Walk>
-__device__ __forceinline__ void atom_step(const ForceArgs& a, long t, Walk walk, AtomSums& s) {
-    const double4 pi = a.pos[t];
-    const int cnt = min(a.ncnt[t], a.cap);
-    double fx, fy, fz;
-    if (!MIXED) {
-        Acc64 q;
-        walk(cnt, pi, q, pair64<LFS>);
-        fx = q.fx;
-        fy = q.fy;
-        fz = q.fz;
-        s.e += q.e;
-        s.w += q.w;
-        s.np += q.np;
-    } else {
-        AccFix q;
-        walk(cnt, pi, q, pair32<LFS>);
-        if (!(q.big < kFixLimit)) atomicOr(&a.sc->fix_saturated, 1);
-        fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
-        fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
-        fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
-        s.ef += fix_value(q.e, q.n);
-        s.wf += fix_value(q.w, q.n);
-        s.np += q.np;
+    if (t < a.n) {
+        const double4 pi = a.pos[t];
+        const int cnt = min(a.ncnt[t], a.cap);
+        double fx, fy, fz;
+        if (!MIXED) {
+            Acc64 q;
+            walk_row(a, t, cnt, pi, q, pair64<LFS>);
+            fx = q.fx;
+            fy = q.fy;
+            fz = q.fz;
+            e = q.e;
+            w = q.w;
+            np = q.np;
+        } else {
+            AccFix q;
+            walk_row(a, t, cnt, pi, q, pair32<LFS>);
+            if (!(q.big < kFixLimit)) atomicOr(&a.sc->fix_saturated, 1);
+            fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
+            fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
+            fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
+            ef = fix_value(q.e, q.n);
+            wf = fix_value(q.w, q.n);
+            np = q.np;
+        }
+        a.frc[t] = make_double4(fx, fy, fz, 0.0);
+        if (KICK) {
+            double4 v = a.vel[t];
+            const double ch = a.half_dt * (1.0 / v.w);
+            v.x = fma(ch, fx, v.x);
+            v.y = fma(ch, fy, v.y);
+            v.z = fma(ch, fz, v.z);
+            a.vel[t] = v;
+            ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+        }
     }
-    a.frc[t] = make_double4(fx, fy, fz, 0.0);
-    if (KICK) {
-        double4 v = a.vel[t];
-        const double ch = a.half_dt * (1.0 / v.w);
-        v.x = fma(ch, fx, v.x);
-        v.y = fma(ch, fy, v.y);
-        v.z = fma(ch, fz, v.z);
-        a.vel[t] = v;
-        s.ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
-    }
-}
-
-// Sum over a workgroup of nw waves; valid in thread 0.
-template <class T, class WaveFn>
-__device__ __forceinline__ T group_reduce(T v, T* smem, WaveFn wf, int nw) {
-    v = wf(v);
-    if ((threadIdx.x & (kWave - 1)) == 0) smem[threadIdx.x / kWave] = v;
-    __syncthreads();
-    T r = T(0);
-    if (threadIdx.x == 0)
-        for (int w = 0; w < nw; ++w) r += smem[w];
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ void write_partials(const ForceArgs& a, long b, const AtomSums& s,
-                                               double* sm_d, long long* sm_l, int nw) {
-    const double be = group_reduce(s.e, sm_d, wave_sum, nw);
-    const double bw = group_reduce(s.w, sm_d, wave_sum, nw);
-    const double bk = group_reduce(s.ke, sm_d, wave_sum, nw);
-    const long long bef = group_reduce(s.ef, sm_l, wave_sum_ll, nw);
-    const long long bwf = group_reduce(s.wf, sm_l, wave_sum_ll, nw);
-    const long long bnp = group_reduce(s.np, sm_l, wave_sum_ll, nw);
+    // workgroup partials
+    const double be = block_reduce(e, sm_d, wave_sum);
+    const double bw = block_reduce(w, sm_d, wave_sum);
+    const double bk = block_reduce(ke, sm_d, wave_sum);
+    const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
+    const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
+    const long long bnp = block_reduce(np, sm_l, wave_sum_ll);
     if (threadIdx.x == 0) {
+        const long b = blk;
         a.part_d[3 * b + 0] = be;
         a.part_d[3 * b + 1] = bw;
         a.part_d[3 * b + 2] = bk;
@@ -400,83 +327,16 @@ __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, const
     }
 }
 
-// Untiled pair kernel (unusable grids): 256 atoms per workgroup in the
-// XCD-aware order, partners gathered from global memory.
-template <bool LFS, bool MIXED, bool KICK>
-__global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
-    __shared__ double sm_d[kBlock / kWave];
-    __shared__ long long sm_l[kBlock / kWave];
-    const long blk = xcd_block();
-    if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
-    const long t = blk * kBlock + threadIdx.x;
-    AtomSums s;
-    if (t < a.n) {
-        auto walk = [&](int cnt, const double4& pi, auto& q, auto pf) { walk_row(a, t, cnt, pi, q, pf); };
-        atom_step<LFS, MIXED, KICK>(a, t, walk, s);
-    }
-    write_partials(a, blk, s, sm_d, sm_l, kBlock / kWave);
-}
-
-// Tiled pair kernel: one workgroup per tile. The tile's staged candidates
-// (the same cells, in the same order, as the tiled build) have their
-// positions copied into LDS with coalesced loads; every partner of every row
-// is then read from LDS.
-constexpr size_t kForceTileLds = static_cast<size_t>(kTileCap) * 3 * sizeof(double);
-
-template <bool LFS, bool MIXED, bool KICK>
-__global__ __launch_bounds__(kTileThreads) void k_lj_tiles(ForceArgs a) {
-    extern __shared__ double lds[];
-    __shared__ TileStage ts;
-    __shared__ double sm_d[kTileThreads / kWave];
-    __shared__ long long sm_l[kTileThreads / kWave];
-    const long tile = xcd_block();
-    if (tile >= a.tiles.count) return;  // padding workgroup of the XCD-aware grid
-    tile_stage_table(ts, a.grid, a.tiles.nseg, a.cell_start, tile);
-    // a successful build guarantees total <= kTileCap; the clamp keeps a
-    // failed (flagged) build's tile inside the LDS arrays
-    const int total = min(ts.off[ts.nq], kTileCap);
-    double* lx = lds;
-    double* ly = lx + kTileCap;
-    double* lz = ly + kTileCap;
-    for (int m = threadIdx.x; m < total; m += blockDim.x) {
-        const double4 p = a.pos[staged_slot(ts, a.cell_start, m)];
-        lx[m] = p.x;
-        ly[m] = p.y;
-        lz[m] = p.z;
-    }
-    __syncthreads();
-    const int nx = a.grid.nc[0];
-    const int rb = nx * (ts.y + a.grid.nc[1] * ts.z);
-    const int first = a.cell_start[rb + ts.x0], end = a.cell_start[rb + ts.x1];
-    AtomSums s;
-    for (long t = first + static_cast<long>(threadIdx.x); t < end; t += blockDim.x) {
-        auto walk = [&](int cnt, const double4& pi, auto& q, auto pf) {
-            walk_tile_row(a, t, cnt, pi, lx, ly, lz, q, pf);
-        };
-        atom_step<LFS, MIXED, KICK>(a, t, walk, s);
-    }
-    write_partials(a, tile, s, sm_d, sm_l, kTileThreads / kWave);
-}
-
 __global__ __launch_bounds__(kBlock) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
 template <bool LFS, bool MIXED>
-void dispatch_kick(const ForceArgs& a, hipStream_t s) {
-    if (a.tiles.on) {
-        const dim3 g((static_cast<unsigned>(a.tiles.count) + kXcd - 1) / kXcd * kXcd);
-        if (a.kick)
-            hipLaunchKernelGGL((k_lj_tiles<LFS, MIXED, true>), g, dim3(kTileThreads), kForceTileLds, s, a);
-        else
-            hipLaunchKernelGGL((k_lj_tiles<LFS, MIXED, false>), g, dim3(kTileThreads), kForceTileLds, s, a);
-    } else {
-        const dim3 g(xcd_grid(a.n));
-        if (a.kick)
-            hipLaunchKernelGGL((k_lj<LFS, MIXED, true>), g, dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_lj<LFS, MIXED, false>), g, dim3(kBlock), 0, s, a);
-    }
+void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
+    if (a.kick)
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, true>), g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, false>), g, dim3(kBlock), 0, s, a);
 }
 
 // ---- direct (no list) evaluations: brute force and cells at the cutoff ----
@@ -670,36 +530,18 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
 }
 
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
+    const dim3 g(xcd_grid(a.n));
     if (a.lj.lfs) {
-        if (a.mixed) dispatch_kick<true, true>(a, s);
-        else dispatch_kick<true, false>(a, s);
+        if (a.mixed) dispatch_kick<true, true>(a, g, s);
+        else dispatch_kick<true, false>(a, g, s);
     } else {
-        if (a.mixed) dispatch_kick<false, true>(a, s);
-        else dispatch_kick<false, false>(a, s);
+        if (a.mixed) dispatch_kick<false, true>(a, g, s);
+        else dispatch_kick<false, false>(a, g, s);
     }
 }
 
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
-    const int nb = a.tiles.on ? a.tiles.count : static_cast<int>(grid_of(a.n));
-    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kBlock), 0, s, a, nb);
-}
-
-hipError_t force_tiles_init() {
-    const void* fns[] = {
-        reinterpret_cast<const void*>(k_lj_tiles<false, false, false>),
-        reinterpret_cast<const void*>(k_lj_tiles<false, false, true>),
-        reinterpret_cast<const void*>(k_lj_tiles<false, true, false>),
-        reinterpret_cast<const void*>(k_lj_tiles<false, true, true>),
-        reinterpret_cast<const void*>(k_lj_tiles<true, false, false>),
-        reinterpret_cast<const void*>(k_lj_tiles<true, false, true>),
-        reinterpret_cast<const void*>(k_lj_tiles<true, true, false>),
-        reinterpret_cast<const void*>(k_lj_tiles<true, true, true>)};
-    for (const void* f : fns) {
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 static_cast<int>(kForceTileLds));
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(grid_of(a.n)));
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

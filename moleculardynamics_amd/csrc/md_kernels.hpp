@@ -20,36 +20,9 @@ struct Grid {
     int usable;
 };
 
-// Tiles of a usable cell grid for the list build and the pair kernel. Tile
-// (k, y, z) owns the atoms of cells [x0, x1) of row (y, z), with
-// x0 = k nx / nseg and x1 = (k + 1) nx / nseg; its staged candidates are the
-// cells x0-1 .. x1 (mod nx) of the nine rows (y + dy, z + dz), dz outer and dy
-// inner over -1..1, in slot order within a cell -- for every atom of the
-// tile, its 27-cell stencil in the untiled build's order. The tiled build
-// stores 16-bit indices into that staged array (bit 15: orientation) in the
-// hexadecet layout (hex_index), and the tiled pair kernel stages the same
-// candidates' positions in LDS.
-constexpr int kTileSegMax = 16;    // cells per segment
-constexpr int kTileCap = 2304;     // staged candidates per tile
-constexpr int kTileThreads = 192;  // workgroup size of the tiled kernels
-constexpr uint32_t kTileFlip = 0x8000u;
-struct Tiles {
-    int on;     // 1: tiled build and pair kernel
-    int nseg;   // segments per row
-    int count;  // nseg * ny * nz
-};
-
-// Entry k of atom t in the 16-bit hexadecet layout: every 64 consecutive
-// atoms own a [cap/16][64][16] tile, so an atom's entries 16g..16g+15 are 32
-// contiguous bytes. cap is a multiple of 16.
-__host__ __device__ __forceinline__ long hex_index(long t, int k, int cap) {
-    return (((t >> 6) * static_cast<long>(cap >> 4) + (k >> 4)) * 64 + (t & 63)) * 16 + (k & 15);
-}
-
 struct NeighArgs {
     long n;
     Grid grid;
-    Tiles tiles;
     Box box;
     double rl2;  // (rc + skin)^2 as the reference computes it
     double4 *pos, *vel, *frc, *ref;
@@ -68,10 +41,6 @@ struct NeighArgs {
 void launch_rebuild(const NeighArgs& a, hipStream_t s);
 // Number of scan tiles (entries of NeighArgs::tile_sum) for ncell cells.
 long scan_tiles(long ncell);
-// Raise the dynamic-LDS limits of the tiled build and pair kernels (once per
-// device, before their first launch).
-hipError_t neigh_tiles_init();
-hipError_t force_tiles_init();
 
 // LJ pair kernel over the full Verlet list.
 struct LJConst {
@@ -92,11 +61,6 @@ struct ForceArgs {
     const int* ncnt;
     long npad;
     int cap;         // row capacity: rows longer than this overflowed and are clamped
-    // tiled pair kernel (tiles.on): the grid and cell ranges of the last build
-    Grid grid;
-    Tiles tiles;
-    const int* cell_start;
-    int nparts;      // per-workgroup partials written (tiles or 256-atom blocks)
     double half_dt;  // kick coefficient h (v += h/m f) when kick != 0
     int kick;
     int mixed;

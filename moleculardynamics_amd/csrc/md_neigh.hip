@@ -19,7 +19,6 @@
 // Invariant: cell_count is all zero between rebuilds (k_scan clears it).
 #include "md_common.hpp"
 #include "md_kernels.hpp"
-#include "md_tiles.hpp"
 
 namespace mdk {
 
@@ -288,111 +287,6 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     if (cnt > cap) atomicOr(&sc->overflow, 1);
 }
 
-// Tiled commit + build (usable grid). One workgroup per tile: the tile's
-// staged candidates (positions and upload indices, in the order of
-// tile_stage_table) are copied into LDS with coalesced loads, then each atom
-// of the tile scans its 27 cells there -- the same candidates in the same
-// order as k_commit_build, with the same exact test -- and stores the staged
-// index of each partner (16 bits, bit 15 the orientation). Entries collect
-// sixteen at a time in the thread's LDS column and go out as one 32-byte
-// write. A tile whose candidates exceed kTileCap sets bit 2 of the overflow
-// flag and builds nothing; the host then retries with shorter tiles.
-constexpr size_t kBuildTileLds =
-    static_cast<size_t>(kTileCap) * (3 * sizeof(double) + sizeof(int)) +
-    16 * kTileThreads * sizeof(uint16_t);
-
-__global__ __launch_bounds__(kTileThreads) void k_build_tiles(NeighArgs a) {
-    if (*a.flag == 0) return;
-    extern __shared__ double lds[];
-    __shared__ TileStage ts;
-    const long tile = xcd_block();
-    if (tile >= a.tiles.count) return;  // padding workgroup of the XCD-aware grid
-    tile_stage_table(ts, a.grid, a.tiles.nseg, a.cell_start, tile);
-    const int total = ts.off[ts.nq];
-    const int nx = a.grid.nc[0];
-    const int rb = nx * (ts.y + a.grid.nc[1] * ts.z);
-    const int first = a.cell_start[rb + ts.x0], end = a.cell_start[rb + ts.x1];
-    if (total > kTileCap) {
-        // too many candidates for LDS: commit the new order (the state must
-        // stay whole) and flag the tile; no rows are built
-        for (int t = first + static_cast<int>(threadIdx.x); t < end; t += blockDim.x) {
-            const double4 pi = a.pos2[t];
-            const int oi = a.orig2[t];
-            a.pos[t] = pi;
-            a.ref[t] = pi;
-            a.vel[t] = a.vel2[t];
-            a.frc[t] = a.frc2[t];
-            a.orig[t] = oi;
-            a.slot[oi] = t;
-            a.cell_of[t] = a.cell2[t];
-            a.ncnt[t] = 0;
-        }
-        if (threadIdx.x == 0) atomicOr(&a.sc->overflow, 2);
-        return;
-    }
-    double* lx = lds;
-    double* ly = lx + kTileCap;
-    double* lz = ly + kTileCap;
-    int* lo = reinterpret_cast<int*>(lz + kTileCap);
-    uint16_t* st = reinterpret_cast<uint16_t*>(lo + kTileCap);
-    for (int m = threadIdx.x; m < total; m += blockDim.x) {
-        const int s = staged_slot(ts, a.cell_start, m);
-        const double4 p = a.pos2[s];
-        lx[m] = p.x;
-        ly[m] = p.y;
-        lz[m] = p.z;
-        lo[m] = a.orig2[s];
-    }
-    __syncthreads();
-    uint16_t* nbr16 = reinterpret_cast<uint16_t*>(a.nbr);
-    for (int t = first + static_cast<int>(threadIdx.x); t < end; t += blockDim.x) {
-        const double4 pi = a.pos2[t];
-        const int oi = a.orig2[t];
-        const int ci = a.cell2[t];
-        a.pos[t] = pi;
-        a.ref[t] = pi;
-        a.vel[t] = a.vel2[t];
-        a.frc[t] = a.frc2[t];
-        a.orig[t] = oi;
-        a.slot[oi] = t;
-        a.cell_of[t] = ci;
-        const int j = ci % nx - ts.x0 + 1;  // staged column of the home cell
-        uint16_t* row = nbr16 + hex_index(t, 0, a.cap);
-        auto flush = [&](int grp) {
-            uint32_t w[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                w[q] = static_cast<uint32_t>(st[(2 * q) * kTileThreads + threadIdx.x]) |
-                       (static_cast<uint32_t>(st[(2 * q + 1) * kTileThreads + threadIdx.x]) << 16);
-            uint4* dst = reinterpret_cast<uint4*>(row + static_cast<long>(grp) * 1024);
-            dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-            dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
-        };
-        int cnt = 0;
-        for (int r = 0; r < 9; ++r) {
-            const int b = ts.off[r * ts.s2 + j - 1], e = ts.off[r * ts.s2 + j + 2];
-            for (int m = b; m < e; ++m) {
-                const int oj = lo[m];
-                const bool flip = oi > oj;
-                double dx, dy, dz;
-                row_delta(a.box, pi, make_double4(lx[m], ly[m], lz[m], 0.0), flip, dx, dy, dz);
-                if (norm2_ref(dx, dy, dz) < a.rl2 && oj != oi) {
-                    if (cnt < a.cap) {
-                        st[(cnt & 15) * kTileThreads + threadIdx.x] =
-                            static_cast<uint16_t>(m | (flip ? kTileFlip : 0u));
-                        if ((cnt & 15) == 15) flush(cnt >> 4);
-                    }
-                    ++cnt;
-                }
-            }
-        }
-        const int kept = cnt < a.cap ? cnt : a.cap;
-        if (kept & 15) flush(kept >> 4);
-        a.ncnt[t] = cnt;
-        if (cnt > a.cap) atomicOr(&a.sc->overflow, 1);
-    }
-}
-
 inline unsigned grid_capped(long n) {
     const long g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g < static_cast<long>(kGridCap) ? (g > 0 ? g : 1) : kGridCap);
@@ -413,23 +307,11 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_cells, gc, b, 0, s, a.cell_start, ncell, a.orig, a.order, a.flag);
     hipLaunchKernelGGL(k_permute, gn, b, 0, s, a.order, n, a.pos, a.vel, a.frc, a.orig, a.cell_of,
                        a.pos2, a.vel2, a.frc2, a.orig2, a.cell2, a.flag);
-    if (a.tiles.on) {
-        const unsigned g8 = (static_cast<unsigned>(a.tiles.count) + kXcd - 1) / kXcd * kXcd;
-        hipLaunchKernelGGL(k_build_tiles, dim3(g8), dim3(kTileThreads), kBuildTileLds, s, a);
-    } else {
-        hipLaunchKernelGGL(k_commit_build, dim3(xcd_grid(n)), b, 0, s, n, a.grid, a.box, a.rl2,
-                           a.pos2, a.vel2, a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel,
-                           a.frc, a.ref, a.orig, a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc,
-                           a.flag);
-    }
+    hipLaunchKernelGGL(k_commit_build, dim3(xcd_grid(n)), b, 0, s, n, a.grid, a.box, a.rl2, a.pos2,
+                       a.vel2, a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref,
+                       a.orig, a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
 }
 
 long scan_tiles(long ncell) { return (ncell + kScanTile - 1) / kScanTile; }
-
-hipError_t neigh_tiles_init() {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_build_tiles),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(kBuildTileLds));
-}
 
 }  // namespace mdk
