@@ -40,6 +40,10 @@ struct md_ctx {
     long ncell = 0, ncell_alloc = 0;
     int *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *order = nullptr;
     int* tile_sum = nullptr;
+    // periodic fixed-point positions of the list (see alloc_neigh): the
+    // committed order and the permuted order of a rebuild
+    int4 *qpos = nullptr, *qpos2 = nullptr;
+    mdk::Quant quant{};
     uint32_t* nbr = nullptr;
     int* ncnt = nullptr;
     long npad = 0;
@@ -207,6 +211,30 @@ md_status alloc_neigh(md_ctx* c) {
         MD_HIP(c, alloc(c->tile_sum, mdk::scan_tiles(c->ncell)));
         c->ncell_alloc = c->ncell;
     }
+    // Periodic fixed-point positions (unit L / 2^32 per axis) are kept when
+    // the cell grid is usable (every list pair then lies well inside half a
+    // box, where the int32 wrap is the minimum image) and the unit is at most
+    // 2^-25 rc, a quarter of FP32's relative precision at the cutoff
+    // (L <= 128 rc). The decision bands bound the r2 error: per axis the
+    // difference is within one unit u, so |r2' - r2| <= 2 sqrt(3) r u + 3 u^2.
+    double umax = 0.0;
+    for (int d = 0; d < 3; ++d) umax = std::max(umax, c->L[d] / 4294967296.0);
+    const bool fixed_point = c->grid.usable && c->rc > 0.0 && umax <= c->rc / 33554432.0;
+    if (fixed_point) {
+        MD_HIP(c, alloc(c->qpos, c->n));
+        MD_HIP(c, alloc(c->qpos2, c->n));
+        for (int d = 0; d < 3; ++d) {
+            c->quant.inv[d] = 4294967296.0 / c->L[d];
+            c->quant.unit[d] = c->L[d] / 4294967296.0;
+        }
+        const double rl = c->rc + c->skin;
+        c->quant.band_l = 4.0 * umax / rl + 4.0 * (umax / rl) * (umax / rl) + 1e-12;
+        c->quant.band_c = 4.0 * umax / c->rc + 4.0 * (umax / c->rc) * (umax / c->rc) + 1e-12;
+    } else {
+        release(c->qpos);
+        release(c->qpos2);
+        c->quant = mdk::Quant{};
+    }
     c->npad = (c->n + 63) / 64 * 64;
     c->cap = c->cap_user > 0 ? (c->cap_user + 7) / 8 * 8 : auto_capacity(c);
     const long need = static_cast<long>(c->cap) * c->npad;
@@ -229,6 +257,7 @@ mdk::NeighArgs neigh_args(md_ctx* c) {
     a.orig2 = c->orig2; a.cell2 = c->cell2;
     a.cell_of = c->cell_of; a.cell_count = c->cell_count; a.cell_start = c->cell_start;
     a.cell_fill = c->cell_fill; a.order = c->order; a.tile_sum = c->tile_sum;
+    a.qpos = c->qpos; a.qpos2 = c->qpos2; a.quant = c->quant;
     a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
     a.sc = c->sc;
     a.flag = &c->sc->rebuild_flag;
@@ -243,6 +272,9 @@ mdk::ForceArgs force_args(md_ctx* c, int precision) {
     a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.orig = c->orig;
     a.nbr = c->nbr; a.ncnt = c->ncnt; a.npad = c->npad; a.cap = c->cap;
     a.mixed = precision == MD_MIXED ? 1 : 0;
+    // the mixed kernel reads the quantised positions when the list keeps them
+    a.qpos = (a.mixed && c->qpos != nullptr) ? c->qpos : nullptr;
+    a.quant = c->quant;
     a.part_d = c->part_d; a.part_l = c->part_l;
     a.sc = c->sc;
     return a;
@@ -349,6 +381,7 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->stage);
     release(c->cell_count); release(c->cell_start); release(c->cell_fill); release(c->order);
     release(c->tile_sum);
+    release(c->qpos); release(c->qpos2);
     release(c->nbr); release(c->ncnt);
     release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
     release(c->ci_atoms);
@@ -521,6 +554,8 @@ md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, dou
     if (source == MD_PAIRS_VERLET) {
         if (!c->list_built) return fail(c, MD_ERR_STATE, "the Verlet list has not been built");
         mdk::ForceArgs a = force_args(c, precision);
+        if (a.qpos != nullptr)
+            mdk::launch_quantize(c->n, c->quant, c->pos, c->orig, c->qpos, c->stream);
         hipEvent_t e0{};
         ev_begin(c, 0, &e0);
         mdk::launch_lj_force(a, c->stream);
@@ -754,6 +789,11 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     mdk::StepArgs sa = step_args(c);
     mdk::NeighArgs na = neigh_args(c);
     mdk::ForceArgs fa = force_args(c, precision);
+    if (fa.qpos != nullptr) {
+        // every pre-step pass refreshes the quantised positions for the mixed kernel
+        sa.qpos = c->qpos;
+        sa.quant = c->quant;
+    }
     fa.kick = 1;
     fa.half_dt = 0.5 * c->integ.dt;
     fa.chain = nhc ? c->chain : nullptr;

@@ -194,9 +194,9 @@ __host__ __device__ __forceinline__ long nbr_index(long t, int k, int cap) {
 // w = 24 eps (2 s6*s6 - s6) with the products fused into the subtraction
 // (one rounding where the two terms cancel near the potential minimum).
 // Returns false outside the cutoff.
-__device__ __forceinline__ bool lj_terms_f32(double r2d, double rc2d, float sig2, float eps,
-                                             float& inv, float& u, float& w) {
-    if (!(r2d < rc2d)) return false;
+// The terms for a pair already known to be inside the cutoff.
+__device__ __forceinline__ void lj_terms_f32_in(double r2d, float sig2, float eps, float& inv,
+                                                float& u, float& w) {
     const float r2 = static_cast<float>(r2d);
     float y = __builtin_amdgcn_rcpf(r2);
     const float e = __builtin_fmaf(-r2, y, 1.0f);
@@ -206,6 +206,12 @@ __device__ __forceinline__ bool lj_terms_f32(double r2d, double rc2d, float sig2
     const float s6 = (s2 * s2) * s2;
     u = (4.0f * eps) * __builtin_fmaf(s6, s6, -s6);
     w = (24.0f * eps) * __builtin_fmaf(2.0f * s6, s6, -s6);
+}
+
+__device__ __forceinline__ bool lj_terms_f32(double r2d, double rc2d, float sig2, float eps,
+                                             float& inv, float& u, float& w) {
+    if (!(r2d < rc2d)) return false;
+    lj_terms_f32_in(r2d, sig2, eps, inv, u, w);
     return true;
 }
 

@@ -16,6 +16,10 @@
 // MIXED: d is formed in FP64 and rounded to FP32; r2, s2, s6, u, w and the
 // force components are FP32; per-atom force, energy and virial accumulate
 // in 64-bit fixed point (scale 2^32), so sums are exact integer additions.
+// When the list keeps periodic fixed-point positions (Quant), the mixed
+// kernel gathers those (16 bytes per partner instead of 32) and takes d from
+// them in FP64; a row with a cutoff decision too close to rc is evaluated
+// again from the FP64 positions (pair32q).
 //
 // E and W come out of a fixed reduction tree: wave shuffles, a 4-wave LDS
 // step, per-workgroup partials, and a one-workgroup finish kernel summing the
@@ -175,10 +179,39 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
 // big is the largest |term| seen, checked against kFixLimit at the end.
 struct AccFix {
     unsigned long long fx = 0, fy = 0, fz = 0, e = 0, w = 0;
-    unsigned long long n = 0;
-    long long np = 0;
+    unsigned int n = 0;  // terms in this row (at most the row capacity)
+    int np = 0;
     float big = 0.0f;
+    bool redo = false;  // a quantised cutoff decision fell in the guard band
 };
+
+// FP32 terms of a pair inside the cutoff, accumulated in fixed point.
+template <bool LFS>
+__device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx, double dy,
+                                      double dz, bool flip, AccFix& q) {
+    float inv, u, ww;
+    lj_terms_f32_in(r2d, a.lj.sig2f, a.lj.epsf, inv, u, ww);
+    const float xf = static_cast<float>(dx);
+    const float yf = static_cast<float>(dy);
+    const float zf = static_cast<float>(dz);
+    const float r2 = static_cast<float>(r2d);
+    if (LFS) {
+        const float r = sqrtf(r2);
+        u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
+        ww = ww + r * a.lj.dudr_rcf;
+    }
+    const float c = ww * inv;
+    const float tx = c * xf, ty = c * yf, tz = c * zf;
+    q.e += fix_biased(u);
+    q.w += fix_biased(ww);
+    q.fx += fix_biased(tx);
+    q.fy += fix_biased(ty);
+    q.fz += fix_biased(tz);
+    q.n += 1;
+    q.np += flip ? 0 : 1;
+    q.big = fmaxf(q.big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
+                               fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
+}
 
 template <bool LFS>
 __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, uint32_t jj,
@@ -187,29 +220,32 @@ __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, ui
     double dx, dy, dz;
     row_delta(a.box, pi, pj, flip, dx, dy, dz);
     const double r2d = norm2_ref(dx, dy, dz);
-    float inv, u, ww;
-    if (lj_terms_f32(r2d, a.lj.rc2, a.lj.sig2f, a.lj.epsf, inv, u, ww)) {
-        const float xf = static_cast<float>(dx);
-        const float yf = static_cast<float>(dy);
-        const float zf = static_cast<float>(dz);
-        const float r2 = static_cast<float>(r2d);
-        if (LFS) {
-            const float r = sqrtf(r2);
-            u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
-            ww = ww + r * a.lj.dudr_rcf;
-        }
-        const float c = ww * inv;
-        const float tx = c * xf, ty = c * yf, tz = c * zf;
-        q.e += fix_biased(u);
-        q.w += fix_biased(ww);
-        q.fx += fix_biased(tx);
-        q.fy += fix_biased(ty);
-        q.fz += fix_biased(tz);
-        q.n += 1;
-        q.np += flip ? 0 : 1;
-        q.big = fmaxf(q.big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
-                                   fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
+    if (r2d < a.lj.rc2) acc32<LFS>(a, r2d, dx, dy, dz, flip, q);
+}
+
+// Mixed pair from fixed-point positions: one 16-byte gather per partner. The
+// fixed-point r2 is within band_c (host-computed from the unit L / 2^32)
+// relative of the exact one, so it decides the cutoff outside that band
+// around rc2. A pair inside the band marks the
+// row for redo: the row is then evaluated again with pair32 (d and the
+// cutoff test from the FP64 positions), so the set of pairs inside the
+// cutoff is always the one the FP64 kernel uses. Band hits are rare (about
+// 1e-5 of the rows), and the redo loop runs after the quantised one, so it
+// adds no register pressure to it. d is r_i - r_j in both orientations of an
+// entry (within 1e-8 of the exact difference).
+template <bool LFS>
+__device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, uint32_t jj,
+                                        const int4& qj, AccFix& q) {
+    const bool flip = (jj & kFlipBit) != 0;
+    double dx, dy, dz;
+    qdelta(a.quant, qi, qj, dx, dy, dz);
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    if (!(r2d < a.lj.rc2 * (1.0 + a.quant.band_c))) return;
+    if (!(r2d < a.lj.rc2 * (1.0 - a.quant.band_c))) {
+        q.redo = true;
+        return;
     }
+    acc32<LFS>(a, r2d, dx, dy, dz, flip, q);
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -223,6 +259,8 @@ __device__ __forceinline__ void load_octet(const uint32_t* p, u32x4& lo, u32x4& 
     hi = __builtin_nontemporal_load(v + 1);
 }
 
+// Four entries: the four partner gathers are issued together, then the four
+// pair terms are evaluated in row order.
 template <class Acc, class PairFn>
 __device__ __forceinline__ void quad(const ForceArgs& a, const double4& pi, const u32x4& j, Acc& q,
                                      PairFn pf) {
@@ -236,13 +274,26 @@ __device__ __forceinline__ void quad(const ForceArgs& a, const double4& pi, cons
     pf(a, pi, j.w, p3, q);
 }
 
+template <bool LFS>
+__device__ __forceinline__ void quad_q(const ForceArgs& a, const int4& qi, const u32x4& j,
+                                       AccFix& q) {
+    const int4 p0 = a.qpos[j.x & kIndexMask];
+    const int4 p1 = a.qpos[j.y & kIndexMask];
+    const int4 p2 = a.qpos[j.z & kIndexMask];
+    const int4 p3 = a.qpos[j.w & kIndexMask];
+    pair32q<LFS>(a, qi, j.x, p0, q);
+    pair32q<LFS>(a, qi, j.y, p1, q);
+    pair32q<LFS>(a, qi, j.z, p2, q);
+    pair32q<LFS>(a, qi, j.w, p3, q);
+}
+
 // Row walk over the octet layout: the next octet's indices are loaded while
-// the current one is processed, and each half of an octet issues its four
-// partner gathers together before evaluating the four pair terms in row
-// order (the summation order is the row order).
-template <class Acc, class PairFn>
-__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, const double4& pi,
-                                         Acc& q, PairFn pf) {
+// the current one is processed; quad_fn(j) handles four entries, one_fn(e)
+// a single entry of the last partial octet. Either way the summation order
+// is the row order.
+template <class QuadFn, class OneFn>
+__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, QuadFn quad_fn,
+                                         OneFn one_fn) {
     const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
     const int full = cnt >> 3;
     u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
@@ -250,8 +301,8 @@ __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, co
     for (int g = 0; g < full; ++g) {
         u32x4 nlo = {0u, 0u, 0u, 0u}, nhi = {0u, 0u, 0u, 0u};
         if (g + 1 < full) load_octet(row + (g + 1) * 512, nlo, nhi);
-        quad(a, pi, lo, q, pf);
-        quad(a, pi, hi, q, pf);
+        quad_fn(lo);
+        quad_fn(hi);
         lo = nlo;
         hi = nhi;
     }
@@ -261,11 +312,20 @@ __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, co
         const uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
         for (int r = 0; r < 7; ++r)
-            if (r < rem) pf(a, pi, e[r], a.pos[e[r] & kIndexMask], q);
+            if (r < rem) one_fn(e[r]);
     }
 }
 
-template <bool LFS, bool MIXED, bool KICK>
+template <class Acc, class PairFn>
+__device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt,
+                                             const double4& pi, Acc& q, PairFn pf) {
+    walk_row(
+        a, t, cnt, [&](const u32x4& j) { quad(a, pi, j, q, pf); },
+        [&](uint32_t e) { pf(a, pi, e, a.pos[e & kIndexMask], q); });
+}
+
+// QP: the mixed kernel reads quantised positions (a.qpos) for the partners.
+template <bool LFS, bool MIXED, bool QP, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
     __shared__ long long sm_l[kBlock / kWave];
@@ -280,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         double fx, fy, fz;
         if (!MIXED) {
             Acc64 q;
-            walk_row(a, t, cnt, pi, q, pair64<LFS>);
+            walk_row_pos(a, t, cnt, pi, q, pair64<LFS>);
             fx = q.fx;
             fy = q.fy;
             fz = q.fz;
@@ -289,7 +349,18 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             np = q.np;
         } else {
             AccFix q;
-            walk_row(a, t, cnt, pi, q, pair32<LFS>);
+            if (QP) {
+                const int4 qi = a.qpos[t];
+                walk_row(
+                    a, t, cnt, [&](const u32x4& j) { quad_q<LFS>(a, qi, j, q); },
+                    [&](uint32_t e) { pair32q<LFS>(a, qi, e, a.qpos[e & kIndexMask], q); });
+                if (q.redo) {
+                    q = AccFix{};
+                    walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
+                }
+            } else {
+                walk_row_pos(a, t, cnt, pi, q, pair32<LFS>);
+            }
             if (!(q.big < kFixLimit)) atomicOr(&a.sc->fix_saturated, 1);
             fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
             fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
@@ -331,12 +402,12 @@ __global__ __launch_bounds__(kBlock) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
-template <bool LFS, bool MIXED>
+template <bool LFS, bool MIXED, bool QP>
 void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
     if (a.kick)
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, true>), g, dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, true>), g, dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, false>), g, dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, false>), g, dim3(kBlock), 0, s, a);
 }
 
 // ---- direct (no list) evaluations: brute force and cells at the cutoff ----
@@ -531,12 +602,15 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
 
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
     const dim3 g(xcd_grid(a.n));
+    const bool qp = a.qpos != nullptr;
     if (a.lj.lfs) {
-        if (a.mixed) dispatch_kick<true, true>(a, g, s);
-        else dispatch_kick<true, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<true, false, false>(a, g, s);
+        else if (qp) dispatch_kick<true, true, true>(a, g, s);
+        else dispatch_kick<true, true, false>(a, g, s);
     } else {
-        if (a.mixed) dispatch_kick<false, true>(a, g, s);
-        else dispatch_kick<false, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<false, false, false>(a, g, s);
+        else if (qp) dispatch_kick<false, true, true>(a, g, s);
+        else dispatch_kick<false, true, false>(a, g, s);
     }
 }
 

@@ -139,6 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
         }
         a.pos[t] = p;
         a.vel[t] = v;
+        if (a.qpos != nullptr) a.qpos[t] = quantize(a.quant, p, a.orig[t]);
         if (a.check) d = displacement(a.box, p, a.ref[t]);
     }
     double t1 = d, t2 = 0.0;
@@ -367,6 +368,13 @@ __global__ void k_set_flag(int* flag, int v) {
     if (threadIdx.x == 0) *flag = v;
 }
 
+__global__ void k_quantize(long n, Quant q, const double4* __restrict__ pos,
+                           const int* __restrict__ orig, int4* __restrict__ qpos) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    qpos[t] = quantize(q, pos[t], orig[t]);
+}
+
 inline unsigned grid_of(long n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -380,6 +388,11 @@ void launch_pre_step(const StepArgs& a, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_pre<0>, g, dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_pre_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(g.x));
+}
+
+void launch_quantize(long n, const Quant& q, const double4* pos, const int* orig, int4* qpos,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_quantize, dim3(grid_of(n)), dim3(kBlock), 0, s, n, q, pos, orig, qpos);
 }
 
 void launch_check(const StepArgs& a, hipStream_t s) {

@@ -20,6 +20,41 @@ struct Grid {
     int usable;
 };
 
+// Periodic fixed-point positions. Each coordinate is stored as a uint32 that
+// maps the box length onto 2^32 (X = round((x / L + 1/2) 2^32) mod 2^32),
+// and the fourth word is the atom's upload index. The difference of two
+// coordinates is then one wrapping 32-bit subtraction, read as int32: the
+// minimum image falls out of two's-complement arithmetic with no fold, and
+// d = dX (L / 2^32) is formed in FP64. The error of d is below one unit
+// L / 2^32 per axis (2.5e-8 at L = 107), and one 16-byte load carries a
+// partner's position and orientation key.
+struct Quant {
+    double inv[3];   // 2^32 / L
+    double unit[3];  // L / 2^32
+    double band_l;   // relative half-width of the decision band around rl^2
+    double band_c;   // the same around rc^2
+};
+
+__device__ __forceinline__ uint32_t quant_axis(double x, double inv) {
+    // the int64 conversion keeps values outside the box; the uint32 cast then
+    // takes them modulo 2^32, which is the periodic wrap
+    return static_cast<uint32_t>(static_cast<long long>(rint(fma(x, inv, 2147483648.0))));
+}
+
+__device__ __forceinline__ int4 quantize(const Quant& q, const double4& p, int orig) {
+    return make_int4(static_cast<int>(quant_axis(p.x, q.inv[0])),
+                     static_cast<int>(quant_axis(p.y, q.inv[1])),
+                     static_cast<int>(quant_axis(p.z, q.inv[2])), orig);
+}
+
+// d = r_a - r_b (minimum image) from fixed-point positions, FP64.
+__device__ __forceinline__ void qdelta(const Quant& q, const int4& a, const int4& b, double& dx,
+                                       double& dy, double& dz) {
+    dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(a.x) - static_cast<uint32_t>(b.x))) * q.unit[0];
+    dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(a.y) - static_cast<uint32_t>(b.y))) * q.unit[1];
+    dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(a.z) - static_cast<uint32_t>(b.z))) * q.unit[2];
+}
+
 struct NeighArgs {
     long n;
     Grid grid;
@@ -31,6 +66,10 @@ struct NeighArgs {
     int *orig2, *cell2;
     int *cell_of, *cell_count, *cell_start, *cell_fill, *order;
     int* tile_sum;  // scan_tiles(cell count) entries
+    // fixed-point positions (or null): the committed order / the permuted
+    // order that the candidate scan reads
+    int4 *qpos, *qpos2;
+    Quant quant;
     uint32_t* nbr;
     int* ncnt;
     long npad;
@@ -61,6 +100,8 @@ struct ForceArgs {
     const int* ncnt;
     long npad;
     int cap;         // row capacity: rows longer than this overflowed and are clamped
+    const int4* qpos;  // fixed-point positions for the mixed kernel (or null)
+    Quant quant;
     double half_dt;  // kick coefficient h (v += h/m f) when kick != 0
     int kick;
     int mixed;
@@ -124,6 +165,8 @@ struct StepArgs {
     int check;        // compute the displacement check and set the rebuild flag
     double* part_d;   // 2 per group (top1, top2)
     Scalars* sc;
+    int4* qpos;       // refreshed from the new positions when not null
+    Quant quant;
 };
 void launch_pre_step(const StepArgs& a, hipStream_t s);
 // Displacement check alone (md_neigh_needs_rebuild).
@@ -148,5 +191,9 @@ void launch_drift(long n, const Box& box, double4* pos, const double4* vel, doub
 void launch_langevin_o(long n, double4* vel, const int* orig, double c1, double c2, double T,
                        uint64_t seed, uint64_t counter, hipStream_t s);
 void launch_scale(long n, double4* vel, double sfac, hipStream_t s);
+// qpos[t] = quantize(pos[t], orig[t]) (before a mixed force call whose
+// positions were not written by launch_pre_step).
+void launch_quantize(long n, const Quant& q, const double4* pos, const int* orig, int4* qpos,
+                     hipStream_t s);
 
 }  // namespace mdk

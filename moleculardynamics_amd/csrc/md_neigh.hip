@@ -169,20 +169,20 @@ __global__ __launch_bounds__(kBlock) void k_sort_cells(const int* __restrict__ c
     }
 }
 
-__global__ void k_permute(const int* __restrict__ order, long n, const double4* __restrict__ pos,
-                          const double4* __restrict__ vel, const double4* __restrict__ frc,
-                          const int* __restrict__ orig, const int* __restrict__ cell_of,
-                          double4* __restrict__ pos2, double4* __restrict__ vel2,
-                          double4* __restrict__ frc2, int* __restrict__ orig2,
-                          int* __restrict__ cell2, const int* flag) {
-    if (*flag == 0) return;
-    GRID_STRIDE(t, n) {
-        const int s = order[t];
-        pos2[t] = pos[s];
-        vel2[t] = vel[s];
-        frc2[t] = frc[s];
-        orig2[t] = orig[s];
-        cell2[t] = cell_of[s];
+// Gathers the state into the new order (and the fixed-point positions, when
+// the list keeps them).
+__global__ void k_permute(NeighArgs a) {
+    if (*a.flag == 0) return;
+    GRID_STRIDE(t, a.n) {
+        const int s = a.order[t];
+        const double4 p = a.pos[s];
+        const int o = a.orig[s];
+        a.pos2[t] = p;
+        a.vel2[t] = a.vel[s];
+        a.frc2[t] = a.frc[s];
+        a.orig2[t] = o;
+        a.cell2[t] = a.cell_of[s];
+        if (a.qpos2 != nullptr) a.qpos2[t] = quantize(a.quant, p, o);
     }
 }
 
@@ -198,33 +198,46 @@ __global__ void k_permute(const int* __restrict__ order, long n, const double4* 
 // order within a cell. Candidates are read four at a time -- positions and
 // upload indices of the four issued together -- so every thread keeps four
 // gathers in flight instead of one dependent load chain per candidate.
-__global__ __launch_bounds__(kBlock) void k_commit_build(
-    long n, Grid g, Box box, double rl2, const double4* __restrict__ pos2,
-    const double4* __restrict__ vel2, const double4* __restrict__ frc2,
-    const int* __restrict__ orig2, const int* __restrict__ cell2, const int* __restrict__ cell_start,
-    double4* __restrict__ pos, double4* __restrict__ vel, double4* __restrict__ frc,
-    double4* __restrict__ ref, int* __restrict__ orig, int* __restrict__ slot,
-    int* __restrict__ cell_of, uint32_t* __restrict__ nbr, int* __restrict__ ncnt, int cap,
-    Scalars* sc, const int* flag) {
-    if (*flag == 0) return;
+//
+// Q: the candidates are read as fixed-point positions (Quant): one 16-byte
+// load carries a candidate's position and upload index, and the fixed-point
+// r2 decides every candidate outside a band of relative half-width band_l
+// around rl2; inside the band the exact FP64 test above decides, from the
+// FP64 positions. The pair set is therefore the same.
+template <bool Q>
+__global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
+    if (*a.flag == 0) return;
+    const long n = a.n;
+    const Grid& g = a.grid;
+    const Box& box = a.box;
+    const double rl2 = a.rl2;
+    const double4* __restrict__ pos2 = a.pos2;
+    const int* __restrict__ orig2 = a.orig2;
+    const int* __restrict__ cell_start = a.cell_start;
+    const int cap = a.cap;
     const long tl = xcd_block() * kBlock + threadIdx.x;
     if (tl >= n) return;
     const int t = static_cast<int>(tl);
     const double4 pi = pos2[t];
     const int oi = orig2[t];
-    const int ci = cell2[t];
-    pos[t] = pi;
-    ref[t] = pi;
-    vel[t] = vel2[t];
-    frc[t] = frc2[t];
-    orig[t] = oi;
-    slot[oi] = t;
-    cell_of[t] = ci;
+    const int ci = a.cell2[t];
+    a.pos[t] = pi;
+    a.ref[t] = pi;
+    a.vel[t] = a.vel2[t];
+    a.frc[t] = a.frc2[t];
+    a.orig[t] = oi;
+    a.slot[oi] = t;
+    a.cell_of[t] = ci;
+    int4 qi{};
+    if (Q) {
+        qi = a.qpos2[t];
+        a.qpos[t] = qi;
+    }
     // Entries are collected eight at a time in this thread's LDS column and
     // written as one 32-byte octet, so the list goes out in whole sectors
     // instead of 4-byte partial writes.
     __shared__ uint32_t stage[8][kBlock];
-    uint32_t* row = nbr + oct_index(t, 0, cap);
+    uint32_t* row = a.nbr + oct_index(t, 0, cap);
     auto flush = [&](int grp) {
         uint4* dst = reinterpret_cast<uint4*>(row + static_cast<long>(grp) * 512);
         dst[0] = make_uint4(stage[0][threadIdx.x], stage[1][threadIdx.x], stage[2][threadIdx.x],
@@ -233,29 +246,54 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
                             stage[7][threadIdx.x]);
     };
     int cnt = 0;
+    auto push = [&](int j, bool flip) {
+        if (cnt < cap) {
+            stage[cnt & 7][threadIdx.x] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+            if ((cnt & 7) == 7) flush(cnt >> 3);
+        }
+        ++cnt;
+    };
     auto visit = [&](int j, const double4& q, int oj) {
         const bool flip = oi > oj;
         double dx, dy, dz;
         row_delta(box, pi, q, flip, dx, dy, dz);
-        if (norm2_ref(dx, dy, dz) < rl2 && j != t) {
-            if (cnt < cap) {
-                stage[cnt & 7][threadIdx.x] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                if ((cnt & 7) == 7) flush(cnt >> 3);
-            }
-            ++cnt;
+        if (norm2_ref(dx, dy, dz) < rl2 && j != t) push(j, flip);
+    };
+    const double lo = rl2 * (1.0 - a.quant.band_l), hi = rl2 * (1.0 + a.quant.band_l);
+    auto visit_q = [&](int j, const int4& qj) {
+        double dx, dy, dz;
+        qdelta(a.quant, qi, qj, dx, dy, dz);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < hi)) return;
+        if (!(r2 < lo)) {
+            visit(j, pos2[j], qj.w);  // inside the band: the exact test
+        } else if (j != t) {
+            push(j, oi > qj.w);
         }
     };
     auto range = [&](int b, int e) {
         int j = b;
-        for (; j + 4 <= e; j += 4) {
-            const double4 q0 = pos2[j], q1 = pos2[j + 1], q2 = pos2[j + 2], q3 = pos2[j + 3];
-            const int o0 = orig2[j], o1 = orig2[j + 1], o2 = orig2[j + 2], o3 = orig2[j + 3];
-            visit(j, q0, o0);
-            visit(j + 1, q1, o1);
-            visit(j + 2, q2, o2);
-            visit(j + 3, q3, o3);
+        if (Q) {
+            for (; j + 4 <= e; j += 4) {
+                const int4 q0 = a.qpos2[j], q1 = a.qpos2[j + 1], q2 = a.qpos2[j + 2],
+                           q3 = a.qpos2[j + 3];
+                visit_q(j, q0);
+                visit_q(j + 1, q1);
+                visit_q(j + 2, q2);
+                visit_q(j + 3, q3);
+            }
+            for (; j < e; ++j) visit_q(j, a.qpos2[j]);
+        } else {
+            for (; j + 4 <= e; j += 4) {
+                const double4 q0 = pos2[j], q1 = pos2[j + 1], q2 = pos2[j + 2], q3 = pos2[j + 3];
+                const int o0 = orig2[j], o1 = orig2[j + 1], o2 = orig2[j + 2], o3 = orig2[j + 3];
+                visit(j, q0, o0);
+                visit(j + 1, q1, o1);
+                visit(j + 2, q2, o2);
+                visit(j + 3, q3, o3);
+            }
+            for (; j < e; ++j) visit(j, pos2[j], orig2[j]);
         }
-        for (; j < e; ++j) visit(j, pos2[j], orig2[j]);
     };
     if (g.usable) {
         const int nx = g.nc[0];
@@ -283,8 +321,8 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(
     }
     const int kept = cnt < cap ? cnt : cap;
     if (kept & 7) flush(kept >> 3);  // the last, partial octet (slots past the row are not read)
-    ncnt[t] = cnt;
-    if (cnt > cap) atomicOr(&sc->overflow, 1);
+    a.ncnt[t] = cnt;
+    if (cnt > cap) atomicOr(&a.sc->overflow, 1);
 }
 
 inline unsigned grid_capped(long n) {
@@ -305,11 +343,11 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
     hipLaunchKernelGGL(k_sort_cells, gc, b, 0, s, a.cell_start, ncell, a.orig, a.order, a.flag);
-    hipLaunchKernelGGL(k_permute, gn, b, 0, s, a.order, n, a.pos, a.vel, a.frc, a.orig, a.cell_of,
-                       a.pos2, a.vel2, a.frc2, a.orig2, a.cell2, a.flag);
-    hipLaunchKernelGGL(k_commit_build, dim3(xcd_grid(n)), b, 0, s, n, a.grid, a.box, a.rl2, a.pos2,
-                       a.vel2, a.frc2, a.orig2, a.cell2, a.cell_start, a.pos, a.vel, a.frc, a.ref,
-                       a.orig, a.slot, a.cell_of, a.nbr, a.ncnt, a.cap, a.sc, a.flag);
+    hipLaunchKernelGGL(k_permute, gn, b, 0, s, a);
+    if (a.qpos2 != nullptr)
+        hipLaunchKernelGGL(k_commit_build<true>, dim3(xcd_grid(n)), b, 0, s, a);
+    else
+        hipLaunchKernelGGL(k_commit_build<false>, dim3(xcd_grid(n)), b, 0, s, a);
 }
 
 long scan_tiles(long ncell) { return (ncell + kScanTile - 1) / kScanTile; }

@@ -89,7 +89,12 @@ def test_graph_replay_matches_plain_launches(gpu_ok):
 
 def test_batch_matches_single_system_path(gpu_ok):
     """Same seed, same counter-based noise, fixed-point forces: a replica in
-    the batch and the same system on the plain path follow one trajectory."""
+    the batch and the same system on the plain path follow one trajectory.
+    The batch kernel forms pair differences from the FP64 positions, the
+    plain mixed kernel from cell-relative quantised positions (both within
+    1e-8 of the exact difference). An FP32 rounding of d can then differ by
+    one ulp, so the two agree to 1e-5 after 30 steps rather than bit for bit
+    (a different noise stream or force field would differ by 1e-2)."""
     from moleculardynamics_amd import native
 
     systems, T, seeds, _, _ = _systems(4, master=4300, max_cells=5)
@@ -110,8 +115,8 @@ def test_batch_matches_single_system_path(gpu_ok):
         ctx.run(30, native.MIXED, thermo=False)
         x1, v1, _ = ctx.download()
         ctx.close()
-        assert np.max(np.abs(x1 - pos[k])) < 1e-9
-        assert np.max(np.abs(v1 - vel[k])) < 1e-9
+        assert np.max(np.abs(x1 - pos[k])) < 1e-5
+        assert np.max(np.abs(v1 - vel[k])) < 1e-5
 
 
 def test_batch_thermostat_equipartition(gpu_ok):

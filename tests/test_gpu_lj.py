@@ -286,3 +286,70 @@ def test_mixed_precision_force_error(ctx, orc):
     ctx.forces(native.VERLET, native.MIXED)
     _, _, f32b = ctx.download()
     assert np.array_equal(f32, f32b)
+
+
+def test_mixed_cutoff_band_rows_are_exact(ctx, orc):
+    """Pairs placed 1e-10 inside and outside the cutoff: the mixed kernel's
+    fixed-point r2 cannot decide them (its band is ~4e-9 relative at this box
+    size), so their rows are re-evaluated from the FP64 positions, and the
+    pairs inside the cutoff are exactly the FP64 kernel's (and the oracle's)."""
+    from moleculardynamics_amd import native
+
+    fr = orc.fcc_lattice(6, 0.8442, 0.722, 99)
+    x = _pos(fr)
+    rc = 2.5
+    d = x - x[0]
+    d -= fr.L * np.floor(d / fr.L + 0.5)
+    r = np.sqrt(np.sum(d * d, axis=1))
+    # one atom of the 2.375 shell and one of the 2.656 shell moved onto rc(1 +- 1e-10)
+    b = int(np.argmin(np.abs(r - 2.375)))
+    c = int(np.argmin(np.abs(r - 2.656)))
+    x[b] = x[0] + d[b] / r[b] * rc * (1.0 + 1e-10)
+    x[c] = x[0] + d[c] / r[c] * rc * (1.0 - 1e-10)
+    snap = fr.copy()
+    snap.x, snap.y, snap.z = x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy()
+    ctx.upload(fr.L, x)
+    ctx.set_lj(1.0, 1.0, rc, False)
+    ctx.config_list(rc, 0.3)
+    ctx.build_list()
+    e64, _ = ctx.forces(native.VERLET, native.FP64)
+    n64 = ctx.pair_count()
+    _, _, f64 = ctx.download()
+    e32, _ = ctx.forces(native.VERLET, native.MIXED)
+    n32 = ctx.pair_count()
+    _, _, f32 = ctx.download()
+    f0, e_or, _ = orc.lj_forces(snap, rc=rc, mode="brute")
+    assert n32 == n64
+    assert abs(e64 - e_or) <= 1e-12 * abs(e_or)
+    rms = np.sqrt(np.sum((f32 - f0) ** 2) / np.sum(f0 ** 2))
+    assert rms < 1e-5, rms
+    assert abs(e32 - e_or) / abs(e_or) < 1e-6
+
+
+def test_mixed_without_fixed_point_positions(ctx, orc):
+    """A box longer than 128 rc keeps no fixed-point positions; the mixed
+    kernel then gathers the FP64 positions and still matches the oracle."""
+    from moleculardynamics_amd import native
+
+    fr = orc.fcc_lattice(6, 0.8442, 0.722, 5)
+    rng = np.random.default_rng(3)
+    x = _pos(fr) + rng.uniform(-0.03, 0.03, size=(fr.n, 3))
+    L = np.array([160.0, 160.0, 160.0])  # the crystallite sits in a large empty box
+    rc = 1.2
+    snap = fr.copy()
+    snap.L = L
+    snap.x, snap.y, snap.z = x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy()
+    ctx.upload(L, x)
+    ctx.set_lj(1.0, 1.0, rc, False)
+    ctx.config_list(rc, 0.3)
+    ctx.build_list()
+    ctx.forces(native.VERLET, native.FP64)
+    n64 = ctx.pair_count()
+    e32, _ = ctx.forces(native.VERLET, native.MIXED)
+    n32 = ctx.pair_count()
+    _, _, f32 = ctx.download()
+    f0, e_or, _ = orc.lj_forces(snap, rc=rc, mode="brute")
+    assert n32 == n64
+    rms = np.sqrt(np.sum((f32 - f0) ** 2) / np.sum(f0 ** 2))
+    assert rms < 1e-5, rms
+    assert abs(e32 - e_or) / abs(e_or) < 1e-6
