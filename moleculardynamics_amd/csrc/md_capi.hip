@@ -132,6 +132,10 @@ void set_lj_const(md_ctx* c) {
     k.rcf = static_cast<float>(k.rc);
     k.u_rcf = static_cast<float>(k.u_rc);
     k.dudr_rcf = static_cast<float>(k.dudr_rc);
+    k.e4s = (4.0f * k.epsf) * 4294967296.0f;
+    k.e24s = (24.0f * k.epsf) * 4294967296.0f;
+    k.u_rcs = k.u_rcf * 4294967296.0f;
+    k.dudr_rcs = k.dudr_rcf * 4294967296.0f;
     k.lfs = p.linear_force_shift ? 1 : 0;
 }
 

@@ -274,6 +274,15 @@ __device__ __forceinline__ unsigned long long fix_biased(float v) {
         __double_as_longlong(fma(static_cast<double>(v), kFixScale, kMagic)));
 }
 
+// The same for a term already multiplied by 2^32 in FP32 (exact: a power of
+// two), whose limit is then kFixLimitScaled: one conversion and one FP64 add.
+constexpr float kFixLimitScaled = 2251799813685248.0f;  // 2^51
+
+__device__ __forceinline__ unsigned long long fix_biased_scaled(float vs) {
+    return static_cast<unsigned long long>(
+        __double_as_longlong(static_cast<double>(vs) + kMagic));
+}
+
 __device__ __forceinline__ long long fix_value(unsigned long long acc, unsigned long long n) {
     const unsigned long long bias = static_cast<unsigned long long>(__double_as_longlong(kMagic));
     return static_cast<long long>(acc - n * bias);

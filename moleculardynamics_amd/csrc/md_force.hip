@@ -112,7 +112,8 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
     Scalars* sc = a.sc;
     sc->energy = E;
     sc->virial = W;
-    sc->pairs_in_cut = red_l[2][0];
+    const long long pairs = red_l[2][0] / 2;  // each pair is a term of both its rows
+    sc->pairs_in_cut = pairs;
     double conserved = 0.0;
     if (a.kick) {
         if (a.chain != nullptr) {
@@ -133,13 +134,16 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
         th->virial = W;
         th->kinetic = ke;
         th->conserved = conserved;
-        th->pairs = static_cast<double>(red_l[2][0]);
+        th->pairs = static_cast<double>(pairs);
     }
 }
 
+// Every in-cutoff term is counted in the row that holds it (n). The list is
+// full and both rows of a pair take the same decision (the same r2 bits), so
+// the pairs inside the cutoff are the sum of n over the rows, halved.
 struct Acc64 {
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
-    long long np = 0;
+    int n = 0;
 };
 
 template <bool LFS>
@@ -167,7 +171,7 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
         }
         q.e += u;
         q.w += ww;
-        q.np += flip ? 0 : 1;
+        q.n += 1;
         const double c = ww * inv;
         q.fx += c * dx;
         q.fy += c * dy;
@@ -175,40 +179,46 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
     }
 }
 
-// Biased fixed-point accumulators (fix_biased / fix_value): n terms each;
-// big is the largest |term| seen, checked against kFixLimit at the end.
+// Biased fixed-point accumulators (fix_biased_scaled / fix_value): n terms
+// each; big is the largest |term| seen (in units of 2^-32), checked against
+// kFixLimitScaled at the end.
 struct AccFix {
     unsigned long long fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     unsigned int n = 0;  // terms in this row (at most the row capacity)
-    int np = 0;
     float big = 0.0f;
-    bool redo = false;  // a quantised cutoff decision fell in the guard band
+    bool redo = false;  // a fixed-point cutoff decision fell in the guard band
 };
 
-// FP32 terms of a pair inside the cutoff, accumulated in fixed point.
+// FP32 terms of a pair inside the cutoff, accumulated in fixed point. The
+// energy and force coefficients carry the fixed-point scale 2^32 (e4s, e24s,
+// u_rcs, dudr_rcs), so every FP32 term comes out multiplied by 2^32 exactly:
+// a power-of-two factor commutes with each rounding. The term's fixed-point
+// value round(v 2^32) is then one FP64 add of the bias.
 template <bool LFS>
 __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx, double dy,
-                                      double dz, bool flip, AccFix& q) {
-    float inv, u, ww;
-    lj_terms_f32_in(r2d, a.lj.sig2f, a.lj.epsf, inv, u, ww);
-    const float xf = static_cast<float>(dx);
-    const float yf = static_cast<float>(dy);
-    const float zf = static_cast<float>(dz);
+                                      double dz, AccFix& q) {
     const float r2 = static_cast<float>(r2d);
+    float inv = __builtin_amdgcn_rcpf(r2);
+    inv = __builtin_fmaf(inv, __builtin_fmaf(-r2, inv, 1.0f), inv);
+    const float s2 = a.lj.sig2f * inv;
+    const float s6 = (s2 * s2) * s2;
+    float u = a.lj.e4s * __builtin_fmaf(s6, s6, -s6);
+    float ww = a.lj.e24s * __builtin_fmaf(2.0f * s6, s6, -s6);
     if (LFS) {
         const float r = sqrtf(r2);
-        u = u - a.lj.u_rcf - (r - a.lj.rcf) * a.lj.dudr_rcf;
-        ww = ww + r * a.lj.dudr_rcf;
+        u = u - a.lj.u_rcs - (r - a.lj.rcf) * a.lj.dudr_rcs;
+        ww = ww + r * a.lj.dudr_rcs;
     }
     const float c = ww * inv;
-    const float tx = c * xf, ty = c * yf, tz = c * zf;
-    q.e += fix_biased(u);
-    q.w += fix_biased(ww);
-    q.fx += fix_biased(tx);
-    q.fy += fix_biased(ty);
-    q.fz += fix_biased(tz);
+    const float tx = c * static_cast<float>(dx);
+    const float ty = c * static_cast<float>(dy);
+    const float tz = c * static_cast<float>(dz);
+    q.e += fix_biased_scaled(u);
+    q.w += fix_biased_scaled(ww);
+    q.fx += fix_biased_scaled(tx);
+    q.fy += fix_biased_scaled(ty);
+    q.fz += fix_biased_scaled(tz);
     q.n += 1;
-    q.np += flip ? 0 : 1;
     q.big = fmaxf(q.big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
                                fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
 }
@@ -220,23 +230,23 @@ __device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, ui
     double dx, dy, dz;
     row_delta(a.box, pi, pj, flip, dx, dy, dz);
     const double r2d = norm2_ref(dx, dy, dz);
-    if (r2d < a.lj.rc2) acc32<LFS>(a, r2d, dx, dy, dz, flip, q);
+    if (r2d < a.lj.rc2) acc32<LFS>(a, r2d, dx, dy, dz, q);
 }
 
 // Mixed pair from fixed-point positions: one 16-byte gather per partner. The
 // fixed-point r2 is within band_c (host-computed from the unit L / 2^32)
 // relative of the exact one, so it decides the cutoff outside that band
-// around rc2. A pair inside the band marks the
-// row for redo: the row is then evaluated again with pair32 (d and the
-// cutoff test from the FP64 positions), so the set of pairs inside the
-// cutoff is always the one the FP64 kernel uses. Band hits are rare (about
-// 1e-5 of the rows), and the redo loop runs after the quantised one, so it
-// adds no register pressure to it. d is r_i - r_j in both orientations of an
-// entry (within 1e-8 of the exact difference).
+// around rc2. A pair inside the band marks the row for redo: the row is then
+// evaluated again with pair32 (d and the cutoff test from the FP64
+// positions), so the set of pairs inside the cutoff is always the one the
+// FP64 kernel uses. Band hits are rare (about 1e-5 of the rows), and the redo
+// loop runs after the fixed-point one, so it adds no register pressure to it.
+// d is r_i - r_j in both orientations of an entry (within one unit of the
+// exact difference per axis), and the two rows of a pair see r2 bits that
+// are the same.
 template <bool LFS>
-__device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, uint32_t jj,
-                                        const int4& qj, AccFix& q) {
-    const bool flip = (jj & kFlipBit) != 0;
+__device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, const int4& qj,
+                                        AccFix& q) {
     double dx, dy, dz;
     qdelta(a.quant, qi, qj, dx, dy, dz);
     const double r2d = dx * dx + dy * dy + dz * dz;
@@ -245,7 +255,7 @@ __device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, uint
         q.redo = true;
         return;
     }
-    acc32<LFS>(a, r2d, dx, dy, dz, flip, q);
+    acc32<LFS>(a, r2d, dx, dy, dz, q);
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -259,50 +269,28 @@ __device__ __forceinline__ void load_octet(const uint32_t* p, u32x4& lo, u32x4& 
     hi = __builtin_nontemporal_load(v + 1);
 }
 
-// Four entries: the four partner gathers are issued together, then the four
-// pair terms are evaluated in row order.
-template <class Acc, class PairFn>
-__device__ __forceinline__ void quad(const ForceArgs& a, const double4& pi, const u32x4& j, Acc& q,
-                                     PairFn pf) {
-    const double4 p0 = a.pos[j.x & kIndexMask];
-    const double4 p1 = a.pos[j.y & kIndexMask];
-    const double4 p2 = a.pos[j.z & kIndexMask];
-    const double4 p3 = a.pos[j.w & kIndexMask];
-    pf(a, pi, j.x, p0, q);
-    pf(a, pi, j.y, p1, q);
-    pf(a, pi, j.z, p2, q);
-    pf(a, pi, j.w, p3, q);
-}
-
-template <bool LFS>
-__device__ __forceinline__ void quad_q(const ForceArgs& a, const int4& qi, const u32x4& j,
-                                       AccFix& q) {
-    const int4 p0 = a.qpos[j.x & kIndexMask];
-    const int4 p1 = a.qpos[j.y & kIndexMask];
-    const int4 p2 = a.qpos[j.z & kIndexMask];
-    const int4 p3 = a.qpos[j.w & kIndexMask];
-    pair32q<LFS>(a, qi, j.x, p0, q);
-    pair32q<LFS>(a, qi, j.y, p1, q);
-    pair32q<LFS>(a, qi, j.z, p2, q);
-    pair32q<LFS>(a, qi, j.w, p3, q);
-}
-
-// Row walk over the octet layout: the next octet's indices are loaded while
-// the current one is processed; quad_fn(j) handles four entries, one_fn(e)
-// a single entry of the last partial octet. Either way the summation order
-// is the row order.
-template <class QuadFn, class OneFn>
-__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, QuadFn quad_fn,
-                                         OneFn one_fn) {
+// Row walk over the octet layout. Per octet: the eight partner gathers are
+// issued first (gather), then the next octet's indices (loads return in
+// issue order, so waiting for a gather never waits for the index stream,
+// which comes from HBM), then the eight pair terms are evaluated in row order
+// (eval). The last octet's indices are requested once more in place of a
+// next one, so the loop has no branch around its loads. one(e) handles an
+// entry of the last, partial octet.
+template <class P, class GatherFn, class EvalFn, class OneFn>
+__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, GatherFn gather,
+                                         EvalFn eval, OneFn one) {
     const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
     const int full = cnt >> 3;
     u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
     if (full > 0) load_octet(row, lo, hi);
     for (int g = 0; g < full; ++g) {
-        u32x4 nlo = {0u, 0u, 0u, 0u}, nhi = {0u, 0u, 0u, 0u};
-        if (g + 1 < full) load_octet(row + (g + 1) * 512, nlo, nhi);
-        quad_fn(lo);
-        quad_fn(hi);
+        P p[8];
+        gather(lo, p);
+        gather(hi, p + 4);
+        u32x4 nlo, nhi;
+        load_octet(row + min(g + 1, full - 1) * 512, nlo, nhi);
+        eval(lo, p);
+        eval(hi, p + 4);
         lo = nlo;
         hi = nhi;
     }
@@ -312,19 +300,53 @@ __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, Qu
         const uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
         for (int r = 0; r < 7; ++r)
-            if (r < rem) one_fn(e[r]);
+            if (r < rem) one(e[r]);
     }
 }
 
+// Partners from the FP64 positions.
 template <class Acc, class PairFn>
 __device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt,
                                              const double4& pi, Acc& q, PairFn pf) {
-    walk_row(
-        a, t, cnt, [&](const u32x4& j) { quad(a, pi, j, q, pf); },
+    walk_row<double4>(
+        a, t, cnt,
+        [&](const u32x4& j, double4* p) {
+            p[0] = a.pos[j.x & kIndexMask];
+            p[1] = a.pos[j.y & kIndexMask];
+            p[2] = a.pos[j.z & kIndexMask];
+            p[3] = a.pos[j.w & kIndexMask];
+        },
+        [&](const u32x4& j, const double4* p) {
+            pf(a, pi, j.x, p[0], q);
+            pf(a, pi, j.y, p[1], q);
+            pf(a, pi, j.z, p[2], q);
+            pf(a, pi, j.w, p[3], q);
+        },
         [&](uint32_t e) { pf(a, pi, e, a.pos[e & kIndexMask], q); });
 }
 
-// QP: the mixed kernel reads quantised positions (a.qpos) for the partners.
+// Partners from the fixed-point positions (mixed kernel).
+template <bool LFS>
+__device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt,
+                                               const int4& qi, AccFix& q) {
+    walk_row<int4>(
+        a, t, cnt,
+        [&](const u32x4& j, int4* p) {
+            p[0] = a.qpos[j.x & kIndexMask];
+            p[1] = a.qpos[j.y & kIndexMask];
+            p[2] = a.qpos[j.z & kIndexMask];
+            p[3] = a.qpos[j.w & kIndexMask];
+        },
+        [&](const u32x4& j, const int4* p) {
+            pair32q<LFS>(a, qi, p[0], q);
+            pair32q<LFS>(a, qi, p[1], q);
+            pair32q<LFS>(a, qi, p[2], q);
+            pair32q<LFS>(a, qi, p[3], q);
+        },
+        [&](uint32_t e) { pair32q<LFS>(a, qi, a.qpos[e & kIndexMask], q); });
+}
+
+// QP: the mixed kernel reads fixed-point positions (a.qpos) for the partners.
 template <bool LFS, bool MIXED, bool QP, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
@@ -333,41 +355,37 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
     const long t = blk * kBlock + threadIdx.x;
     double e = 0.0, w = 0.0, ke = 0.0;
-    long long ef = 0, wf = 0, np = 0;
+    long long ef = 0, wf = 0, nt = 0;
     if (t < a.n) {
-        const double4 pi = a.pos[t];
         const int cnt = min(a.ncnt[t], a.cap);
         double fx, fy, fz;
         if (!MIXED) {
             Acc64 q;
-            walk_row_pos(a, t, cnt, pi, q, pair64<LFS>);
+            walk_row_pos(a, t, cnt, a.pos[t], q, pair64<LFS>);
             fx = q.fx;
             fy = q.fy;
             fz = q.fz;
             e = q.e;
             w = q.w;
-            np = q.np;
+            nt = q.n;
         } else {
             AccFix q;
             if (QP) {
-                const int4 qi = a.qpos[t];
-                walk_row(
-                    a, t, cnt, [&](const u32x4& j) { quad_q<LFS>(a, qi, j, q); },
-                    [&](uint32_t e) { pair32q<LFS>(a, qi, e, a.qpos[e & kIndexMask], q); });
+                walk_row_fixed<LFS>(a, t, cnt, a.qpos[t], q);
                 if (q.redo) {
                     q = AccFix{};
                     walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
                 }
             } else {
-                walk_row_pos(a, t, cnt, pi, q, pair32<LFS>);
+                walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
             }
-            if (!(q.big < kFixLimit)) atomicOr(&a.sc->fix_saturated, 1);
+            if (!(q.big < kFixLimitScaled)) atomicOr(&a.sc->fix_saturated, 1);
             fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
             fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
             fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
             ef = fix_value(q.e, q.n);
             wf = fix_value(q.w, q.n);
-            np = q.np;
+            nt = q.n;
         }
         a.frc[t] = make_double4(fx, fy, fz, 0.0);
         if (KICK) {
@@ -380,13 +398,13 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
         }
     }
-    // workgroup partials
+    // workgroup partials (the in-cutoff term count is twice the pair count)
     const double be = block_reduce(e, sm_d, wave_sum);
     const double bw = block_reduce(w, sm_d, wave_sum);
     const double bk = block_reduce(ke, sm_d, wave_sum);
     const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
     const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
-    const long long bnp = block_reduce(np, sm_l, wave_sum_ll);
+    const long long bnt = block_reduce(nt, sm_l, wave_sum_ll);
     if (threadIdx.x == 0) {
         const long b = blk;
         a.part_d[3 * b + 0] = be;
@@ -394,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         a.part_d[3 * b + 2] = bk;
         a.part_l[3 * b + 0] = bef;
         a.part_l[3 * b + 1] = bwf;
-        a.part_l[3 * b + 2] = bnp;
+        a.part_l[3 * b + 2] = bnt;
     }
 }
 

@@ -85,6 +85,9 @@ long scan_tiles(long ncell);
 struct LJConst {
     double sig2, eps, rc, rc2, u_rc, dudr_rc;
     float sig2f, epsf, rc2f, rcf, u_rcf, dudr_rcf;
+    // FP32 coefficients times the fixed-point scale 2^32 (mixed kernel):
+    // 4 eps, 24 eps, u(rc), u'(rc)
+    float e4s, e24s, u_rcs, dudr_rcs;
     int lfs;
 };
 
