@@ -769,6 +769,7 @@ md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
         // Q_1 = dof T tau^2, Q_k = T tau^2 (nose_hoover.cpp:45-49)
         for (int k = 0; k < ch.M; ++k) ch.Q[k] = it->temperature * it->tau * it->tau;
         ch.Q[0] = static_cast<double>(it->dof) * it->temperature * it->tau * it->tau;
+        for (int k = 0; k < ch.M; ++k) ch.iQ[k] = 1.0 / ch.Q[k];
         MD_HIP(c, hipMemcpy(c->chain, &ch, sizeof(ch), hipMemcpyHostToDevice));
     } else if (it->kind != MD_VELOCITY_VERLET && it->kind != MD_LANGEVIN_BAOAB) {
         return fail(c, MD_ERR_INVALID_ARGUMENT, "unknown integrator kind");

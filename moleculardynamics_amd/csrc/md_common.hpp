@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace mdk {
 
@@ -63,7 +64,25 @@ struct Chain {
     int M;
     double dof, T;
     double Q[kMaxChain], xi[kMaxChain], p[kMaxChain];
+    double iQ[kMaxChain];  // 1 / Q
 };
+
+// Calls f(std::integral_constant<int, j>) for j = N-1 .. 0 / 0 .. N-1, so
+// that every index inside f is a compile-time constant.
+template <int N, class F>
+__device__ __forceinline__ void unroll_down(F& f) {
+    if constexpr (N > 0) {
+        f(std::integral_constant<int, N - 1>{});
+        unroll_down<N - 1>(f);
+    }
+}
+template <int N, int J = 0, class F>
+__device__ __forceinline__ void unroll_up(F& f) {
+    if constexpr (J < N) {
+        f(std::integral_constant<int, J>{});
+        unroll_up<N, J + 1>(f);
+    }
+}
 
 struct Box {
     double L[3];

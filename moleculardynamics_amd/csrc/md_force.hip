@@ -34,12 +34,45 @@ namespace mdk {
 
 namespace {
 
+// Working copy of the chain for thread 0 of a finish kernel: the first MM
+// links in registers (every index a compile-time constant after unrolling),
+// so the chain's long dependent sequence of FP64 operations does no memory
+// access. Divisions by Q use the reciprocals iQ computed once on the host.
+template <int MM>
+struct ChainRegs {
+    int M;
+    double dof, T;
+    double p[MM], xi[MM], iQ[MM];
+
+    __device__ void load(const Chain& c) {
+        M = c.M;
+        dof = c.dof;
+        T = c.T;
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+            p[k] = k < M ? c.p[k] : 0.0;
+            xi[k] = k < M ? c.xi[k] : 0.0;
+            iQ[k] = k < M ? c.iQ[k] : 0.0;
+        }
+    }
+    __device__ void store(Chain& c) const {
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+            if (k < M) {
+                c.p[k] = p[k];
+                c.xi[k] = xi[k];
+            }
+        }
+    }
+};
+
 // One MTK96 chain propagation of length delta with Suzuki-Yoshida weights
 // (integrate/nose_hoover.cpp:63-124). The chain rescales every velocity by a
 // common factor, so the kinetic energy after a rescale is KE * f^2; it is
 // tracked that way instead of re-summing the velocities. Returns the product
 // of the three rescale factors and updates ke.
-__device__ double chain_half(Chain& c, double& ke, double delta) {
+template <int MM>
+__device__ double chain_half(ChainRegs<MM>& c, double& ke, double delta) {
     const double w1 = 1.0 / (2.0 - cbrt(2.0));
     const double ws[3] = {w1, 1.0 - 2.0 * w1, w1};
     const int M = c.M;
@@ -47,84 +80,132 @@ __device__ double chain_half(Chain& c, double& ke, double delta) {
     for (int s = 0; s < 3; ++s) {
         const double d = ws[s] * delta;
         const double d2 = 0.5 * d, d4 = 0.25 * d;
-        auto link = [&](int j) {
-            const bool next = j + 1 < M;
-            // p[j+1] is not touched in between: one exp serves both damping factors
-            const double damp = next ? exp(-d4 * c.p[j + 1] / c.Q[j + 1]) : 1.0;
-            if (next) c.p[j] *= damp;
-            const double g = (j == 0) ? 2.0 * ke - c.dof * c.T
-                                      : c.p[j - 1] * c.p[j - 1] / c.Q[j - 1] - c.T;
-            c.p[j] += d2 * g;
-            if (next) c.p[j] *= damp;
+        auto link = [&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j >= M) return;
+            double damp = 1.0;
+            if constexpr (j + 1 < MM) {
+                // p[j+1] is not touched in between: one exp serves both damping factors
+                if (j + 1 < M) damp = exp(-d4 * c.p[j + 1] * c.iQ[j + 1]);
+            }
+            double g;
+            if constexpr (j == 0) {
+                g = 2.0 * ke - c.dof * c.T;
+            } else {
+                g = c.p[j - 1] * c.p[j - 1] * c.iQ[j - 1] - c.T;
+            }
+            c.p[j] = (c.p[j] * damp + d2 * g) * damp;
         };
-        for (int j = M - 1; j >= 0; --j) link(j);
-        for (int k = 0; k < M; ++k) c.xi[k] += d * c.p[k] / c.Q[k];
-        const double f = exp(-d * c.p[0] / c.Q[0]);
+        unroll_down<MM>(link);
+#pragma unroll
+        for (int k = 0; k < MM; ++k)
+            if (k < M) c.xi[k] += d * c.p[k] * c.iQ[k];
+        const double f = exp(-d * c.p[0] * c.iQ[0]);
         scale *= f;
         ke *= f * f;
-        for (int j = 0; j < M; ++j) link(j);
+        unroll_up<MM>(link);
     }
     return scale;
 }
 
-__device__ double chain_conserved(const Chain& c) {
+template <int MM>
+__device__ double chain_conserved(const ChainRegs<MM>& c) {
     double acc = c.dof * c.T * c.xi[0];
-    for (int k = 1; k < c.M; ++k) acc += c.T * c.xi[k];
-    for (int k = 0; k < c.M; ++k) acc += 0.5 * c.p[k] * c.p[k] / c.Q[k];
+#pragma unroll
+    for (int k = 1; k < MM; ++k)
+        if (k < c.M) acc += c.T * c.xi[k];
+#pragma unroll
+    for (int k = 0; k < MM; ++k)
+        if (k < c.M) acc += 0.5 * c.p[k] * c.p[k] * c.iQ[k];
     return acc;
 }
 
-// Final reduction (one workgroup, launched after the pair kernel): the nb
-// per-workgroup partials summed in a fixed order.
+// The chain's part of a step on thread 0: one or two half-chains, the
+// conserved contribution, the rescale factor. Short chains (the usual M = 3)
+// run in registers; longer ones in a 16-link copy.
+struct ChainStep {
+    double scale, conserved;
+};
+
+template <int MM>
+__device__ ChainStep chain_step_m(Chain* chain, double& ke, double delta, bool twice) {
+    ChainRegs<MM> c;
+    c.load(*chain);
+    ChainStep r;
+    r.scale = chain_half(c, ke, delta);
+    r.conserved = chain_conserved(c);
+    if (twice) {
+        double ke2 = ke;
+        r.scale *= chain_half(c, ke2, delta);
+    }
+    c.store(*chain);
+    return r;
+}
+
+__device__ ChainStep chain_step(Chain* chain, double& ke, double delta, bool twice) {
+    if (chain->M <= 4) return chain_step_m<4>(chain, ke, delta, twice);
+    return chain_step_m<kMaxChain>(chain, ke, delta, twice);
+}
+
+// Final reduction (one workgroup of kFinish threads, launched after the pair
+// kernel): the nb per-workgroup partials summed in a fixed order -- a strided
+// share per thread, wave shuffles, then the waves in index order. Thread 0
+// then advances the Nose-Hoover chain (chain_step).
+constexpr int kFinish = 1024;
+
 __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
-    __shared__ double red_d[3][kBlock];
-    __shared__ long long red_l[3][kBlock];
-    for (int q = 0; q < 3; ++q) {
-        double sd = 0.0;
-        long long sl = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) {
-            sd += a.part_d[3 * b + q];
-            sl += a.part_l[3 * b + q];
+    __shared__ double red_d[3][kFinish / kWave];
+    __shared__ long long red_l[3][kFinish / kWave];
+    double sd[3] = {0.0, 0.0, 0.0};
+    long long sl[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += kFinish) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            sd[q] += a.part_d[3 * b + q];
+            sl[q] += a.part_l[3 * b + q];
         }
-        red_d[q][threadIdx.x] = sd;
-        red_l[q][threadIdx.x] = sl;
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double vd = wave_sum(sd[q]);
+        const long long vl = wave_sum_ll(sl[q]);
+        if (lane == 0) {
+            red_d[q][wid] = vd;
+            red_l[q][wid] = vl;
+        }
     }
     __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            for (int q = 0; q < 3; ++q) {
-                red_d[q][threadIdx.x] += red_d[q][threadIdx.x + off];
-                red_l[q][threadIdx.x] += red_l[q][threadIdx.x + off];
-            }
-        }
-        __syncthreads();
-    }
     if (threadIdx.x != 0) return;
+    double tot_d[3] = {0.0, 0.0, 0.0};
+    long long tot_l[3] = {0, 0, 0};
+    for (int w = 0; w < kFinish / kWave; ++w) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            tot_d[q] += red_d[q][w];
+            tot_l[q] += red_l[q][w];
+        }
+    }
     double E, W;
     if (mixed) {
-        E = 0.5 * (static_cast<double>(red_l[0][0]) * kFixInv);
-        W = 0.5 * (static_cast<double>(red_l[1][0]) * kFixInv);
+        E = 0.5 * (static_cast<double>(tot_l[0]) * kFixInv);
+        W = 0.5 * (static_cast<double>(tot_l[1]) * kFixInv);
     } else {
-        E = 0.5 * red_d[0][0];
-        W = 0.5 * red_d[1][0];
+        E = 0.5 * tot_d[0];
+        W = 0.5 * tot_d[1];
     }
-    double ke = red_d[2][0];
+    double ke = tot_d[2];
     Scalars* sc = a.sc;
     sc->energy = E;
     sc->virial = W;
-    const long long pairs = red_l[2][0] / 2;  // each pair is a term of both its rows
+    const long long pairs = tot_l[2] / 2;  // each pair is a term of both its rows
     sc->pairs_in_cut = pairs;
     double conserved = 0.0;
     if (a.kick) {
         if (a.chain != nullptr) {
-            Chain& c = *a.chain;
-            double s = chain_half(c, ke, a.chain_delta);
-            conserved = chain_conserved(c);
-            if (a.chain_twice) {
-                double ke2 = ke;
-                s *= chain_half(c, ke2, a.chain_delta);
-            }
-            sc->vscale = s;
+            const ChainStep r = chain_step(a.chain, ke, a.chain_delta, a.chain_twice != 0);
+            conserved = r.conserved;
+            sc->vscale = r.scale;
         }
         sc->kinetic = ke;
     }
@@ -416,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_lj_finish(ForceArgs a, int nb) {
+__global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
@@ -607,7 +688,7 @@ __global__ void k_sort_index(const int* __restrict__ start, long m, int* __restr
 __global__ void k_chain_first(Chain* chain, Scalars* sc, double delta) {
     if (threadIdx.x != 0) return;
     double ke = sc->kinetic;
-    sc->vscale = chain_half(*chain, ke, delta);
+    sc->vscale = chain_step(chain, ke, delta, false).scale;
 }
 
 inline unsigned grid_of(long n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
@@ -633,7 +714,8 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
 }
 
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(grid_of(a.n)));
+    // the pair kernel writes partials for its xcd_grid(n) blocks (padding blocks none)
+    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(grid_of(a.n)));
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

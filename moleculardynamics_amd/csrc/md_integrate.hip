@@ -80,27 +80,27 @@ __device__ __forceinline__ double displacement(const Box& b, const double4& p, c
     return sqrt(norm2_ref(dx, dy, dz));
 }
 
+// One workgroup of kFinish threads: strided shares, wave top-2 by shuffles,
+// then the waves' pairs merged by thread 0.
+constexpr int kFinish = 1024;
+
 __device__ void finish_check(const StepArgs& a, int nb) {
-    __shared__ double t1s[kBlock], t2s[kBlock];
+    __shared__ double t1s[kFinish / kWave], t2s[kFinish / kWave];
     double t1 = 0.0, t2 = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock)
+    for (int b = threadIdx.x; b < nb; b += kFinish)
         top2_merge(t1, t2, a.part_d[2 * b], a.part_d[2 * b + 1]);
-    t1s[threadIdx.x] = t1;
-    t2s[threadIdx.x] = t2;
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            double x1 = t1s[threadIdx.x], x2 = t2s[threadIdx.x];
-            top2_merge(x1, x2, t1s[threadIdx.x + off], t2s[threadIdx.x + off]);
-            t1s[threadIdx.x] = x1;
-            t2s[threadIdx.x] = x2;
-        }
-        __syncthreads();
+    wave_top2(t1, t2);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        t1s[threadIdx.x / kWave] = t1;
+        t2s[threadIdx.x / kWave] = t2;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        a.sc->top1 = t1s[0];
-        a.sc->top2 = t2s[0];
-        a.sc->rebuild_flag = (t1s[0] + t2s[0]) > a.skin ? 1 : 0;
+        double b1 = 0.0, b2 = 0.0;
+        for (int w = 0; w < kFinish / kWave; ++w) top2_merge(b1, b2, t1s[w], t2s[w]);
+        a.sc->top1 = b1;
+        a.sc->top2 = b2;
+        a.sc->rebuild_flag = (b1 + b2) > a.skin ? 1 : 0;
         a.sc->vscale = 1.0;
     }
 }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
 
 // One workgroup after k_pre: rebuild decision from the partials; the chain
 // rescale applied by k_pre is consumed.
-__global__ __launch_bounds__(kBlock) void k_pre_finish(StepArgs a, int nb) {
+__global__ __launch_bounds__(kFinish) void k_pre_finish(StepArgs a, int nb) {
     if (a.check) {
         finish_check(a, nb);
     } else if (threadIdx.x == 0) {
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_check(StepArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_check_finish(StepArgs a, int nb) {
+__global__ __launch_bounds__(kFinish) void k_check_finish(StepArgs a, int nb) {
     const double keep = a.sc->vscale;
     finish_check(a, nb);
     if (threadIdx.x == 0) a.sc->vscale = keep;
@@ -387,7 +387,7 @@ void launch_pre_step(const StepArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_pre<2>, g, dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_pre<0>, g, dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_pre_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(g.x));
+    hipLaunchKernelGGL(k_pre_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(g.x));
 }
 
 void launch_quantize(long n, const Quant& q, const double4* pos, const int* orig, int4* qpos,
@@ -397,7 +397,7 @@ void launch_quantize(long n, const Quant& q, const double4* pos, const int* orig
 
 void launch_check(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_check, dim3(grid_of(a.n)), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_check_finish, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(grid_of(a.n)));
+    hipLaunchKernelGGL(k_check_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(grid_of(a.n)));
 }
 
 void launch_apply_scale(long n, double4* vel, Scalars* sc, hipStream_t s) {

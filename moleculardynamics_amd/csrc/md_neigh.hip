@@ -10,7 +10,7 @@
 // an atom's cell (cells at least rc + skin wide, at least 3 per side), or from
 // every atom when the grid is unusable (the reference's brute-force fallback).
 //
-// Seven kernels: bin -> tile count -> scan -> place -> sort-within-cell ->
+// Six kernels: bin -> tile count -> scan -> place -> sort-within-cell and
 // permute -> commit+build. Every kernel takes the device rebuild flag and
 // returns at once when it is 0, so a multi-step run launches the rebuild
 // unconditionally and keeps the decision on the device. The O(N) passes use
@@ -25,6 +25,7 @@ namespace mdk {
 namespace {
 
 constexpr unsigned kGridCap = 2048;
+constexpr long kSortCap = 8192;
 
 __device__ __forceinline__ int cell_coord(double r, double L, double w, int n) {
     // floor((wrap(r) + L/2) / width) clamped to [0, n-1] (cell_list.cpp:31-38);
@@ -35,24 +36,55 @@ __device__ __forceinline__ int cell_coord(double r, double L, double w, int n) {
     return k < 0 ? 0 : (k >= n ? n - 1 : k);
 }
 
-#define GRID_STRIDE(i, n)                                                              \
-    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
-         i += static_cast<long>(gridDim.x) * blockDim.x)
+// Whole waves stride over the atoms together (t = base + lane, the active
+// lanes a prefix of the wave), so lanes can cooperate.
+#define WAVE_STRIDE(base, n)                                                                  \
+    for (long base = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) & ~(kWave - 1L); \
+         base < (n); base += static_cast<long>(gridDim.x) * blockDim.x)
+
+// Runs of equal keys in consecutive lanes. The atoms are in the previous
+// build's cell order, so a wave covers a few cells and its lanes form a few
+// runs; one atomic per run (by its first lane, with the run length) replaces
+// one per lane on the same counter. rank is the lane's place in its run and
+// head the run's first lane.
+struct Run {
+    int len, rank, head;
+    bool first;
+};
+
+__device__ __forceinline__ Run wave_run(int key, bool active) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int prev = __shfl_up(key, 1, kWave);
+    const bool first = active && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(first);
+    const int nact = __popcll(__ballot(active));
+    const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
+    Run r;
+    r.first = first;
+    r.head = 63 - __clzll(heads & upto);
+    r.rank = lane - r.head;
+    const unsigned long long after = heads & ~upto;
+    r.len = (after ? __ffsll(static_cast<long long>(after)) - 1 : nact) - lane;
+    return r;
+}
 
 __global__ void k_bin(const double4* __restrict__ pos, long n, Grid g, int* __restrict__ cell_of,
                       int* __restrict__ cell_count, const int* flag) {
     if (*flag == 0) return;
-    GRID_STRIDE(t, n) {
+    WAVE_STRIDE(base, n) {
+        const long t = base + (threadIdx.x & (kWave - 1));
+        const bool active = t < n;
         int c = 0;
-        if (g.usable) {
+        if (active && g.usable) {
             const double4 p = pos[t];
             const int cx = cell_coord(p.x, g.L[0], g.w[0], g.nc[0]);
             const int cy = cell_coord(p.y, g.L[1], g.w[1], g.nc[1]);
             const int cz = cell_coord(p.z, g.L[2], g.w[2], g.nc[2]);
             c = cx + g.nc[0] * (cy + g.nc[1] * cz);
         }
-        cell_of[t] = c;
-        atomicAdd(&cell_count[c], 1);
+        const Run r = wave_run(c, active);
+        if (active) cell_of[t] = c;
+        if (r.first) atomicAdd(&cell_count[c], r.len);
     }
 }
 
@@ -124,65 +156,66 @@ __global__ __launch_bounds__(kScanTile) void k_scan(int* __restrict__ cell_count
 __global__ void k_place(const int* __restrict__ cell_of, long n, int* __restrict__ cell_fill,
                         int* __restrict__ order, const int* flag) {
     if (*flag == 0) return;
-    GRID_STRIDE(t, n) {
-        const int at = atomicAdd(&cell_fill[cell_of[t]], 1);
-        order[at] = static_cast<int>(t);
+    WAVE_STRIDE(base, n) {
+        const long t = base + (threadIdx.x & (kWave - 1));
+        const bool active = t < n;
+        const int c = active ? cell_of[t] : 0;
+        const Run r = wave_run(c, active);
+        int at = 0;
+        if (r.first) at = atomicAdd(&cell_fill[c], r.len);
+        at = __shfl(at, r.head, kWave) + r.rank;
+        if (active) order[at] = static_cast<int>(t);
     }
 }
 
 // Within each cell, order the slots by upload index, which makes the new
 // order a pure function of the positions (independent of atomic arrival
-// order). One wave per cell: with up to 64 entries each lane holds one and
-// its rank is the number of smaller keys (upload indices are distinct);
-// larger cells fall back to an insertion sort by one lane.
-__global__ __launch_bounds__(kBlock) void k_sort_cells(const int* __restrict__ cell_start, long ncell,
-                                                       const int* __restrict__ orig,
-                                                       int* __restrict__ order, const int* flag) {
-    if (*flag == 0) return;
+// order), and gather the state into that order. One wave per cell: a slot's
+// rank is the number of smaller keys in its cell (upload indices are
+// distinct); with up to 64 atoms each lane holds one and the keys come by
+// shuffles, a larger cell is ranked 64 slots at a time against keys read
+// from memory.
+__device__ __forceinline__ void gather_slot(const NeighArgs& a, int dst, int v, int key, int c) {
+    const double4 p = a.pos[v];
+    a.pos2[dst] = p;
+    a.vel2[dst] = a.vel[v];
+    a.frc2[dst] = a.frc[v];
+    a.orig2[dst] = key;
+    a.cell2[dst] = c;
+    if (a.qpos2 != nullptr) a.qpos2[dst] = quantize(a.quant, p, key);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a, long ncell) {
+    if (*a.flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
     for (long c = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; c < ncell;
          c += waves) {
-        const int b = cell_start[c], e = cell_start[c + 1];
+        const int b = a.cell_start[c], e = a.cell_start[c + 1];
         const int cnt = e - b;
+        const int cell = a.grid.usable ? static_cast<int>(c) : 0;
         if (cnt <= kWave) {
             int v = 0, key = 0x7fffffff;
             if (lane < cnt) {
-                v = order[b + lane];
-                key = orig[v];
+                v = a.order[b + lane];
+                key = a.orig[v];
             }
             int rank = 0;
             for (int j = 0; j < cnt; ++j) rank += (__shfl(key, j, kWave) < key) ? 1 : 0;
-            if (lane < cnt) order[b + rank] = v;
-        } else if (lane == 0) {
-            for (int i = b + 1; i < e; ++i) {
-                const int v = order[i];
-                const int key = orig[v];
-                int j = i - 1;
-                while (j >= b && orig[order[j]] > key) {
-                    order[j + 1] = order[j];
-                    --j;
+            if (lane < cnt) gather_slot(a, b + rank, v, key, cell);
+        } else {
+            for (int k0 = 0; k0 < cnt; k0 += kWave) {
+                const int k = k0 + lane;
+                int v = 0, key = 0x7fffffff;
+                if (k < cnt) {
+                    v = a.order[b + k];
+                    key = a.orig[v];
                 }
-                order[j + 1] = v;
+                int rank = 0;
+                for (int j = 0; j < cnt; ++j) rank += (a.orig[a.order[b + j]] < key) ? 1 : 0;
+                if (k < cnt) gather_slot(a, b + rank, v, key, cell);
             }
         }
-    }
-}
-
-// Gathers the state into the new order (and the fixed-point positions, when
-// the list keeps them).
-__global__ void k_permute(NeighArgs a) {
-    if (*a.flag == 0) return;
-    GRID_STRIDE(t, a.n) {
-        const int s = a.order[t];
-        const double4 p = a.pos[s];
-        const int o = a.orig[s];
-        a.pos2[t] = p;
-        a.vel2[t] = a.vel[s];
-        a.frc2[t] = a.frc[s];
-        a.orig2[t] = o;
-        a.cell2[t] = a.cell_of[s];
-        if (a.qpos2 != nullptr) a.qpos2[t] = quantize(a.quant, p, o);
     }
 }
 
@@ -233,43 +266,46 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
         qi = a.qpos2[t];
         a.qpos[t] = qi;
     }
-    // Entries are collected eight at a time in this thread's LDS column and
-    // written as one 32-byte octet, so the list goes out in whole sectors
-    // instead of 4-byte partial writes.
-    __shared__ uint32_t stage[8][kBlock];
+    // Entries go to a 16-slot ring in this thread's LDS column without a
+    // branch: every candidate writes the next free slot and only a kept one
+    // advances the count. Whole octets leave as 32-byte writes once eight are
+    // pending, checked after every group of up to four candidates, so at most
+    // 11 are pending and the ring never overwrites one.
+    __shared__ uint32_t ring[16][kBlock];
+    const int tid = threadIdx.x;
     uint32_t* row = a.nbr + oct_index(t, 0, cap);
-    auto flush = [&](int grp) {
-        uint4* dst = reinterpret_cast<uint4*>(row + static_cast<long>(grp) * 512);
-        dst[0] = make_uint4(stage[0][threadIdx.x], stage[1][threadIdx.x], stage[2][threadIdx.x],
-                            stage[3][threadIdx.x]);
-        dst[1] = make_uint4(stage[4][threadIdx.x], stage[5][threadIdx.x], stage[6][threadIdx.x],
-                            stage[7][threadIdx.x]);
-    };
-    int cnt = 0;
-    auto push = [&](int j, bool flip) {
-        if (cnt < cap) {
-            stage[cnt & 7][threadIdx.x] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-            if ((cnt & 7) == 7) flush(cnt >> 3);
+    int cnt = 0, done = 0;  // entries kept / entries written out (a multiple of 8)
+    auto write_octet = [&]() {
+        const int s = done & 15;
+        if (done < cap) {
+            uint4* dst = reinterpret_cast<uint4*>(row + static_cast<long>(done >> 3) * 512);
+            dst[0] = make_uint4(ring[s][tid], ring[s + 1][tid], ring[s + 2][tid], ring[s + 3][tid]);
+            dst[1] = make_uint4(ring[s + 4][tid], ring[s + 5][tid], ring[s + 6][tid],
+                                ring[s + 7][tid]);
         }
-        ++cnt;
+        done += 8;
     };
-    auto visit = [&](int j, const double4& q, int oj) {
-        const bool flip = oi > oj;
+    auto drain = [&]() {
+        if (cnt - done >= 8) write_octet();
+    };
+    auto put = [&](int j, bool keep, bool flip) {
+        ring[cnt & 15][tid] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+        cnt += (keep && j != t) ? 1 : 0;
+    };
+    auto exact = [&](const double4& q, bool flip) {
         double dx, dy, dz;
         row_delta(box, pi, q, flip, dx, dy, dz);
-        if (norm2_ref(dx, dy, dz) < rl2 && j != t) push(j, flip);
+        return norm2_ref(dx, dy, dz) < rl2;
     };
     const double lo = rl2 * (1.0 - a.quant.band_l), hi = rl2 * (1.0 + a.quant.band_l);
     auto visit_q = [&](int j, const int4& qj) {
         double dx, dy, dz;
         qdelta(a.quant, qi, qj, dx, dy, dz);
         const double r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 < hi)) return;
-        if (!(r2 < lo)) {
-            visit(j, pos2[j], qj.w);  // inside the band: the exact test
-        } else if (j != t) {
-            push(j, oi > qj.w);
-        }
+        const bool flip = oi > qj.w;
+        bool keep = r2 < lo;
+        if (!keep && r2 < hi) keep = exact(pos2[j], flip);  // inside the band: the exact test
+        put(j, keep, flip);
     };
     auto range = [&](int b, int e) {
         int j = b;
@@ -281,18 +317,27 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
                 visit_q(j + 1, q1);
                 visit_q(j + 2, q2);
                 visit_q(j + 3, q3);
+                drain();
             }
-            for (; j < e; ++j) visit_q(j, a.qpos2[j]);
+            for (; j < e; ++j) {
+                visit_q(j, a.qpos2[j]);
+                drain();
+            }
         } else {
             for (; j + 4 <= e; j += 4) {
                 const double4 q0 = pos2[j], q1 = pos2[j + 1], q2 = pos2[j + 2], q3 = pos2[j + 3];
                 const int o0 = orig2[j], o1 = orig2[j + 1], o2 = orig2[j + 2], o3 = orig2[j + 3];
-                visit(j, q0, o0);
-                visit(j + 1, q1, o1);
-                visit(j + 2, q2, o2);
-                visit(j + 3, q3, o3);
+                put(j, exact(q0, oi > o0), oi > o0);
+                put(j + 1, exact(q1, oi > o1), oi > o1);
+                put(j + 2, exact(q2, oi > o2), oi > o2);
+                put(j + 3, exact(q3, oi > o3), oi > o3);
+                drain();
             }
-            for (; j < e; ++j) visit(j, pos2[j], orig2[j]);
+            for (; j < e; ++j) {
+                const int oj = orig2[j];
+                put(j, exact(pos2[j], oi > oj), oi > oj);
+                drain();
+            }
         }
     };
     if (g.usable) {
@@ -319,8 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     } else {
         range(0, static_cast<int>(n));
     }
-    const int kept = cnt < cap ? cnt : cap;
-    if (kept & 7) flush(kept >> 3);  // the last, partial octet (slots past the row are not read)
+    if (cnt > done) write_octet();  // the last, partial octet (slots past the row are not read)
     a.ncnt[t] = cnt;
     if (cnt > cap) atomicOr(&a.sc->overflow, 1);
 }
@@ -335,15 +379,17 @@ inline unsigned grid_capped(long n) {
 void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     const long n = a.n;
     const long ncell = a.grid.usable ? static_cast<long>(a.grid.nc[0]) * a.grid.nc[1] * a.grid.nc[2] : 1;
-    const dim3 gn(grid_capped(n)), gc(grid_capped(ncell)), b(kBlock);
+    // k_sort_permute takes one wave per cell
+    const long sort_blocks = (ncell + kBlock / kWave - 1) / (kBlock / kWave);
+    const dim3 gn(grid_capped(n)), b(kBlock);
+    const dim3 gc(static_cast<unsigned>(sort_blocks < kSortCap ? sort_blocks : kSortCap));
     const dim3 gt(scan_tiles(ncell)), bt(kScanTile);
     hipLaunchKernelGGL(k_bin, gn, b, 0, s, a.pos, n, a.grid, a.cell_of, a.cell_count, a.flag);
     hipLaunchKernelGGL(k_tile_count, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.flag);
     hipLaunchKernelGGL(k_scan, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.cell_start,
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
-    hipLaunchKernelGGL(k_sort_cells, gc, b, 0, s, a.cell_start, ncell, a.orig, a.order, a.flag);
-    hipLaunchKernelGGL(k_permute, gn, b, 0, s, a);
+    hipLaunchKernelGGL(k_sort_permute, gc, b, 0, s, a, ncell);
     if (a.qpos2 != nullptr)
         hipLaunchKernelGGL(k_commit_build<true>, dim3(xcd_grid(n)), b, 0, s, a);
     else
