@@ -43,6 +43,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(key: str):
+    """HBM bytes per launch of the pair kernel from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "k_lj_lj1m_nhc_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def run_gpu(L, pos, vel, precision: int, steps: int, warmup: int):
     from moleculardynamics_amd import native
 
@@ -405,7 +419,8 @@ def main() -> int:
         "rebuilds": {"fp64": d["rebuilds"], "mixed": m["rebuilds"]},
         "roofline": {"bound": "valu", "kernel": "k_lj (LJ pair kernel, mixed)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
+                     "frac": achieved / PEAK_FP32,
+                     "traffic": profiled_traffic("traffic_bytes_per_launch_mixed"),
                      "launch_ms": m["force_ms"],
                      "step_frac": (max(168.0 * n / PEAK_HBM, F_PAIR * pairs / PEAK_FP32)
                                    / (m["ms_per_step"] * 1e-3))},

@@ -52,7 +52,7 @@ struct md_ctx {
 
     // cells at the LJ cutoff (MD_PAIRS_CELLS)
     int *ci_cell_of = nullptr, *ci_count = nullptr, *ci_start = nullptr, *ci_fill = nullptr,
-        *ci_atoms = nullptr;
+        *ci_atoms = nullptr, *ci_tmp = nullptr;
     long ci_ncell_alloc = 0;
 
     // reductions and device scalars
@@ -388,7 +388,7 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->qpos); release(c->qpos2);
     release(c->nbr); release(c->ncnt);
     release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
-    release(c->ci_atoms);
+    release(c->ci_atoms); release(c->ci_tmp);
     release(c->part_d); release(c->part_l); release(c->sc); release(c->chain); release(c->com4);
     release(c->thermo_dev);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -591,8 +591,9 @@ md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, dou
             }
             MD_HIP(c, alloc(c->ci_cell_of, c->n));
             MD_HIP(c, alloc(c->ci_atoms, c->n));
+            MD_HIP(c, alloc(c->ci_tmp, c->n));
             mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start,
-                                  c->ci_fill, c->ci_atoms, c->stream);
+                                  c->ci_fill, c->ci_tmp, c->ci_atoms, c->stream);
             a.grid = g;
             a.cell_of = c->ci_cell_of;
             a.cell_start = c->ci_start;
@@ -893,8 +894,9 @@ md_status md_lj_forces_cells(md_ctx* c, double cell_cutoff, double* energy, doub
     }
     MD_HIP(c, alloc(c->ci_cell_of, c->n));
     MD_HIP(c, alloc(c->ci_atoms, c->n));
+    MD_HIP(c, alloc(c->ci_tmp, c->n));
     mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start, c->ci_fill,
-                          c->ci_atoms, c->stream);
+                          c->ci_tmp, c->ci_atoms, c->stream);
     mdk::DirectArgs a{};
     a.n = c->n;
     a.box = c->box;

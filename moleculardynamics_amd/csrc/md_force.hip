@@ -671,17 +671,29 @@ __global__ void k_place_index(const int* __restrict__ cell_of, long n, int* __re
     cell_atoms[atomicAdd(&fill[cell_of[t]], 1)] = static_cast<int>(t);
 }
 
-__global__ void k_sort_index(const int* __restrict__ start, long m, int* __restrict__ cell_atoms) {
-    const long c = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= m) return;
-    for (int i = start[c] + 1; i < start[c + 1]; ++i) {
-        const int v = cell_atoms[i];
-        int j = i - 1;
-        while (j >= start[c] && cell_atoms[j] > v) {
-            cell_atoms[j + 1] = cell_atoms[j];
-            --j;
+// Within each cell, the slot indices placed by k_place_index (in atomic
+// arrival order, in tmp) go to cell_atoms in ascending order, so the index
+// is a pure function of the positions. One wave per cell: a slot's rank is
+// its count of smaller indices in the cell, the other indices read 64 at a
+// time and broadcast by shuffles.
+__global__ __launch_bounds__(kBlock) void k_rank_index(const int* __restrict__ start, long m,
+                                                       const int* __restrict__ tmp,
+                                                       int* __restrict__ cell_atoms) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
+    for (long c = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; c < m;
+         c += waves) {
+        const int b = start[c], cnt = start[c + 1] - b;
+        for (int k0 = 0; k0 < cnt; k0 += kWave) {
+            const int key = k0 + lane < cnt ? tmp[b + k0 + lane] : 0x7fffffff;
+            int rank = 0;
+            for (int j0 = 0; j0 < cnt; j0 += kWave) {
+                const int other = j0 + lane < cnt ? tmp[b + j0 + lane] : 0x7fffffff;
+                const int lim = min(kWave, cnt - j0);
+                for (int q = 0; q < lim; ++q) rank += __shfl(other, q, kWave) < key ? 1 : 0;
+            }
+            if (k0 + lane < cnt) cell_atoms[b + rank] = key;
         }
-        cell_atoms[j + 1] = v;
     }
 }
 
@@ -728,15 +740,18 @@ void launch_lj_direct(const DirectArgs& a, hipStream_t s) {
 }
 
 void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,
-                      int* cell_start, int* cell_fill, int* cell_atoms, hipStream_t s) {
+                      int* cell_start, int* cell_fill, int* cell_tmp, int* cell_atoms,
+                      hipStream_t s) {
     const long m = static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2];
     hipLaunchKernelGGL(k_zero_int, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_count, m);
     hipLaunchKernelGGL(k_bin_index, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, g, cell_of,
                        cell_count);
     hipLaunchKernelGGL(k_scan_plain, dim3(1), dim3(1024), 0, s, cell_count, m, cell_start, cell_fill);
     hipLaunchKernelGGL(k_place_index, dim3(grid_of(n)), dim3(kBlock), 0, s, cell_of, n, cell_fill,
-                       cell_atoms);
-    hipLaunchKernelGGL(k_sort_index, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_start, m, cell_atoms);
+                       cell_tmp);
+    const long rank_blocks = (m + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL(k_rank_index, dim3(static_cast<unsigned>(rank_blocks < 8192 ? rank_blocks : 8192)),
+                       dim3(kBlock), 0, s, cell_start, m, cell_tmp, cell_atoms);
 }
 
 }  // namespace mdk

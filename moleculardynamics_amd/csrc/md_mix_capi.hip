@@ -53,7 +53,7 @@ struct md_mix {
     int cap = 0;
     long nbr_alloc = 0;
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr,
-        *cell_atoms = nullptr;
+        *cell_atoms = nullptr, *cell_tmp = nullptr;
     long ncell_alloc = 0;
     double *part_ew = nullptr, *part_q = nullptr, *part_d = nullptr, *gamma = nullptr,
            *hist = nullptr, *tables = nullptr, *ew = nullptr;
@@ -195,7 +195,7 @@ md_status build_list(md_mix* m, const double4* p, double4* ref) {
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (m->grid.usable)
             mdk::launch_bin_index(m->n, p, m->grid, m->cell_of, m->cell_count, m->cell_start,
-                                  m->cell_fill, m->cell_atoms, m->stream);
+                                  m->cell_fill, m->cell_tmp, m->cell_atoms, m->stream);
         mdk::MixListArgs a{};
         a.n = m->n;
         a.grid = m->grid;
@@ -395,6 +395,7 @@ md_status md_mix_destroy(md_mix* m) {
     free_dev(m->nbr); free_dev(m->ncnt);
     free_dev(m->cell_of); free_dev(m->cell_count); free_dev(m->cell_start); free_dev(m->cell_fill);
     free_dev(m->cell_atoms);
+    free_dev(m->cell_tmp);
     free_dev(m->part_ew); free_dev(m->part_q); free_dev(m->part_d); free_dev(m->gamma);
     free_dev(m->hist); free_dev(m->tables); free_dev(m->ew); free_dev(m->sc);
     free_dev(m->ckpt); free_dev(m->frames); free_dev(m->lx); free_dev(m->lv); free_dev(m->lu);
@@ -428,6 +429,7 @@ md_status md_mix_upload(md_mix* m, long n, const double* L, const double* x, con
     MHIP(m, malloc_dev(m->pos, n)); MHIP(m, malloc_dev(m->vel, n)); MHIP(m, malloc_dev(m->frc, n));
     MHIP(m, malloc_dev(m->ref, n)); MHIP(m, malloc_dev(m->ncnt, n));
     MHIP(m, malloc_dev(m->cell_of, n)); MHIP(m, malloc_dev(m->cell_atoms, n));
+    MHIP(m, malloc_dev(m->cell_tmp, n));
     MHIP(m, malloc_dev(m->part_ew, 2 * n));
     MHIP(m, malloc_dev(m->part_q, static_cast<long>(mdk::kMaxTypes) * mdk::kMixQ * n));
     MHIP(m, malloc_dev(m->part_d, 4 * nblocks(n) + 8));

@@ -19,7 +19,11 @@
 // the conjugate of S(k), so the +k and -k terms are equal):
 //  k_sk     one thread per k-vector, sites staged in LDS tiles, partial S(k)
 //           per site tile, then k_sk_reduce sums the tiles in order;
-//  k_recip  one thread per site, loops over the k-vectors of the half space.
+//  k_recip  (site tile, column chunk) blocks: a thread walks its site over
+//           the chunk's columns of consecutive nz with one complex product
+//           per k-vector, and k_recip_reduce adds the chunks in order.
+// The real-space kernel takes kWaterTpa lanes per site, the list build one
+// wave per site, so a 12,288-site system fills the chip.
 // The mixed path evaluates erfc in FP32 with Abramowitz & Stegun 7.1.26
 // (|error| <= 1.5e-7); the k-space sums stay FP64.
 #include "md_common.hpp"
@@ -31,6 +35,7 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kSiteTile = 16;  // sites per LDS tile of the S(k) kernel
+constexpr int kWaterTpa = 16;  // lanes per site of the real-space kernel
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -51,23 +56,43 @@ __device__ __forceinline__ float erfc_as(float x, float ex2) {
     for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
          i += static_cast<long>(gridDim.x) * blockDim.x)
 
-// Full list of intermolecular partners within rc + skin: cells (index array,
-// no re-sorting) when at least 3 per side fit, all sites otherwise.
+// Full list of intermolecular partners within rc + skin: cells (index array
+// in site order within a cell) when at least 3 per side fit, all sites
+// otherwise. One wave per site tests 64 consecutive candidates at once and
+// keeps them in candidate order by a ballot and a prefix count, so a row is
+// the row of the serial scan (cells in order, sites in index order).
 __global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int* __restrict__ cell_of,
                                                   const int* __restrict__ cell_start,
                                                   const int* __restrict__ cell_atoms) {
-    GRID_STRIDE(i, a.n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = (1ull << lane) - 1;
+    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
+    for (long i = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; i < a.n;
+         i += waves) {
         const double4 pi = a.pos[i];
         const long mi = i / 3;
         int cnt = 0;
-        auto visit = [&](int j) {
-            if (j / 3 == mi) return;
-            const bool flip = i > j;
-            double dx, dy, dz;
-            pair_delta(a.box, pi, a.pos[j], flip, dx, dy, dz);
-            if (norm2_ref(dx, dy, dz) < a.rl2) {
-                if (cnt < a.cap) a.nbr[nbr_index(i, cnt, a.cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                ++cnt;
+        // candidates idx[b..e), or the sites b..e when idx is null
+        auto scan = [&](const int* idx, int b, int e) {
+            for (int k0 = b; k0 < e; k0 += kWave) {
+                const int k = k0 + lane;
+                bool keep = false;
+                int j = 0;
+                if (k < e) {
+                    j = idx != nullptr ? idx[k] : k;
+                    if (j / 3 != mi) {
+                        double dx, dy, dz;
+                        pair_delta(a.box, pi, a.pos[j], i > j, dx, dy, dz);
+                        keep = norm2_ref(dx, dy, dz) < a.rl2;
+                    }
+                }
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int at = cnt + __popcll(m & below);
+                    if (at < a.cap)
+                        a.nbr[nbr_index(i, at, a.cap)] = static_cast<uint32_t>(j) | (i > j ? kFlipBit : 0u);
+                }
+                cnt += __popcll(m);
             }
         };
         if (g.usable) {
@@ -79,31 +104,38 @@ __global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int
                         const int nb = (cx + ox + g.nc[0]) % g.nc[0] +
                                        g.nc[0] * ((cy + oy + g.nc[1]) % g.nc[1] +
                                                   g.nc[1] * ((cz + oz + g.nc[2]) % g.nc[2]));
-                        for (int k = cell_start[nb]; k < cell_start[nb + 1]; ++k) visit(cell_atoms[k]);
+                        scan(cell_atoms, cell_start[nb], cell_start[nb + 1]);
                     }
         } else {
-            for (int j = 0; j < static_cast<int>(a.n); ++j) visit(j);
+            scan(nullptr, 0, static_cast<int>(a.n));
         }
-        a.ncnt[i] = cnt;
-        if (cnt > a.cap) atomicOr(a.overflow, 1);
-        a.ref[i] = pi;
+        if (lane == 0) {
+            a.ncnt[i] = cnt;
+            if (cnt > a.cap) atomicOr(a.overflow, 1);
+            a.ref[i] = pi;
+        }
     }
 }
 
 // Real-space Coulomb + oxygen LJ, intramolecular correction; forces are
 // written (not accumulated), the reciprocal kernel adds its part.
+// kWaterTpa lanes per site: lane s of a site takes row entries s, s + kWaterTpa,
+// ... and the intramolecular partner s (s < 2); the lanes' sums are combined
+// by a fixed shuffle tree.
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
     __shared__ double sm[kBlock / kWave];
-    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long g = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long i = g / kWaterTpa;
+    const int sub = static_cast<int>(g % kWaterTpa);
     double ereal = 0.0, elj = 0.0, eintra = 0.0, wsum = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     if (i < a.n) {
         const double4 pi = a.pos[i];
         const bool oi = (i % 3) == 0;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
         const int cnt = min(a.ncnt[i], a.cap);
         const uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
-        for (int k = 0; k < cnt; ++k) {
+        for (int k = sub; k < cnt; k += kWaterTpa) {
             const uint32_t jj = row[k * 64];
             const int j = static_cast<int>(jj & kIndexMask);
             const bool flip = (jj & kFlipBit) != 0;
@@ -141,10 +173,10 @@ __global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
             fy += c * dy;
             fz += c * dz;
         }
-        // intramolecular correction: the two partners of this site
-        const long m0 = (i / 3) * 3;
-        for (long j = m0; j < m0 + 3; ++j) {
-            if (j == i) continue;
+        // intramolecular correction: lanes 0 and 1 of the site take its two partners
+        if (sub < 2) {
+            const long m0 = (i / 3) * 3;
+            const long j = m0 + ((m0 + sub) >= i ? sub + 1 : sub);
             const double4 pj = a.pos[j];
             const bool flip = i > j;
             double dx, dy, dz;
@@ -160,8 +192,13 @@ __global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
             fy += c * dy;
             fz += c * dz;
         }
-        a.frc[i] = make_double4(fx, fy, fz, 0.0);
     }
+    for (int off = kWaterTpa / 2; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, kWave);
+        fy += __shfl_xor(fy, off, kWave);
+        fz += __shfl_xor(fz, off, kWave);
+    }
+    if (i < a.n && sub == 0) a.frc[i] = make_double4(fx, fy, fz, 0.0);
     const double b0 = block_reduce(ereal, sm, wave_sum);
     const double b1 = block_reduce(elj, sm, wave_sum);
     const double b2 = block_reduce(eintra, sm, wave_sum);
@@ -250,41 +287,75 @@ __global__ void k_sk_reduce(WaterArgs a, int groups) {
             im += p.y;
         }
         a.sk[kv] = make_double2(re, im);
+        a.skf[kv] = make_double2(4.0 * a.kfac[kv] * re, 4.0 * a.kfac[kv] * im);
         e = 2.0 * a.kfac[kv] * (re * re + im * im);
     }
     const double b = block_reduce(e, sm, wave_sum);
     if (threadIdx.x == 0) a.part2[blockIdx.x] = b;
 }
 
-// Reciprocal forces: f_i += sum_k 2 * [2 factor q_i (Re S sin - Im S cos)] k
-// over the half space (the -k term equals the +k term).
+__device__ __forceinline__ double2 site_phase(const double2* t, int nn, long n) {
+    // e^{i n theta} of one site for n of either sign; t points at the site's n = 0 entry
+    const double2 v = t[static_cast<long>(nn < 0 ? -nn : nn) * n];
+    return nn < 0 ? make_double2(v.x, -v.y) : v;
+}
+
+// Reciprocal forces: f_i += q_i sum_k [4 factor(k) (Re S sin - Im S cos)] k
+// over the half space (the -k term equals the +k term; skf = 4 factor S).
+// Block (site tile, column chunk): each thread walks the chunk's columns for
+// its site, advancing the phase along a column by e^{i k0z z} (one complex
+// product per k-vector after the column's first). Along a column kx and ky
+// are fixed, so the force needs only sum t and sum t nz per column. The
+// chunk's sums go to fpart[chunk]; k_recip_reduce adds the chunks in order.
 __global__ __launch_bounds__(kBlock) void k_recip(WaterArgs a) {
     const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    const double q = a.pos[i].w;
-    const double2* tx = a.eik + (0L * (a.kmax + 1)) * a.n + i;
-    const double2* ty = a.eik + (1L * (a.kmax + 1)) * a.n + i;
-    const double2* tz = a.eik + (2L * (a.kmax + 1)) * a.n + i;
+    const int chunk = blockIdx.y;
+    const int c0 = static_cast<int>(static_cast<long>(a.ncol) * chunk / a.nchunk);
+    const int c1 = static_cast<int>(static_cast<long>(a.ncol) * (chunk + 1) / a.nchunk);
+    const long n = a.n;
+    const double2* tx = a.eik + (0L * (a.kmax + 1)) * n + i;
+    const double2* ty = a.eik + (1L * (a.kmax + 1)) * n + i;
+    const double2* tz = a.eik + (2L * (a.kmax + 1)) * n + i;
+    const double2 step = tz[n];  // e^{i k0z z}
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int kv = 0; kv < a.nk; ++kv) {
-        const int3 nv = a.kvec[kv];
-        const double2 ex = tx[static_cast<long>(nv.x < 0 ? -nv.x : nv.x) * a.n];
-        const double2 ey = ty[static_cast<long>(nv.y < 0 ? -nv.y : nv.y) * a.n];
-        const double2 ez = tz[static_cast<long>(nv.z < 0 ? -nv.z : nv.z) * a.n];
-        const double2 e = cmul(cmul(nv.x < 0 ? make_double2(ex.x, -ex.y) : ex,
-                                    nv.y < 0 ? make_double2(ey.x, -ey.y) : ey),
-                               nv.z < 0 ? make_double2(ez.x, -ez.y) : ez);
-        const double2 S = a.sk[kv];
-        const double coeff = 2.0 * (a.kfac[kv] * 2.0 * q * (S.x * e.y - S.y * e.x));
-        fx += coeff * (a.k0[0] * nv.x);
-        fy += coeff * (a.k0[1] * nv.y);
-        fz += coeff * (a.k0[2] * nv.z);
+    for (int c = c0; c < c1; ++c) {
+        const int4 col = a.kcol[c];
+        const double2* s = a.skf + a.kcol0[c];
+        double2 e = cmul(cmul(site_phase(tx, col.x, n), site_phase(ty, col.y, n)),
+                         site_phase(tz, col.z, n));
+        double sum_t = 0.0, sum_tz = 0.0, nz = static_cast<double>(col.z);
+        for (int m = 0; m < col.w; ++m) {
+            const double2 sf = s[m];
+            const double t = sf.x * e.y - sf.y * e.x;
+            sum_t += t;
+            sum_tz += t * nz;
+            nz += 1.0;
+            e = cmul(e, step);
+        }
+        fx += sum_t * (a.k0[0] * col.x);
+        fy += sum_t * (a.k0[1] * col.y);
+        fz += sum_tz * a.k0[2];
     }
-    double4 f = a.frc[i];
-    f.x += fx;
-    f.y += fy;
-    f.z += fz;
-    a.frc[i] = f;
+    a.fpart[chunk * n + i] = make_double4(fx, fy, fz, 0.0);
+}
+
+__global__ void k_recip_reduce(WaterArgs a) {
+    GRID_STRIDE(i, a.n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int c = 0; c < a.nchunk; ++c) {
+            const double4 p = a.fpart[c * a.n + i];
+            fx += p.x;
+            fy += p.y;
+            fz += p.z;
+        }
+        const double q = a.pos[i].w;
+        double4 f = a.frc[i];
+        f.x += q * fx;
+        f.y += q * fy;
+        f.z += q * fz;
+        a.frc[i] = f;
+    }
 }
 
 // ---- rigid-body constraints: one thread per molecule ------------------------
@@ -478,28 +549,46 @@ inline unsigned gridn(long n) {
 
 void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
                        const int* cell_atoms, hipStream_t s) {
-    hipLaunchKernelGGL(k_wlist, dim3(gridn(a.n)), dim3(kBlock), 0, s, a, g, cell_of, cell_start, cell_atoms);
+    // one wave per site
+    hipLaunchKernelGGL(k_wlist, dim3(gridn(a.n * kWave)), dim3(kBlock), 0, s, a, g, cell_of,
+                       cell_start, cell_atoms);
+}
+
+long water_real_blocks(long n) { return static_cast<long>(gridn(n * kWaterTpa)); }
+
+int water_recip_chunks(long n, int ncol) {
+    // enough (site tile, column chunk) blocks to fill the chip
+    const long tiles = gridn(n);
+    long c = (2048 + tiles - 1) / tiles;
+    if (c > ncol) c = ncol;
+    return static_cast<int>(c < 1 ? 1 : c);
 }
 
 void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s) {
+    const dim3 gr(static_cast<unsigned>(water_real_blocks(a.n)));
     if (mixed)
-        hipLaunchKernelGGL(k_wreal<true>, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_wreal<true>, gr, dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL(k_wreal<false>, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_wreal<false>, gr, dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_weik, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
     const int groups = water_site_groups(a.n);
     const int spg = static_cast<int>((a.n + groups - 1) / groups);
     hipLaunchKernelGGL(k_sk, dim3((a.nk + kBlock - 1) / kBlock, groups), dim3(kBlock), 0, s, a, spg);
     hipLaunchKernelGGL(k_sk_reduce, dim3((a.nk + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, groups);
-    hipLaunchKernelGGL(k_recip, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_recip, dim3(gridn(a.n), a.nchunk), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_recip_reduce, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
 }
 
+// One thread per molecule in single-wave workgroups, so a few thousand
+// molecules spread over as many compute units as there are waves.
+inline unsigned wave_groups(long m) { return static_cast<unsigned>(m > 0 ? (m + kWave - 1) / kWave : 1); }
+
 void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_shake, dim3(gridn(a.n / 3)), dim3(kBlock), 0, s, a, refp, status);
+    hipLaunchKernelGGL(k_shake, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, refp, status);
 }
 
 void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_rattle, dim3(gridn(a.n / 3)), dim3(kBlock), 0, s, a, status);
+    hipLaunchKernelGGL(k_rattle, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, status);
 }
 
 void water_launch_cvv_first(const WaterArgs& a, double dt, double4* start, double4* free_pos,

@@ -34,6 +34,13 @@ struct WaterArgs {
     double2* eik;                // [3][kmax+1][n]
     double2* skpart;             // [groups][nk]
     double2* sk;                 // [nk]
+    // The half-space k-vectors as columns of consecutive nz: column c holds
+    // (nx, ny, first nz, count), its k-vectors are kcol0[c] .. kcol0[c] + count - 1.
+    const int4* kcol;
+    const int* kcol0;
+    int ncol, nchunk;            // columns; column chunks of the reciprocal force kernel
+    double2* skf;                // [nk] 4 factor(k) S(k)
+    double4* fpart;              // [nchunk][n] reciprocal force partials
     int* overflow;
     double oh, hh, tol;
     int max_iter;
@@ -41,6 +48,10 @@ struct WaterArgs {
 
 void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
                        const int* cell_atoms, hipStream_t s);
+// Column chunks of the reciprocal force kernel for n sites and ncol columns.
+int water_recip_chunks(long n, int ncol);
+// Workgroups of the real-space kernel (each writes 4 partials to part).
+long water_real_blocks(long n);
 void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s);
 void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hipStream_t s);
 void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s);

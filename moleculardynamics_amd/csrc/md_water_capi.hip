@@ -42,10 +42,14 @@ struct md_water {
             *freep = nullptr;
     uint32_t* nbr = nullptr;
     int *ncnt = nullptr, *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr,
-        *cell_fill = nullptr, *cell_atoms = nullptr, *status = nullptr, *flags = nullptr;
+        *cell_fill = nullptr, *cell_atoms = nullptr, *cell_tmp = nullptr, *status = nullptr,
+        *flags = nullptr;
     double *part = nullptr, *part2 = nullptr, *kfac = nullptr;
     int3* kvec = nullptr;
-    double2 *eik = nullptr, *skpart = nullptr, *sk = nullptr;
+    double2 *eik = nullptr, *skpart = nullptr, *sk = nullptr, *skf = nullptr;
+    int4* kcol = nullptr;
+    int* kcol0 = nullptr;
+    double4* fpart = nullptr;
     long builds = 0;
     bool list_built = false;
 };
@@ -78,7 +82,7 @@ md_status build_list(md_water* w) {
     const mdk::Grid& g = w->grid;
     if (g.usable) {
         mdk::launch_bin_index(w->n, w->pos, g, w->cell_of, w->cell_count, w->cell_start,
-                              w->cell_fill, w->cell_atoms, w->stream);
+                              w->cell_fill, w->cell_tmp, w->cell_atoms, w->stream);
     }
     mdk::water_launch_list(w->a, g, w->cell_of, w->cell_start, w->cell_atoms, w->stream);
     WHIP(w, hipGetLastError());
@@ -99,7 +103,7 @@ md_status compute(md_water* w, bool mixed, double* terms) {
     }
     mdk::water_launch_forces(w->a, mixed, w->stream);
     WHIP(w, hipGetLastError());
-    const long nb = wblocks(w->n);
+    const long nb = mdk::water_real_blocks(w->n);
     const long nbk = (w->a.nk + mdk::kBlock - 1) / mdk::kBlock;
     std::vector<double> part(static_cast<size_t>(4 * nb)), part2(static_cast<size_t>(nbk));
     WHIP(w, hipStreamSynchronize(w->stream));
@@ -333,22 +337,41 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
             }
     const int nk = static_cast<int>(kv.size());
     const int groups = mdk::water_site_groups(n);
+    // columns: runs of consecutive k-vectors with the same (nx, ny) and nz rising by one
+    std::vector<int4> cols;
+    std::vector<int> col0;
+    for (int k = 0; k < nk; ++k) {
+        if (k > 0 && kv[k].x == kv[k - 1].x && kv[k].y == kv[k - 1].y && kv[k].z == kv[k - 1].z + 1) {
+            cols.back().w += 1;
+        } else {
+            cols.push_back(make_int4(kv[k].x, kv[k].y, kv[k].z, 1));
+            col0.push_back(k);
+        }
+    }
+    const int ncol = static_cast<int>(cols.size());
+    const int nchunk = mdk::water_recip_chunks(n, std::max(1, ncol));
     WHIP(w, walloc(w->pos, n)); WHIP(w, walloc(w->vel, n)); WHIP(w, walloc(w->frc, n));
     WHIP(w, walloc(w->ref, n)); WHIP(w, walloc(w->start, n)); WHIP(w, walloc(w->freep, n));
     const long npad = (n + 63) / 64 * 64;
     WHIP(w, walloc(w->nbr, cap * npad)); WHIP(w, walloc(w->ncnt, n));
     WHIP(w, walloc(w->cell_of, n)); WHIP(w, walloc(w->cell_count, ncell));
     WHIP(w, walloc(w->cell_start, ncell + 1)); WHIP(w, walloc(w->cell_fill, ncell));
-    WHIP(w, walloc(w->cell_atoms, n)); WHIP(w, walloc(w->status, n / 3)); WHIP(w, walloc(w->flags, 4));
-    WHIP(w, walloc(w->part, 4 * wblocks(n) + 8)); WHIP(w, walloc(w->part2, wblocks(std::max<long>(n, nk)) + 8));
+    WHIP(w, walloc(w->cell_atoms, n)); WHIP(w, walloc(w->cell_tmp, n)); WHIP(w, walloc(w->status, n / 3)); WHIP(w, walloc(w->flags, 4));
+    WHIP(w, walloc(w->part, 4 * mdk::water_real_blocks(n) + 8)); WHIP(w, walloc(w->part2, wblocks(std::max<long>(n, nk)) + 8));
     WHIP(w, walloc(w->kvec, std::max(1, nk))); WHIP(w, walloc(w->kfac, std::max(1, nk)));
     WHIP(w, walloc(w->eik, 3L * (km + 1) * n));
     WHIP(w, walloc(w->skpart, static_cast<long>(groups) * std::max(1, nk)));
     WHIP(w, walloc(w->sk, std::max(1, nk)));
+    WHIP(w, walloc(w->skf, std::max(1, nk)));
+    WHIP(w, walloc(w->kcol, std::max(1, ncol)));
+    WHIP(w, walloc(w->kcol0, std::max(1, ncol)));
+    WHIP(w, walloc(w->fpart, static_cast<long>(nchunk) * n));
     WHIP(w, hipMemset(w->flags, 0, 4 * sizeof(int)));
     if (nk > 0) {
         WHIP(w, hipMemcpy(w->kvec, kv.data(), nk * sizeof(int3), hipMemcpyHostToDevice));
         WHIP(w, hipMemcpy(w->kfac, kf.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        WHIP(w, hipMemcpy(w->kcol, cols.data(), ncol * sizeof(int4), hipMemcpyHostToDevice));
+        WHIP(w, hipMemcpy(w->kcol0, col0.data(), ncol * sizeof(int), hipMemcpyHostToDevice));
     }
     std::vector<double4> hp(static_cast<size_t>(n)), hv(static_cast<size_t>(n)),
         hf(static_cast<size_t>(n), make_double4(0, 0, 0, 0));
@@ -377,7 +400,9 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     a.part = w->part; a.part2 = w->part2;
     a.kmax = km; a.nk = nk; a.kvec = w->kvec; a.kfac = w->kfac;
     for (int d = 0; d < 3; ++d) a.k0[d] = k0[d];
-    a.eik = w->eik; a.skpart = w->skpart; a.sk = w->sk;
+    a.eik = w->eik; a.skpart = w->skpart; a.sk = w->sk; a.skf = w->skf;
+    a.kcol = w->kcol; a.kcol0 = w->kcol0; a.ncol = ncol; a.nchunk = nk > 0 ? nchunk : 0;
+    a.fpart = w->fpart;
     a.overflow = w->flags;
     a.oh = p->r_oh;
     a.hh = 2.0 * p->r_oh * std::sin(0.5 * p->angle_hoh * kPi / 180.0);
@@ -394,9 +419,12 @@ md_status md_water_destroy(md_water* w) {
                     static_cast<void*>(w->nbr), static_cast<void*>(w->ncnt), static_cast<void*>(w->cell_of),
                     static_cast<void*>(w->cell_count), static_cast<void*>(w->cell_start),
                     static_cast<void*>(w->cell_fill), static_cast<void*>(w->cell_atoms),
+                    static_cast<void*>(w->cell_tmp),
                     static_cast<void*>(w->status), static_cast<void*>(w->flags), static_cast<void*>(w->part),
                     static_cast<void*>(w->part2), static_cast<void*>(w->kvec), static_cast<void*>(w->kfac),
-                    static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk)})
+                    static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk),
+                    static_cast<void*>(w->skf), static_cast<void*>(w->kcol), static_cast<void*>(w->kcol0),
+                    static_cast<void*>(w->fpart)})
         if (q) (void)hipFree(q);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
