@@ -10,10 +10,12 @@
 //      (x, y, z FP64 arrays; up to 160 KB per workgroup on gfx950) and the
 //      displacement top-2 is reduced in the workgroup;
 //   2. when the sum of the two largest displacements exceeds the skin, the
-//      replica's rows are rebuilt by the reference's exact pair test over
-//      all atom pairs in LDS -- the same pair set as the reference's cell or
-//      brute-force build, including the single-nearest-image semantics of
-//      boxes shorter than 2 (rc + skin);
+//      replica's rows are rebuilt by the reference's exact pair test: from
+//      the 27 cells around each atom (a counting sort by cell in LDS) when the
+//      replica's cell grid is usable, over all atom pairs in LDS otherwise --
+//      the same pair set as the reference's cell or brute-force build,
+//      including the single-nearest-image semantics of boxes shorter than
+//      2 (rc + skin);
 //   3. forces: `tpa` lanes per atom walk its row (a per-size-class choice
 //      made by the autotuner), gathering partners from LDS; FP32 pair terms
 //      with 64-bit fixed-point accumulation (or FP64); the final B half-kick
@@ -56,6 +58,103 @@ __device__ __forceinline__ double4 drift(const Box& b, double4 p, const double4&
     p.y = fold(fma(h, v.y, p.y), b.L[1], b.lo[1], b.hi1[1], b.hi2[1]);
     p.z = fold(fma(h, v.z, p.z), b.L[2], b.lo[2], b.hi1[2], b.hi2[2]);
     return p;
+}
+
+// Cell of a coordinate: floor((wrap(r) + L/2) / width) clamped to [0, n-1]
+// (neighbor/cell_list.cpp:31-38), wrap the reference fold.
+__device__ __forceinline__ int rep_cell_coord(double r, double L, double w, int n) {
+#pragma clang fp contract(off)
+    const double f = fold_ref(r, L);
+    const int k = static_cast<int>(floor((f + 0.5 * L) / w));
+    return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
+// Rebuild of a replica with a usable cell grid, in the workgroup's LDS:
+// counting sort of the atoms by cell (LDS atomics, then the atoms of a cell
+// ordered by index, so the order is a pure function of the positions), and
+// each atom's row from the 27 surrounding cells in (z, y, x) order with the
+// reference's exact pair test. The pair set is the all-pairs set: every pair
+// within rc + skin lies in neighbouring cells.
+__device__ void rebuild_cells(const BatchArgs& a, const BatchRep& R, const double* xs,
+                              const double* ys, const double* zs) {
+    extern __shared__ double lds[];
+    const int n = R.n;
+    int* cidx = reinterpret_cast<int*>(lds + 3 * a.nmax);
+    int* tmp = cidx + a.nmax;
+    int* srt = tmp + a.nmax;
+    int* cstart = srt + a.nmax;
+    int* cfill = cstart + a.maxcell + 1;
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    const int ncell = nx * ny * nz;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int c = tid; c < ncell; c += bd) cfill[c] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += bd) {
+        const int cx = rep_cell_coord(xs[i], R.box.L[0], R.w[0], nx);
+        const int cy = rep_cell_coord(ys[i], R.box.L[1], R.w[1], ny);
+        const int cz = rep_cell_coord(zs[i], R.box.L[2], R.w[2], nz);
+        const int c = cx + nx * (cy + ny * cz);
+        cidx[i] = c;
+        atomicAdd(&cfill[c], 1);
+    }
+    __syncthreads();
+    if (tid < kWave) {  // exclusive scan of the counts by the first wave
+        const int lane = tid;
+        int run = 0;
+        for (int c0 = 0; c0 < ncell; c0 += kWave) {
+            const int c = c0 + lane;
+            const int v = c < ncell ? cfill[c] : 0;
+            int x = v;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int y = __shfl_up(x, off, kWave);
+                if (lane >= off) x += y;
+            }
+            if (c < ncell) cstart[c] = run + x - v;
+            run += __shfl(x, kWave - 1, kWave);
+        }
+        if (lane == 0) cstart[ncell] = run;
+    }
+    __syncthreads();
+    for (int c = tid; c < ncell; c += bd) cfill[c] = cstart[c];
+    __syncthreads();
+    for (int i = tid; i < n; i += bd) tmp[atomicAdd(&cfill[cidx[i]], 1)] = i;
+    __syncthreads();
+    for (int i = tid; i < n; i += bd) {
+        const int c = cidx[i], b = cstart[c], e = cstart[c + 1];
+        int r = 0;
+        for (int k = b; k < e; ++k) r += tmp[k] < i ? 1 : 0;
+        srt[b + r] = i;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += bd) {
+        const long g = R.off + i;
+        const double4 pi = make_double4(xs[i], ys[i], zs[i], 0.0);
+        const int c = cidx[i];
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        int cnt = 0;
+        uint32_t* row = a.nbr + g * a.cap;
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int nb = (cx + ox + nx) % nx + nx * ((cy + oy + ny) % ny + ny * ((cz + oz + nz) % nz));
+                    for (int k = cstart[nb]; k < cstart[nb + 1]; ++k) {
+                        const int j = srt[k];
+                        if (j == i) continue;
+                        const bool flip = i > j;
+                        double dx, dy, dz;
+                        row_delta(R.box, pi, make_double4(xs[j], ys[j], zs[j], 0.0), flip, dx, dy, dz);
+                        if (norm2_ref(dx, dy, dz) < a.rl2) {
+                            if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+                            ++cnt;
+                        }
+                    }
+                }
+        a.ncnt[g] = cnt;
+        if (cnt > a.cap) atomicOr(a.overflow, 1);
+        a.ref[g] = pi;
+    }
+    if (tid == 0) a.builds[blockIdx.x] += 1;
+    __syncthreads();
 }
 
 // Workgroup sum (any power-of-two workgroup of wave64s); result in all threads.
@@ -142,7 +241,9 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
     }
 
     // ---- 2. rebuild the replica's rows ----------------------------------
-    if (s_rebuild) {
+    if (s_rebuild && R.usable && a.maxcell > 0) {
+        rebuild_cells(a, R, xs, ys, zs);
+    } else if (s_rebuild) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const long g = R.off + i;
             const double4 pi = make_double4(xs[i], ys[i], zs[i], 0.0);
@@ -293,8 +394,14 @@ __global__ void k_batch_advance(unsigned long long* step) {
 
 }  // namespace
 
+size_t batch_lds_bytes(int nmax, int maxcell) {
+    size_t b = static_cast<size_t>(3) * nmax * sizeof(double);
+    if (maxcell > 0) b += (static_cast<size_t>(3) * nmax + 2 * static_cast<size_t>(maxcell) + 1) * sizeof(int);
+    return b;
+}
+
 hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s) {
-    const size_t shmem = static_cast<size_t>(3) * a.nmax * sizeof(double);
+    const size_t shmem = batch_lds_bytes(a.nmax, a.maxcell);
     if (a.mixed)
         hipLaunchKernelGGL(k_batch<true>, dim3(a.nrep), dim3(block), shmem, s, a);
     else

@@ -23,6 +23,7 @@ struct md_batch {
     int nrep = 0;
     long natoms = 0;
     int nmax = 0;
+    int maxcell = 0;
     int block = 512;
     int mixed = 1;
     double dt = 0.0, skin = 0.0, friction = 0.0;
@@ -71,6 +72,7 @@ mdk::BatchArgs args_of(md_batch* b, int mode, double* thermo, long thermo_steps)
     a.reps = b->d_reps;
     a.nrep = b->nrep;
     a.nmax = b->nmax;
+    a.maxcell = b->maxcell;
     a.mode = mode;
     a.mixed = b->mixed;
     a.pos = b->pos; a.vel = b->vel; a.frc = b->frc; a.ref = b->ref;
@@ -201,10 +203,23 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
     const double rl = lj->cutoff + skin;
     long cap = static_cast<long>(std::ceil(2.0 * 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho_max + 32.0));
     b->cap = static_cast<int>(std::min<long>((cap + 7) / 8 * 8, b->nmax));
-    const size_t lds = static_cast<size_t>(3) * b->nmax * sizeof(double);
+    // each replica's cell grid at rc + skin (the reference's rule)
+    for (auto& R : b->reps) {
+        bool usable = true;
+        for (int d = 0; d < 3; ++d) {
+            R.nc[d] = std::max(1, static_cast<int>(std::floor(R.box.L[d] / rl)));
+            R.w[d] = R.box.L[d] / R.nc[d];
+            usable = usable && R.nc[d] >= 3;
+        }
+        R.usable = usable ? 1 : 0;
+        if (usable) b->maxcell = std::max(b->maxcell, R.nc[0] * R.nc[1] * R.nc[2]);
+    }
     int dev = 0, maxsh = 0;
     BHIP(b, hipGetDevice(&dev));
     BHIP(b, hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // cell tables when they fit next to the positions, else all-pairs rebuilds
+    if (mdk::batch_lds_bytes(b->nmax, b->maxcell) + 2048 > static_cast<size_t>(maxsh)) b->maxcell = 0;
+    const size_t lds = mdk::batch_lds_bytes(b->nmax, b->maxcell);
     if (lds + 2048 > static_cast<size_t>(maxsh))
         return bfail(b, MD_ERR_INVALID_ARGUMENT,
                      "largest replica does not fit the workgroup's LDS (" + std::to_string(maxsh) + " B)");
