@@ -89,46 +89,77 @@ __device__ __forceinline__ void sigma_terms(const MixTab& T, const PairTerms& p,
 
 // ---- list ----------------------------------------------------------------
 
+// One wave per atom: 64 consecutive candidates are tested at once and kept
+// in candidate order by a ballot and a prefix count, so the row is the row
+// of the serial scan (27 cells in (z, y, x) order, atoms in index order
+// within a cell, or every atom when the grid is unusable).
 __global__ __launch_bounds__(kBlock) void k_mix_list(MixListArgs a) {
-    const long il = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
-    if (il >= a.n) return;
-    const int i = static_cast<int>(il);
-    const double4 pi = a.pos[i];
-    if (a.ref != nullptr) a.ref[i] = pi;
-    uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
-    int cnt = 0;
-    auto visit = [&](int j) {
-        if (j == i) return;
-        const bool flip = i > j;
-        double dx, dy, dz;
-        pair_delta(a.box, pi, a.pos[j], flip, dx, dy, dz);
-        if (norm2_ref(dx, dy, dz) < a.rl2) {
-            if (cnt < a.cap) row[cnt * 64] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-            ++cnt;
-        }
-    };
-    const Grid& g = a.grid;
-    if (g.usable) {
-        const int ci = a.cell_of[i];
-        const int cx = ci % g.nc[0];
-        const int cy = (ci / g.nc[0]) % g.nc[1];
-        const int cz = ci / (g.nc[0] * g.nc[1]);
-        for (int oz = -1; oz <= 1; ++oz) {
-            const int zz = (cz + oz + g.nc[2]) % g.nc[2];
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const int xx = (cx + ox + g.nc[0]) % g.nc[0];
-                    const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
-                    for (int m = a.cell_start[c]; m < a.cell_start[c + 1]; ++m) visit(a.cell_atoms[m]);
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = (1ull << lane) - 1;
+    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
+    for (long il = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; il < a.n;
+         il += waves) {
+        const int i = static_cast<int>(il);
+        const double4 pi = a.pos[i];
+        uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
+        int cnt = 0;
+        auto scan = [&](const int* idx, int b, int e) {
+            for (int k0 = b; k0 < e; k0 += kWave) {
+                const int k = k0 + lane;
+                bool keep = false;
+                int j = 0;
+                if (k < e) {
+                    j = idx != nullptr ? idx[k] : k;
+                    if (j != i) {
+                        double dx, dy, dz;
+                        pair_delta(a.box, pi, a.pos[j], i > j, dx, dy, dz);
+                        keep = norm2_ref(dx, dy, dz) < a.rl2;
+                    }
+                }
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int at = cnt + __popcll(m & below);
+                    if (at < a.cap) row[at * 64] = static_cast<uint32_t>(j) | (i > j ? kFlipBit : 0u);
+                }
+                cnt += __popcll(m);
+            }
+        };
+        const Grid& g = a.grid;
+        if (g.usable) {
+            const int ci = a.cell_of[i];
+            const int cx = ci % g.nc[0];
+            const int cy = (ci / g.nc[0]) % g.nc[1];
+            const int cz = ci / (g.nc[0] * g.nc[1]);
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int zz = (cz + oz + g.nc[2]) % g.nc[2];
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int yy = (cy + oy + g.nc[1]) % g.nc[1];
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int xx = (cx + ox + g.nc[0]) % g.nc[0];
+                        const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
+                        scan(a.cell_atoms, a.cell_start[c], a.cell_start[c + 1]);
+                    }
                 }
             }
+        } else {
+            scan(nullptr, 0, static_cast<int>(a.n));
         }
-    } else {
-        for (int j = 0; j < static_cast<int>(a.n); ++j) visit(j);
+        if (lane == 0) {
+            if (a.ref != nullptr) a.ref[i] = pi;
+            a.ncnt[i] = cnt;
+            if (cnt > a.cap) atomicOr(&a.sc->overflow, 1);
+        }
     }
-    a.ncnt[i] = cnt;
-    if (cnt > a.cap) atomicOr(&a.sc->overflow, 1);
+}
+
+// kMixTpa lanes per atom in the force and adjoint passes: lane s of an atom
+// takes row entries s, s + kMixTpa, ..., and the lanes' sums are combined by
+// a fixed shuffle tree (lanes of an atom are adjacent).
+constexpr int kMixTpa = 8;
+
+__device__ __forceinline__ double mix_lane_sum(double v) {
+    for (int off = kMixTpa / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
 }
 
 // ---- forces (+ parameter derivatives) ---------------------------------------
@@ -136,20 +167,22 @@ __global__ __launch_bounds__(kBlock) void k_mix_list(MixListArgs a) {
 template <bool LFS, bool DERIV>
 __global__ __launch_bounds__(kBlock) void k_mix_force(MixArgs a) {
 #pragma clang fp contract(off)
-    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long i = gt / kMixTpa;
+    const int sub = static_cast<int>(gt % kMixTpa);
+    const bool valid = i < a.n;
     const MixTab& T = a.tab;
-    const double4 pi = a.pos[i];
+    const double4 pi = valid ? a.pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     const int ti = static_cast<int>(pi.w);
-    const int cnt = min(a.ncnt[i], a.cap);
-    const uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
+    const int cnt = valid ? min(a.ncnt[i], a.cap) : 0;
+    const uint32_t* row = a.nbr + nbr_index(valid ? i : 0, 0, a.cap);
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, w = 0.0;
     double q[kMaxTypes][kMixQ];
 #pragma unroll
     for (int b = 0; b < kMaxTypes; ++b)
 #pragma unroll
         for (int s = 0; s < kMixQ; ++s) q[b][s] = 0.0;
-    for (int m = 0; m < cnt; ++m) {
+    for (int m = sub; m < cnt; m += kMixTpa) {
         const uint32_t en = row[m * 64];
         const double4 pj = a.pos[en & kIndexMask];
         PairTerms p;
@@ -178,6 +211,18 @@ __global__ __launch_bounds__(kBlock) void k_mix_force(MixArgs a) {
             }
         }
     }
+    fx = mix_lane_sum(fx);
+    fy = mix_lane_sum(fy);
+    fz = mix_lane_sum(fz);
+    e = mix_lane_sum(e);
+    w = mix_lane_sum(w);
+    if (DERIV) {
+#pragma unroll
+        for (int b = 0; b < kMaxTypes; ++b)
+#pragma unroll
+            for (int s = 0; s < kMixQ; ++s) q[b][s] = mix_lane_sum(q[b][s]);
+    }
+    if (!valid || sub != 0) return;
     a.frc[i] = make_double4(fx, fy, fz, 0.0);
     a.part_ew[2 * i] = e;
     a.part_ew[2 * i + 1] = w;
@@ -195,20 +240,22 @@ __global__ __launch_bounds__(kBlock) void k_mix_force(MixArgs a) {
 template <bool LFS>
 __global__ __launch_bounds__(kBlock) void k_mix_adjoint(MixArgs a) {
 #pragma clang fp contract(off)
-    const long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long i = gt / kMixTpa;
+    const int sub = static_cast<int>(gt % kMixTpa);
+    const bool valid = i < a.n;
     const MixTab& T = a.tab;
-    const double4 pi = a.pos[i];
-    const double4 vi = a.vec[i];
+    const double4 pi = valid ? a.pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 vi = valid ? a.vec[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     const int ti = static_cast<int>(pi.w);
-    const int cnt = min(a.ncnt[i], a.cap);
-    const uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
+    const int cnt = valid ? min(a.ncnt[i], a.cap) : 0;
+    const uint32_t* row = a.nbr + nbr_index(valid ? i : 0, 0, a.cap);
     const double wE = a.w_energy, wW = a.w_virial;
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double G[kMaxTypes][kMixG];
 #pragma unroll
     for (int b = 0; b < kMaxTypes; ++b) G[b][0] = G[b][1] = 0.0;
-    for (int m = 0; m < cnt; ++m) {
+    for (int m = sub; m < cnt; m += kMixTpa) {
         const uint32_t en = row[m * 64];
         const uint32_t j = en & kIndexMask;
         const double4 pj = a.pos[j];
@@ -251,6 +298,15 @@ __global__ __launch_bounds__(kBlock) void k_mix_adjoint(MixArgs a) {
             }
         }
     }
+    gx = mix_lane_sum(gx);
+    gy = mix_lane_sum(gy);
+    gz = mix_lane_sum(gz);
+#pragma unroll
+    for (int b = 0; b < kMaxTypes; ++b) {
+        G[b][0] = mix_lane_sum(G[b][0]);
+        G[b][1] = mix_lane_sum(G[b][1]);
+    }
+    if (!valid || sub != 0) return;
     double4 o = a.accumulate ? a.out[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     o.x += gx;
     o.y += gy;
@@ -391,11 +447,11 @@ inline unsigned blocks(long n) { return static_cast<unsigned>((n + kBlock - 1) /
 }  // namespace
 
 void launch_mix_list(const MixListArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_mix_list, dim3(blocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_mix_list, dim3(blocks(a.n * kWave)), dim3(kBlock), 0, s, a);  // a wave per atom
 }
 
 void launch_mix_force(const MixArgs& a, int deriv, hipStream_t s) {
-    const dim3 g(blocks(a.n)), b(kBlock);
+    const dim3 g(blocks(a.n * kMixTpa)), b(kBlock);
     if (a.tab.lfs) {
         if (deriv) hipLaunchKernelGGL((k_mix_force<true, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_mix_force<true, false>), g, b, 0, s, a);
@@ -406,7 +462,7 @@ void launch_mix_force(const MixArgs& a, int deriv, hipStream_t s) {
 }
 
 void launch_mix_adjoint(const MixArgs& a, hipStream_t s) {
-    const dim3 g(blocks(a.n)), b(kBlock);
+    const dim3 g(blocks(a.n * kMixTpa)), b(kBlock);
     if (a.tab.lfs) hipLaunchKernelGGL(k_mix_adjoint<true>, g, b, 0, s, a);
     else hipLaunchKernelGGL(k_mix_adjoint<false>, g, b, 0, s, a);
 }
