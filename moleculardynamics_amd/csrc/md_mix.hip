@@ -130,14 +130,21 @@ __global__ __launch_bounds__(kBlock) void k_mix_list(MixListArgs a) {
             const int cx = ci % g.nc[0];
             const int cy = (ci / g.nc[0]) % g.nc[1];
             const int cz = ci / (g.nc[0] * g.nc[1]);
+            // cells x-1, x, x+1 of a (y, z) row are contiguous in the cell-sorted
+            // index unless the row wraps at x: one range instead of three
+            const bool inner = cx >= 1 && cx + 1 < g.nc[0];
             for (int oz = -1; oz <= 1; ++oz) {
                 const int zz = (cz + oz + g.nc[2]) % g.nc[2];
                 for (int oy = -1; oy <= 1; ++oy) {
                     const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                    for (int ox = -1; ox <= 1; ++ox) {
-                        const int xx = (cx + ox + g.nc[0]) % g.nc[0];
-                        const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
-                        scan(a.cell_atoms, a.cell_start[c], a.cell_start[c + 1]);
+                    const int rb = g.nc[0] * (yy + g.nc[1] * zz);
+                    if (inner) {
+                        scan(a.cell_atoms, a.cell_start[rb + cx - 1], a.cell_start[rb + cx + 2]);
+                    } else {
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const int c = rb + (cx + ox + g.nc[0]) % g.nc[0];
+                            scan(a.cell_atoms, a.cell_start[c], a.cell_start[c + 1]);
+                        }
                     }
                 }
             }
@@ -322,16 +329,41 @@ __global__ __launch_bounds__(kBlock) void k_mix_adjoint(MixArgs a) {
 
 // ---- reductions ---------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_mix_sum_ew(long n, const double* __restrict__ part,
-                                                       double* e_out, double* w_out) {
-    __shared__ double sm[kBlock / kWave];
-    double e = 0.0, w = 0.0;
-    for (long i = threadIdx.x; i < n; i += kBlock) {
-        e += part[2 * i];
-        w += part[2 * i + 1];
+// The per-atom reductions below run in one workgroup of kSumThreads
+// threads. Thread t sums atoms t, t + kSumThreads, ... into kSumWays
+// interleaved partial sums (independent loads in flight), which are then
+// combined in a fixed order, the threads by wave shuffles and the waves in
+// index order: the result does not depend on timing.
+constexpr int kSumThreads = 1024;
+constexpr int kSumWays = 4;
+
+__device__ double sum_fixed(double v, double* sm) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < kSumThreads / kWave; ++w) r += sm[w];
+    return r;
+}
+
+__global__ __launch_bounds__(kSumThreads) void k_mix_sum_ew(long n, const double* __restrict__ part,
+                                                            double* e_out, double* w_out) {
+    __shared__ double sm[kSumThreads / kWave];
+    double e[kSumWays] = {}, w[kSumWays] = {};
+    for (long i0 = threadIdx.x; i0 < n; i0 += static_cast<long>(kSumThreads) * kSumWays) {
+#pragma unroll
+        for (int k = 0; k < kSumWays; ++k) {
+            const long i = i0 + static_cast<long>(k) * kSumThreads;
+            if (i < n) {
+                e[k] += part[2 * i];
+                w[k] += part[2 * i + 1];
+            }
+        }
     }
-    const double be = block_reduce(e, sm, wave_sum);
-    const double bw = block_reduce(w, sm, wave_sum);
+    const double be = sum_fixed((e[0] + e[1]) + (e[2] + e[3]), sm);
+    const double bw = sum_fixed((w[0] + w[1]) + (w[2] + w[3]), sm);
     if (threadIdx.x == 0) {
         *e_out = be;
         *w_out = bw;
@@ -339,20 +371,25 @@ __global__ __launch_bounds__(kBlock) void k_mix_sum_ew(long n, const double* __r
 }
 
 // One workgroup per (a, b, k): sum over atoms of type a of part_q[i][b][k].
-__global__ __launch_bounds__(kBlock) void k_mix_tables(long n, const double4* __restrict__ pos,
-                                                       int nt, int q,
-                                                       const double* __restrict__ part,
-                                                       double* tables, int accumulate) {
-    __shared__ double sm[kBlock / kWave];
+__global__ __launch_bounds__(kSumThreads) void k_mix_tables(long n, const double4* __restrict__ pos,
+                                                            int nt, int q,
+                                                            const double* __restrict__ part,
+                                                            double* tables, int accumulate) {
+    __shared__ double sm[kSumThreads / kWave];
     const int id = blockIdx.x;  // (a * nt + b) * q + k
     const int k = id % q;
     const int b = (id / q) % nt;
     const int ta = id / (q * nt);
     const long stride = static_cast<long>(kMaxTypes) * q;
-    double s = 0.0;
-    for (long i = threadIdx.x; i < n; i += kBlock)
-        if (static_cast<int>(pos[i].w) == ta) s += part[i * stride + b * q + k];
-    const double r = block_reduce(s, sm, wave_sum);
+    double s[kSumWays] = {};
+    for (long i0 = threadIdx.x; i0 < n; i0 += static_cast<long>(kSumThreads) * kSumWays) {
+#pragma unroll
+        for (int m = 0; m < kSumWays; ++m) {
+            const long i = i0 + static_cast<long>(m) * kSumThreads;
+            if (i < n && static_cast<int>(pos[i].w) == ta) s[m] += part[i * stride + b * q + k];
+        }
+    }
+    const double r = sum_fixed((s[0] + s[1]) + (s[2] + s[3]), sm);
     if (threadIdx.x == 0) tables[id] = accumulate ? tables[id] + r : r;
 }
 
@@ -468,13 +505,13 @@ void launch_mix_adjoint(const MixArgs& a, hipStream_t s) {
 }
 
 void launch_mix_sum_ew(long n, const double* part_ew, double* e_out, double* w_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_mix_sum_ew, dim3(1), dim3(kBlock), 0, s, n, part_ew, e_out, w_out);
+    hipLaunchKernelGGL(k_mix_sum_ew, dim3(1), dim3(kSumThreads), 0, s, n, part_ew, e_out, w_out);
 }
 
 void launch_mix_tables(long n, const double4* pos, int nt, int q, const double* part_q,
                        double* tables, int accumulate, hipStream_t s) {
-    hipLaunchKernelGGL(k_mix_tables, dim3(nt * nt * q), dim3(kBlock), 0, s, n, pos, nt, q, part_q,
-                       tables, accumulate);
+    hipLaunchKernelGGL(k_mix_tables, dim3(nt * nt * q), dim3(kSumThreads), 0, s, n, pos, nt, q,
+                       part_q, tables, accumulate);
 }
 
 void launch_mix_sd(long n, const Box& box, double4* pos, const double4* frc, double step,
