@@ -234,6 +234,7 @@ md_status alloc_neigh(md_ctx* c) {
         const double rl = c->rc + c->skin;
         c->quant.band_l = 4.0 * umax / rl + 4.0 * (umax / rl) * (umax / rl) + 1e-12;
         c->quant.band_c = 4.0 * umax / c->rc + 4.0 * (umax / c->rc) * (umax / c->rc) + 1e-12;
+        c->quant.cubic = (c->quant.unit[0] == c->quant.unit[1] && c->quant.unit[0] == c->quant.unit[2]) ? 1 : 0;
     } else {
         release(c->qpos);
         release(c->qpos2);

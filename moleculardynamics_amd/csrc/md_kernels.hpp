@@ -33,6 +33,7 @@ struct Quant {
     double unit[3];  // L / 2^32
     double band_l;   // relative half-width of the decision band around rl^2
     double band_c;   // the same around rc^2
+    int cubic;       // one unit on every axis: r2 may be taken in units squared
 };
 
 __device__ __forceinline__ uint32_t quant_axis(double x, double inv) {

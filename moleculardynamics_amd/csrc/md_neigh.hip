@@ -236,8 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a, long ncell
 // load carries a candidate's position and upload index, and the fixed-point
 // r2 decides every candidate outside a band of relative half-width band_l
 // around rl2; inside the band the exact FP64 test above decides, from the
-// FP64 positions. The pair set is therefore the same.
-template <bool Q>
+// FP64 positions. The pair set is therefore the same. CUBIC: the box has one
+// unit on every axis, so r2 is compared in units squared (the integer
+// differences converted exactly, no scaling per axis).
+template <bool Q, bool CUBIC>
 __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     if (*a.flag == 0) return;
     const long n = a.n;
@@ -297,11 +299,20 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
         row_delta(box, pi, q, flip, dx, dy, dz);
         return norm2_ref(dx, dy, dz) < rl2;
     };
-    const double lo = rl2 * (1.0 - a.quant.band_l), hi = rl2 * (1.0 + a.quant.band_l);
+    const double u2 = CUBIC ? a.quant.unit[0] * a.quant.unit[0] : 1.0;
+    const double lo = rl2 * (1.0 - a.quant.band_l) / u2, hi = rl2 * (1.0 + a.quant.band_l) / u2;
     auto visit_q = [&](int j, const int4& qj) {
-        double dx, dy, dz;
-        qdelta(a.quant, qi, qj, dx, dy, dz);
-        const double r2 = dx * dx + dy * dy + dz * dz;
+        double r2;
+        if (CUBIC) {
+            const double ux = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+            const double uy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+            const double uz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+            r2 = ux * ux + uy * uy + uz * uz;
+        } else {
+            double dx, dy, dz;
+            qdelta(a.quant, qi, qj, dx, dy, dz);
+            r2 = dx * dx + dy * dy + dz * dz;
+        }
         const bool flip = oi > qj.w;
         bool keep = r2 < lo;
         if (!keep && r2 < hi) keep = exact(pos2[j], flip);  // inside the band: the exact test
@@ -390,10 +401,12 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
     hipLaunchKernelGGL(k_sort_permute, gc, b, 0, s, a, ncell);
-    if (a.qpos2 != nullptr)
-        hipLaunchKernelGGL(k_commit_build<true>, dim3(xcd_grid(n)), b, 0, s, a);
+    if (a.qpos2 != nullptr && a.quant.cubic)
+        hipLaunchKernelGGL((k_commit_build<true, true>), dim3(xcd_grid(n)), b, 0, s, a);
+    else if (a.qpos2 != nullptr)
+        hipLaunchKernelGGL((k_commit_build<true, false>), dim3(xcd_grid(n)), b, 0, s, a);
     else
-        hipLaunchKernelGGL(k_commit_build<false>, dim3(xcd_grid(n)), b, 0, s, a);
+        hipLaunchKernelGGL((k_commit_build<false, false>), dim3(xcd_grid(n)), b, 0, s, a);
 }
 
 long scan_tiles(long ncell) { return (ncell + kScanTile - 1) / kScanTile; }
