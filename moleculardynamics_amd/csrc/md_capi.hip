@@ -594,7 +594,7 @@ md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, dou
             MD_HIP(c, alloc(c->ci_atoms, c->n));
             MD_HIP(c, alloc(c->ci_tmp, c->n));
             mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start,
-                                  c->ci_fill, c->ci_tmp, c->ci_atoms, c->stream);
+                                  c->ci_fill, c->ci_tmp, c->ci_atoms, nullptr, c->stream);
             a.grid = g;
             a.cell_of = c->ci_cell_of;
             a.cell_start = c->ci_start;
@@ -897,7 +897,7 @@ md_status md_lj_forces_cells(md_ctx* c, double cell_cutoff, double* energy, doub
     MD_HIP(c, alloc(c->ci_atoms, c->n));
     MD_HIP(c, alloc(c->ci_tmp, c->n));
     mdk::launch_bin_index(c->n, c->pos, g, c->ci_cell_of, c->ci_count, c->ci_start, c->ci_fill,
-                          c->ci_tmp, c->ci_atoms, c->stream);
+                          c->ci_tmp, c->ci_atoms, nullptr, c->stream);
     mdk::DirectArgs a{};
     a.n = c->n;
     a.box = c->box;

@@ -615,8 +615,12 @@ __global__ __launch_bounds__(kBlock) void k_direct_finish(DirectArgs a, int nb) 
     finish_direct(a, nb);
 }
 
+// The cell-index kernels return at once when flag is given and 0 (a device
+// rebuild decision); a null flag always builds.
 __global__ void k_bin_index(const double4* __restrict__ pos, long n, Grid g,
-                            int* __restrict__ cell_of, int* __restrict__ cell_count) {
+                            int* __restrict__ cell_of, int* __restrict__ cell_count,
+                            const int* flag) {
+    if (flag != nullptr && *flag == 0) return;
     const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const double4 p = pos[t];
@@ -632,14 +636,16 @@ __global__ void k_bin_index(const double4* __restrict__ pos, long n, Grid g,
     atomicAdd(&cell_count[cell], 1);
 }
 
-__global__ void k_zero_int(int* a, long n) {
+__global__ void k_zero_int(int* a, long n, const int* flag) {
+    if (flag != nullptr && *flag == 0) return;
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0;
 }
 
 __global__ __launch_bounds__(1024) void k_scan_plain(const int* __restrict__ cnt, long m,
                                                      int* __restrict__ start,
-                                                     int* __restrict__ fill) {
+                                                     int* __restrict__ fill, const int* flag) {
+    if (flag != nullptr && *flag == 0) return;
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const long chunk = (m + 1023) / 1024;
@@ -665,7 +671,8 @@ __global__ __launch_bounds__(1024) void k_scan_plain(const int* __restrict__ cnt
 }
 
 __global__ void k_place_index(const int* __restrict__ cell_of, long n, int* __restrict__ fill,
-                              int* __restrict__ cell_atoms) {
+                              int* __restrict__ cell_atoms, const int* flag) {
+    if (flag != nullptr && *flag == 0) return;
     const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     cell_atoms[atomicAdd(&fill[cell_of[t]], 1)] = static_cast<int>(t);
@@ -678,7 +685,9 @@ __global__ void k_place_index(const int* __restrict__ cell_of, long n, int* __re
 // time and broadcast by shuffles.
 __global__ __launch_bounds__(kBlock) void k_rank_index(const int* __restrict__ start, long m,
                                                        const int* __restrict__ tmp,
-                                                       int* __restrict__ cell_atoms) {
+                                                       int* __restrict__ cell_atoms,
+                                                       const int* flag) {
+    if (flag != nullptr && *flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
     for (long c = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; c < m;
@@ -741,17 +750,18 @@ void launch_lj_direct(const DirectArgs& a, hipStream_t s) {
 
 void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,
                       int* cell_start, int* cell_fill, int* cell_tmp, int* cell_atoms,
-                      hipStream_t s) {
+                      const int* flag, hipStream_t s) {
     const long m = static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2];
-    hipLaunchKernelGGL(k_zero_int, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_count, m);
+    hipLaunchKernelGGL(k_zero_int, dim3(grid_of(m)), dim3(kBlock), 0, s, cell_count, m, flag);
     hipLaunchKernelGGL(k_bin_index, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, g, cell_of,
-                       cell_count);
-    hipLaunchKernelGGL(k_scan_plain, dim3(1), dim3(1024), 0, s, cell_count, m, cell_start, cell_fill);
+                       cell_count, flag);
+    hipLaunchKernelGGL(k_scan_plain, dim3(1), dim3(1024), 0, s, cell_count, m, cell_start, cell_fill,
+                       flag);
     hipLaunchKernelGGL(k_place_index, dim3(grid_of(n)), dim3(kBlock), 0, s, cell_of, n, cell_fill,
-                       cell_tmp);
+                       cell_tmp, flag);
     const long rank_blocks = (m + kBlock / kWave - 1) / (kBlock / kWave);
     hipLaunchKernelGGL(k_rank_index, dim3(static_cast<unsigned>(rank_blocks < 8192 ? rank_blocks : 8192)),
-                       dim3(kBlock), 0, s, cell_start, m, cell_tmp, cell_atoms);
+                       dim3(kBlock), 0, s, cell_start, m, cell_tmp, cell_atoms, flag);
 }
 
 }  // namespace mdk

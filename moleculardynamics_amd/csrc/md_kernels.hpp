@@ -147,9 +147,10 @@ struct DirectArgs {
 void launch_lj_direct(const DirectArgs& a, hipStream_t s);
 // Bins atoms into cells without permuting them: cell_atoms holds slot
 // indices grouped by cell (ordered by slot within a cell).
+// flag (or null): a device rebuild decision; 0 skips the build.
 void launch_bin_index(long n, const double4* pos, const Grid& g, int* cell_of, int* cell_count,
                       int* cell_start, int* cell_fill, int* cell_tmp, int* cell_atoms,
-                      hipStream_t s);
+                      const int* flag, hipStream_t s);
 
 // Integrator per-atom passes before the force call, with the displacement
 // check folded in.

@@ -195,7 +195,7 @@ md_status build_list(md_mix* m, const double4* p, double4* ref) {
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (m->grid.usable)
             mdk::launch_bin_index(m->n, p, m->grid, m->cell_of, m->cell_count, m->cell_start,
-                                  m->cell_fill, m->cell_tmp, m->cell_atoms, m->stream);
+                                  m->cell_fill, m->cell_tmp, m->cell_atoms, nullptr, m->stream);
         mdk::MixListArgs a{};
         a.n = m->n;
         a.grid = m->grid;

@@ -64,6 +64,7 @@ __device__ __forceinline__ float erfc_as(float x, float ex2) {
 __global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int* __restrict__ cell_of,
                                                   const int* __restrict__ cell_start,
                                                   const int* __restrict__ cell_atoms) {
+    if (a.build_flag != nullptr && *a.build_flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long below = (1ull << lane) - 1;
     const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
@@ -409,6 +410,7 @@ __global__ void k_shake(WaterArgs a, const double4* __restrict__ refp, int* __re
         a.pos[b + 1] = r[1];
         a.pos[b + 2] = r[2];
         status[m] = res;
+        if (res < 0 && a.fail != nullptr) atomicMin(&a.fail[0], res);
     }
 }
 
@@ -449,6 +451,7 @@ __global__ void k_rattle(WaterArgs a, int* __restrict__ status) {
         a.vel[b + 1] = v[1];
         a.vel[b + 2] = v[2];
         status[m] = res;
+        if (res < 0 && a.fail != nullptr) atomicMin(&a.fail[1], res);
     }
 }
 
@@ -540,6 +543,28 @@ __global__ __launch_bounds__(kBlock) void k_wkinetic(WaterArgs a) {
     if (threadIdx.x == 0) a.part2[blockIdx.x] = b;
 }
 
+// In-order sums of the energy partials (one workgroup of kBlock threads):
+// thread t takes partials t, t + kBlock, ...; then the threads in index order.
+__global__ __launch_bounds__(kBlock) void k_wenergy_sum(WaterArgs a, int nbr, int nbk, double* out) {
+    __shared__ double sm[kBlock / kWave];
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nbr; b += kBlock)
+        for (int q = 0; q < 4; ++q) v[q] += a.part[4 * b + q];
+    for (int b = threadIdx.x; b < nbk; b += kBlock) v[4] += a.part2[b];
+    for (int q = 0; q < 5; ++q) {
+        const double r = block_reduce(v[q], sm, wave_sum);
+        if (threadIdx.x == 0) out[q] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_wkinetic_sum(WaterArgs a, int nb, double* out) {
+    __shared__ double sm[kBlock / kWave];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) v += a.part2[b];
+    const double r = block_reduce(v, sm, wave_sum);
+    if (threadIdx.x == 0) *out = r;
+}
+
 inline unsigned gridn(long n) {
     const long g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g > 0 ? g : 1);
@@ -606,6 +631,15 @@ void water_launch_cvv_kick(const WaterArgs& a, double dt, hipStream_t s) {
 
 void water_launch_check(const WaterArgs& a, int* flag, hipStream_t s) {
     hipLaunchKernelGGL(k_wcheck, dim3(1), dim3(1024), 0, s, a, flag);
+}
+
+void water_launch_energy_sum(const WaterArgs& a, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_wenergy_sum, dim3(1), dim3(kBlock), 0, s, a,
+                       static_cast<int>(water_real_blocks(a.n)), static_cast<int>(gridn(a.nk)), out);
+}
+
+void water_launch_kinetic_sum(const WaterArgs& a, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_wkinetic_sum, dim3(1), dim3(kBlock), 0, s, a, static_cast<int>(gridn(a.n)), out);
 }
 
 void water_launch_kinetic(const WaterArgs& a, hipStream_t s) {

@@ -41,6 +41,11 @@ struct WaterArgs {
     int ncol, nchunk;            // columns; column chunks of the reciprocal force kernel
     double2* skf;                // [nk] 4 factor(k) S(k)
     double4* fpart;              // [nchunk][n] reciprocal force partials
+    // device-driven runs: SHAKE / RATTLE failure codes (atomic minimum of the
+    // per-molecule status, 0 = none) and the rebuild flag that gates the list
+    // kernels (null: always build)
+    int* fail;
+    const int* build_flag;
     int* overflow;
     double oh, hh, tol;
     int max_iter;
@@ -61,6 +66,11 @@ void water_launch_cvv_correct(const WaterArgs& a, double dt, const double4* free
 void water_launch_cvv_kick(const WaterArgs& a, double dt, hipStream_t s);
 void water_launch_check(const WaterArgs& a, int* flag, hipStream_t s);
 void water_launch_kinetic(const WaterArgs& a, hipStream_t s);
+// One workgroup: the real-space partials (4 per real-space workgroup) and the
+// k-space partials summed in order into out = {real, lj, intra, virial, recip}.
+void water_launch_energy_sum(const WaterArgs& a, double* out, hipStream_t s);
+// One workgroup: the kinetic partials of water_launch_kinetic summed into *out.
+void water_launch_kinetic_sum(const WaterArgs& a, double* out, hipStream_t s);
 int water_site_groups(long n);
 
 }  // namespace mdk

@@ -52,6 +52,8 @@ struct md_water {
     double4* fpart = nullptr;
     long builds = 0;
     bool list_built = false;
+    double* eterm = nullptr;  // per-step device sums of md_water_run: 5 energy sums + KE
+    long eterm_steps = 0;
 };
 
 namespace {
@@ -82,7 +84,7 @@ md_status build_list(md_water* w) {
     const mdk::Grid& g = w->grid;
     if (g.usable) {
         mdk::launch_bin_index(w->n, w->pos, g, w->cell_of, w->cell_count, w->cell_start,
-                              w->cell_fill, w->cell_tmp, w->cell_atoms, w->stream);
+                              w->cell_fill, w->cell_tmp, w->cell_atoms, nullptr, w->stream);
     }
     mdk::water_launch_list(w->a, g, w->cell_of, w->cell_start, w->cell_atoms, w->stream);
     WHIP(w, hipGetLastError());
@@ -96,27 +98,28 @@ md_status build_list(md_water* w) {
 }
 
 // Forces on the current positions; terms = {disp, lrc, real, recip, self, intra}.
+void terms_of(const md_water* w, const double* sums, double* terms);
+
 md_status compute(md_water* w, bool mixed, double* terms) {
     if (!w->list_built) {
         md_status st = build_list(w);
         if (st != MD_OK) return st;
     }
     mdk::water_launch_forces(w->a, mixed, w->stream);
+    mdk::water_launch_energy_sum(w->a, w->eterm, w->stream);
     WHIP(w, hipGetLastError());
-    const long nb = mdk::water_real_blocks(w->n);
-    const long nbk = (w->a.nk + mdk::kBlock - 1) / mdk::kBlock;
-    std::vector<double> part(static_cast<size_t>(4 * nb)), part2(static_cast<size_t>(nbk));
+    double sums[5];
+    WHIP(w, hipMemcpyAsync(sums, w->eterm, sizeof(sums), hipMemcpyDeviceToHost, w->stream));
     WHIP(w, hipStreamSynchronize(w->stream));
-    WHIP(w, hipMemcpy(part.data(), w->part, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-    WHIP(w, hipMemcpy(part2.data(), w->part2, part2.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double real = 0, lj = 0, intra = 0, vir = 0, recip = 0;
-    for (long b = 0; b < nb; ++b) {
-        real += part[4 * b];
-        lj += part[4 * b + 1];
-        intra += part[4 * b + 2];
-        vir += part[4 * b + 3];
-    }
-    for (long b = 0; b < nbk; ++b) recip += part2[b];
+    terms_of(w, sums, terms);
+    return MD_OK;
+}
+
+// The six terms {disp, lrc, real, recip, self, intra} from the device sums
+// {real, lj, intra, virial, recip} of one force evaluation.
+void terms_of(const md_water* w, const double* sums, double* terms) {
+    double real = sums[0], lj = sums[1], intra = sums[2];
+    const double recip = sums[4];
     // every intermolecular and intramolecular pair was seen from both sites
     real *= 0.5;
     lj *= 0.5;
@@ -137,7 +140,6 @@ md_status compute(md_water* w, bool mixed, double* terms) {
     if (terms) {
         terms[0] = lj; terms[1] = lrc; terms[2] = real; terms[3] = recip; terms[4] = self; terms[5] = intra;
     }
-    return MD_OK;
 }
 
 md_status status_check(md_water* w, const char* what, int* iters) {
@@ -367,6 +369,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     WHIP(w, walloc(w->kcol0, std::max(1, ncol)));
     WHIP(w, walloc(w->fpart, static_cast<long>(nchunk) * n));
     WHIP(w, hipMemset(w->flags, 0, 4 * sizeof(int)));
+    WHIP(w, walloc(w->eterm, 6));
+    w->eterm_steps = 1;
     if (nk > 0) {
         WHIP(w, hipMemcpy(w->kvec, kv.data(), nk * sizeof(int3), hipMemcpyHostToDevice));
         WHIP(w, hipMemcpy(w->kfac, kf.data(), nk * sizeof(double), hipMemcpyHostToDevice));
@@ -424,7 +428,7 @@ md_status md_water_destroy(md_water* w) {
                     static_cast<void*>(w->part2), static_cast<void*>(w->kvec), static_cast<void*>(w->kfac),
                     static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk),
                     static_cast<void*>(w->skf), static_cast<void*>(w->kcol), static_cast<void*>(w->kcol0),
-                    static_cast<void*>(w->fpart)})
+                    static_cast<void*>(w->fpart), static_cast<void*>(w->eterm)})
         if (q) (void)hipFree(q);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
@@ -479,38 +483,69 @@ md_status md_water_rattle(md_water* w, int* iterations) {
 // rebuilt when the two largest displacements exceed the skin.
 md_status md_water_run(md_water* w, long nsteps, double dt, int precision, double* pe, double* ke) {
     if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (nsteps <= 0) return MD_OK;
+    if (!w->list_built) {
+        md_status st = build_list(w);
+        if (st != MD_OK) return st;
+    }
+    if (nsteps > w->eterm_steps) {
+        WHIP(w, walloc(w->eterm, 6 * nsteps));
+        w->eterm_steps = nsteps;
+    }
+    // The whole run is enqueued without a host round trip: the rebuild is
+    // decided and gated on the device (flags[1]), SHAKE/RATTLE failures are
+    // collected by atomic minimum (flags[2], flags[3]) and the step's
+    // energies and kinetic energy are summed on the device; all of it is
+    // read once, after the last step. A failure is reported for the run.
+    const bool mixed = precision == MD_MIXED;
+    mdk::WaterArgs a = w->a;
+    a.fail = w->flags + 2;
+    a.build_flag = w->flags + 1;
+    WHIP(w, hipMemsetAsync(w->flags + 2, 0, 2 * sizeof(int), w->stream));
+    const mdk::Grid& g = w->grid;
     for (long k = 0; k < nsteps; ++k) {
-        mdk::water_launch_cvv_first(w->a, dt, w->start, w->freep, w->stream);
-        mdk::water_launch_shake(w->a, w->start, w->status, w->stream);
-        md_status st = status_check(w, "RigidWater: SHAKE", nullptr);
-        if (st != MD_OK) return st;
-        mdk::water_launch_cvv_correct(w->a, dt, w->freep, w->stream);
-        mdk::water_launch_check(w->a, w->flags + 1, w->stream);
-        int flag = 0;
-        WHIP(w, hipMemcpyAsync(&flag, w->flags + 1, sizeof(int), hipMemcpyDeviceToHost, w->stream));
-        WHIP(w, hipStreamSynchronize(w->stream));
-        if (flag) {
-            st = build_list(w);
-            if (st != MD_OK) return st;
-        }
-        double t[6];
-        st = compute(w, precision == MD_MIXED, t);
-        if (st != MD_OK) return st;
-        mdk::water_launch_cvv_kick(w->a, dt, w->stream);
-        mdk::water_launch_rattle(w->a, w->status, w->stream);
-        st = status_check(w, "RigidWater: RATTLE", nullptr);
-        if (st != MD_OK) return st;
-        if (pe) pe[k] = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
+        mdk::water_launch_cvv_first(a, dt, w->start, w->freep, w->stream);
+        mdk::water_launch_shake(a, w->start, w->status, w->stream);
+        mdk::water_launch_cvv_correct(a, dt, w->freep, w->stream);
+        mdk::water_launch_check(a, w->flags + 1, w->stream);
+        if (g.usable)
+            mdk::launch_bin_index(w->n, w->pos, g, w->cell_of, w->cell_count, w->cell_start,
+                                  w->cell_fill, w->cell_tmp, w->cell_atoms, w->flags + 1, w->stream);
+        mdk::water_launch_list(a, g, w->cell_of, w->cell_start, w->cell_atoms, w->stream);
+        mdk::water_launch_forces(a, mixed, w->stream);
+        mdk::water_launch_energy_sum(a, w->eterm + 6 * k, w->stream);
+        mdk::water_launch_cvv_kick(a, dt, w->stream);
+        mdk::water_launch_rattle(a, w->status, w->stream);
         if (ke) {
-            mdk::water_launch_kinetic(w->a, w->stream);
-            const long nb = wblocks(w->n);
-            std::vector<double> part(static_cast<size_t>(nb));
-            WHIP(w, hipStreamSynchronize(w->stream));
-            WHIP(w, hipMemcpy(part.data(), w->part2, nb * sizeof(double), hipMemcpyDeviceToHost));
-            double s = 0.0;
-            for (double v : part) s += v;
-            ke[k] = s;
+            mdk::water_launch_kinetic(a, w->stream);
+            mdk::water_launch_kinetic_sum(a, w->eterm + 6 * k + 5, w->stream);
         }
+    }
+    WHIP(w, hipGetLastError());
+    int f[4] = {0, 0, 0, 0};
+    WHIP(w, hipMemcpyAsync(f, w->flags, sizeof(f), hipMemcpyDeviceToHost, w->stream));
+    std::vector<double> e;
+    if (pe || ke) {
+        e.resize(static_cast<size_t>(6 * nsteps));
+        WHIP(w, hipMemcpyAsync(e.data(), w->eterm, e.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               w->stream));
+    }
+    WHIP(w, hipStreamSynchronize(w->stream));
+    for (int c = 0; c < 2; ++c) {
+        const char* what = c == 0 ? "RigidWater: SHAKE" : "RigidWater: RATTLE";
+        if (f[2 + c] == -2) return wfail(w, MD_ERR_RUNTIME, std::string(what) + ": constraint became singular");
+        if (f[2 + c] < 0)
+            return wfail(w, MD_ERR_RUNTIME, std::string(what) + " failed to converge in " +
+                                                std::to_string(w->p.max_iterations) + " iterations");
+    }
+    if (f[0]) return wfail(w, MD_ERR_CAPACITY, "water neighbour row capacity exceeded");
+    for (long k = 0; k < nsteps; ++k) {
+        if (pe) {
+            double t[6];
+            terms_of(w, &e[6 * k], t);
+            pe[k] = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
+        }
+        if (ke) ke[k] = e[6 * k + 5];
     }
     return MD_OK;
 }
