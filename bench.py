@@ -202,7 +202,7 @@ def bench_batch(args) -> int:
     atoms = int(sum(s[1].shape[0] for s in systems))
     b = native.Batch(systems, T, seeds, precision=native.MIXED)
     b.prime()
-    tpa, block, _ = b.autotune(trial_steps=5)
+    tpa, pieces, block, _ = b.autotune(trial_steps=10)
     b.run(max(args.warmup, 100), segment=100)
     steps = 10000
     t0 = time.perf_counter()
@@ -229,6 +229,7 @@ def bench_batch(args) -> int:
     plain_rate = 256 * plain_steps / plain_wall
     extra = {"replicas": 256, "atoms_total": atoms, "block": int(block),
              "tpa_classes": sorted(set(int(t) for t in tpa)),
+             "workgroups_per_step": int(sum(int(p) for p in pieces)),
              "plain_path_replica_steps_per_s": plain_rate,
              "batch_vs_plain": batch_rate / plain_rate,
              "plain_path_sample": f"256 replicas x {plain_steps} steps, one md_ctx each, "

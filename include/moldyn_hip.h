@@ -227,8 +227,9 @@ md_status md_fcc_lattice(int cells, double density, double temperature, uint64_t
  * process (apps/moldyn_run.cpp:570-777). A batch is many independent LJ
  * systems, each with its own box, Verlet list (verlet_list.cpp:8-69) and
  * LangevinBAOAB thermostat (langevin.hpp:36-88), advanced together: one
- * kernel launch per step for all replicas, multi-step segments captured in a
- * HIP graph. Forces use MD_MIXED or MD_FP64 pair arithmetic; LJ Truncated. */
+ * set of three launches per step for all replicas (each replica split into
+ * pieces, one workgroup each), multi-step segments captured in a HIP graph.
+ * Forces use MD_MIXED or MD_FP64 pair arithmetic; LJ Truncated. */
 typedef struct md_batch md_batch;
 
 /* The replica recipe: from Rng(master_seed) (util/random.hpp:37-48), per
@@ -254,13 +255,15 @@ md_status md_batch_prime(md_batch* b);
  * NULL) receives nsteps x nrep rows of (E, W, KE, step). */
 md_status md_batch_run(md_batch* b, long nsteps, long segment, int use_graph, double* thermo_out);
 /* Lanes per atom of each replica's force rows (1, 2, 4, 8, 16) and the
- * workgroup size (256, 512, 1024). */
+ * workgroup size of the rebuild/force kernel (256, 512, 1024). */
 md_status md_batch_set_tpa(md_batch* b, const int* tpa, int block);
-/* Times candidate (workgroup size, per-size-class lanes-per-atom) choices on
- * the batch itself from a saved state, applies the fastest, restores the
- * state. */
-md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* block_out,
-                            double* best_ms_per_step);
+/* Workgroups (pieces) each replica is split into: 1 .. min(32, n / 32). */
+md_status md_batch_set_pieces(md_batch* b, const int* pieces);
+/* Times candidate (workgroup size, per-size-class piece counts and
+ * lanes-per-atom) choices on the batch itself from a saved state, applies
+ * the fastest, restores the state. Any output may be NULL. */
+md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* pieces_out,
+                            int* block_out, double* best_ms_per_step);
 md_status md_batch_download(md_batch* b, double* x, double* y, double* z, double* vx, double* vy,
                             double* vz, double* fx, double* fy, double* fz);
 /* Per-replica list build counts (VerletList::rebuildCount). */

@@ -1,28 +1,31 @@
-// Batched independent replicas: one kernel launch advances every replica by
-// one BAOAB step (integrate/langevin.hpp:36-88) with its own Verlet list
+// Batched independent replicas: three launches advance every replica by one
+// BAOAB step (integrate/langevin.hpp:36-88) with its own Verlet list
 // (neighbor/verlet_list.cpp:8-69) and LJ forces
 // (forcefield/lennard_jones.cpp:50-88).
 //
-// One workgroup owns one replica for the whole step, so every phase that
-// needs the replica's atoms together (the top-2 displacement check, the
-// rebuild, the force rows) synchronises with workgroup barriers only:
-//   1. B, A, O, A per atom; the replica's positions are staged in LDS
-//      (x, y, z FP64 arrays; up to 160 KB per workgroup on gfx950) and the
-//      displacement top-2 is reduced in the workgroup;
-//   2. when the sum of the two largest displacements exceeds the skin, the
-//      replica's rows are rebuilt by the reference's exact pair test: from
-//      the 27 cells around each atom (a counting sort by cell in LDS) when the
-//      replica's cell grid is usable, over all atom pairs in LDS otherwise --
-//      the same pair set as the reference's cell or brute-force build,
-//      including the single-nearest-image semantics of boxes shorter than
-//      2 (rc + skin);
-//   3. forces: `tpa` lanes per atom walk its row (a per-size-class choice
-//      made by the autotuner), gathering partners from LDS; FP32 pair terms
-//      with 64-bit fixed-point accumulation (or FP64); the final B half-kick
-//      and the replica's E, W, KE.
-// The step counter lives on the device and a one-thread kernel after each
-// step advances it, so a segment of identical launches can be captured in a
-// HIP graph and replayed.
+// Each replica is split into pieces, one workgroup each (the piece count per
+// replica is a per-size-class choice of the autotuner); the pieces of all
+// replicas form one grid, longest first:
+//   1. k_batch_pre: B, A, O, A over the piece's atoms and the piece's two
+//      largest displacements since the last build;
+//   2. k_batch_force: every piece stages its whole replica's positions in
+//      LDS (FP64, in slot order: the atoms sorted by cell at the last build).
+//      When the sum of the replica's two largest displacements exceeds the
+//      skin, the piece re-sorts the replica by cell in LDS (a counting sort;
+//      the atoms of a cell ordered by index, so the order is a pure function
+//      of the positions and every piece computes the same one) and rebuilds
+//      the rows of its own slots by the reference's exact pair test: from the
+//      27 cells around each atom when the replica's cell grid is usable, over
+//      all atom pairs otherwise -- the same pair set as the reference's cell
+//      or brute-force build, including the single-nearest-image semantics of
+//      boxes shorter than 2 (rc + skin). Then `tpa` lanes per slot walk its
+//      row, gathering partners from LDS: FP32 pair terms with 64-bit
+//      fixed-point accumulation (or FP64), the final B half-kick, and the
+//      piece's E, W, KE partials;
+//   3. k_batch_sum: one workgroup sums each replica's piece partials in piece
+//      order into the thermo ring and advances the device step counter.
+// A segment of identical launches can therefore be captured in a HIP graph
+// and replayed.
 #include "md_common.hpp"
 #include "md_batch.hpp"
 
@@ -69,33 +72,50 @@ __device__ __forceinline__ int rep_cell_coord(double r, double L, double w, int 
     return k < 0 ? 0 : (k >= n ? n - 1 : k);
 }
 
-// Rebuild of a replica with a usable cell grid, in the workgroup's LDS:
-// counting sort of the atoms by cell (LDS atomics, then the atoms of a cell
-// ordered by index, so the order is a pure function of the positions), and
-// each atom's row from the 27 surrounding cells in (z, y, x) order with the
-// reference's exact pair test. The pair set is the all-pairs set: every pair
-// within rc + skin lies in neighbouring cells.
-__device__ void rebuild_cells(const BatchArgs& a, const BatchRep& R, const double* xs,
-                              const double* ys, const double* zs) {
+// The workgroup's LDS: the replica's positions and slot -> atom map, then
+// the cell tables of the rebuild.
+struct Stage {
+    double *xs, *ys, *zs;
+    int* ol;     // slot -> atom
+    int* cidx;   // atom -> cell
+    int* tmp;    // atoms grouped by cell, in atomic order
+    int* cstart; // ncell + 1 offsets
+    int* cfill;
+};
+
+__device__ __forceinline__ Stage stage_of(const BatchArgs& a) {
     extern __shared__ double lds[];
+    Stage st;
+    st.xs = lds;
+    st.ys = lds + a.nmax;
+    st.zs = lds + 2 * a.nmax;
+    st.ol = reinterpret_cast<int*>(lds + 3 * a.nmax);
+    st.cidx = st.ol + a.nmax;
+    st.tmp = st.cidx + a.nmax;
+    st.cstart = st.tmp + a.nmax;
+    st.cfill = st.cstart + a.maxcell + 1;
+    return st;
+}
+
+// Counting sort of the replica's atoms by cell into slot order: the atoms of
+// a cell in index order. Stages positions and the slot map in LDS; for the
+// piece's own atoms [s0, s1) writes the new slot and the build-time position.
+__device__ void sort_slots(const BatchArgs& a, const BatchRep& R, const Stage& st, int s0, int s1) {
+    const double4* P = a.pos + R.off;
     const int n = R.n;
-    int* cidx = reinterpret_cast<int*>(lds + 3 * a.nmax);
-    int* tmp = cidx + a.nmax;
-    int* srt = tmp + a.nmax;
-    int* cstart = srt + a.nmax;
-    int* cfill = cstart + a.maxcell + 1;
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     const int ncell = nx * ny * nz;
     const int tid = threadIdx.x, bd = blockDim.x;
-    for (int c = tid; c < ncell; c += bd) cfill[c] = 0;
+    for (int c = tid; c < ncell; c += bd) st.cfill[c] = 0;
     __syncthreads();
     for (int i = tid; i < n; i += bd) {
-        const int cx = rep_cell_coord(xs[i], R.box.L[0], R.w[0], nx);
-        const int cy = rep_cell_coord(ys[i], R.box.L[1], R.w[1], ny);
-        const int cz = rep_cell_coord(zs[i], R.box.L[2], R.w[2], nz);
+        const double4 p = P[i];
+        const int cx = rep_cell_coord(p.x, R.box.L[0], R.w[0], nx);
+        const int cy = rep_cell_coord(p.y, R.box.L[1], R.w[1], ny);
+        const int cz = rep_cell_coord(p.z, R.box.L[2], R.w[2], nz);
         const int c = cx + nx * (cy + ny * cz);
-        cidx[i] = c;
-        atomicAdd(&cfill[c], 1);
+        st.cidx[i] = c;
+        atomicAdd(&st.cfill[c], 1);
     }
     __syncthreads();
     if (tid < kWave) {  // exclusive scan of the counts by the first wave
@@ -103,61 +123,148 @@ __device__ void rebuild_cells(const BatchArgs& a, const BatchRep& R, const doubl
         int run = 0;
         for (int c0 = 0; c0 < ncell; c0 += kWave) {
             const int c = c0 + lane;
-            const int v = c < ncell ? cfill[c] : 0;
+            const int v = c < ncell ? st.cfill[c] : 0;
             int x = v;
             for (int off = 1; off < kWave; off <<= 1) {
                 const int y = __shfl_up(x, off, kWave);
                 if (lane >= off) x += y;
             }
-            if (c < ncell) cstart[c] = run + x - v;
+            if (c < ncell) st.cstart[c] = run + x - v;
             run += __shfl(x, kWave - 1, kWave);
         }
-        if (lane == 0) cstart[ncell] = run;
+        if (lane == 0) st.cstart[ncell] = run;
     }
     __syncthreads();
-    for (int c = tid; c < ncell; c += bd) cfill[c] = cstart[c];
+    for (int c = tid; c < ncell; c += bd) st.cfill[c] = st.cstart[c];
     __syncthreads();
-    for (int i = tid; i < n; i += bd) tmp[atomicAdd(&cfill[cidx[i]], 1)] = i;
+    for (int i = tid; i < n; i += bd) st.tmp[atomicAdd(&st.cfill[st.cidx[i]], 1)] = i;
     __syncthreads();
     for (int i = tid; i < n; i += bd) {
-        const int c = cidx[i], b = cstart[c], e = cstart[c + 1];
+        const int c = st.cidx[i], b = st.cstart[c], e = st.cstart[c + 1];
         int r = 0;
-        for (int k = b; k < e; ++k) r += tmp[k] < i ? 1 : 0;
-        srt[b + r] = i;
+        for (int k = b; k < e; ++k) r += st.tmp[k] < i ? 1 : 0;
+        const int s = b + r;
+        const double4 p = P[i];
+        st.ol[s] = i;
+        st.xs[s] = p.x;
+        st.ys[s] = p.y;
+        st.zs[s] = p.z;
+        if (i >= s0 && i < s1) {
+            a.slot_of[R.off + i] = s;
+            a.ref[R.off + i] = p;
+        }
     }
-    __syncthreads();
-    for (int i = tid; i < n; i += bd) {
-        const long g = R.off + i;
-        const double4 pi = make_double4(xs[i], ys[i], zs[i], 0.0);
-        const int c = cidx[i];
-        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-        int cnt = 0;
-        uint32_t* row = a.nbr + g * a.cap;
-        for (int oz = -1; oz <= 1; ++oz)
-            for (int oy = -1; oy <= 1; ++oy)
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const int nb = (cx + ox + nx) % nx + nx * ((cy + oy + ny) % ny + ny * ((cz + oz + nz) % nz));
-                    for (int k = cstart[nb]; k < cstart[nb + 1]; ++k) {
-                        const int j = srt[k];
-                        if (j == i) continue;
-                        const bool flip = i > j;
-                        double dx, dy, dz;
-                        row_delta(R.box, pi, make_double4(xs[j], ys[j], zs[j], 0.0), flip, dx, dy, dz);
-                        if (norm2_ref(dx, dy, dz) < a.rl2) {
-                            if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                            ++cnt;
-                        }
-                    }
-                }
-        a.ncnt[g] = cnt;
-        if (cnt > a.cap) atomicOr(a.overflow, 1);
-        a.ref[g] = pi;
-    }
-    if (tid == 0) a.builds[blockIdx.x] += 1;
     __syncthreads();
 }
 
-// Workgroup sum (any power-of-two workgroup of wave64s); result in all threads.
+// Appends to a row the slots k in [ks, ke) within rc + skin of slot s
+// (neighbor/verlet_list.cpp:26-50's test: the lower atom index first).
+__device__ __forceinline__ int scan_range(const BatchArgs& a, const BatchRep& R, const Stage& st, int s,
+                                          int oi, const double4& pi, int ks, int ke, uint32_t* row,
+                                          int cnt) {
+    for (int k = ks; k < ke; ++k) {
+        if (k == s) continue;
+        const bool flip = oi > st.ol[k];
+        double dx, dy, dz;
+        row_delta(R.box, pi, make_double4(st.xs[k], st.ys[k], st.zs[k], 0.0), flip, dx, dy, dz);
+        if (norm2_ref(dx, dy, dz) < a.rl2) {
+            if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(k) | (flip ? kFlipBit : 0u);
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+// Rows of the piece's slots [s0, s1): cell-sorted replicas scan the 3 x 3
+// runs of x-adjacent cells around the slot's cell (contiguous in slot order,
+// plus the periodic image cell at either end of the x axis); other replicas
+// scan all slots.
+__device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& st, bool cells, int s0,
+                           int s1) {
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    for (int s = s0 + static_cast<int>(threadIdx.x); s < s1; s += blockDim.x) {
+        const long g = R.off + s;
+        const int oi = st.ol[s];
+        const double4 pi = make_double4(st.xs[s], st.ys[s], st.zs[s], 0.0);
+        uint32_t* row = a.nbr + g * a.cap;
+        int cnt = 0;
+        if (cells) {
+            const int c = st.cidx[oi];
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            const int xa = cx > 0 ? cx - 1 : 0, xb = cx < nx - 1 ? cx + 1 : nx - 1;
+            const int xw = cx == 0 ? nx - 1 : (cx == nx - 1 ? 0 : -1);
+            for (int oz = -1; oz <= 1; ++oz)
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int b = nx * ((cy + oy + ny) % ny + ny * ((cz + oz + nz) % nz));
+                    cnt = scan_range(a, R, st, s, oi, pi, st.cstart[b + xa], st.cstart[b + xb + 1], row, cnt);
+                    if (xw >= 0) cnt = scan_range(a, R, st, s, oi, pi, st.cstart[b + xw], st.cstart[b + xw + 1], row, cnt);
+                }
+        } else {
+            cnt = scan_range(a, R, st, s, oi, pi, 0, R.n, row, cnt);
+        }
+        a.ncnt[g] = cnt;
+        a.orig[g] = oi;
+        if (cnt > a.cap) atomicOr(a.overflow, 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_pre(BatchArgs a) {
+    __shared__ double red_d[4], red_d2[4];
+    const int4 pc = a.pieces[blockIdx.x];
+    const BatchRep R = a.reps[pc.x];
+    const uint64_t step = *a.step;
+    const double h = 0.5 * a.dt;
+    const Box box = R.box;
+    const double c1 = R.c1, c2 = R.c2, T = R.T;
+    const uint32_t k0 = R.seed_lo, k1 = R.seed_hi;
+    const long off = R.off;
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = pc.y + static_cast<int>(threadIdx.x); i < pc.z; i += blockDim.x) {
+        const long g = off + i;
+        double4 p = a.pos[g];
+        double4 v = a.vel[g];
+        const double4 f = a.frc[g];
+        const double c = h * (1.0 / v.w);
+        v.x = fma(c, f.x, v.x);
+        v.y = fma(c, f.y, v.y);
+        v.z = fma(c, f.z, v.z);
+        p = drift(box, p, v, h);
+        const U4 q0 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
+                                  static_cast<uint32_t>(step >> 32), 0u}, k0, k1);
+        const U4 q1 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
+                                  static_cast<uint32_t>(step >> 32), 1u}, k0, k1);
+        const double r1 = sqrt(-2.0 * log(u53(q0.x, q0.y)));
+        const double r2 = sqrt(-2.0 * log(u53(q1.x, q1.y)));
+        double s1, cs1, s2, cs2;
+        sincospi(2.0 * u53(q0.z, q0.w), &s1, &cs1);
+        sincospi(2.0 * u53(q1.z, q1.w), &s2, &cs2);
+        const double k = c2 * sqrt(T / v.w);
+        v.x = c1 * v.x + (r1 * cs1) * k;
+        v.y = c1 * v.y + (r1 * s1) * k;
+        v.z = c1 * v.z + (r2 * cs2) * k;
+        p = drift(box, p, v, h);
+        a.pos[g] = p;
+        a.vel[g] = v;
+        const double4 r = a.ref[g];
+        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        min_image(box, dx, dy, dz);
+        top2_merge(t1, t2, sqrt(norm2_ref(dx, dy, dz)), 0.0);
+    }
+    wave_top2(t1, t2);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        red_d[wid] = t1;
+        red_d2[wid] = t2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b1 = 0.0, b2 = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x / kWave); ++w) top2_merge(b1, b2, red_d[w], red_d2[w]);
+        a.ptop[pc.w] = make_double2(b1, b2);
+    }
+}
+
+// Workgroup sum (any power-of-two workgroup of wave64s); result in thread 0.
 template <class T, class WaveFn>
 __device__ __forceinline__ T wg_sum(T v, T* smem, WaveFn wf) {
     v = wf(v);
@@ -172,126 +279,86 @@ __device__ __forceinline__ T wg_sum(T v, T* smem, WaveFn wf) {
 }
 
 template <bool MIXED>
-__global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
-    extern __shared__ double lds[];
-    __shared__ double red_d[16], red_d2[16];
+__global__ __launch_bounds__(1024) void k_batch_force(BatchArgs a) {
+    __shared__ double red_d[16];
     __shared__ long long red_l[16];
     __shared__ int s_rebuild;
-    const BatchRep R = a.reps[blockIdx.x];
-    const int n = R.n;
-    double* xs = lds;
-    double* ys = lds + a.nmax;
-    double* zs = lds + 2 * a.nmax;
-    const uint64_t step = *a.step;
+    const int4 pc = a.pieces[blockIdx.x];
+    const BatchRep R = a.reps[pc.x];
+    const int n = R.n, s0 = pc.y, s1 = pc.z;
+    const long off = R.off;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const Stage st = stage_of(a);
     const double h = 0.5 * a.dt;
-
-    // ---- 1. pre-force passes and displacement check -----------------------
-    double t1 = 0.0, t2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const long g = R.off + i;
-        double4 p = a.pos[g];
+    if (tid == 0) {
+        int rb = 1;
         if (a.mode == 1) {
-            double4 v = a.vel[g];
-            const double4 f = a.frc[g];
-            const double c = h * (1.0 / v.w);
-            v.x = fma(c, f.x, v.x);
-            v.y = fma(c, f.y, v.y);
-            v.z = fma(c, f.z, v.z);
-            p = drift(R.box, p, v, h);
-            const U4 q0 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
-                                      static_cast<uint32_t>(step >> 32), 0u}, R.seed_lo, R.seed_hi);
-            const U4 q1 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
-                                      static_cast<uint32_t>(step >> 32), 1u}, R.seed_lo, R.seed_hi);
-            const double r1 = sqrt(-2.0 * log(u53(q0.x, q0.y)));
-            const double r2 = sqrt(-2.0 * log(u53(q1.x, q1.y)));
-            double s1, c1, s2, c2;
-            sincospi(2.0 * u53(q0.z, q0.w), &s1, &c1);
-            sincospi(2.0 * u53(q1.z, q1.w), &s2, &c2);
-            const double k = R.c2 * sqrt(R.T / v.w);
-            v.x = R.c1 * v.x + (r1 * c1) * k;
-            v.y = R.c1 * v.y + (r1 * s1) * k;
-            v.z = R.c1 * v.z + (r2 * c2) * k;
-            p = drift(R.box, p, v, h);
-            a.pos[g] = p;
-            a.vel[g] = v;
-            const double4 r = a.ref[g];
-            double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-            min_image(R.box, dx, dy, dz);
-            const double d = sqrt(norm2_ref(dx, dy, dz));
-            top2_merge(t1, t2, d, 0.0);
-        }
-        xs[i] = p.x;
-        ys[i] = p.y;
-        zs[i] = p.z;
-    }
-    wave_top2(t1, t2);
-    {
-        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-        if (lane == 0) {
-            red_d[wid] = t1;
-            red_d2[wid] = t2;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
             double b1 = 0.0, b2 = 0.0;
-            for (int w = 0; w < static_cast<int>(blockDim.x / kWave); ++w) top2_merge(b1, b2, red_d[w], red_d2[w]);
-            s_rebuild = (a.mode == 0) || (b1 + b2 > a.skin);
-        }
-        __syncthreads();
-    }
-
-    // ---- 2. rebuild the replica's rows ----------------------------------
-    if (s_rebuild && R.usable && a.maxcell > 0) {
-        rebuild_cells(a, R, xs, ys, zs);
-    } else if (s_rebuild) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const long g = R.off + i;
-            const double4 pi = make_double4(xs[i], ys[i], zs[i], 0.0);
-            int cnt = 0;
-            uint32_t* row = a.nbr + g * a.cap;
-            for (int j = 0; j < n; ++j) {
-                if (j == i) continue;
-                const bool flip = i > j;
-                double dx, dy, dz;
-                row_delta(R.box, pi, make_double4(xs[j], ys[j], zs[j], 0.0), flip, dx, dy, dz);
-                if (norm2_ref(dx, dy, dz) < a.rl2) {
-                    if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
-                    ++cnt;
-                }
+            for (int k = 0; k < R.npiece; ++k) {
+                const double2 t = a.ptop[R.piece0 + k];
+                top2_merge(b1, b2, t.x, t.y);
             }
-            a.ncnt[g] = cnt;
-            if (cnt > a.cap) atomicOr(a.overflow, 1);
-            a.ref[g] = pi;
+            rb = b1 + b2 > a.skin ? 1 : 0;
         }
-        if (threadIdx.x == 0) a.builds[blockIdx.x] += 1;
+        s_rebuild = rb;
+    }
+    __syncthreads();
+
+    // ---- 1. stage the replica in slot order; rebuild the piece's rows ------
+    if (!s_rebuild) {
+        for (int i = tid; i < n; i += bd) {
+            const double4 p = a.pos[off + i];
+            const int s = a.slot_of[off + i];
+            st.xs[s] = p.x;
+            st.ys[s] = p.y;
+            st.zs[s] = p.z;
+        }
+        for (int s = s0 + tid; s < s1; s += bd) st.ol[s] = a.orig[off + s];
+        __syncthreads();
+    } else {
+        const bool cells = R.usable && a.maxcell > 0;
+        if (cells) {
+            sort_slots(a, R, st, s0, s1);
+        } else {
+            for (int i = tid; i < n; i += bd) {
+                const double4 p = a.pos[off + i];
+                st.xs[i] = p.x;
+                st.ys[i] = p.y;
+                st.zs[i] = p.z;
+                st.ol[i] = i;
+                if (i >= s0 && i < s1) a.ref[off + i] = p;
+            }
+            __syncthreads();
+        }
+        build_rows(a, R, st, cells, s0, s1);
+        if (tid == 0 && s0 == 0) a.builds[pc.x] += 1;
         __syncthreads();
     }
 
-    // ---- 3. forces, second half-kick, replica sums ------------------------
+    // ---- 2. forces, second half-kick, piece partials ------------------------
     const int tpa = R.tpa;
-    const int groups = blockDim.x / tpa;
-    const int sub = threadIdx.x % tpa;
+    const int groups = bd / tpa;
+    const int sub = tid % tpa;
     double e64 = 0.0, w64 = 0.0, ke = 0.0;
     long long ef = 0, wf = 0;
     float big = 0.0f;
-    for (int base = 0; base < n; base += groups) {
-        const int i = base + static_cast<int>(threadIdx.x) / tpa;
-        const bool valid = i < n;
+    for (int base = s0; base < s1; base += groups) {
+        const int s = base + tid / tpa;
+        const bool valid = s < s1;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         long long ax = 0, ay = 0, az = 0;
         if (valid) {
-            const long g = R.off + i;
+            const long g = off + s;
             const int cnt = min(a.ncnt[g], a.cap);
             const uint32_t* row = a.nbr + g * a.cap;
-            const double xi = xs[i], yi = ys[i], zi = zs[i];
+            const double4 pi = make_double4(st.xs[s], st.ys[s], st.zs[s], 0.0);
             unsigned long long bx = 0, by = 0, bz = 0, be = 0, bw = 0, nt = 0;
             for (int k = sub; k < cnt; k += tpa) {
                 const uint32_t jj = row[k];
                 const int j = static_cast<int>(jj & kIndexMask);
                 const bool flip = (jj & kFlipBit) != 0;
                 double dx, dy, dz;
-                row_delta(R.box, make_double4(xi, yi, zi, 0.0), make_double4(xs[j], ys[j], zs[j], 0.0),
-                          flip, dx, dy, dz);
+                row_delta(R.box, pi, make_double4(st.xs[j], st.ys[j], st.zs[j], 0.0), flip, dx, dy, dz);
                 if (MIXED) {
                     const float xf = static_cast<float>(dx), yf = static_cast<float>(dy),
                                 zf = static_cast<float>(dz);
@@ -332,16 +399,16 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
                 wf += fix_value(bw, nt);
             }
         }
-        // combine the tpa lanes of an atom (lanes of a group are adjacent)
-        for (int off = 1; off < tpa; off <<= 1) {
+        // combine the tpa lanes of a slot (lanes of a group are adjacent)
+        for (int o = 1; o < tpa; o <<= 1) {
             if (MIXED) {
-                ax += __shfl_xor(ax, off, kWave);
-                ay += __shfl_xor(ay, off, kWave);
-                az += __shfl_xor(az, off, kWave);
+                ax += __shfl_xor(ax, o, kWave);
+                ay += __shfl_xor(ay, o, kWave);
+                az += __shfl_xor(az, o, kWave);
             } else {
-                fx += __shfl_xor(fx, off, kWave);
-                fy += __shfl_xor(fy, off, kWave);
-                fz += __shfl_xor(fz, off, kWave);
+                fx += __shfl_xor(fx, o, kWave);
+                fy += __shfl_xor(fy, o, kWave);
+                fz += __shfl_xor(fz, o, kWave);
             }
         }
         if (valid && sub == 0) {
@@ -350,72 +417,106 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs a) {
                 fy = static_cast<double>(ay) * kFixInv;
                 fz = static_cast<double>(az) * kFixInv;
             }
-            const long g = R.off + i;
-            a.frc[g] = make_double4(fx, fy, fz, 0.0);
+            const long ga = off + st.ol[s];
+            a.frc[ga] = make_double4(fx, fy, fz, 0.0);
             if (a.mode == 1) {
-                double4 v = a.vel[g];
+                double4 v = a.vel[ga];
                 const double c = h * (1.0 / v.w);
                 v.x = fma(c, fx, v.x);
                 v.y = fma(c, fy, v.y);
                 v.z = fma(c, fz, v.z);
-                a.vel[g] = v;
+                a.vel[ga] = v;
                 ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
             }
         }
     }
     if (!(big < kFixLimit)) atomicOr(a.fix_sat, 1);
     if (a.thermo != nullptr) {
-        double E, W;
+        long long* out = a.ppart + 3L * pc.w;
         if (MIXED) {
             const long long se = wg_sum(ef, red_l, wave_sum_ll);
             const long long sw = wg_sum(wf, red_l, wave_sum_ll);
-            E = 0.5 * (static_cast<double>(se) * kFixInv);
-            W = 0.5 * (static_cast<double>(sw) * kFixInv);
+            if (tid == 0) {
+                out[0] = se;
+                out[1] = sw;
+            }
         } else {
-            E = 0.5 * wg_sum(e64, red_d, wave_sum);
-            W = 0.5 * wg_sum(w64, red_d, wave_sum);
+            const double se = wg_sum(e64, red_d, wave_sum);
+            const double sw = wg_sum(w64, red_d, wave_sum);
+            if (tid == 0) {
+                out[0] = __double_as_longlong(se);
+                out[1] = __double_as_longlong(sw);
+            }
         }
-        const double K = wg_sum(ke, red_d, wave_sum);
-        if (threadIdx.x == 0) {
-            double* th = a.thermo + (static_cast<long>(step % a.thermo_steps) * a.nrep + blockIdx.x) * 4;
+        const double sk = wg_sum(ke, red_d, wave_sum);
+        if (tid == 0) out[2] = __double_as_longlong(sk);
+    }
+}
+
+// Per-replica sums of the piece partials (in piece order) into the thermo
+// ring, then the step counter advances (one workgroup: every thread has read
+// the step before thread 0 writes it).
+__global__ __launch_bounds__(256) void k_batch_sum(BatchArgs a) {
+    const uint64_t step = *a.step;
+    if (a.thermo != nullptr) {
+        for (int r = threadIdx.x; r < a.nrep; r += blockDim.x) {
+            const int p0 = a.reps[r].piece0, np = a.reps[r].npiece;
+            const long long* pp = a.ppart + 3L * p0;
+            double E, W, K = 0.0;
+            if (a.mixed) {
+                long long se = 0, sw = 0;
+                for (int k = 0; k < np; ++k) {
+                    se += pp[3 * k];
+                    sw += pp[3 * k + 1];
+                }
+                E = 0.5 * (static_cast<double>(se) * kFixInv);
+                W = 0.5 * (static_cast<double>(sw) * kFixInv);
+            } else {
+                double se = 0.0, sw = 0.0;
+                for (int k = 0; k < np; ++k) {
+                    se += __longlong_as_double(pp[3 * k]);
+                    sw += __longlong_as_double(pp[3 * k + 1]);
+                }
+                E = 0.5 * se;
+                W = 0.5 * sw;
+            }
+            for (int k = 0; k < np; ++k) K += __longlong_as_double(pp[3 * k + 2]);
+            double* th = a.thermo + (static_cast<long>(step % a.thermo_steps) * a.nrep + r) * 4;
             th[0] = E;
             th[1] = W;
             th[2] = K;
             th[3] = static_cast<double>(step);
         }
     }
-}
-
-// Advances the device step counter after every replica finished the step
-// (a separate one-thread launch: stream order makes it the barrier).
-__global__ void k_batch_advance(unsigned long long* step) {
-    if (threadIdx.x == 0) *step += 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *a.step = step + 1;
 }
 
 }  // namespace
 
 size_t batch_lds_bytes(int nmax, int maxcell) {
-    size_t b = static_cast<size_t>(3) * nmax * sizeof(double);
-    if (maxcell > 0) b += (static_cast<size_t>(3) * nmax + 2 * static_cast<size_t>(maxcell) + 1) * sizeof(int);
+    size_t b = static_cast<size_t>(nmax) * (3 * sizeof(double) + sizeof(int));
+    if (maxcell > 0) b += (static_cast<size_t>(2) * nmax + 2 * static_cast<size_t>(maxcell) + 1) * sizeof(int);
     return b;
 }
 
 hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s) {
     const size_t shmem = batch_lds_bytes(a.nmax, a.maxcell);
+    if (a.mode == 1) hipLaunchKernelGGL(k_batch_pre, dim3(a.npieces), dim3(256), 0, s, a);
     if (a.mixed)
-        hipLaunchKernelGGL(k_batch<true>, dim3(a.nrep), dim3(block), shmem, s, a);
+        hipLaunchKernelGGL(k_batch_force<true>, dim3(a.npieces), dim3(block), shmem, s, a);
     else
-        hipLaunchKernelGGL(k_batch<false>, dim3(a.nrep), dim3(block), shmem, s, a);
-    if (a.mode == 1) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, a.step);
+        hipLaunchKernelGGL(k_batch_force<false>, dim3(a.npieces), dim3(block), shmem, s, a);
+    if (a.mode == 1) hipLaunchKernelGGL(k_batch_sum, dim3(1), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t batch_allow_lds(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch<true>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_force<true>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(bytes));
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch<false>),
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_force<false>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
 }
 

@@ -1,4 +1,4 @@
-// Launch interface of the batched-replica step kernel (md_batch.hip).
+// Launch interface of the batched-replica step kernels (md_batch.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,7 +11,8 @@
 namespace mdk {
 
 // One replica: its slice [off, off + n) of the flat atom arrays, its box,
-// Langevin constants, noise key and the lanes-per-atom of its force rows.
+// Langevin constants, noise key, the lanes-per-atom of its force rows and
+// the pieces (workgroups) it is split into.
 struct BatchRep {
     long off;
     int n;
@@ -24,23 +25,35 @@ struct BatchRep {
     int nc[3];
     double w[3];
     int usable;
+    int piece0;  // the replica's pieces are piece0 .. piece0 + npiece - 1
+    int npiece;
 };
 
+// A piece is one workgroup's share of a replica: atoms [s0, s1) in the
+// integration kernel and slots [s0, s1) in the rebuild/force kernel.
+// Fields: x = replica, y = s0, z = s1, w = piece index (replica-contiguous
+// numbering of the per-piece partial arrays).
 struct BatchArgs {
     const BatchRep* reps;
-    int nrep;
+    const int4* pieces;  // launch order (longest first)
+    int nrep, npieces;
     int nmax;        // largest replica (LDS staging size)
     int maxcell;     // largest usable cell grid (LDS cell tables), 0: every rebuild all-pairs
     int mode;        // 0: build rows + forces only; 1: one full BAOAB step
     int mixed;
-    double4 *pos, *vel, *frc, *ref;
-    uint32_t* nbr;   // row-major rows of `cap` entries per atom
-    int* ncnt;
+    double4 *pos, *vel, *frc, *ref;  // by atom index
+    // Slot order: each replica's atoms sorted by cell at the last build;
+    // rows, the LDS staging and the force pass run in slot order.
+    int* slot_of;    // atom -> slot
+    int* orig;       // slot -> atom
+    uint32_t* nbr;   // row-major rows of `cap` slot entries per slot
+    int* ncnt;       // by slot
     int cap;
     LJConst lj;
     double rl2, skin, dt;
-    unsigned long long* step;  // device step counter (advanced by the kernel)
-    unsigned int* ticket;
+    double2* ptop;       // per piece: the two largest displacements since the build
+    long long* ppart;    // per piece: E, W (fixed point or FP64 bits), KE (FP64 bits)
+    unsigned long long* step;  // device step counter (advanced by the sum kernel)
     double* thermo;  // [thermo_steps][nrep][4] ring (E, W, KE, step) or null
     long thermo_steps;
     long long* builds;  // per replica
@@ -49,8 +62,8 @@ struct BatchArgs {
 };
 
 hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s);
-// Dynamic LDS of one workgroup: the positions, and with maxcell > 0 the
-// cell tables of the in-LDS rebuild.
+// Dynamic LDS of one rebuild/force workgroup: the positions and slot map,
+// and with maxcell > 0 the cell tables of the in-LDS rebuild.
 size_t batch_lds_bytes(int nmax, int maxcell);
 hipError_t batch_allow_lds(size_t bytes);
 

@@ -1,12 +1,13 @@
 // C ABI of the batched-replica path (include/moldyn_hip.h, md_batch_*):
-// flat device state for many independent systems, one kernel launch per
+// flat device state for many independent systems, one set of launches per
 // step for all of them, HIP-graph-captured multi-step segments, and the
-// lanes-per-atom autotuner.
+// pieces / lanes-per-atom autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <new>
 #include <random>
@@ -33,15 +34,24 @@ struct md_batch {
     std::vector<long> off;
     mdk::BatchRep* d_reps = nullptr;
     double4 *pos = nullptr, *vel = nullptr, *frc = nullptr, *ref = nullptr;
+    int *slot_of = nullptr, *orig = nullptr;
     uint32_t* nbr = nullptr;
     int* ncnt = nullptr;
     int cap = 0;
-    unsigned long long* step = nullptr;  // [0] step counter, followed by the ticket
+    // pieces: per replica the density (work estimate), the largest allowed
+    // piece count and the default one; the launch-order table on the device
+    std::vector<double> rho;
+    std::vector<int> max_pieces, def_pieces;
+    int piece_slots = 0;
+    int npieces = 0;
+    int4* d_pieces = nullptr;
+    double2* ptop = nullptr;
+    long long* ppart = nullptr;
+    unsigned long long* step = nullptr;  // [0] step counter
     long long* builds = nullptr;
     int* flags = nullptr;                // [0] overflow, [1] fixed-point saturation
     double* thermo = nullptr;
     long thermo_alloc = 0;
-    double* stage = nullptr;
     bool primed = false;
 };
 
@@ -70,20 +80,24 @@ hipError_t dalloc(T*& p, long count) {
 mdk::BatchArgs args_of(md_batch* b, int mode, double* thermo, long thermo_steps) {
     mdk::BatchArgs a{};
     a.reps = b->d_reps;
+    a.pieces = b->d_pieces;
     a.nrep = b->nrep;
+    a.npieces = b->npieces;
     a.nmax = b->nmax;
     a.maxcell = b->maxcell;
     a.mode = mode;
     a.mixed = b->mixed;
     a.pos = b->pos; a.vel = b->vel; a.frc = b->frc; a.ref = b->ref;
+    a.slot_of = b->slot_of; a.orig = b->orig;
     a.nbr = b->nbr; a.ncnt = b->ncnt; a.cap = b->cap;
+    a.ptop = b->ptop;
+    a.ppart = b->ppart;
     a.lj = b->ljc;
     const double rl = b->lj.cutoff + b->skin;
     a.rl2 = rl * rl;
     a.skin = b->skin;
     a.dt = b->dt;
     a.step = b->step;
-    a.ticket = reinterpret_cast<unsigned int*>(b->step + 1);
     a.thermo = thermo;
     a.thermo_steps = thermo_steps > 0 ? thermo_steps : 1;
     a.builds = b->builds;
@@ -105,6 +119,36 @@ md_status upload_reps(md_batch* b) {
     BHIP(b, hipMemcpy(b->d_reps, b->reps.data(), b->reps.size() * sizeof(mdk::BatchRep),
                       hipMemcpyHostToDevice));
     return MD_OK;
+}
+
+// Splits replica r into reps[r].npiece even ranges and orders all pieces by
+// estimated work (atoms x (density + 0.05)), largest first, so the longest
+// workgroups start first; the partial arrays keep a replica-contiguous
+// numbering (piece0 .. piece0 + npiece - 1).
+md_status build_pieces(md_batch* b) {
+    struct Item {
+        double w;
+        int4 pc;
+    };
+    std::vector<Item> items;
+    int p = 0;
+    for (int r = 0; r < b->nrep; ++r) {
+        mdk::BatchRep& R = b->reps[r];
+        R.piece0 = p;
+        for (int k = 0; k < R.npiece; ++k) {
+            const int s0 = static_cast<int>(static_cast<long>(k) * R.n / R.npiece);
+            const int s1 = static_cast<int>(static_cast<long>(k + 1) * R.n / R.npiece);
+            items.push_back({(s1 - s0) * (b->rho[r] + 0.05), make_int4(r, s0, s1, p)});
+            ++p;
+        }
+    }
+    if (p > b->piece_slots) return bfail(b, MD_ERR_INVALID_ARGUMENT, "too many pieces");
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.w > y.w; });
+    std::vector<int4> table(items.size());
+    for (size_t k = 0; k < items.size(); ++k) table[k] = items[k].pc;
+    b->npieces = p;
+    BHIP(b, hipMemcpy(b->d_pieces, table.data(), table.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return upload_reps(b);
 }
 
 }  // namespace
@@ -187,7 +231,7 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
         mdk::BatchRep& R = b->reps[r];
         R.off = total;
         R.n = static_cast<int>(n[r]);
-        R.tpa = 1;
+        R.tpa = 4;
         R.seed_lo = static_cast<uint32_t>(seeds[r]);
         R.seed_hi = static_cast<uint32_t>(seeds[r] >> 32);
         const double Lr[3] = {L[3 * r], L[3 * r + 1], L[3 * r + 2]};
@@ -197,7 +241,8 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
         R.c2 = c2;
         total += n[r];
         b->nmax = std::max(b->nmax, R.n);
-        rho_max = std::max(rho_max, n[r] / (Lr[0] * Lr[1] * Lr[2]));
+        b->rho.push_back(n[r] / (Lr[0] * Lr[1] * Lr[2]));
+        rho_max = std::max(rho_max, b->rho.back());
     }
     b->natoms = total;
     const double rl = lj->cutoff + skin;
@@ -224,12 +269,32 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
         return bfail(b, MD_ERR_INVALID_ARGUMENT,
                      "largest replica does not fit the workgroup's LDS (" + std::to_string(maxsh) + " B)");
     BHIP(b, mdk::batch_allow_lds(lds));
+    // pieces: the default count balances atoms x density over two workgroups
+    // per CU; a replica has at most 32 pieces of at least 32 atoms
+    int ncu = 0;
+    BHIP(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    double work = 0.0;
+    for (int r = 0; r < nrep; ++r) work += n[r] * b->rho[r];
+    const double target = work / (2.0 * std::max(1, ncu));
+    b->max_pieces.resize(static_cast<size_t>(nrep));
+    b->def_pieces.resize(static_cast<size_t>(nrep));
+    b->piece_slots = 0;
+    for (int r = 0; r < nrep; ++r) {
+        const int mp = static_cast<int>(std::max<long>(1, std::min<long>(32, n[r] / 32)));
+        const int dp = static_cast<int>(std::ceil(n[r] * b->rho[r] / target));
+        b->max_pieces[r] = mp;
+        b->def_pieces[r] = std::max(1, std::min(mp, dp));
+        b->reps[r].npiece = b->def_pieces[r];
+        b->piece_slots += mp;
+    }
     BHIP(b, dalloc(b->d_reps, nrep));
     BHIP(b, dalloc(b->pos, total)); BHIP(b, dalloc(b->vel, total));
     BHIP(b, dalloc(b->frc, total)); BHIP(b, dalloc(b->ref, total));
+    BHIP(b, dalloc(b->slot_of, total)); BHIP(b, dalloc(b->orig, total));
     BHIP(b, dalloc(b->nbr, total * b->cap)); BHIP(b, dalloc(b->ncnt, total));
     BHIP(b, dalloc(b->step, 2)); BHIP(b, dalloc(b->builds, nrep)); BHIP(b, dalloc(b->flags, 2));
-    BHIP(b, dalloc(b->stage, 9 * total));
+    BHIP(b, dalloc(b->d_pieces, b->piece_slots)); BHIP(b, dalloc(b->ptop, b->piece_slots));
+    BHIP(b, dalloc(b->ppart, 3L * b->piece_slots));
     BHIP(b, hipMemset(b->step, 0, 2 * sizeof(unsigned long long)));
     BHIP(b, hipMemset(b->builds, 0, nrep * sizeof(long long)));
     BHIP(b, hipMemset(b->flags, 0, 2 * sizeof(int)));
@@ -243,7 +308,13 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
     BHIP(b, hipMemcpy(b->ref, p.data(), total * sizeof(double4), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->vel, v.data(), total * sizeof(double4), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->frc, f.data(), total * sizeof(double4), hipMemcpyHostToDevice));
-    return upload_reps(b);
+    // identity slot order until a replica's first cell-sorted build
+    std::vector<int> ident(static_cast<size_t>(total));
+    for (int r = 0; r < nrep; ++r)
+        for (int i = 0; i < n[r]; ++i) ident[static_cast<size_t>(b->reps[r].off + i)] = i;
+    BHIP(b, hipMemcpy(b->slot_of, ident.data(), total * sizeof(int), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->orig, ident.data(), total * sizeof(int), hipMemcpyHostToDevice));
+    return build_pieces(b);
 }
 
 md_status md_batch_destroy(md_batch* b) {
@@ -254,7 +325,9 @@ md_status md_batch_destroy(md_batch* b) {
                     static_cast<void*>(b->ref), static_cast<void*>(b->nbr),
                     static_cast<void*>(b->ncnt), static_cast<void*>(b->step),
                     static_cast<void*>(b->builds), static_cast<void*>(b->flags),
-                    static_cast<void*>(b->thermo), static_cast<void*>(b->stage)})
+                    static_cast<void*>(b->thermo), static_cast<void*>(b->slot_of),
+                    static_cast<void*>(b->orig), static_cast<void*>(b->d_pieces),
+                    static_cast<void*>(b->ptop), static_cast<void*>(b->ppart)})
         if (p) (void)hipFree(p);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -279,10 +352,23 @@ md_status md_batch_set_tpa(md_batch* b, const int* tpa, int block) {
         const int t = tpa[r];
         if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
             return bfail(b, MD_ERR_INVALID_ARGUMENT, "lanes per atom must be 1, 2, 4, 8 or 16");
-        b->reps[r].tpa = t;
     }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int r = 0; r < b->nrep; ++r) b->reps[r].tpa = tpa[r];
     b->block = block;
     return upload_reps(b);
+}
+
+md_status md_batch_set_pieces(md_batch* b, const int* pieces) {
+    if (b == nullptr || pieces == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    for (int r = 0; r < b->nrep; ++r)
+        if (pieces[r] < 1 || pieces[r] > b->max_pieces[r])
+            return bfail(b, MD_ERR_INVALID_ARGUMENT,
+                         "replica " + std::to_string(r) + ": pieces must be in [1, " +
+                             std::to_string(b->max_pieces[r]) + "]");
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int r = 0; r < b->nrep; ++r) b->reps[r].npiece = pieces[r];
+    return build_pieces(b);
 }
 
 md_status md_batch_run(md_batch* b, long nsteps, long segment, int use_graph, double* thermo_out) {
@@ -373,12 +459,14 @@ md_status md_batch_rebuild_counts(md_batch* b, long* counts) {
 }
 
 // Autotuner: for each workgroup size in {256, 512, 1024}, coordinate descent
-// over the lanes-per-atom of each replica size class in {1, 2, 4, 8, 16},
-// timing `trial_steps` launches from the same saved state with HIP events;
-// the best (block, per-class lanes) is applied and the state restored, so
-// tuning leaves the trajectory untouched.
-md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* block_out,
-                            double* best_ms_per_step) {
+// over each replica size class (largest first): the class's piece counts as
+// {1/4, 1/2, 1, 2, 4} times the work-balanced default, then its lanes per atom
+// in {1, 2, 4, 8, 16}; each candidate is timed over `trial_steps` steps from
+// the same saved state with HIP events. The fastest (block, pieces, lanes) is
+// applied and the state restored, so tuning leaves the trajectory untouched
+// (fixed-point forces and sums do not depend on the split).
+md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* pieces_out,
+                            int* block_out, double* best_ms_per_step) {
     if (b == nullptr) return MD_ERR_INVALID_ARGUMENT;
     if (!b->primed) return bfail(b, MD_ERR_STATE, "md_batch_prime has not been called");
     if (trial_steps < 1) trial_steps = 5;
@@ -386,43 +474,66 @@ md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* blo
     // save
     double4 *sp = nullptr, *sv = nullptr, *sf = nullptr, *sr = nullptr;
     uint32_t* snb = nullptr;
-    int* sc = nullptr;
+    int *sc = nullptr, *sslot = nullptr, *sorig = nullptr;
     unsigned long long sstep[2];
     std::vector<long long> sbuilds(static_cast<size_t>(b->nrep));
+    auto release = [&]() {
+        for (void* q : {static_cast<void*>(sp), static_cast<void*>(sv), static_cast<void*>(sf),
+                        static_cast<void*>(sr), static_cast<void*>(snb), static_cast<void*>(sc),
+                        static_cast<void*>(sslot), static_cast<void*>(sorig)})
+            if (q) (void)hipFree(q);
+    };
+    struct Guard {
+        std::function<void()> f;
+        ~Guard() { f(); }
+    } guard{release};
     BHIP(b, dalloc(sp, n)); BHIP(b, dalloc(sv, n)); BHIP(b, dalloc(sf, n)); BHIP(b, dalloc(sr, n));
     BHIP(b, dalloc(snb, n * b->cap)); BHIP(b, dalloc(sc, n));
+    BHIP(b, dalloc(sslot, n)); BHIP(b, dalloc(sorig, n));
     BHIP(b, hipStreamSynchronize(b->stream));
-    BHIP(b, hipMemcpy(sp, b->pos, n * sizeof(double4), hipMemcpyDeviceToDevice));
-    BHIP(b, hipMemcpy(sv, b->vel, n * sizeof(double4), hipMemcpyDeviceToDevice));
-    BHIP(b, hipMemcpy(sf, b->frc, n * sizeof(double4), hipMemcpyDeviceToDevice));
-    BHIP(b, hipMemcpy(sr, b->ref, n * sizeof(double4), hipMemcpyDeviceToDevice));
-    BHIP(b, hipMemcpy(snb, b->nbr, n * b->cap * sizeof(uint32_t), hipMemcpyDeviceToDevice));
-    BHIP(b, hipMemcpy(sc, b->ncnt, n * sizeof(int), hipMemcpyDeviceToDevice));
+    const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
+    BHIP(b, hipMemcpy(sp, b->pos, n * sizeof(double4), d2d));
+    BHIP(b, hipMemcpy(sv, b->vel, n * sizeof(double4), d2d));
+    BHIP(b, hipMemcpy(sf, b->frc, n * sizeof(double4), d2d));
+    BHIP(b, hipMemcpy(sr, b->ref, n * sizeof(double4), d2d));
+    BHIP(b, hipMemcpy(snb, b->nbr, n * b->cap * sizeof(uint32_t), d2d));
+    BHIP(b, hipMemcpy(sc, b->ncnt, n * sizeof(int), d2d));
+    BHIP(b, hipMemcpy(sslot, b->slot_of, n * sizeof(int), d2d));
+    BHIP(b, hipMemcpy(sorig, b->orig, n * sizeof(int), d2d));
     BHIP(b, hipMemcpy(sstep, b->step, sizeof(sstep), hipMemcpyDeviceToHost));
     BHIP(b, hipMemcpy(sbuilds.data(), b->builds, b->nrep * sizeof(long long), hipMemcpyDeviceToHost));
     auto restore = [&]() -> md_status {
-        BHIP(b, hipMemcpy(b->pos, sp, n * sizeof(double4), hipMemcpyDeviceToDevice));
-        BHIP(b, hipMemcpy(b->vel, sv, n * sizeof(double4), hipMemcpyDeviceToDevice));
-        BHIP(b, hipMemcpy(b->frc, sf, n * sizeof(double4), hipMemcpyDeviceToDevice));
-        BHIP(b, hipMemcpy(b->ref, sr, n * sizeof(double4), hipMemcpyDeviceToDevice));
-        BHIP(b, hipMemcpy(b->nbr, snb, n * b->cap * sizeof(uint32_t), hipMemcpyDeviceToDevice));
-        BHIP(b, hipMemcpy(b->ncnt, sc, n * sizeof(int), hipMemcpyDeviceToDevice));
+        BHIP(b, hipMemcpy(b->pos, sp, n * sizeof(double4), d2d));
+        BHIP(b, hipMemcpy(b->vel, sv, n * sizeof(double4), d2d));
+        BHIP(b, hipMemcpy(b->frc, sf, n * sizeof(double4), d2d));
+        BHIP(b, hipMemcpy(b->ref, sr, n * sizeof(double4), d2d));
+        BHIP(b, hipMemcpy(b->nbr, snb, n * b->cap * sizeof(uint32_t), d2d));
+        BHIP(b, hipMemcpy(b->ncnt, sc, n * sizeof(int), d2d));
+        BHIP(b, hipMemcpy(b->slot_of, sslot, n * sizeof(int), d2d));
+        BHIP(b, hipMemcpy(b->orig, sorig, n * sizeof(int), d2d));
         BHIP(b, hipMemcpy(b->step, sstep, sizeof(sstep), hipMemcpyHostToDevice));
         BHIP(b, hipMemcpy(b->builds, sbuilds.data(), b->nrep * sizeof(long long), hipMemcpyHostToDevice));
         return MD_OK;
     };
-    // size classes
-    std::map<int, std::vector<int>> classes;
+    // size classes, largest first
+    std::map<int, std::vector<int>, std::greater<int>> classes;
     for (int r = 0; r < b->nrep; ++r) classes[b->reps[r].n].push_back(r);
     hipEvent_t e0, e1;
     BHIP(b, hipEventCreate(&e0));
     BHIP(b, hipEventCreate(&e1));
-    const mdk::BatchArgs a = args_of(b, 1, nullptr, 0);
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } events{e0, e1};
     auto trial = [&](double* ms) -> md_status {
         md_status st = restore();
         if (st != MD_OK) return st;
-        st = upload_reps(b);
+        st = build_pieces(b);
         if (st != MD_OK) return st;
+        const mdk::BatchArgs a = args_of(b, 1, nullptr, 0);
         BHIP(b, hipEventRecord(e0, b->stream));
         for (int k = 0; k < trial_steps; ++k) BHIP(b, mdk::launch_batch(a, b->block, b->stream));
         BHIP(b, hipEventRecord(e1, b->stream));
@@ -432,52 +543,78 @@ md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* blo
         *ms = f / trial_steps;
         return MD_OK;
     };
+    auto scaled = [&](int r, double m) {
+        const int v = static_cast<int>(std::lround(b->def_pieces[r] * m));
+        return std::max(1, std::min(b->max_pieces[r], v));
+    };
     double best = 1e30;
     int best_block = b->block;
-    std::vector<int> best_tpa(static_cast<size_t>(b->nrep), 1);
+    std::vector<int> best_tpa(static_cast<size_t>(b->nrep), 4), best_np(static_cast<size_t>(b->nrep), 1);
     const int lanes[5] = {1, 2, 4, 8, 16};
+    const double mult[5] = {0.25, 0.5, 1.0, 2.0, 4.0};
     for (int block : {256, 512, 1024}) {
         b->block = block;
-        for (auto& r : b->reps) r.tpa = 1;
+        for (int r = 0; r < b->nrep; ++r) {
+            b->reps[r].tpa = 4;
+            b->reps[r].npiece = b->def_pieces[r];
+        }
         double cur = 0.0;
         md_status st = trial(&cur);
         if (st != MD_OK) return st;
         for (auto& kv : classes) {
-            int keep = 1;
+            double keep_m = 1.0;
+            for (double m : mult) {
+                if (m == 1.0) continue;
+                for (int r : kv.second) b->reps[r].npiece = scaled(r, m);
+                double ms = 0.0;
+                st = trial(&ms);
+                if (st != MD_OK) return st;
+                if (ms < cur) {
+                    cur = ms;
+                    keep_m = m;
+                }
+            }
+            for (int r : kv.second) b->reps[r].npiece = scaled(r, keep_m);
+            int keep_t = 4;
             for (int t : lanes) {
-                if (t == 1) continue;
+                if (t == 4) continue;
                 for (int r : kv.second) b->reps[r].tpa = t;
                 double ms = 0.0;
                 st = trial(&ms);
                 if (st != MD_OK) return st;
                 if (ms < cur) {
                     cur = ms;
-                    keep = t;
+                    keep_t = t;
                 }
             }
-            for (int r : kv.second) b->reps[r].tpa = keep;
+            for (int r : kv.second) b->reps[r].tpa = keep_t;
         }
         if (cur < best) {
             best = cur;
             best_block = block;
-            for (int r = 0; r < b->nrep; ++r) best_tpa[r] = b->reps[r].tpa;
+            for (int r = 0; r < b->nrep; ++r) {
+                best_tpa[r] = b->reps[r].tpa;
+                best_np[r] = b->reps[r].npiece;
+            }
         }
     }
     b->block = best_block;
-    for (int r = 0; r < b->nrep; ++r) b->reps[r].tpa = best_tpa[r];
+    for (int r = 0; r < b->nrep; ++r) {
+        b->reps[r].tpa = best_tpa[r];
+        b->reps[r].npiece = best_np[r];
+    }
     md_status st = restore();
     if (st != MD_OK) return st;
-    st = upload_reps(b);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (void* p : {static_cast<void*>(sp), static_cast<void*>(sv), static_cast<void*>(sf),
-                    static_cast<void*>(sr), static_cast<void*>(snb), static_cast<void*>(sc)})
-        (void)hipFree(p);
+    st = build_pieces(b);
+    if (st != MD_OK) return st;
+    BHIP(b, hipStreamSynchronize(b->stream));
     if (tpa_out)
         for (int r = 0; r < b->nrep; ++r) tpa_out[r] = best_tpa[r];
+    if (pieces_out)
+        for (int r = 0; r < b->nrep; ++r) pieces_out[r] = best_np[r];
     if (block_out) *block_out = best_block;
     if (best_ms_per_step) *best_ms_per_step = best;
-    return st;
+    return MD_OK;
 }
 
 }  // extern "C"

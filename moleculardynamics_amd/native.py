@@ -125,7 +125,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_batch_prime.argtypes = [vp]
     lib.md_batch_run.argtypes = [vp, C.c_long, C.c_long, C.c_int, _dp]
     lib.md_batch_set_tpa.argtypes = [vp, _ip, C.c_int]
-    lib.md_batch_autotune.argtypes = [vp, C.c_int, _ip, _ip, _dp]
+    lib.md_batch_set_pieces.argtypes = [vp, _ip]
+    lib.md_batch_autotune.argtypes = [vp, C.c_int, _ip, _ip, _ip, _dp]
     lib.md_batch_download.argtypes = [vp] + [_dp] * 9
     lib.md_batch_rebuild_counts.argtypes = [vp, _lp]
     wp = C.POINTER(WaterParams)
@@ -346,7 +347,7 @@ def batch_draw(count: int, master_seed: int):
 
 
 class Batch:
-    """md_batch: many independent replicas advanced by one launch per step."""
+    """md_batch: many independent replicas advanced by one set of launches per step."""
 
     def __init__(self, systems, temperatures, seeds, *, sigma=1.0, epsilon=1.0, cutoff=2.5,
                  skin=0.3, dt=0.005, friction=1.0, precision=MIXED):
@@ -396,13 +397,20 @@ class Batch:
         t = np.ascontiguousarray(tpa, dtype=np.int32)
         self._chk(self.lib.md_batch_set_tpa(self.h, t.ctypes.data_as(C.POINTER(C.c_int)), block))
 
+    def set_pieces(self, pieces):
+        p = np.ascontiguousarray(pieces, dtype=np.int32)
+        self._chk(self.lib.md_batch_set_pieces(self.h, p.ctypes.data_as(C.POINTER(C.c_int))))
+
     def autotune(self, trial_steps=5):
+        """Returns (lanes per atom, pieces per replica, block, ms per step)."""
         t = np.zeros(self.nrep, dtype=np.int32)
+        p = np.zeros(self.nrep, dtype=np.int32)
         blk = C.c_int()
         ms = C.c_double()
         self._chk(self.lib.md_batch_autotune(self.h, trial_steps, t.ctypes.data_as(C.POINTER(C.c_int)),
-                                             C.byref(blk), C.byref(ms)))
-        return t, blk.value, ms.value
+                                             p.ctypes.data_as(C.POINTER(C.c_int)), C.byref(blk),
+                                             C.byref(ms)))
+        return t, p, blk.value, ms.value
 
     def download(self):
         n = int(self.sizes.sum())

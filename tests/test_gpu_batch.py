@@ -1,5 +1,6 @@
-"""Batched uneven replicas (one launch per step for all replicas, HIP-graph
-segments) against the oracle and against the single-system path."""
+"""Batched uneven replicas (one set of launches per step for all replicas,
+each split into pieces, HIP-graph segments) against the oracle and against
+the single-system path."""
 import numpy as np
 import pytest
 
@@ -87,6 +88,31 @@ def test_graph_replay_matches_plain_launches(gpu_ok):
     assert np.array_equal(th0, th1)
 
 
+def test_piece_split_keeps_trajectory(gpu_ok):
+    """Rows, slot order and fixed-point sums do not depend on how a replica is
+    split into workgroups: one piece per replica and the largest split give
+    the same trajectory, E and W bit for bit, through several rebuilds (the
+    kinetic energy is an FP64 sum whose grouping follows the split)."""
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, _, _ = _systems(10, master=4600)
+    n = np.array([s[1].shape[0] for s in systems])
+    outs = []
+    for pieces in (np.ones_like(n), np.maximum(1, np.minimum(32, n // 32))):
+        b = native.Batch(systems, T, seeds, precision=native.MIXED)
+        b.set_pieces(pieces)
+        b.prime()
+        th = b.run(150, segment=50, thermo=True)
+        outs.append((b.download(), th, b.rebuild_counts()))
+        b.close()
+    (s0, th0, c0), (s1, th1, c1) = outs
+    assert np.all(c0 > 1) and np.array_equal(c0, c1)
+    for a, c in zip(s0[0] + s0[1] + s0[2], s1[0] + s1[1] + s1[2]):
+        assert np.array_equal(a, c)
+    assert np.array_equal(th0[..., [0, 1, 3]], th1[..., [0, 1, 3]])
+    assert np.allclose(th0[..., 2], th1[..., 2], rtol=1e-12, atol=0.0)
+
+
 def test_batch_matches_single_system_path(gpu_ok):
     """Same seed, same counter-based noise, fixed-point forces: a replica in
     the batch and the same system on the plain path follow one trajectory.
@@ -148,8 +174,10 @@ def test_autotune_keeps_trajectory(gpu_ok):
     ref.close()
     b = native.Batch(systems, T, seeds)
     b.prime()
-    tpa, block, ms = b.autotune(trial_steps=3)
+    tpa, pieces, block, ms = b.autotune(trial_steps=3)
     assert block in (256, 512, 1024) and set(np.unique(tpa)) <= {1, 2, 4, 8, 16} and ms > 0
+    n = np.array([s[1].shape[0] for s in systems])
+    assert np.all(pieces >= 1) and np.all(pieces <= np.maximum(1, np.minimum(32, n // 32)))
     b.run(20, graph=False)
     got = b.download()
     b.close()
