@@ -15,10 +15,11 @@
 //      the atoms of a cell ordered by index, so the order is a pure function
 //      of the positions and every piece computes the same one) and rebuilds
 //      the rows of its own slots by the reference's exact pair test: from the
-//      27 cells around each atom when the replica's cell grid is usable, over
-//      all atom pairs otherwise -- the same pair set as the reference's cell
-//      or brute-force build, including the single-nearest-image semantics of
-//      boxes shorter than 2 (rc + skin). Then `tpa` lanes per slot walk its
+//      5 x 5 x 5 cells of width >= (rc + skin)/2 around each atom when 5 or
+//      more such cells fit on every axis, over all atom pairs otherwise --
+//      the same pair set as the reference's cell or brute-force build,
+//      including the single-nearest-image semantics of boxes shorter than
+//      2 (rc + skin). Then `tpa` lanes per slot walk its
 //      row, gathering partners from LDS: FP32 pair terms with 64-bit
 //      fixed-point accumulation (or FP64), the final B half-kick, and the
 //      piece's E, W, KE partials;
@@ -76,11 +77,9 @@ __device__ __forceinline__ int rep_cell_coord(double r, double L, double w, int 
 // the cell tables of the rebuild.
 struct Stage {
     double *xs, *ys, *zs;
-    int* ol;     // slot -> atom
-    int* cidx;   // atom -> cell
-    int* tmp;    // atoms grouped by cell, in atomic order
-    int* cstart; // ncell + 1 offsets
-    int* cfill;
+    int* ol;    // slot -> atom; during a sort, the atoms grouped by cell
+    int* cidx;  // atom -> cell, then atom -> slot
+    int* cend;  // per cell: count, start, then end offset
 };
 
 __device__ __forceinline__ Stage stage_of(const BatchArgs& a) {
@@ -91,11 +90,11 @@ __device__ __forceinline__ Stage stage_of(const BatchArgs& a) {
     st.zs = lds + 2 * a.nmax;
     st.ol = reinterpret_cast<int*>(lds + 3 * a.nmax);
     st.cidx = st.ol + a.nmax;
-    st.tmp = st.cidx + a.nmax;
-    st.cstart = st.tmp + a.nmax;
-    st.cfill = st.cstart + a.maxcell + 1;
+    st.cend = st.cidx + a.nmax;
     return st;
 }
+
+__device__ __forceinline__ int cell_begin(const Stage& st, int c) { return c > 0 ? st.cend[c - 1] : 0; }
 
 // Counting sort of the replica's atoms by cell into slot order: the atoms of
 // a cell in index order. Stages positions and the slot map in LDS; for the
@@ -106,7 +105,7 @@ __device__ void sort_slots(const BatchArgs& a, const BatchRep& R, const Stage& s
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     const int ncell = nx * ny * nz;
     const int tid = threadIdx.x, bd = blockDim.x;
-    for (int c = tid; c < ncell; c += bd) st.cfill[c] = 0;
+    for (int c = tid; c < ncell; c += bd) st.cend[c] = 0;
     __syncthreads();
     for (int i = tid; i < n; i += bd) {
         const double4 p = P[i];
@@ -115,35 +114,38 @@ __device__ void sort_slots(const BatchArgs& a, const BatchRep& R, const Stage& s
         const int cz = rep_cell_coord(p.z, R.box.L[2], R.w[2], nz);
         const int c = cx + nx * (cy + ny * cz);
         st.cidx[i] = c;
-        atomicAdd(&st.cfill[c], 1);
+        atomicAdd(&st.cend[c], 1);
     }
     __syncthreads();
-    if (tid < kWave) {  // exclusive scan of the counts by the first wave
+    if (tid < kWave) {  // exclusive scan of the counts in place by the first wave
         const int lane = tid;
         int run = 0;
         for (int c0 = 0; c0 < ncell; c0 += kWave) {
             const int c = c0 + lane;
-            const int v = c < ncell ? st.cfill[c] : 0;
+            const int v = c < ncell ? st.cend[c] : 0;
             int x = v;
             for (int off = 1; off < kWave; off <<= 1) {
                 const int y = __shfl_up(x, off, kWave);
                 if (lane >= off) x += y;
             }
-            if (c < ncell) st.cstart[c] = run + x - v;
+            if (c < ncell) st.cend[c] = run + x - v;
             run += __shfl(x, kWave - 1, kWave);
         }
-        if (lane == 0) st.cstart[ncell] = run;
     }
     __syncthreads();
-    for (int c = tid; c < ncell; c += bd) st.cfill[c] = st.cstart[c];
+    // group by cell (the start offsets advance to the end offsets)
+    for (int i = tid; i < n; i += bd) st.ol[atomicAdd(&st.cend[st.cidx[i]], 1)] = i;
     __syncthreads();
-    for (int i = tid; i < n; i += bd) st.tmp[atomicAdd(&st.cfill[st.cidx[i]], 1)] = i;
+    // slot = cell start + rank of the atom index within its cell
+    for (int i = tid; i < n; i += bd) {
+        const int c = st.cidx[i], b = cell_begin(st, c), e = st.cend[c];
+        int r = 0;
+        for (int k = b; k < e; ++k) r += st.ol[k] < i ? 1 : 0;
+        st.cidx[i] = b + r;
+    }
     __syncthreads();
     for (int i = tid; i < n; i += bd) {
-        const int c = st.cidx[i], b = st.cstart[c], e = st.cstart[c + 1];
-        int r = 0;
-        for (int k = b; k < e; ++k) r += st.tmp[k] < i ? 1 : 0;
-        const int s = b + r;
+        const int s = st.cidx[i];
         const double4 p = P[i];
         st.ol[s] = i;
         st.xs[s] = p.x;
@@ -175,10 +177,11 @@ __device__ __forceinline__ int scan_range(const BatchArgs& a, const BatchRep& R,
     return cnt;
 }
 
-// Rows of the piece's slots [s0, s1): cell-sorted replicas scan the 3 x 3
-// runs of x-adjacent cells around the slot's cell (contiguous in slot order,
-// plus the periodic image cell at either end of the x axis); other replicas
-// scan all slots.
+// Rows of the piece's slots [s0, s1). Cell-sorted replicas (cells of at
+// least (rc + skin)/2, 5 or more per axis) scan the 5 x 5 runs of five
+// x-adjacent cells around the slot's cell -- contiguous in slot order, split
+// in two where the run wraps the x axis; every pair within rc + skin lies in
+// such a run. Other replicas scan all slots.
 __device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& st, bool cells, int s0,
                            int s1) {
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
@@ -189,16 +192,23 @@ __device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& s
         uint32_t* row = a.nbr + g * a.cap;
         int cnt = 0;
         if (cells) {
-            const int c = st.cidx[oi];
-            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-            const int xa = cx > 0 ? cx - 1 : 0, xb = cx < nx - 1 ? cx + 1 : nx - 1;
-            const int xw = cx == 0 ? nx - 1 : (cx == nx - 1 ? 0 : -1);
-            for (int oz = -1; oz <= 1; ++oz)
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int b = nx * ((cy + oy + ny) % ny + ny * ((cz + oz + nz) % nz));
-                    cnt = scan_range(a, R, st, s, oi, pi, st.cstart[b + xa], st.cstart[b + xb + 1], row, cnt);
-                    if (xw >= 0) cnt = scan_range(a, R, st, s, oi, pi, st.cstart[b + xw], st.cstart[b + xw + 1], row, cnt);
+            const int cx = rep_cell_coord(pi.x, R.box.L[0], R.w[0], nx);
+            const int cy = rep_cell_coord(pi.y, R.box.L[1], R.w[1], ny);
+            const int cz = rep_cell_coord(pi.z, R.box.L[2], R.w[2], nz);
+            const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            for (int oz = -2; oz <= 2; ++oz) {
+                const int zz = (cz + oz + nz) % nz;
+                for (int oy = -2; oy <= 2; ++oy) {
+                    const int b = nx * ((cy + oy + ny) % ny + ny * zz);
+                    cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + xa), st.cend[b + xb], row, cnt);
+                    if (cx < 2)
+                        cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + nx + cx - 2),
+                                         st.cend[b + nx - 1], row, cnt);
+                    else if (cx + 2 >= nx)
+                        cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b), st.cend[b + cx + 2 - nx],
+                                         row, cnt);
                 }
+            }
         } else {
             cnt = scan_range(a, R, st, s, oi, pi, 0, R.n, row, cnt);
         }
@@ -496,7 +506,7 @@ __global__ __launch_bounds__(256) void k_batch_sum(BatchArgs a) {
 
 size_t batch_lds_bytes(int nmax, int maxcell) {
     size_t b = static_cast<size_t>(nmax) * (3 * sizeof(double) + sizeof(int));
-    if (maxcell > 0) b += (static_cast<size_t>(2) * nmax + 2 * static_cast<size_t>(maxcell) + 1) * sizeof(int);
+    if (maxcell > 0) b += (static_cast<size_t>(nmax) + static_cast<size_t>(maxcell)) * sizeof(int);
     return b;
 }
 

@@ -20,8 +20,9 @@ struct BatchRep {
     uint32_t seed_lo, seed_hi;
     Box box;
     double T, c1, c2;
-    // the replica's cell grid at rc + skin (neighbor/cell_list.cpp:8-14);
-    // usable when 3 or more cells fit on every axis
+    // the replica's rebuild grid: cells of at least (rc + skin)/2 searched
+    // 5 x 5 x 5 (the cell index rule of neighbor/cell_list.cpp:31-38 at that
+    // width); usable when 5 or more cells fit on every axis
     int nc[3];
     double w[3];
     int usable;
@@ -63,7 +64,8 @@ struct BatchArgs {
 
 hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s);
 // Dynamic LDS of one rebuild/force workgroup: the positions and slot map,
-// and with maxcell > 0 the cell tables of the in-LDS rebuild.
+// and with maxcell > 0 the atom -> cell map and cell offsets of the in-LDS
+// rebuild.
 size_t batch_lds_bytes(int nmax, int maxcell);
 hipError_t batch_allow_lds(size_t bytes);
 

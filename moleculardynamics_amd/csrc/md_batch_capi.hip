@@ -248,13 +248,15 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
     const double rl = lj->cutoff + skin;
     long cap = static_cast<long>(std::ceil(2.0 * 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho_max + 32.0));
     b->cap = static_cast<int>(std::min<long>((cap + 7) / 8 * 8, b->nmax));
-    // each replica's cell grid at rc + skin (the reference's rule)
+    // each replica's rebuild grid: cells of at least (rc + skin)/2 (with a
+    // 1e-9 relative margin, so that rounding cannot put a pair within rc + skin
+    // three cells apart), searched 5 x 5 x 5; usable with 5 or more per axis
     for (auto& R : b->reps) {
         bool usable = true;
         for (int d = 0; d < 3; ++d) {
-            R.nc[d] = std::max(1, static_cast<int>(std::floor(R.box.L[d] / rl)));
+            R.nc[d] = std::max(1, static_cast<int>(std::floor(2.0 * R.box.L[d] / (rl * (1.0 + 1e-9)))));
             R.w[d] = R.box.L[d] / R.nc[d];
-            usable = usable && R.nc[d] >= 3;
+            usable = usable && R.nc[d] >= 5;
         }
         R.usable = usable ? 1 : 0;
         if (usable) b->maxcell = std::max(b->maxcell, R.nc[0] * R.nc[1] * R.nc[2]);
