@@ -159,33 +159,52 @@ __device__ void sort_slots(const BatchArgs& a, const BatchRep& R, const Stage& s
     __syncthreads();
 }
 
-// Appends to a row the slots k in [ks, ke) within rc + skin of slot s
-// (neighbor/verlet_list.cpp:26-50's test: the lower atom index first).
+constexpr int kBuildLanes = 8;  // lanes per slot in the row build
+
+// Appends to a row, in slot order, the slots k in [ks, ke) within rc + skin
+// of slot s (neighbor/verlet_list.cpp:26-50's test: the lower atom index
+// first). The kBuildLanes lanes of a slot test consecutive candidates
+// together; a ballot compacts the accepted ones (`gshift` is the group's
+// first lane in the wave).
 __device__ __forceinline__ int scan_range(const BatchArgs& a, const BatchRep& R, const Stage& st, int s,
                                           int oi, const double4& pi, int ks, int ke, uint32_t* row,
-                                          int cnt) {
-    for (int k = ks; k < ke; ++k) {
-        if (k == s) continue;
-        const bool flip = oi > st.ol[k];
-        double dx, dy, dz;
-        row_delta(R.box, pi, make_double4(st.xs[k], st.ys[k], st.zs[k], 0.0), flip, dx, dy, dz);
-        if (norm2_ref(dx, dy, dz) < a.rl2) {
-            if (cnt < a.cap) row[cnt] = static_cast<uint32_t>(k) | (flip ? kFlipBit : 0u);
-            ++cnt;
+                                          int cnt, int sub, int gshift) {
+    for (int k0 = ks; k0 < ke; k0 += kBuildLanes) {
+        const int k = k0 + sub;
+        bool acc = false;
+        uint32_t ent = 0;
+        if (k < ke && k != s) {
+            const bool flip = oi > st.ol[k];
+            double dx, dy, dz;
+            row_delta(R.box, pi, make_double4(st.xs[k], st.ys[k], st.zs[k], 0.0), flip, dx, dy, dz);
+            acc = norm2_ref(dx, dy, dz) < a.rl2;
+            ent = static_cast<uint32_t>(k) | (flip ? kFlipBit : 0u);
         }
+        const uint32_t gm = static_cast<uint32_t>(__ballot(acc) >> gshift) & ((1u << kBuildLanes) - 1u);
+        if (acc) {
+            const int pos = cnt + __popc(gm & ((1u << sub) - 1u));
+            if (pos < a.cap) row[pos] = ent;
+        }
+        cnt += __popc(gm);
     }
     return cnt;
 }
 
-// Rows of the piece's slots [s0, s1). Cell-sorted replicas (cells of at
-// least (rc + skin)/2, 5 or more per axis) scan the 5 x 5 runs of five
-// x-adjacent cells around the slot's cell -- contiguous in slot order, split
-// in two where the run wraps the x axis; every pair within rc + skin lies in
-// such a run. Other replicas scan all slots.
+// Rows of the piece's slots [s0, s1), kBuildLanes lanes per slot.
+// Cell-sorted replicas (cells of at least (rc + skin)/2, 5 or more per axis)
+// scan the 5 x 5 runs of five x-adjacent cells around the slot's cell --
+// contiguous in slot order, split in two where the run wraps the x axis;
+// every pair within rc + skin lies in such a run. Other replicas scan all
+// slots.
 __device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& st, bool cells, int s0,
                            int s1) {
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
-    for (int s = s0 + static_cast<int>(threadIdx.x); s < s1; s += blockDim.x) {
+    const int tid = threadIdx.x;
+    const int groups = blockDim.x / kBuildLanes, sub = tid % kBuildLanes;
+    const int gshift = (tid & (kWave - 1)) & ~(kBuildLanes - 1);
+    for (int base = s0; base < s1; base += groups) {
+        const int s = base + tid / kBuildLanes;
+        if (s >= s1) continue;
         const long g = R.off + s;
         const int oi = st.ol[s];
         const double4 pi = make_double4(st.xs[s], st.ys[s], st.zs[s], 0.0);
@@ -200,21 +219,24 @@ __device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& s
                 const int zz = (cz + oz + nz) % nz;
                 for (int oy = -2; oy <= 2; ++oy) {
                     const int b = nx * ((cy + oy + ny) % ny + ny * zz);
-                    cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + xa), st.cend[b + xb], row, cnt);
+                    cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + xa), st.cend[b + xb], row, cnt,
+                                     sub, gshift);
                     if (cx < 2)
                         cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + nx + cx - 2),
-                                         st.cend[b + nx - 1], row, cnt);
+                                         st.cend[b + nx - 1], row, cnt, sub, gshift);
                     else if (cx + 2 >= nx)
                         cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b), st.cend[b + cx + 2 - nx],
-                                         row, cnt);
+                                         row, cnt, sub, gshift);
                 }
             }
         } else {
-            cnt = scan_range(a, R, st, s, oi, pi, 0, R.n, row, cnt);
+            cnt = scan_range(a, R, st, s, oi, pi, 0, R.n, row, cnt, sub, gshift);
         }
-        a.ncnt[g] = cnt;
-        a.orig[g] = oi;
-        if (cnt > a.cap) atomicOr(a.overflow, 1);
+        if (sub == 0) {
+            a.ncnt[g] = cnt;
+            a.orig[g] = oi;
+            if (cnt > a.cap) atomicOr(a.overflow, 1);
+        }
     }
 }
 
