@@ -202,7 +202,9 @@ def bench_batch(args) -> int:
     atoms = int(sum(s[1].shape[0] for s in systems))
     b = native.Batch(systems, T, seeds, precision=native.MIXED)
     b.prime()
-    tpa, pieces, block, _ = b.autotune(trial_steps=10)
+    # tune from a state whose list ages are spread, over trials that span rebuilds
+    b.run(200, segment=100)
+    tpa, pieces, block, _ = b.autotune(trial_steps=30)
     b.run(max(args.warmup, 100), segment=100)
     steps = 10000
     t0 = time.perf_counter()

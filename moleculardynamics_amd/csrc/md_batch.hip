@@ -8,7 +8,9 @@
 // replicas form one grid, longest first:
 //   1. k_batch_pre: B, A, O, A over the piece's atoms and the piece's two
 //      largest displacements since the last build;
-//   2. k_batch_force: every piece stages its whole replica's positions in
+//   2. k_batch_plan (one workgroup): each replica's rebuild decision and the
+//      force kernel's queue, pieces of rebuilding replicas first;
+//   3. k_batch_force: every piece stages its whole replica's positions in
 //      LDS (FP64, in slot order: the atoms sorted by cell at the last build).
 //      When the sum of the replica's two largest displacements exceeds the
 //      skin, the piece re-sorts the replica by cell in LDS (a counting sort;
@@ -23,7 +25,7 @@
 //      row, gathering partners from LDS: FP32 pair terms with 64-bit
 //      fixed-point accumulation (or FP64), the final B half-kick, and the
 //      piece's E, W, KE partials;
-//   3. k_batch_sum: one workgroup sums each replica's piece partials in piece
+//   4. k_batch_sum: one workgroup sums each replica's piece partials in piece
 //      order into the thermo ring and advances the device step counter.
 // A segment of identical launches can therefore be captured in a HIP graph
 // and replayed.
@@ -216,9 +218,12 @@ __device__ void build_rows(const BatchArgs& a, const BatchRep& R, const Stage& s
             const int cz = rep_cell_coord(pi.z, R.box.L[2], R.w[2], nz);
             const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
             for (int oz = -2; oz <= 2; ++oz) {
-                const int zz = (cz + oz + nz) % nz;
+                int zz = cz + oz;
+                zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
                 for (int oy = -2; oy <= 2; ++oy) {
-                    const int b = nx * ((cy + oy + ny) % ny + ny * zz);
+                    int yy = cy + oy;
+                    yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
+                    const int b = nx * (yy + ny * zz);
                     cnt = scan_range(a, R, st, s, oi, pi, cell_begin(st, b + xa), st.cend[b + xb], row, cnt,
                                      sub, gshift);
                     if (cx < 2)
@@ -310,30 +315,69 @@ __device__ __forceinline__ T wg_sum(T v, T* smem, WaveFn wf) {
     return r;
 }
 
+// The step's plan (one workgroup of 1024 threads): each replica's rebuild
+// decision from its pieces' displacement top-2 (always in mode 0), then the
+// launch queue of the force kernel -- the pieces of rebuilding replicas first,
+// each part in the static longest-first order, so the long rebuild pieces
+// start first.
+__global__ __launch_bounds__(1024) void k_batch_plan(BatchArgs a) {
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int r = tid; r < a.nrep; r += bd) {
+        int rb = 1;
+        if (a.mode == 1) {
+            const int p0 = a.reps[r].piece0, np = a.reps[r].npiece;
+            double b1 = 0.0, b2 = 0.0;
+            for (int k = 0; k < np; ++k) {
+                const double2 t = a.ptop[p0 + k];
+                top2_merge(b1, b2, t.x, t.y);
+            }
+            rb = b1 + b2 > a.skin ? 1 : 0;
+        }
+        a.rflag[r] = rb;
+    }
+    __syncthreads();
+    // stable partition: thread t owns pieces [t*per, (t+1)*per)
+    const int per = (a.npieces + bd - 1) / bd;
+    const int q0 = min(tid * per, a.npieces), q1 = min(q0 + per, a.npieces);
+    int c1 = 0;
+    for (int q = q0; q < q1; ++q) c1 += a.rflag[a.pieces[q].x];
+    // exclusive block scan of c1
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    int x = c1;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) wsum[wid] = x;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < static_cast<int>(bd / kWave); ++w) {
+        before += w < wid ? wsum[w] : 0;
+        total += wsum[w];
+    }
+    int p1 = before + x - c1;          // next slot for a rebuilding piece
+    int p0 = total + (q0 - (p1));      // next slot for the others
+    for (int q = q0; q < q1; ++q) {
+        const int4 pc = a.pieces[q];
+        if (a.rflag[pc.x]) a.queue[p1++] = pc;
+        else a.queue[p0++] = pc;
+    }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(1024) void k_batch_force(BatchArgs a) {
     __shared__ double red_d[16];
     __shared__ long long red_l[16];
     __shared__ int s_rebuild;
-    const int4 pc = a.pieces[blockIdx.x];
+    const int4 pc = a.queue[blockIdx.x];
     const BatchRep R = a.reps[pc.x];
     const int n = R.n, s0 = pc.y, s1 = pc.z;
     const long off = R.off;
     const int tid = threadIdx.x, bd = blockDim.x;
     const Stage st = stage_of(a);
     const double h = 0.5 * a.dt;
-    if (tid == 0) {
-        int rb = 1;
-        if (a.mode == 1) {
-            double b1 = 0.0, b2 = 0.0;
-            for (int k = 0; k < R.npiece; ++k) {
-                const double2 t = a.ptop[R.piece0 + k];
-                top2_merge(b1, b2, t.x, t.y);
-            }
-            rb = b1 + b2 > a.skin ? 1 : 0;
-        }
-        s_rebuild = rb;
-    }
+    if (tid == 0) s_rebuild = a.rflag[pc.x];
     __syncthreads();
 
     // ---- 1. stage the replica in slot order; rebuild the piece's rows ------
@@ -535,6 +579,7 @@ size_t batch_lds_bytes(int nmax, int maxcell) {
 hipError_t launch_batch(const BatchArgs& a, int block, hipStream_t s) {
     const size_t shmem = batch_lds_bytes(a.nmax, a.maxcell);
     if (a.mode == 1) hipLaunchKernelGGL(k_batch_pre, dim3(a.npieces), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(1024), 0, s, a);
     if (a.mixed)
         hipLaunchKernelGGL(k_batch_force<true>, dim3(a.npieces), dim3(block), shmem, s, a);
     else

@@ -36,7 +36,9 @@ struct BatchRep {
 // numbering of the per-piece partial arrays).
 struct BatchArgs {
     const BatchRep* reps;
-    const int4* pieces;  // launch order (longest first)
+    const int4* pieces;  // static order (longest first)
+    int4* queue;         // this step's launch order of the force kernel
+    int* rflag;          // per replica: rebuild this step
     int nrep, npieces;
     int nmax;        // largest replica (LDS staging size)
     int maxcell;     // largest usable cell grid (LDS cell tables), 0: every rebuild all-pairs

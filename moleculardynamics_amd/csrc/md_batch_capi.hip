@@ -38,13 +38,15 @@ struct md_batch {
     uint32_t* nbr = nullptr;
     int* ncnt = nullptr;
     int cap = 0;
-    // pieces: per replica the density (work estimate), the largest allowed
-    // piece count and the default one; the launch-order table on the device
+    // pieces: per replica the density (work estimate) and the largest
+    // allowed piece count; the launch-order table on the device
     std::vector<double> rho;
-    std::vector<int> max_pieces, def_pieces;
+    std::vector<int> max_pieces;
     int piece_slots = 0;
     int npieces = 0;
     int4* d_pieces = nullptr;
+    int4* queue = nullptr;
+    int* rflag = nullptr;
     double2* ptop = nullptr;
     long long* ppart = nullptr;
     unsigned long long* step = nullptr;  // [0] step counter
@@ -56,6 +58,10 @@ struct md_batch {
 };
 
 namespace {
+
+// defaults before tuning: lanes per atom of the force rows, atoms per piece
+constexpr int kDefaultTpa = 8;
+constexpr int kDefaultPer = 384;
 
 md_status bfail(md_batch* b, md_status s, const std::string& m) {
     if (b) b->err = m;
@@ -81,6 +87,8 @@ mdk::BatchArgs args_of(md_batch* b, int mode, double* thermo, long thermo_steps)
     mdk::BatchArgs a{};
     a.reps = b->d_reps;
     a.pieces = b->d_pieces;
+    a.queue = b->queue;
+    a.rflag = b->rflag;
     a.nrep = b->nrep;
     a.npieces = b->npieces;
     a.nmax = b->nmax;
@@ -231,7 +239,7 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
         mdk::BatchRep& R = b->reps[r];
         R.off = total;
         R.n = static_cast<int>(n[r]);
-        R.tpa = 4;
+        R.tpa = kDefaultTpa;
         R.seed_lo = static_cast<uint32_t>(seeds[r]);
         R.seed_hi = static_cast<uint32_t>(seeds[r] >> 32);
         const double Lr[3] = {L[3 * r], L[3 * r + 1], L[3 * r + 2]};
@@ -271,22 +279,15 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
         return bfail(b, MD_ERR_INVALID_ARGUMENT,
                      "largest replica does not fit the workgroup's LDS (" + std::to_string(maxsh) + " B)");
     BHIP(b, mdk::batch_allow_lds(lds));
-    // pieces: the default count balances atoms x density over two workgroups
-    // per CU; a replica has at most 32 pieces of at least 32 atoms
-    int ncu = 0;
-    BHIP(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    double work = 0.0;
-    for (int r = 0; r < nrep; ++r) work += n[r] * b->rho[r];
-    const double target = work / (2.0 * std::max(1, ncu));
+    // pieces: by default kDefaultPer atoms each; a replica has at most 32
+    // pieces of at least 32 atoms
     b->max_pieces.resize(static_cast<size_t>(nrep));
-    b->def_pieces.resize(static_cast<size_t>(nrep));
     b->piece_slots = 0;
     for (int r = 0; r < nrep; ++r) {
         const int mp = static_cast<int>(std::max<long>(1, std::min<long>(32, n[r] / 32)));
-        const int dp = static_cast<int>(std::ceil(n[r] * b->rho[r] / target));
+        const int dp = static_cast<int>((n[r] + kDefaultPer - 1) / kDefaultPer);
         b->max_pieces[r] = mp;
-        b->def_pieces[r] = std::max(1, std::min(mp, dp));
-        b->reps[r].npiece = b->def_pieces[r];
+        b->reps[r].npiece = std::max(1, std::min(mp, dp));
         b->piece_slots += mp;
     }
     BHIP(b, dalloc(b->d_reps, nrep));
@@ -296,6 +297,7 @@ md_status md_batch_create(void* hip_stream, int nrep, const long* n, const doubl
     BHIP(b, dalloc(b->nbr, total * b->cap)); BHIP(b, dalloc(b->ncnt, total));
     BHIP(b, dalloc(b->step, 2)); BHIP(b, dalloc(b->builds, nrep)); BHIP(b, dalloc(b->flags, 2));
     BHIP(b, dalloc(b->d_pieces, b->piece_slots)); BHIP(b, dalloc(b->ptop, b->piece_slots));
+    BHIP(b, dalloc(b->queue, b->piece_slots)); BHIP(b, dalloc(b->rflag, nrep));
     BHIP(b, dalloc(b->ppart, 3L * b->piece_slots));
     BHIP(b, hipMemset(b->step, 0, 2 * sizeof(unsigned long long)));
     BHIP(b, hipMemset(b->builds, 0, nrep * sizeof(long long)));
@@ -329,7 +331,8 @@ md_status md_batch_destroy(md_batch* b) {
                     static_cast<void*>(b->builds), static_cast<void*>(b->flags),
                     static_cast<void*>(b->thermo), static_cast<void*>(b->slot_of),
                     static_cast<void*>(b->orig), static_cast<void*>(b->d_pieces),
-                    static_cast<void*>(b->ptop), static_cast<void*>(b->ppart)})
+                    static_cast<void*>(b->ptop), static_cast<void*>(b->ppart),
+                    static_cast<void*>(b->queue), static_cast<void*>(b->rflag)})
         if (p) (void)hipFree(p);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -461,12 +464,15 @@ md_status md_batch_rebuild_counts(md_batch* b, long* counts) {
 }
 
 // Autotuner: for each workgroup size in {256, 512, 1024}, coordinate descent
-// over each replica size class (largest first): the class's piece counts as
-// {1/4, 1/2, 1, 2, 4} times the work-balanced default, then its lanes per atom
-// in {1, 2, 4, 8, 16}; each candidate is timed over `trial_steps` steps from
-// the same saved state with HIP events. The fastest (block, pieces, lanes) is
-// applied and the state restored, so tuning leaves the trajectory untouched
-// (fixed-point forces and sums do not depend on the split).
+// over each replica size class (largest first): the class's atoms per piece
+// in {128, 192, 256, 384, 512, 768, 1024, whole replica}, then its lanes per
+// atom in {1, 2, 4, 8, 16}; each candidate is timed over `trial_steps` steps
+// from the same saved state with HIP events. The fastest (block, pieces,
+// lanes) is applied and the state restored, so tuning leaves the trajectory
+// untouched (fixed-point forces and sums do not depend on the split). The
+// trials should span list rebuilds: a rebuilding piece costs several force
+// pieces, so tune from a state some steps after md_batch_prime, over a few
+// tens of steps.
 md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* pieces_out,
                             int* block_out, double* best_ms_per_step) {
     if (b == nullptr) return MD_ERR_INVALID_ARGUMENT;
@@ -545,41 +551,44 @@ md_status md_batch_autotune(md_batch* b, int trial_steps, int* tpa_out, int* pie
         *ms = f / trial_steps;
         return MD_OK;
     };
-    auto scaled = [&](int r, double m) {
-        const int v = static_cast<int>(std::lround(b->def_pieces[r] * m));
+    // pieces of a replica for a target of `per` atoms per piece (0: whole)
+    auto split = [&](int r, int per) {
+        if (per <= 0) return 1;
+        const int v = (b->reps[r].n + per - 1) / per;
         return std::max(1, std::min(b->max_pieces[r], v));
     };
     double best = 1e30;
     int best_block = b->block;
-    std::vector<int> best_tpa(static_cast<size_t>(b->nrep), 4), best_np(static_cast<size_t>(b->nrep), 1);
+    std::vector<int> best_tpa(static_cast<size_t>(b->nrep), kDefaultTpa),
+        best_np(static_cast<size_t>(b->nrep), 1);
     const int lanes[5] = {1, 2, 4, 8, 16};
-    const double mult[5] = {0.25, 0.5, 1.0, 2.0, 4.0};
+    const int per_opts[8] = {kDefaultPer, 128, 192, 256, 512, 768, 1024, 0};
     for (int block : {256, 512, 1024}) {
         b->block = block;
         for (int r = 0; r < b->nrep; ++r) {
-            b->reps[r].tpa = 4;
-            b->reps[r].npiece = b->def_pieces[r];
+            b->reps[r].tpa = kDefaultTpa;
+            b->reps[r].npiece = split(r, kDefaultPer);
         }
         double cur = 0.0;
         md_status st = trial(&cur);
         if (st != MD_OK) return st;
         for (auto& kv : classes) {
-            double keep_m = 1.0;
-            for (double m : mult) {
-                if (m == 1.0) continue;
-                for (int r : kv.second) b->reps[r].npiece = scaled(r, m);
+            int keep_p = kDefaultPer;
+            for (int per : per_opts) {
+                if (per == kDefaultPer) continue;
+                for (int r : kv.second) b->reps[r].npiece = split(r, per);
                 double ms = 0.0;
                 st = trial(&ms);
                 if (st != MD_OK) return st;
                 if (ms < cur) {
                     cur = ms;
-                    keep_m = m;
+                    keep_p = per;
                 }
             }
-            for (int r : kv.second) b->reps[r].npiece = scaled(r, keep_m);
-            int keep_t = 4;
+            for (int r : kv.second) b->reps[r].npiece = split(r, keep_p);
+            int keep_t = kDefaultTpa;
             for (int t : lanes) {
-                if (t == 4) continue;
+                if (t == kDefaultTpa) continue;
                 for (int r : kv.second) b->reps[r].tpa = t;
                 double ms = 0.0;
                 st = trial(&ms);
