@@ -203,12 +203,13 @@ def bench_batch(args) -> int:
     b = native.Batch(systems, T, seeds, precision=native.MIXED)
     b.prime()
     # tune from a state whose list ages are spread, over trials that span rebuilds
-    b.run(200, segment=100)
+    graph = not args.no_graph
+    b.run(200, segment=100, graph=graph)
     tpa, pieces, block, _ = b.autotune(trial_steps=30)
-    b.run(max(args.warmup, 100), segment=100)
+    b.run(max(args.warmup, 100), segment=100, graph=graph)
     steps = 10000
     t0 = time.perf_counter()
-    b.run(steps, segment=100)
+    b.run(steps, segment=100, graph=graph)
     wall = time.perf_counter() - t0
     b.close()
     batch_rate = 256 * steps / wall  # replica-steps per second
@@ -240,7 +241,8 @@ def bench_batch(args) -> int:
                  "replica-steps/s", steps, 100, 1e3 * wall / steps,
                  "f32 pair / i64 fixed-point accumulate / f64 state",
                  "256 LJ replicas of 108-4000 atoms, rho* U[0.5,0.9], T* U[0.7,1.5], rc=2.5, "
-                 "BAOAB gamma=1 dt*=0.005, 10000 steps in 100-step graph segments", extra)
+                 "BAOAB gamma=1 dt*=0.005, 10000 steps in "
+                 + ("100-step graph segments" if graph else "plain launches"), extra)
 
 
 def bench_water(args) -> int:
@@ -332,6 +334,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch workload: plain launches instead of HIP-graph segments "
+                         "(for kernel traces)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--workload", default="lj1m",
                     choices=["lj1m", "lj864", "water", "batch", "mixture"],
