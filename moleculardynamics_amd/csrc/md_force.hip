@@ -261,23 +261,51 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
 }
 
 // Biased fixed-point accumulators (fix_biased_scaled / fix_value): n terms
-// each; big is the largest |term| seen (in units of 2^-32), checked against
-// kFixLimitScaled at the end.
+// each; r2min is the smallest r2 of an accumulated term, from which the end
+// of the row bounds every term's magnitude (fix_may_saturate).
 struct AccFix {
     unsigned long long fx = 0, fy = 0, fz = 0, e = 0, w = 0;
-    unsigned int n = 0;  // terms in this row (at most the row capacity)
-    float big = 0.0f;
+    unsigned int n = 0;  // biases added (at most the row capacity)
+    unsigned int k = 0;  // terms inside the cutoff
+    unsigned int r2min = 0x7f800000u;  // FP32 bits of the smallest r2 of a term
     bool redo = false;  // a fixed-point cutoff decision fell in the guard band
 };
+
+// True when some term of a row whose pairs all have r2 >= r2min could reach
+// the fixed-point limit kFixLimitScaled (in units of 2^-32). With s6 =
+// (sigma^2 / r2)^3 falling in r2, every term is bounded at r2min:
+// |u| <= e4s (s6^2 + s6) [+ |u_rcs| + rc |dudr_rcs|],
+// |W| <= e24s (2 s6^2 + s6) [+ rc |dudr_rcs|], |f_x| <= |W| / r.
+// The check is conservative (bounds, 1e-6 relative margin for the FP32
+// rounding of the terms) and costs one minimum per pair.
+__device__ __forceinline__ bool fix_may_saturate(const LJConst& k, unsigned int r2min_bits) {
+    const float r2min = __uint_as_float(r2min_bits);
+    if (!(r2min < __builtin_huge_valf())) return false;
+    const double r2 = r2min;
+    const double s2 = static_cast<double>(k.sig2f) / r2;
+    const double s6 = s2 * s2 * s2;
+    double ub = fabs(static_cast<double>(k.e4s)) * (s6 * s6 + s6);
+    double wb = fabs(static_cast<double>(k.e24s)) * (2.0 * s6 * s6 + s6);
+    if (k.lfs) {
+        const double sh = static_cast<double>(k.rcf) * fabs(static_cast<double>(k.dudr_rcs));
+        ub += fabs(static_cast<double>(k.u_rcs)) + sh;
+        wb += sh;
+    }
+    const double tb = wb / sqrt(r2);
+    return fmax(ub, fmax(wb, tb)) * (1.0 + 1e-6) >= static_cast<double>(kFixLimitScaled);
+}
 
 // FP32 terms of a pair inside the cutoff, accumulated in fixed point. The
 // energy and force coefficients carry the fixed-point scale 2^32 (e4s, e24s,
 // u_rcs, dudr_rcs), so every FP32 term comes out multiplied by 2^32 exactly:
 // a power-of-two factor commutes with each rounding. The term's fixed-point
 // value round(v 2^32) is then one FP64 add of the bias.
+// IN: the pair is inside the cutoff. acc32 is also called branch-free with
+// IN false for a pair outside: its terms are then zeroed, adding only the
+// bias (and not counted), so a row's lanes need no divergent update.
 template <bool LFS>
 __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx, double dy,
-                                      double dz, AccFix& q) {
+                                      double dz, AccFix& q, bool in = true) {
     const float r2 = static_cast<float>(r2d);
     float inv = __builtin_amdgcn_rcpf(r2);
     inv = __builtin_fmaf(inv, __builtin_fmaf(-r2, inv, 1.0f), inv);
@@ -290,6 +318,8 @@ __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx,
         u = u - a.lj.u_rcs - (r - a.lj.rcf) * a.lj.dudr_rcs;
         ww = ww + r * a.lj.dudr_rcs;
     }
+    u = in ? u : 0.0f;
+    ww = in ? ww : 0.0f;
     const float c = ww * inv;
     const float tx = c * static_cast<float>(dx);
     const float ty = c * static_cast<float>(dy);
@@ -299,9 +329,10 @@ __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx,
     q.fx += fix_biased_scaled(tx);
     q.fy += fix_biased_scaled(ty);
     q.fz += fix_biased_scaled(tz);
-    q.n += 1;
-    q.big = fmaxf(q.big, fmaxf(fmaxf(fabsf(u), fabsf(ww)),
-                               fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz)))));
+    q.n += 1;  // every call adds one bias
+    q.k += in ? 1u : 0u;
+    // r2 > 0, so its bits order as unsigned integers
+    q.r2min = min(q.r2min, in ? __float_as_uint(r2) : 0x7f800000u);
 }
 
 template <bool LFS>
@@ -331,12 +362,11 @@ __device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, cons
     double dx, dy, dz;
     qdelta(a.quant, qi, qj, dx, dy, dz);
     const double r2d = dx * dx + dy * dy + dz * dz;
-    if (!(r2d < a.lj.rc2 * (1.0 + a.quant.band_c))) return;
-    if (!(r2d < a.lj.rc2 * (1.0 - a.quant.band_c))) {
-        q.redo = true;
-        return;
-    }
-    acc32<LFS>(a, r2d, dx, dy, dz, q);
+    const bool in = r2d < a.lj.rc2 * (1.0 - a.quant.band_c);
+    // no branch per pair: the band test only sets the row's flag, and a pair
+    // outside the cutoff adds zero terms
+    q.redo = q.redo || (!in && r2d < a.lj.rc2 * (1.0 + a.quant.band_c));
+    acc32<LFS>(a, r2d, dx, dy, dz, q, in);
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -460,13 +490,13 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             } else {
                 walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
             }
-            if (!(q.big < kFixLimitScaled)) atomicOr(&a.sc->fix_saturated, 1);
+            if (fix_may_saturate(a.lj, q.r2min)) atomicOr(&a.sc->fix_saturated, 1);
             fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
             fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
             fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
             ef = fix_value(q.e, q.n);
             wf = fix_value(q.w, q.n);
-            nt = q.n;
+            nt = q.k;
         }
         a.frc[t] = make_double4(fx, fy, fz, 0.0);
         if (KICK) {
