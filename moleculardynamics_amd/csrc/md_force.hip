@@ -268,7 +268,9 @@ struct AccFix {
     unsigned int n = 0;  // biases added (at most the row capacity)
     unsigned int k = 0;  // terms inside the cutoff
     unsigned int r2min = 0x7f800000u;  // FP32 bits of the smallest r2 of a term
-    bool redo = false;  // a fixed-point cutoff decision fell in the guard band
+    // pairs below the upper edge of the guard band (fixed-point walk); a row
+    // with kb != k had a cutoff decision inside the band
+    unsigned int kb = 0;
 };
 
 // True when some term of a row whose pairs all have r2 >= r2min could reach
@@ -310,16 +312,17 @@ __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx,
     float inv = __builtin_amdgcn_rcpf(r2);
     inv = __builtin_fmaf(inv, __builtin_fmaf(-r2, inv, 1.0f), inv);
     const float s2 = a.lj.sig2f * inv;
-    const float s6 = (s2 * s2) * s2;
+    float s6 = (s2 * s2) * s2;
+    if (!LFS) s6 = in ? s6 : 0.0f;  // zero s6 zeroes every Truncated term
     float u = a.lj.e4s * __builtin_fmaf(s6, s6, -s6);
     float ww = a.lj.e24s * __builtin_fmaf(2.0f * s6, s6, -s6);
     if (LFS) {
         const float r = sqrtf(r2);
         u = u - a.lj.u_rcs - (r - a.lj.rcf) * a.lj.dudr_rcs;
         ww = ww + r * a.lj.dudr_rcs;
+        u = in ? u : 0.0f;
+        ww = in ? ww : 0.0f;
     }
-    u = in ? u : 0.0f;
-    ww = in ? ww : 0.0f;
     const float c = ww * inv;
     const float tx = c * static_cast<float>(dx);
     const float ty = c * static_cast<float>(dy);
@@ -331,8 +334,10 @@ __device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx,
     q.fz += fix_biased_scaled(tz);
     q.n += 1;  // every call adds one bias
     q.k += in ? 1u : 0u;
-    // r2 > 0, so its bits order as unsigned integers
-    q.r2min = min(q.r2min, in ? __float_as_uint(r2) : 0x7f800000u);
+    // r2 > 0, so its bits order as unsigned integers. A pair outside the
+    // cutoff has r2 above every pair inside, so it changes r2min only in a
+    // row without terms, where the bound stays far below the limit.
+    q.r2min = min(q.r2min, __float_as_uint(r2));
 }
 
 template <bool LFS>
@@ -363,9 +368,9 @@ __device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, cons
     qdelta(a.quant, qi, qj, dx, dy, dz);
     const double r2d = dx * dx + dy * dy + dz * dz;
     const bool in = r2d < a.lj.rc2 * (1.0 - a.quant.band_c);
-    // no branch per pair: the band test only sets the row's flag, and a pair
-    // outside the cutoff adds zero terms
-    q.redo = q.redo || (!in && r2d < a.lj.rc2 * (1.0 + a.quant.band_c));
+    // no branch per pair: the band test only counts, and a pair outside the
+    // cutoff adds zero terms
+    q.kb += r2d < a.lj.rc2 * (1.0 + a.quant.band_c) ? 1u : 0u;
     acc32<LFS>(a, r2d, dx, dy, dz, q, in);
 }
 
@@ -483,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             AccFix q;
             if (QP) {
                 walk_row_fixed<LFS>(a, t, cnt, a.qpos[t], q);
-                if (q.redo) {
+                if (q.kb != q.k) {
                     q = AccFix{};
                     walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
                 }
