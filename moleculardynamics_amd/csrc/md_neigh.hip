@@ -235,8 +235,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a, long ncell
 // Q: the candidates are read as fixed-point positions (Quant): one 16-byte
 // load carries a candidate's position and upload index, and the fixed-point
 // r2 decides every candidate outside a band of relative half-width band_l
-// around rl2; inside the band the exact FP64 test above decides, from the
-// FP64 positions. The pair set is therefore the same. CUBIC: the box has one
+// around rl2. A candidate inside the band is only counted; a row with such a
+// candidate (rare) is then built again with the exact FP64 test above, from
+// the FP64 positions. The pair set is therefore the same. CUBIC: the box has one
 // unit on every axis, so r2 is compared in units squared (the integer
 // differences converted exactly, no scaling per axis).
 template <bool Q, bool CUBIC>
@@ -300,6 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
         return norm2_ref(dx, dy, dz) < rl2;
     };
     const double u2 = CUBIC ? a.quant.unit[0] * a.quant.unit[0] : 1.0;
+    unsigned int hits = 0;  // fixed-point decisions inside the band
     const double lo = rl2 * (1.0 - a.quant.band_l) / u2, hi = rl2 * (1.0 + a.quant.band_l) / u2;
     auto visit_q = [&](int j, const int4& qj) {
         double r2;
@@ -314,13 +316,14 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
             r2 = dx * dx + dy * dy + dz * dz;
         }
         const bool flip = oi > qj.w;
-        bool keep = r2 < lo;
-        if (!keep && r2 < hi) keep = exact(pos2[j], flip);  // inside the band: the exact test
-        put(j, keep, flip);
+        // no branch per candidate: a decision inside the band is only counted,
+        // and the row is then built again with the exact test
+        hits += (r2 >= lo && r2 < hi) ? 1u : 0u;
+        put(j, r2 < lo, flip);
     };
-    auto range = [&](int b, int e) {
+    auto range = [&](int b, int e, bool fixed) {
         int j = b;
-        if (Q) {
+        if (Q && fixed) {
             for (; j + 4 <= e; j += 4) {
                 const int4 q0 = a.qpos2[j], q1 = a.qpos2[j + 1], q2 = a.qpos2[j + 2],
                            q3 = a.qpos2[j + 3];
@@ -351,29 +354,37 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
             }
         }
     };
-    if (g.usable) {
-        const int nx = g.nc[0];
-        const int cx = ci % nx;
-        const int cy = (ci / nx) % g.nc[1];
-        const int cz = ci / (nx * g.nc[1]);
-        const bool inner = cx >= 1 && cx + 1 < nx;
-        for (int oz = -1; oz <= 1; ++oz) {
-            const int zz = (cz + oz + g.nc[2]) % g.nc[2];
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                const int rb = nx * (yy + g.nc[1] * zz);
-                if (inner) {
-                    range(cell_start[rb + cx - 1], cell_start[rb + cx + 2]);
-                } else {
-                    for (int ox = -1; ox <= 1; ++ox) {
-                        const int c = rb + (cx + ox + nx) % nx;
-                        range(cell_start[c], cell_start[c + 1]);
+    auto stencil = [&](bool fixed) {
+        if (g.usable) {
+            const int nx = g.nc[0];
+            const int cx = ci % nx;
+            const int cy = (ci / nx) % g.nc[1];
+            const int cz = ci / (nx * g.nc[1]);
+            const bool inner = cx >= 1 && cx + 1 < nx;
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int zz = (cz + oz + g.nc[2]) % g.nc[2];
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int yy = (cy + oy + g.nc[1]) % g.nc[1];
+                    const int rb = nx * (yy + g.nc[1] * zz);
+                    if (inner) {
+                        range(cell_start[rb + cx - 1], cell_start[rb + cx + 2], fixed);
+                    } else {
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const int c = rb + (cx + ox + nx) % nx;
+                            range(cell_start[c], cell_start[c + 1], fixed);
+                        }
                     }
                 }
             }
+        } else {
+            range(0, static_cast<int>(n), fixed);
         }
-    } else {
-        range(0, static_cast<int>(n));
+    };
+    stencil(Q);
+    if (Q && hits != 0) {  // rare: a fixed-point r2 inside the band, the row again exactly
+        cnt = 0;
+        done = 0;
+        stencil(false);
     }
     if (cnt > done) write_octet();  // the last, partial octet (slots past the row are not read)
     a.ncnt[t] = cnt;
