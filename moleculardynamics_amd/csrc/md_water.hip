@@ -34,7 +34,9 @@ namespace mdk {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kSiteTile = 16;  // sites per LDS tile of the S(k) kernel
+constexpr int kSiteTile = 32;  // sites per LDS tile of the S(k) kernel
+constexpr int kSegLen = 8;     // k-vectors per column segment of the S(k) kernel
+constexpr int kSkSub = 8;      // lanes per segment of the S(k) kernel (sites split among them)
 constexpr int kWaterTpa = 16;  // lanes per site of the real-space kernel
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -237,19 +239,29 @@ __device__ __forceinline__ double2 phase(const double2* t, int nn) {
     return nn < 0 ? make_double2(v.x, -v.y) : v;
 }
 
-// Partial structure factors: block (k-chunk, site-tile-group). Each thread
-// owns one k-vector; sites are staged in LDS (their q and the three
-// per-axis tables) kSiteTile at a time.
+// Partial structure factors: block (segment chunk, site group). A segment is
+// up to kSegLen k-vectors of one column (same nx, ny; nz rising by one).
+// kSkSub adjacent lanes own a segment and take every kSkSub-th site of the
+// group; per site one phase product e^{i(k_x x + k_y y + k_z0 z)} from the
+// per-axis tables, then e^{i k0z z} steps along the segment. Sites are staged
+// in LDS (their q and the three per-axis tables) kSiteTile at a time; the
+// lanes of a segment are combined by a fixed shuffle tree at the end.
 __global__ __launch_bounds__(kBlock) void k_sk(WaterArgs a, int sites_per_group) {
     __shared__ double2 tab[kSiteTile][3][kMaxK + 1];
     __shared__ double qs[kSiteTile];
-    const int kv = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = kv < a.nk;
-    int3 nv = make_int3(0, 0, 0);
-    if (valid) nv = a.kvec[kv];
+    const int gt = blockIdx.x * kBlock + threadIdx.x;
+    const int sg = gt / kSkSub, sub = gt % kSkSub;
+    const bool valid = sg < a.nseg;
+    int4 sv = make_int4(0, 0, 0, 0);
+    if (valid) sv = a.kseg[sg];
     const long s0 = static_cast<long>(blockIdx.y) * sites_per_group;
     const long s1 = min(s0 + sites_per_group, a.n);
-    double re = 0.0, im = 0.0;
+    double re[kSegLen], im[kSegLen];
+#pragma unroll
+    for (int m = 0; m < kSegLen; ++m) {
+        re[m] = 0.0;
+        im[m] = 0.0;
+    }
     for (long t0 = s0; t0 < s1; t0 += kSiteTile) {
         const int cnt = static_cast<int>(min(static_cast<long>(kSiteTile), s1 - t0));
         __syncthreads();
@@ -262,15 +274,34 @@ __global__ __launch_bounds__(kBlock) void k_sk(WaterArgs a, int sites_per_group)
         for (int s = threadIdx.x; s < cnt; s += kBlock) qs[s] = a.pos[t0 + s].w;
         __syncthreads();
         if (valid) {
-            for (int s = 0; s < cnt; ++s) {
-                const double2 e = cmul(cmul(phase(tab[s][0], nv.x), phase(tab[s][1], nv.y)),
-                                       phase(tab[s][2], nv.z));
-                re += qs[s] * e.x;
-                im += qs[s] * e.y;
+            for (int s = sub; s < cnt; s += kSkSub) {
+                double2 e = cmul(cmul(phase(tab[s][0], sv.x), phase(tab[s][1], sv.y)),
+                                 phase(tab[s][2], sv.z));
+                const double2 step = tab[s][2][1];
+                const double q = qs[s];
+#pragma unroll
+                for (int m = 0; m < kSegLen; ++m) {
+                    re[m] += q * e.x;
+                    im[m] += q * e.y;
+                    e = cmul(e, step);
+                }
             }
         }
     }
-    if (valid) a.skpart[static_cast<long>(blockIdx.y) * a.nk + kv] = make_double2(re, im);
+#pragma unroll
+    for (int o = 1; o < kSkSub; o <<= 1) {
+#pragma unroll
+        for (int m = 0; m < kSegLen; ++m) {
+            re[m] += __shfl_xor(re[m], o, kWave);
+            im[m] += __shfl_xor(im[m], o, kWave);
+        }
+    }
+    if (valid && sub == 0) {
+        double2* out = a.skpart + static_cast<long>(blockIdx.y) * a.nk + a.kseg0[sg];
+#pragma unroll
+        for (int m = 0; m < kSegLen; ++m)
+            if (m < sv.w) out[m] = make_double2(re[m], im[m]);
+    }
 }
 
 // S(k) = sum of the site-group partials in order; E_k term and the force
@@ -598,7 +629,8 @@ void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_weik, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
     const int groups = water_site_groups(a.n);
     const int spg = static_cast<int>((a.n + groups - 1) / groups);
-    hipLaunchKernelGGL(k_sk, dim3((a.nk + kBlock - 1) / kBlock, groups), dim3(kBlock), 0, s, a, spg);
+    hipLaunchKernelGGL(k_sk, dim3((a.nseg * kSkSub + kBlock - 1) / kBlock, groups), dim3(kBlock), 0, s, a,
+                       spg);
     hipLaunchKernelGGL(k_sk_reduce, dim3((a.nk + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, groups);
     hipLaunchKernelGGL(k_recip, dim3(gridn(a.n), a.nchunk), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_recip_reduce, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);

@@ -39,6 +39,11 @@ struct WaterArgs {
     const int4* kcol;
     const int* kcol0;
     int ncol, nchunk;            // columns; column chunks of the reciprocal force kernel
+    // the columns cut into segments of at most 8 k-vectors: (nx, ny, first nz,
+    // count) and the index of the segment's first k-vector
+    const int4* kseg;
+    const int* kseg0;
+    int nseg;
     double2* skf;                // [nk] 4 factor(k) S(k)
     double4* fpart;              // [nchunk][n] reciprocal force partials
     // device-driven runs: SHAKE / RATTLE failure codes (atomic minimum of the

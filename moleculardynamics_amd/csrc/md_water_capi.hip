@@ -49,6 +49,9 @@ struct md_water {
     double2 *eik = nullptr, *skpart = nullptr, *sk = nullptr, *skf = nullptr;
     int4* kcol = nullptr;
     int* kcol0 = nullptr;
+    int4* kseg = nullptr;
+    int* kseg0 = nullptr;
+    int nseg = 0;
     double4* fpart = nullptr;
     long builds = 0;
     bool list_built = false;
@@ -352,6 +355,15 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     }
     const int ncol = static_cast<int>(cols.size());
     const int nchunk = mdk::water_recip_chunks(n, std::max(1, ncol));
+    // segments of at most 8 k-vectors per column (the S(k) kernel's unit)
+    std::vector<int4> segs;
+    std::vector<int> seg0;
+    for (int c = 0; c < ncol; ++c)
+        for (int m = 0; m < cols[c].w; m += 8) {
+            segs.push_back(make_int4(cols[c].x, cols[c].y, cols[c].z + m, std::min(8, cols[c].w - m)));
+            seg0.push_back(col0[c] + m);
+        }
+    w->nseg = static_cast<int>(segs.size());
     WHIP(w, walloc(w->pos, n)); WHIP(w, walloc(w->vel, n)); WHIP(w, walloc(w->frc, n));
     WHIP(w, walloc(w->ref, n)); WHIP(w, walloc(w->start, n)); WHIP(w, walloc(w->freep, n));
     const long npad = (n + 63) / 64 * 64;
@@ -367,6 +379,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     WHIP(w, walloc(w->skf, std::max(1, nk)));
     WHIP(w, walloc(w->kcol, std::max(1, ncol)));
     WHIP(w, walloc(w->kcol0, std::max(1, ncol)));
+    WHIP(w, walloc(w->kseg, std::max(1, w->nseg)));
+    WHIP(w, walloc(w->kseg0, std::max(1, w->nseg)));
     WHIP(w, walloc(w->fpart, static_cast<long>(nchunk) * n));
     WHIP(w, hipMemset(w->flags, 0, 4 * sizeof(int)));
     WHIP(w, walloc(w->eterm, 6));
@@ -376,6 +390,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
         WHIP(w, hipMemcpy(w->kfac, kf.data(), nk * sizeof(double), hipMemcpyHostToDevice));
         WHIP(w, hipMemcpy(w->kcol, cols.data(), ncol * sizeof(int4), hipMemcpyHostToDevice));
         WHIP(w, hipMemcpy(w->kcol0, col0.data(), ncol * sizeof(int), hipMemcpyHostToDevice));
+        WHIP(w, hipMemcpy(w->kseg, segs.data(), w->nseg * sizeof(int4), hipMemcpyHostToDevice));
+        WHIP(w, hipMemcpy(w->kseg0, seg0.data(), w->nseg * sizeof(int), hipMemcpyHostToDevice));
     }
     std::vector<double4> hp(static_cast<size_t>(n)), hv(static_cast<size_t>(n)),
         hf(static_cast<size_t>(n), make_double4(0, 0, 0, 0));
@@ -406,6 +422,7 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     for (int d = 0; d < 3; ++d) a.k0[d] = k0[d];
     a.eik = w->eik; a.skpart = w->skpart; a.sk = w->sk; a.skf = w->skf;
     a.kcol = w->kcol; a.kcol0 = w->kcol0; a.ncol = ncol; a.nchunk = nk > 0 ? nchunk : 0;
+    a.kseg = w->kseg; a.kseg0 = w->kseg0; a.nseg = nk > 0 ? w->nseg : 0;
     a.fpart = w->fpart;
     a.overflow = w->flags;
     a.oh = p->r_oh;
@@ -428,6 +445,7 @@ md_status md_water_destroy(md_water* w) {
                     static_cast<void*>(w->part2), static_cast<void*>(w->kvec), static_cast<void*>(w->kfac),
                     static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk),
                     static_cast<void*>(w->skf), static_cast<void*>(w->kcol), static_cast<void*>(w->kcol0),
+                    static_cast<void*>(w->kseg), static_cast<void*>(w->kseg0),
                     static_cast<void*>(w->fpart), static_cast<void*>(w->eterm)})
         if (q) (void)hipFree(q);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
