@@ -58,11 +58,13 @@ __device__ __forceinline__ float erfc_as(float x, float ex2) {
     for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); \
          i += static_cast<long>(gridDim.x) * blockDim.x)
 
-// Full list of intermolecular partners within rc + skin: cells (index array
-// in site order within a cell) when at least 3 per side fit, all sites
-// otherwise. One wave per site tests 64 consecutive candidates at once and
-// keeps them in candidate order by a ballot and a prefix count, so a row is
-// the row of the serial scan (cells in order, sites in index order).
+// Full list of intermolecular partners within rc + skin: from cells of width
+// >= (rc + skin)/2 (index array in site order within a cell), the 5 x 5 runs
+// of five x-adjacent cells around the site's cell, when at least 5 cells fit
+// per side; all sites otherwise. Every pair within rc + skin lies in such a
+// run. One wave per site tests 64 consecutive candidates at once and keeps
+// them in candidate order by a ballot and a prefix count, so a row is the
+// row of the serial scan (runs in order, sites in index order within a cell).
 __global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int* __restrict__ cell_of,
                                                   const int* __restrict__ cell_start,
                                                   const int* __restrict__ cell_atoms) {
@@ -99,16 +101,26 @@ __global__ __launch_bounds__(kBlock) void k_wlist(WaterArgs a, Grid g, const int
             }
         };
         if (g.usable) {
+            const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
             const int c = cell_of[i];
-            const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
-            for (int oz = -1; oz <= 1; ++oz)
-                for (int oy = -1; oy <= 1; ++oy)
-                    for (int ox = -1; ox <= 1; ++ox) {
-                        const int nb = (cx + ox + g.nc[0]) % g.nc[0] +
-                                       g.nc[0] * ((cy + oy + g.nc[1]) % g.nc[1] +
-                                                  g.nc[1] * ((cz + oz + g.nc[2]) % g.nc[2]));
-                        scan(cell_atoms, cell_start[nb], cell_start[nb + 1]);
-                    }
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            for (int oz = -2; oz <= 2; ++oz) {
+                int zz = cz + oz;
+                zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
+                for (int oy = -2; oy <= 2; ++oy) {
+                    int yy = cy + oy;
+                    yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
+                    const int b = nx * (yy + ny * zz);
+                    // five x-adjacent cells are contiguous in the index array,
+                    // split in two where they wrap the x axis
+                    scan(cell_atoms, cell_start[b + xa], cell_start[b + xb + 1]);
+                    if (cx < 2)
+                        scan(cell_atoms, cell_start[b + nx + cx - 2], cell_start[b + nx]);
+                    else if (cx + 2 >= nx)
+                        scan(cell_atoms, cell_start[b], cell_start[b + cx + 3 - nx]);
+                }
+            }
         } else {
             scan(nullptr, 0, static_cast<int>(a.n));
         }
@@ -265,25 +277,34 @@ __global__ __launch_bounds__(kBlock) void k_sk(WaterArgs a, int sites_per_group)
     for (long t0 = s0; t0 < s1; t0 += kSiteTile) {
         const int cnt = static_cast<int>(min(static_cast<long>(kSiteTile), s1 - t0));
         __syncthreads();
-        for (int e = threadIdx.x; e < cnt * 3 * (a.kmax + 1); e += kBlock) {
-            const int s = e / (3 * (a.kmax + 1));
-            const int rem = e % (3 * (a.kmax + 1));
-            const int d = rem / (a.kmax + 1), nn = rem % (a.kmax + 1);
-            tab[s][d][nn] = a.eik[(static_cast<long>(d) * (a.kmax + 1) + nn) * a.n + t0 + s];
+        // thread (d, s) copies one table row; consecutive threads read
+        // consecutive sites of the [axis][n][site] table
+        for (int ds = threadIdx.x; ds < 3 * kSiteTile; ds += kBlock) {
+            const int d = ds / kSiteTile, s = ds % kSiteTile;
+            if (s < cnt) {
+                const double2* src = a.eik + static_cast<long>(d) * (a.kmax + 1) * a.n + t0 + s;
+                for (int nn = 0; nn <= a.kmax; ++nn) tab[s][d][nn] = src[static_cast<long>(nn) * a.n];
+            }
         }
         for (int s = threadIdx.x; s < cnt; s += kBlock) qs[s] = a.pos[t0 + s].w;
         __syncthreads();
         if (valid) {
-            for (int s = sub; s < cnt; s += kSkSub) {
+            // two sites at a time: two independent recursion chains
+            for (int s = sub; s < cnt; s += 2 * kSkSub) {
+                const int s2 = s + kSkSub < cnt ? s + kSkSub : s;
+                const double w2 = s + kSkSub < cnt ? qs[s2] : 0.0;
                 double2 e = cmul(cmul(phase(tab[s][0], sv.x), phase(tab[s][1], sv.y)),
                                  phase(tab[s][2], sv.z));
-                const double2 step = tab[s][2][1];
+                double2 f = cmul(cmul(phase(tab[s2][0], sv.x), phase(tab[s2][1], sv.y)),
+                                 phase(tab[s2][2], sv.z));
+                const double2 step = tab[s][2][1], step2 = tab[s2][2][1];
                 const double q = qs[s];
 #pragma unroll
                 for (int m = 0; m < kSegLen; ++m) {
-                    re[m] += q * e.x;
-                    im[m] += q * e.y;
+                    re[m] += q * e.x + w2 * f.x;
+                    im[m] += q * e.y + w2 * f.y;
                     e = cmul(e, step);
+                    f = cmul(f, step2);
                 }
             }
         }

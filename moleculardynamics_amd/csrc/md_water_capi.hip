@@ -313,10 +313,12 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     w->grid = mdk::Grid{};
     for (int d = 0; d < 3; ++d) {
         w->grid.L[d] = L[d];
-        w->grid.nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / rl)));
+        // cells of at least rl/2 (a 1e-9 relative margin keeps rounding from
+        // putting a pair within rl three cells apart), searched 5 x 5 x 5
+        w->grid.nc[d] = std::max(1, static_cast<int>(std::floor(2.0 * L[d] / (rl * (1.0 + 1e-9)))));
         w->grid.w[d] = L[d] / w->grid.nc[d];
     }
-    w->grid.usable = (w->grid.nc[0] >= 3 && w->grid.nc[1] >= 3 && w->grid.nc[2] >= 3) ? 1 : 0;
+    w->grid.usable = (w->grid.nc[0] >= 5 && w->grid.nc[1] >= 5 && w->grid.nc[2] >= 5) ? 1 : 0;
     const long ncell = static_cast<long>(w->grid.nc[0]) * w->grid.nc[1] * w->grid.nc[2];
     const double rho = n / (L[0] * L[1] * L[2]);
     long cap = static_cast<long>(std::ceil(1.5 * 4.0 / 3.0 * kPi * rl * rl * rl * rho + 64.0));
