@@ -152,6 +152,26 @@ def test_config3_builder_and_ewald_parameters(gpu_ok):
     assert 15 <= kmax <= 24
 
 
+def test_cell_runs_list_matches_oracle(gpu_ok, orc):
+    """A box with six list cells per axis (cells of at least (rc + skin)/2,
+    searched as 5 x 5 runs of five x-adjacent cells, some of them wrapping):
+    forces and energy against the oracle's all-pairs SPC/E + Ewald."""
+    from moleculardynamics_amd import native
+    from oracle.pyoracle import Frame
+
+    L = 49.71853 * 12 / 16  # the config-3 density, 12^3 molecules
+    pos, _ = native.water_lattice(12, L, 77, 300.0)
+    assert np.floor(2 * L / 11.0) == 6
+    fr = Frame(np.array([L, L, L]), pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy())
+    w = native.Water(fr.L, pos, params=native.water_params(cutoff=10.0, kmax=5))
+    _, pe = w.forces()
+    _, _, f = w.download()
+    w.close()
+    f0, pe0 = orc.spce_forces(fr, rc=10.0, kmax=5)
+    assert pe == pytest.approx(pe0, rel=1e-11)
+    assert np.max(np.abs(f - f0)) <= 1e-10 * np.max(np.abs(f0))
+
+
 def test_alpha_independence_of_total(gpu_ok):
     """The converged Ewald total does not depend on alpha (config 1 of NIST,
     two well-converged (alpha, kmax) pairs)."""
