@@ -89,23 +89,30 @@ __device__ __forceinline__ void sigma_terms(const MixTab& T, const PairTerms& p,
 
 // ---- list ----------------------------------------------------------------
 
-// One wave per atom: 64 consecutive candidates are tested at once and kept
-// in candidate order by a ballot and a prefix count, so the row is the row
-// of the serial scan (27 cells in (z, y, x) order, atoms in index order
-// within a cell, or every atom when the grid is unusable).
+constexpr int kListLanes = 8;  // lanes per atom in the list build
+
+// kListLanes lanes per atom test consecutive candidates together and keep
+// them in candidate order by a ballot and a prefix count, so the row is the
+// row of the serial scan: with a usable grid (cells of at least rc + skin
+// over 2, 5 or more per axis) the 5 x 5 runs of five x-adjacent cells around
+// the atom's cell in (z, y) order -- contiguous in the cell-sorted index,
+// split in two where a run wraps the x axis -- atoms in index order within a
+// cell; every atom otherwise. Every pair within rc + skin lies in such a run.
 __global__ __launch_bounds__(kBlock) void k_mix_list(MixListArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const unsigned long long below = (1ull << lane) - 1;
-    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
-    for (long il = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; il < a.n;
-         il += waves) {
+    const int tid = threadIdx.x;
+    const int sub = tid % kListLanes;
+    const int gshift = (tid & (kWave - 1)) & ~(kListLanes - 1);
+    const unsigned below = (1u << sub) - 1u;
+    const long groups = static_cast<long>(gridDim.x) * (kBlock / kListLanes);
+    for (long il = static_cast<long>(blockIdx.x) * (kBlock / kListLanes) + tid / kListLanes; il < a.n;
+         il += groups) {
         const int i = static_cast<int>(il);
         const double4 pi = a.pos[i];
         uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
         int cnt = 0;
         auto scan = [&](const int* idx, int b, int e) {
-            for (int k0 = b; k0 < e; k0 += kWave) {
-                const int k = k0 + lane;
+            for (int k0 = b; k0 < e; k0 += kListLanes) {
+                const int k = k0 + sub;
                 bool keep = false;
                 int j = 0;
                 if (k < e) {
@@ -116,42 +123,39 @@ __global__ __launch_bounds__(kBlock) void k_mix_list(MixListArgs a) {
                         keep = norm2_ref(dx, dy, dz) < a.rl2;
                     }
                 }
-                const unsigned long long m = __ballot(keep);
+                const unsigned gm =
+                    static_cast<unsigned>(__ballot(keep) >> gshift) & ((1u << kListLanes) - 1u);
                 if (keep) {
-                    const int at = cnt + __popcll(m & below);
+                    const int at = cnt + __popc(gm & below);
                     if (at < a.cap) row[at * 64] = static_cast<uint32_t>(j) | (i > j ? kFlipBit : 0u);
                 }
-                cnt += __popcll(m);
+                cnt += __popc(gm);
             }
         };
         const Grid& g = a.grid;
         if (g.usable) {
+            const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
             const int ci = a.cell_of[i];
-            const int cx = ci % g.nc[0];
-            const int cy = (ci / g.nc[0]) % g.nc[1];
-            const int cz = ci / (g.nc[0] * g.nc[1]);
-            // cells x-1, x, x+1 of a (y, z) row are contiguous in the cell-sorted
-            // index unless the row wraps at x: one range instead of three
-            const bool inner = cx >= 1 && cx + 1 < g.nc[0];
-            for (int oz = -1; oz <= 1; ++oz) {
-                const int zz = (cz + oz + g.nc[2]) % g.nc[2];
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                    const int rb = g.nc[0] * (yy + g.nc[1] * zz);
-                    if (inner) {
-                        scan(a.cell_atoms, a.cell_start[rb + cx - 1], a.cell_start[rb + cx + 2]);
-                    } else {
-                        for (int ox = -1; ox <= 1; ++ox) {
-                            const int c = rb + (cx + ox + g.nc[0]) % g.nc[0];
-                            scan(a.cell_atoms, a.cell_start[c], a.cell_start[c + 1]);
-                        }
-                    }
+            const int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
+            const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            for (int oz = -2; oz <= 2; ++oz) {
+                int zz = cz + oz;
+                zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
+                for (int oy = -2; oy <= 2; ++oy) {
+                    int yy = cy + oy;
+                    yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
+                    const int rb = nx * (yy + ny * zz);
+                    scan(a.cell_atoms, a.cell_start[rb + xa], a.cell_start[rb + xb + 1]);
+                    if (cx < 2)
+                        scan(a.cell_atoms, a.cell_start[rb + nx + cx - 2], a.cell_start[rb + nx]);
+                    else if (cx + 2 >= nx)
+                        scan(a.cell_atoms, a.cell_start[rb], a.cell_start[rb + cx + 3 - nx]);
                 }
             }
         } else {
             scan(nullptr, 0, static_cast<int>(a.n));
         }
-        if (lane == 0) {
+        if (sub == 0) {
             if (a.ref != nullptr) a.ref[i] = pi;
             a.ncnt[i] = cnt;
             if (cnt > a.cap) atomicOr(&a.sc->overflow, 1);
@@ -484,7 +488,7 @@ inline unsigned blocks(long n) { return static_cast<unsigned>((n + kBlock - 1) /
 }  // namespace
 
 void launch_mix_list(const MixListArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_mix_list, dim3(blocks(a.n * kWave)), dim3(kBlock), 0, s, a);  // a wave per atom
+    hipLaunchKernelGGL(k_mix_list, dim3(blocks(a.n * kListLanes)), dim3(kBlock), 0, s, a);
 }
 
 void launch_mix_force(const MixArgs& a, int deriv, hipStream_t s) {

@@ -165,10 +165,12 @@ md_status alloc_list(md_mix* m) {
     g = mdk::Grid{};
     for (int d = 0; d < 3; ++d) {
         g.L[d] = m->L[d];
-        g.nc[d] = std::max(1, static_cast<int>(std::floor(m->L[d] / m->rl)));
+        // cells of at least rl/2 (a 1e-9 relative margin keeps rounding from
+        // putting a pair within rl three cells apart), searched 5 x 5 x 5
+        g.nc[d] = std::max(1, static_cast<int>(std::floor(2.0 * m->L[d] / (m->rl * (1.0 + 1e-9)))));
         g.w[d] = m->L[d] / static_cast<double>(g.nc[d]);
     }
-    g.usable = (g.nc[0] >= 3 && g.nc[1] >= 3 && g.nc[2] >= 3) ? 1 : 0;
+    g.usable = (g.nc[0] >= 5 && g.nc[1] >= 5 && g.nc[2] >= 5) ? 1 : 0;
     const long ncell = g.usable ? static_cast<long>(g.nc[0]) * g.nc[1] * g.nc[2] : 1;
     if (ncell > m->ncell_alloc) {
         MHIP(m, malloc_dev(m->cell_count, ncell));
