@@ -113,6 +113,29 @@ def test_piece_split_keeps_trajectory(gpu_ok):
     assert np.allclose(th0[..., 2], th1[..., 2], rtol=1e-12, atol=0.0)
 
 
+def test_invalid_split_and_lanes_are_errors(gpu_ok):
+    """Piece counts outside [1, min(32, n / 32)] and lanes per atom outside
+    {1, 2, 4, 8, 16} are refused with an error, and the batch keeps working."""
+    from moleculardynamics_amd import native
+
+    systems, T, seeds, _, _ = _systems(3, master=4700, max_cells=5)
+    n = np.array([s[1].shape[0] for s in systems])
+    b = native.Batch(systems, T, seeds)
+    b.prime()
+    maxp = np.maximum(1, np.minimum(32, n // 32))
+    with pytest.raises(native.MDError):
+        b.set_pieces(np.zeros_like(n))
+    with pytest.raises(native.MDError):
+        b.set_pieces(maxp + 1)
+    with pytest.raises(native.MDError):
+        b.set_tpa(np.full(len(n), 3), 512)
+    with pytest.raises(native.MDError):
+        b.set_tpa(np.full(len(n), 4), 384)
+    b.set_pieces(maxp)
+    b.run(10, graph=False)
+    b.close()
+
+
 def test_batch_matches_single_system_path(gpu_ok):
     """Same seed, same counter-based noise, fixed-point forces: a replica in
     the batch and the same system on the plain path follow one trajectory.
