@@ -150,6 +150,21 @@ mdk::Grid make_grid(const double L[3], double cutoff) {
     return g;
 }
 
+// The Verlet rebuild's grid: cells of at least rl/2 (a 1e-9 relative margin
+// keeps rounding from putting a pair within rl three cells apart), searched
+// 5 x 5 x 5, usable with 5 or more per axis (then every list pair also lies
+// within 0.4 L, well inside half a box).
+mdk::Grid make_list_grid(const double L[3], double rl) {
+    mdk::Grid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = L[d];
+        g.nc[d] = std::max(1, static_cast<int>(std::floor(2.0 * L[d] / (rl * (1.0 + 1e-9)))));
+        g.w[d] = L[d] / static_cast<double>(g.nc[d]);
+    }
+    g.usable = (g.nc[0] >= 5 && g.nc[1] >= 5 && g.nc[2] >= 5) ? 1 : 0;
+    return g;
+}
+
 md_status need_system(md_ctx* c) {
     if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
     if (c->n <= 0) return fail(c, MD_ERR_STATE, "no system uploaded");
@@ -204,7 +219,7 @@ int auto_capacity(const md_ctx* c) {
 }
 
 md_status alloc_neigh(md_ctx* c) {
-    c->grid = make_grid(c->L, c->rc + c->skin);
+    c->grid = make_list_grid(c->L, c->rc + c->skin);
     c->ncell = c->grid.usable ? static_cast<long>(c->grid.nc[0]) * c->grid.nc[1] * c->grid.nc[2] : 1;
     if (c->ncell > c->ncell_alloc) {
         MD_HIP(c, alloc(c->cell_count, c->ncell));

@@ -6,9 +6,10 @@
 // the list holds every unordered pair with
 //   norm2(minimumImage(r_i - r_j)) < (rc + skin)^2,  i < j by upload index,
 // evaluated with the reference's FP64 operation order, so the pair SET is
-// bit-identical to the reference's. Candidates come from the 27 cells around
-// an atom's cell (cells at least rc + skin wide, at least 3 per side), or from
-// every atom when the grid is unusable (the reference's brute-force fallback).
+// bit-identical to the reference's. Candidates come from the 5 x 5 x 5 cells
+// around an atom's cell (cells at least (rc + skin)/2 wide, at least 5 per
+// side), or from every atom when the grid is unusable; every pair within
+// rc + skin lies in those cells.
 //
 // Six kernels: bin -> tile count -> scan -> place -> sort-within-cell and
 // permute -> commit+build. Every kernel takes the device rebuild flag and
@@ -25,7 +26,6 @@ namespace mdk {
 namespace {
 
 constexpr unsigned kGridCap = 2048;
-constexpr long kSortCap = 8192;
 
 __device__ __forceinline__ int cell_coord(double r, double L, double w, int n) {
     // floor((wrap(r) + L/2) / width) clamped to [0, n-1] (cell_list.cpp:31-38);
@@ -170,12 +170,11 @@ __global__ void k_place(const int* __restrict__ cell_of, long n, int* __restrict
 
 // Within each cell, order the slots by upload index, which makes the new
 // order a pure function of the positions (independent of atomic arrival
-// order), and gather the state into that order. One wave per cell: a slot's
-// rank is the number of smaller keys in its cell (upload indices are
-// distinct); with up to 64 atoms each lane holds one and the keys come by
-// shuffles, a larger cell is ranked 64 slots at a time against keys read
-// from memory.
-__device__ __forceinline__ void gather_slot(const NeighArgs& a, int dst, int v, int key, int c) {
+// order), and gather the state into that order. One thread per atom: its
+// rank is the number of atoms of its cell with a smaller upload index (the
+// cells, at least (rc + skin)/2 wide, hold a few atoms each). With an
+// unusable grid every atom is in cell 0 and its rank is its upload index.
+__device__ __forceinline__ void gather_slot(const NeighArgs& a, int dst, long v, int key, int c) {
     const double4 p = a.pos[v];
     a.pos2[dst] = p;
     a.vel2[dst] = a.vel[v];
@@ -185,50 +184,33 @@ __device__ __forceinline__ void gather_slot(const NeighArgs& a, int dst, int v, 
     if (a.qpos2 != nullptr) a.qpos2[dst] = quantize(a.quant, p, key);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a, long ncell) {
+__global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a) {
     if (*a.flag == 0) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    const long waves = static_cast<long>(gridDim.x) * (kBlock / kWave);
-    for (long c = static_cast<long>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; c < ncell;
-         c += waves) {
-        const int b = a.cell_start[c], e = a.cell_start[c + 1];
-        const int cnt = e - b;
-        const int cell = a.grid.usable ? static_cast<int>(c) : 0;
-        if (cnt <= kWave) {
-            int v = 0, key = 0x7fffffff;
-            if (lane < cnt) {
-                v = a.order[b + lane];
-                key = a.orig[v];
-            }
-            int rank = 0;
-            for (int j = 0; j < cnt; ++j) rank += (__shfl(key, j, kWave) < key) ? 1 : 0;
-            if (lane < cnt) gather_slot(a, b + rank, v, key, cell);
-        } else {
-            for (int k0 = 0; k0 < cnt; k0 += kWave) {
-                const int k = k0 + lane;
-                int v = 0, key = 0x7fffffff;
-                if (k < cnt) {
-                    v = a.order[b + k];
-                    key = a.orig[v];
-                }
-                int rank = 0;
-                for (int j = 0; j < cnt; ++j) rank += (a.orig[a.order[b + j]] < key) ? 1 : 0;
-                if (k < cnt) gather_slot(a, b + rank, v, key, cell);
-            }
+    for (long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; t < a.n;
+         t += static_cast<long>(gridDim.x) * kBlock) {
+        const int key = a.orig[t];
+        if (!a.grid.usable) {
+            gather_slot(a, key, t, key, 0);
+            continue;
         }
+        const int c = a.cell_of[t];
+        const int b = a.cell_start[c], e = a.cell_start[c + 1];
+        int rank = 0;
+        for (int k = b; k < e; ++k) rank += (a.orig[a.order[k]] < key) ? 1 : 0;
+        gather_slot(a, b + rank, t, key, c);
     }
 }
 
 // Commit the new order and build row t of the list from the new positions.
-// One thread per slot scans the 27 surrounding cells (or all atoms when the
-// grid is unusable) and keeps every partner within the list radius. The test
-// is the reference's exact FP64 expression on the pair oriented by upload
-// index, so both rows of a pair see the same bits.
+// One thread per slot scans the 5 x 5 x 5 surrounding cells (or all atoms
+// when the grid is unusable) and keeps every partner within the list radius.
+// The test is the reference's exact FP64 expression on the pair oriented by
+// upload index, so both rows of a pair see the same bits.
 //
-// After the sort, cells x-1, x, x+1 of one (y, z) row are one contiguous slot
-// range unless the row wraps at x, so the 27 cells are scanned as 9 ranges
-// (3 single cells each for a wrapping row), in the order (z, y, x) and slot
-// order within a cell. Candidates are read four at a time -- positions and
+// After the sort, five x-adjacent cells of one (y, z) row are one contiguous
+// slot range unless the row wraps at x (then two ranges), so the cells are
+// scanned as 25 ranges in the order (z, y) and slot order within a range.
+// Candidates are read four at a time -- positions and
 // upload indices of the four issued together -- so every thread keeps four
 // gathers in flight instead of one dependent load chain per candidate.
 //
@@ -356,24 +338,23 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     };
     auto stencil = [&](bool fixed) {
         if (g.usable) {
-            const int nx = g.nc[0];
+            const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
             const int cx = ci % nx;
-            const int cy = (ci / nx) % g.nc[1];
-            const int cz = ci / (nx * g.nc[1]);
-            const bool inner = cx >= 1 && cx + 1 < nx;
-            for (int oz = -1; oz <= 1; ++oz) {
-                const int zz = (cz + oz + g.nc[2]) % g.nc[2];
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int yy = (cy + oy + g.nc[1]) % g.nc[1];
-                    const int rb = nx * (yy + g.nc[1] * zz);
-                    if (inner) {
-                        range(cell_start[rb + cx - 1], cell_start[rb + cx + 2], fixed);
-                    } else {
-                        for (int ox = -1; ox <= 1; ++ox) {
-                            const int c = rb + (cx + ox + nx) % nx;
-                            range(cell_start[c], cell_start[c + 1], fixed);
-                        }
-                    }
+            const int cy = (ci / nx) % ny;
+            const int cz = ci / (nx * ny);
+            const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            for (int oz = -2; oz <= 2; ++oz) {
+                int zz = cz + oz;
+                zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
+                for (int oy = -2; oy <= 2; ++oy) {
+                    int yy = cy + oy;
+                    yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
+                    const int rb = nx * (yy + ny * zz);
+                    range(cell_start[rb + xa], cell_start[rb + xb + 1], fixed);
+                    if (cx < 2)
+                        range(cell_start[rb + nx + cx - 2], cell_start[rb + nx], fixed);
+                    else if (cx + 2 >= nx)
+                        range(cell_start[rb], cell_start[rb + cx + 3 - nx], fixed);
                 }
             }
         } else {
@@ -402,16 +383,14 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     const long n = a.n;
     const long ncell = a.grid.usable ? static_cast<long>(a.grid.nc[0]) * a.grid.nc[1] * a.grid.nc[2] : 1;
     // k_sort_permute takes one wave per cell
-    const long sort_blocks = (ncell + kBlock / kWave - 1) / (kBlock / kWave);
     const dim3 gn(grid_capped(n)), b(kBlock);
-    const dim3 gc(static_cast<unsigned>(sort_blocks < kSortCap ? sort_blocks : kSortCap));
     const dim3 gt(scan_tiles(ncell)), bt(kScanTile);
     hipLaunchKernelGGL(k_bin, gn, b, 0, s, a.pos, n, a.grid, a.cell_of, a.cell_count, a.flag);
     hipLaunchKernelGGL(k_tile_count, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.flag);
     hipLaunchKernelGGL(k_scan, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.cell_start,
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
-    hipLaunchKernelGGL(k_sort_permute, gc, b, 0, s, a, ncell);
+    hipLaunchKernelGGL(k_sort_permute, gn, b, 0, s, a);
     if (a.qpos2 != nullptr && a.quant.cubic)
         hipLaunchKernelGGL((k_commit_build<true, true>), dim3(xcd_grid(n)), b, 0, s, a);
     else if (a.qpos2 != nullptr)
