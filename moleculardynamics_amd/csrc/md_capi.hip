@@ -238,7 +238,9 @@ md_status alloc_neigh(md_ctx* c) {
     // difference is within one unit u, so |r2' - r2| <= 2 sqrt(3) r u + 3 u^2.
     double umax = 0.0;
     for (int d = 0; d < 3; ++d) umax = std::max(umax, c->L[d] / 4294967296.0);
-    const bool fixed_point = c->grid.usable && c->rc > 0.0 && umax <= c->rc / 33554432.0;
+    // (and n < 2^28: the pair kernel forms 32-bit byte offsets of partners)
+    const bool fixed_point = c->grid.usable && c->rc > 0.0 && umax <= c->rc / 33554432.0 &&
+                             c->n < (1L << 28);
     if (fixed_point) {
         MD_HIP(c, alloc(c->qpos, c->n));
         MD_HIP(c, alloc(c->qpos2, c->n));

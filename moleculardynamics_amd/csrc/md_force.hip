@@ -445,13 +445,18 @@ __device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt
 template <bool LFS>
 __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt,
                                                const int4& qi, AccFix& q) {
+    // a partner's byte offset is its entry shifted left by 4 in 32 bits: the
+    // flip bit leaves the word, and the load takes the base from scalar
+    // registers and the offset from one vector register (n < 2^28 on this path)
+    const char* qb = reinterpret_cast<const char*>(a.qpos);
+    auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + (e << 4)); };
     walk_row<int4>(
         a, t, cnt,
         [&](const u32x4& j, int4* p) {
-            p[0] = a.qpos[j.x & kIndexMask];
-            p[1] = a.qpos[j.y & kIndexMask];
-            p[2] = a.qpos[j.z & kIndexMask];
-            p[3] = a.qpos[j.w & kIndexMask];
+            p[0] = at(j.x);
+            p[1] = at(j.y);
+            p[2] = at(j.z);
+            p[3] = at(j.w);
         },
         [&](const u32x4& j, const int4* p) {
             pair32q<LFS>(a, qi, p[0], q);
@@ -459,7 +464,7 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
             pair32q<LFS>(a, qi, p[2], q);
             pair32q<LFS>(a, qi, p[3], q);
         },
-        [&](uint32_t e) { pair32q<LFS>(a, qi, a.qpos[e & kIndexMask], q); });
+        [&](uint32_t e) { pair32q<LFS>(a, qi, at(e), q); });
 }
 
 // QP: the mixed kernel reads fixed-point positions (a.qpos) for the partners.
