@@ -303,12 +303,17 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
         hits += (r2 >= lo && r2 < hi) ? 1u : 0u;
         put(j, r2 < lo, flip);
     };
+    // candidate loads with 32-bit byte offsets from a scalar base (n < 2^28
+    // wherever fixed-point positions are kept)
+    const char* qb = reinterpret_cast<const char*>(a.qpos2);
+    auto qat = [&](int j) {
+        return *reinterpret_cast<const int4*>(qb + (static_cast<uint32_t>(j) << 4));
+    };
     auto range = [&](int b, int e, bool fixed) {
         int j = b;
         if (Q && fixed) {
             for (; j + 4 <= e; j += 4) {
-                const int4 q0 = a.qpos2[j], q1 = a.qpos2[j + 1], q2 = a.qpos2[j + 2],
-                           q3 = a.qpos2[j + 3];
+                const int4 q0 = qat(j), q1 = qat(j + 1), q2 = qat(j + 2), q3 = qat(j + 3);
                 visit_q(j, q0);
                 visit_q(j + 1, q1);
                 visit_q(j + 2, q2);
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
                 drain();
             }
             for (; j < e; ++j) {
-                visit_q(j, a.qpos2[j]);
+                visit_q(j, qat(j));
                 drain();
             }
         } else {
