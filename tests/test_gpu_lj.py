@@ -89,13 +89,18 @@ def test_unusable_cell_list_is_refused(ctx):
     assert np.isfinite(e)
 
 
-@pytest.mark.parametrize("case", ["nist1", "nist3", "fcc864", "fcc108_small_box"])
+@pytest.mark.parametrize("case", ["nist1", "nist3", "fcc864", "fcc6912", "fcc108_small_box"])
 def test_pair_set_bit_exact(ctx, orc, case):
     if case.startswith("nist"):
         fr = load_nist(nist_lj(int(case[-1])))
         rc, skin = 3.0, 0.3
     elif case == "fcc864":
         fr = orc.fcc_lattice(6, 0.8442, 0.722, 12345)
+        rc, skin = 2.5, 0.3
+    elif case == "fcc6912":
+        # 14 list cells per axis: runs of five cells in the interior and
+        # split runs at the wrapping faces
+        fr = orc.fcc_lattice(12, 0.8442, 0.722, 12345)
         rc, skin = 2.5, 0.3
     else:  # L = 4.76 < 2*(rc+skin): brute-force build, single nearest image
         fr = orc.fcc_lattice(3, 1.0, 1.0, 7)
@@ -110,9 +115,11 @@ def test_pair_set_bit_exact(ctx, orc, case):
     assert np.array_equal(got, want)
 
 
-def test_pair_set_after_dynamics(ctx, orc):
-    """Re-sorted device order, moved atoms: the set still matches on the same snapshot."""
-    fr = orc.fcc_lattice(6, 0.8442, 0.722, 4242)
+@pytest.mark.parametrize("cells", [6, 12])
+def test_pair_set_after_dynamics(ctx, orc, cells):
+    """Re-sorted device order, moved atoms: the set still matches on the same
+    snapshot (6: 8 list cells per axis; 12: 14)."""
+    fr = orc.fcc_lattice(cells, 0.8442, 0.722, 4242)
     ctx.upload(fr.L, _pos(fr), _vel(fr))
     ctx.set_lj(1.0, 1.0, 2.5, False)
     ctx.config_list(2.5, 0.3)
