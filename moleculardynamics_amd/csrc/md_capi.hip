@@ -69,6 +69,7 @@ struct md_ctx {
     bool integ_set = false;
     md_integrator integ{};
     uint64_t step_counter = 0;
+    unsigned long long step_upload = 0;  // source of the device step counter's upload
     bool chain_primed = false;
 
     md_thermo* thermo_dev = nullptr;
@@ -799,6 +800,12 @@ md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
     return MD_OK;
 }
 
+// md_run replays captured segments of this many steps for systems of at
+// most kGraphMaxAtoms atoms (where launch latency, not the kernels, sets the
+// step time), unless per-kernel timing is on.
+constexpr long kGraphSegment = 32;
+constexpr long kGraphMaxAtoms = 262144;
+
 md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     md_status st = need_system(c);
     if (st != MD_OK) return st;
@@ -829,28 +836,82 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
         mdk::launch_kinetic(c->n, c->vel, c->part_d, c->sc, c->stream);
         mdk::launch_chain_first(c->chain, c->sc, 0.5 * c->integ.dt, c->stream);
     }
-    for (long k = 0; k < nsteps; ++k) {
-        sa.step = c->step_counter++;
-        hipEvent_t e2{};
-        ev_begin(c, 2, &e2);
-        mdk::launch_pre_step(sa, c->stream);
-        ev_end(c, 2, e2);
-        hipEvent_t e1{};
-        ev_begin(c, 1, &e1);
-        mdk::launch_rebuild(na, c->stream);
-        ev_end(c, 1, e1);
-        fa.thermo = thermo ? static_cast<void*>(c->thermo_dev + k) : nullptr;
-        fa.chain_twice = (nhc && k + 1 < nsteps) ? 1 : 0;
-        hipEvent_t e0{};
-        ev_begin(c, 0, &e0);
-        mdk::launch_lj_force(fa, c->stream);
-        ev_end(c, 0, e0);
-        mdk::launch_lj_finish(fa, c->stream);
+    hipGraphExec_t gexec = nullptr;
+    if (!c->timing && c->n <= kGraphMaxAtoms && nsteps > kGraphSegment) {
+        // Launch-bound runs replay a captured segment of kGraphSegment steps:
+        // the step counter (BAOAB noise) and the thermo row come from the
+        // device counter sc->step, advanced by each step's finish. Every
+        // captured step propagates the next step's first half-chain, so the
+        // run's last step is launched directly.
+        c->step_upload = c->step_counter;
+        MD_HIP(c, hipMemcpyAsync(&c->sc->step, &c->step_upload, sizeof(unsigned long long),
+                                 hipMemcpyHostToDevice, c->stream));
+        mdk::StepArgs gs = sa;
+        gs.dev_step = 1;
+        mdk::ForceArgs gf = fa;
+        gf.dev_step = 1;
+        gf.thermo = nullptr;
+        gf.thermo_base = thermo ? static_cast<void*>(c->thermo_dev) : nullptr;
+        gf.step0 = c->step_counter;
+        gf.chain_twice = nhc ? 1 : 0;
+        auto one_step = [&]() {
+            mdk::launch_pre_step(gs, c->stream);
+            mdk::launch_rebuild(na, c->stream);
+            mdk::launch_lj_force(gf, c->stream);
+            mdk::launch_lj_finish(gf, c->stream);
+        };
+        hipGraph_t graph = nullptr;
+        MD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (long k = 0; k < kGraphSegment; ++k) one_step();
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(c, MD_ERR_HIP, std::string("md_run capture: ") +
+                                           hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MD_HIP(c, ie);
+        const long reps = (nsteps - 1) / kGraphSegment;  // the last step always runs directly
+        for (long r = 0; r < reps; ++r) {
+            const hipError_t e = hipGraphLaunch(gexec, c->stream);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(c->stream);
+                (void)hipGraphExecDestroy(gexec);
+                return fail(c, MD_ERR_HIP, std::string("md_run graph launch: ") + hipGetErrorString(e));
+            }
+        }
+        for (long k = reps * kGraphSegment; k < nsteps; ++k) {
+            gf.chain_twice = (nhc && k + 1 < nsteps) ? 1 : 0;
+            one_step();
+        }
+        c->step_counter += static_cast<uint64_t>(nsteps);
+    } else {
+        for (long k = 0; k < nsteps; ++k) {
+            sa.step = c->step_counter++;
+            hipEvent_t e2{};
+            ev_begin(c, 2, &e2);
+            mdk::launch_pre_step(sa, c->stream);
+            ev_end(c, 2, e2);
+            hipEvent_t e1{};
+            ev_begin(c, 1, &e1);
+            mdk::launch_rebuild(na, c->stream);
+            ev_end(c, 1, e1);
+            fa.thermo = thermo ? static_cast<void*>(c->thermo_dev + k) : nullptr;
+            fa.chain_twice = (nhc && k + 1 < nsteps) ? 1 : 0;
+            hipEvent_t e0{};
+            ev_begin(c, 0, &e0);
+            mdk::launch_lj_force(fa, c->stream);
+            ev_end(c, 0, e0);
+            mdk::launch_lj_finish(fa, c->stream);
+        }
     }
     if (nhc) mdk::launch_apply_scale(c->n, c->vel, c->sc, c->stream);
     MD_HIP(c, hipGetLastError());
     mdk::Scalars s{};
-    st = read_scalars(c, &s);
+    st = read_scalars(c, &s);  // synchronises the stream
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (st != MD_OK) return st;
     if (s.overflow) return fail(c, MD_ERR_CAPACITY, "neighbour row capacity exceeded during md_run");
     if (s.fix_saturated)

@@ -55,6 +55,7 @@ struct Scalars {
     int overflow;                    // neighbour row overflow seen
     int fix_saturated;               // fixed-point conversion saturated
     int pad;
+    unsigned long long step;         // step counter of graph-replayed runs (see StepArgs::dev_step)
 };
 
 // Nose-Hoover chain state (MTK96), held on the device so a whole run stays

@@ -209,14 +209,17 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
         }
         sc->kinetic = ke;
     }
-    if (a.thermo != nullptr) {
-        md_thermo* th = static_cast<md_thermo*>(a.thermo);
+    md_thermo* th = static_cast<md_thermo*>(a.thermo);
+    if (a.dev_step && a.thermo_base != nullptr)
+        th = static_cast<md_thermo*>(a.thermo_base) + (sc->step - a.step0);
+    if (th != nullptr) {
         th->potential = E;
         th->virial = W;
         th->kinetic = ke;
         th->conserved = conserved;
         th->pairs = static_cast<double>(pairs);
     }
+    if (a.dev_step) sc->step += 1;
 }
 
 // Every in-cutoff term is counted in the row that holds it (n). The list is

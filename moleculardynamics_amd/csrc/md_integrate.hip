@@ -128,7 +128,8 @@ __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
             const double h = 0.5 * a.dt;
             p = drift(a.box, p, v, h);
             double g0, g1, g2;
-            normals3(a.seed, a.step, static_cast<uint32_t>(a.orig[t]), g0, g1, g2);
+            normals3(a.seed, a.dev_step ? a.sc->step : a.step, static_cast<uint32_t>(a.orig[t]), g0,
+                     g1, g2);
             const double k = a.c2 * sqrt(a.temperature / v.w);
             v.x = a.c1 * v.x + g0 * k;
             v.y = a.c1 * v.y + g1 * k;

@@ -117,6 +117,12 @@ struct ForceArgs {
     double chain_delta; // dt/2 of the chain half-steps
     int chain_twice;    // 1: also propagate the next step's first half-chain
     void* thermo;       // md_thermo* record for this step, or null
+    // graph-replayed segments (dev_step 1): the step's record is
+    // thermo_base[sc->step - step0] (when thermo_base is not null), and the
+    // finish advances sc->step
+    int dev_step;
+    void* thermo_base;
+    unsigned long long step0;
 };
 void launch_lj_force(const ForceArgs& a, hipStream_t s);
 // One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
@@ -167,6 +173,7 @@ struct StepArgs {
     double c1, c2, temperature;  // BAOAB
     uint64_t seed;
     uint64_t step;    // global step counter (BAOAB noise counter)
+    int dev_step;     // 1: the counter is sc->step instead (graph-replayed segments)
     double skin;
     int check;        // compute the displacement check and set the rebuild flag
     double* part_d;   // 2 per group (top1, top2)

@@ -221,6 +221,44 @@ def test_time_reversibility(ctx, orc):
     assert np.max(np.linalg.norm(d, axis=1)) < 1e-8
 
 
+@pytest.mark.parametrize("kind,prec", [(0, "fp64"), (1, "mixed"), (2, "fp64"), (2, "mixed")])
+def test_graph_segments_match_direct_launches(gpu_ok, orc, kind, prec):
+    """md_run replays captured segments for small systems. The same run with
+    per-kernel timing on (direct launches) gives the same trajectory and
+    thermo bit for bit, over several runs with host work in between (the
+    replays after the first one included)."""
+    from moleculardynamics_amd import native
+
+    fr = orc.fcc_lattice(5, 0.8442, 0.9, 31)
+    n = fr.n
+    precision = native.FP64 if prec == "fp64" else native.MIXED
+    outs = []
+    for direct in (True, False):
+        c = native.Context()
+        c.upload(fr.L, _pos(fr), _vel(fr))
+        c.set_lj(1.0, 1.0, 2.5, False)
+        c.config_list(2.5, 0.3)
+        c.build_list()
+        c.forces(native.VERLET, precision)
+        c.set_integrator(kind=kind, dt=0.005, friction=1.0, temperature=0.9, tau=0.5,
+                         chain_length=3, dof=3 * n - 3, seed=77)
+        if direct:
+            c.timing(True)
+        ths = []
+        for steps in (70, 45, 130):
+            ths.append(c.run(steps, precision))
+            c.download()
+        outs.append((c.download(), ths, c.rebuild_count()))
+        c.close()
+    (s0, t0, r0), (s1, t1, r1) = outs
+    assert r0 == r1 and r0 >= 3
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+    for a, b in zip(t0, t1):
+        for key in a:
+            assert np.array_equal(a[key], b[key]), key
+
+
 def test_nhc_short_trajectory_matches_oracle(ctx, orc):
     fr = orc.fcc_lattice(4, 0.8442, 0.722, 20260001)
     n = fr.n
