@@ -442,7 +442,7 @@ md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, 
     MD_HIP(c, alloc(c->orig2, n)); MD_HIP(c, alloc(c->cell2, n)); MD_HIP(c, alloc(c->cell_of, n));
     MD_HIP(c, alloc(c->order, n)); MD_HIP(c, alloc(c->ncnt, n));
     MD_HIP(c, alloc(c->stage, 9 * n));
-    const long nb = blocks_of(n);
+    const long nb = std::max(blocks_of(n), mdk::lj_partials(n));
     MD_HIP(c, alloc(c->part_d, 4 * nb + 8));
     MD_HIP(c, alloc(c->part_l, 3 * nb + 8));
 

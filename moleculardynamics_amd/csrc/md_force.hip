@@ -540,16 +540,164 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     }
 }
 
+// ---- small systems: several lanes per atom ----
+//
+// Below kLanesMaxAtoms atoms one thread per atom leaves most of the chip idle
+// and every thread walks a dependent chain of ~80 gathers. LANES consecutive
+// lanes then share an atom: lane s takes entries s, s + LANES, ... of its row,
+// and the group's sums are combined by shuffles in a fixed butterfly (FP64:
+// deterministic; fixed point: the raw accumulators and bias counts are
+// summed as integers, so the forces, energy and virial are bit for bit those
+// of one lane per atom). A group redoes its row exactly when any of its lanes
+// saw a band hit, which is the row's own condition.
+constexpr long kLanesMaxAtoms = 32768;
+
+inline int lj_lanes(long n) {
+    if (n > kLanesMaxAtoms) return 1;
+    int l = 16;
+    while (l > 1 && static_cast<long>(l) * n > 262144) l >>= 1;
+    return l;
+}
+
+// Workgroups (and partial-sum slots) of the pair kernel for n atoms.
+inline long lj_groups(long n) {
+    const int l = lj_lanes(n);
+    return (static_cast<long>(l) * n + kBlock - 1) / kBlock;
+}
+
+template <int LANES, class T>
+__device__ __forceinline__ T group_sum(T v) {
+    for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+template <int LANES>
+__device__ __forceinline__ unsigned long long group_sum_u64(unsigned long long v) {
+    return static_cast<unsigned long long>(group_sum<LANES>(static_cast<long long>(v)));
+}
+
+template <bool LFS, bool MIXED, bool QP, bool KICK, int LANES>
+__global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
+    __shared__ double sm_d[kBlock / kWave];
+    __shared__ long long sm_l[kBlock / kWave];
+    const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long t = gt / LANES;
+    const int sub = static_cast<int>(gt & (LANES - 1));
+    // a group is whole lanes of one wave (LANES divides 64); every lane of the
+    // block takes part in the shuffles, a lane past the last atom with an
+    // empty row
+    const bool live = t < a.n;
+    const long tt = live ? t : 0;
+    const int cnt = live ? min(a.ncnt[tt], a.cap) : 0;
+    const uint32_t* row = a.nbr + oct_index(tt, 0, a.cap);
+    auto entry = [&](int k) { return row[(k >> 3) * 512 + (k & 7)]; };
+    double e = 0.0, w = 0.0, ke = 0.0;
+    long long ef = 0, wf = 0, nt = 0;
+    double fx, fy, fz;
+    if (!MIXED) {
+        Acc64 q;
+        const double4 pi = a.pos[tt];
+        for (int k = sub; k < cnt; k += LANES) {
+            const uint32_t jj = entry(k);
+            pair64<LFS>(a, pi, jj, a.pos[jj & kIndexMask], q);
+        }
+        fx = group_sum<LANES>(q.fx);
+        fy = group_sum<LANES>(q.fy);
+        fz = group_sum<LANES>(q.fz);
+        e = group_sum<LANES>(q.e);
+        w = group_sum<LANES>(q.w);
+        nt = group_sum<LANES>(static_cast<long long>(q.n));
+    } else {
+        AccFix q;
+        auto exact = [&]() {
+            const double4 pi = a.pos[tt];
+            for (int k = sub; k < cnt; k += LANES) {
+                const uint32_t jj = entry(k);
+                pair32<LFS>(a, pi, jj, a.pos[jj & kIndexMask], q);
+            }
+        };
+        if (QP) {
+            const int4 qi = a.qpos[tt];
+            const char* qb = reinterpret_cast<const char*>(a.qpos);
+            for (int k = sub; k < cnt; k += LANES)
+                pair32q<LFS>(a, qi, *reinterpret_cast<const int4*>(qb + (entry(k) << 4)), q);
+            if (group_sum<LANES>(static_cast<int>(q.kb != q.k)) != 0) {
+                q = AccFix{};
+                exact();
+            }
+        } else {
+            exact();
+        }
+        unsigned int r2min = q.r2min;
+        for (int off = LANES / 2; off > 0; off >>= 1) r2min = min(r2min, static_cast<unsigned int>(__shfl_xor(static_cast<int>(r2min), off, kWave)));
+        const unsigned long long nb = group_sum_u64<LANES>(q.n);
+        if (live && sub == 0 && fix_may_saturate(a.lj, r2min)) atomicOr(&a.sc->fix_saturated, 1);
+        fx = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fx), nb)) * kFixInv;
+        fy = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fy), nb)) * kFixInv;
+        fz = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fz), nb)) * kFixInv;
+        ef = fix_value(group_sum_u64<LANES>(q.e), nb);
+        wf = fix_value(group_sum_u64<LANES>(q.w), nb);
+        nt = group_sum<LANES>(static_cast<long long>(q.k));
+    }
+    if (live && sub == 0) {
+        a.frc[t] = make_double4(fx, fy, fz, 0.0);
+        if (KICK) {
+            double4 v = a.vel[t];
+            const double ch = a.half_dt * (1.0 / v.w);
+            v.x = fma(ch, fx, v.x);
+            v.y = fma(ch, fy, v.y);
+            v.z = fma(ch, fz, v.z);
+            a.vel[t] = v;
+            ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+        }
+    } else {
+        e = w = 0.0;
+        ef = wf = nt = 0;
+    }
+    const double be = block_reduce(e, sm_d, wave_sum);
+    const double bw = block_reduce(w, sm_d, wave_sum);
+    const double bk = block_reduce(ke, sm_d, wave_sum);
+    const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
+    const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
+    const long long bnt = block_reduce(nt, sm_l, wave_sum_ll);
+    if (threadIdx.x == 0) {
+        const long b = blockIdx.x;
+        a.part_d[3 * b + 0] = be;
+        a.part_d[3 * b + 1] = bw;
+        a.part_d[3 * b + 2] = bk;
+        a.part_l[3 * b + 0] = bef;
+        a.part_l[3 * b + 1] = bwf;
+        a.part_l[3 * b + 2] = bnt;
+    }
+}
+
 __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
+template <bool LFS, bool MIXED, bool QP, bool KICK>
+void dispatch_lanes(const ForceArgs& a, hipStream_t s) {
+    const dim3 g(static_cast<unsigned>(lj_groups(a.n))), b(kBlock);
+    switch (lj_lanes(a.n)) {
+        case 16: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 16>), g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 8>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 4>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 2>), g, b, 0, s, a); break;
+    }
+}
+
 template <bool LFS, bool MIXED, bool QP>
 void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
-    if (a.kick)
+    if (lj_lanes(a.n) > 1) {
+        if (a.kick)
+            dispatch_lanes<LFS, MIXED, QP, true>(a, s);
+        else
+            dispatch_lanes<LFS, MIXED, QP, false>(a, s);
+    } else if (a.kick) {
         hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, true>), g, dim3(kBlock), 0, s, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, false>), g, dim3(kBlock), 0, s, a);
+    }
 }
 
 // ---- direct (no list) evaluations: brute force and cells at the cutoff ----
@@ -763,6 +911,8 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
     hipLaunchKernelGGL(k_chain_first, dim3(1), dim3(64), 0, s, chain, sc, delta);
 }
 
+long lj_partials(long n) { return lj_groups(n); }
+
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
     const dim3 g(xcd_grid(a.n));
     const bool qp = a.qpos != nullptr;
@@ -778,8 +928,9 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
 }
 
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
-    // the pair kernel writes partials for its xcd_grid(n) blocks (padding blocks none)
-    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(grid_of(a.n)));
+    // the pair kernel writes partials for its lj_groups(n) blocks (padding blocks of
+    // the XCD-aware grid none)
+    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(lj_groups(a.n)));
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

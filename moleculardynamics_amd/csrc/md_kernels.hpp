@@ -124,6 +124,9 @@ struct ForceArgs {
     void* thermo_base;
     unsigned long long step0;
 };
+// Per-workgroup partial slots that launch_lj_force writes for n atoms (small
+// systems use several lanes per atom, hence more workgroups).
+long lj_partials(long n);
 void launch_lj_force(const ForceArgs& a, hipStream_t s);
 // One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
 // thermo row and NHC chain; launched right after launch_lj_force.

@@ -398,3 +398,42 @@ def test_mixed_without_fixed_point_positions(ctx, orc):
     rms = np.sqrt(np.sum((f32 - f0) ** 2) / np.sum(f0 ** 2))
     assert rms < 1e-5, rms
     assert abs(e32 - e_or) / abs(e_or) < 1e-6
+
+
+def test_lane_split_matches_one_lane_per_atom(gpu_ok, orc):
+    """Systems of up to 32,768 atoms run the pair kernel with several lanes per
+    atom, larger ones with one. A crystallite of 32,768 atoms in a large box,
+    alone and with one isolated extra atom (32,769: one lane per atom), has the
+    same pair set: the fixed-point forces, energy and virial agree bit for bit,
+    the FP64 ones to rounding, and both FP64 results match the oracle."""
+    from moleculardynamics_amd import native
+    from oracle.pyoracle import Frame
+
+    fr = orc.fcc_lattice(16, 0.8442, 0.722, 20260003)
+    rng = np.random.default_rng(5)
+    x = _pos(fr) + rng.uniform(-0.05, 0.05, size=(fr.n, 3))
+    L = 80.0
+    xs = [x, np.vstack([x, [[39.0, 39.0, 39.0]]])]
+    out = []
+    for pos in xs:
+        c = native.Context()
+        c.upload(np.array([L, L, L]), pos)
+        c.set_lj(1.0, 1.0, 2.5, False)
+        c.config_list(2.5, 0.3)
+        c.build_list()
+        e64, w64 = c.forces(native.VERLET, native.FP64)
+        f64 = c.download()[2][: fr.n]
+        e32, w32 = c.forces(native.VERLET, native.MIXED)
+        f32 = c.download()[2][: fr.n]
+        c.close()
+        out.append((e64, w64, f64, e32, w32, f32))
+    (a64, b64, f64a, a32, b32, f32a), (c64, d64, f64b, c32, d32, f32b) = out
+    assert np.array_equal(f32a, f32b) and a32 == c32 and b32 == d32
+    assert _force_err(f64b, f64a) <= REL
+    assert abs(c64 - a64) <= REL * abs(a64) and abs(d64 - b64) <= REL * abs(b64)
+    f0, e0, w0 = orc.lj_forces(Frame(np.array([L, L, L]), x[:, 0].copy(), x[:, 1].copy(),
+                                     x[:, 2].copy()), rc=2.5, mode="cells")
+    assert _force_err(f64a, f0) <= REL and _force_err(f64b, f0) <= REL
+    assert abs(a64 - e0) <= REL * abs(e0) and abs(b64 - w0) <= REL * abs(w0)
+    rms = np.sqrt(np.sum((f32a - f0) ** 2) / np.sum(f0 ** 2))
+    assert rms < 2e-6, rms
