@@ -377,6 +377,127 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     if (cnt > cap) atomicOr(&a.sc->overflow, 1);
 }
 
+// Commit + build for small systems (at most kBuildLanesMaxAtoms atoms): one
+// thread per slot leaves most of the chip idle and walks about 300 candidates
+// in a dependent chain, so kBuildLanes consecutive lanes share a slot. The
+// group takes a range kBuildLanes candidates at a time, lane s candidate
+// base + s, and appends the kept ones in candidate order through a ballot
+// over its lanes: the row, and its order, are the ones k_commit_build writes.
+// A band hit in any lane makes the group build its row again exactly.
+constexpr long kBuildLanesMaxAtoms = 32768;
+constexpr int kBuildLanes = 8;
+
+template <bool Q, bool CUBIC>
+__global__ __launch_bounds__(kBlock) void k_commit_build_lanes(NeighArgs a) {
+    if (*a.flag == 0) return;
+    const long n = a.n;
+    const Grid& g = a.grid;
+    const double rl2 = a.rl2;
+    const int cap = a.cap;
+    const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long tl = gt / kBuildLanes;
+    const int sub = static_cast<int>(gt & (kBuildLanes - 1));
+    if (tl >= n) return;  // whole groups: kBuildLanes divides the workgroup
+    const int t = static_cast<int>(tl);
+    const double4 pi = a.pos2[t];
+    const int oi = a.orig2[t];
+    const int ci = a.cell2[t];
+    int4 qi{};
+    if (Q) qi = a.qpos2[t];
+    if (sub == 0) {
+        a.pos[t] = pi;
+        a.ref[t] = pi;
+        a.vel[t] = a.vel2[t];
+        a.frc[t] = a.frc2[t];
+        a.orig[t] = oi;
+        a.slot[oi] = t;
+        a.cell_of[t] = ci;
+        if (Q) a.qpos[t] = qi;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int head = lane & ~(kBuildLanes - 1);
+    const unsigned long long gmask = ((1ull << kBuildLanes) - 1) << head;
+    const unsigned long long below = (1ull << lane) - 1;
+    const double u2 = CUBIC ? a.quant.unit[0] * a.quant.unit[0] : 1.0;
+    const double lo = rl2 * (1.0 - a.quant.band_l) / u2, hi = rl2 * (1.0 + a.quant.band_l) / u2;
+    int cnt = 0;
+    unsigned int hits = 0;
+    auto keep_entry = [&](bool keep, int j, bool flip) {
+        const unsigned long long m = __ballot(keep) & gmask;
+        const int at = cnt + __popcll(m & below);
+        if (keep && at < cap)
+            a.nbr[oct_index(t, at, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+        cnt += __popcll(m);
+    };
+    auto range = [&](int b, int e, bool fixed) {
+        for (int base = b; base < e; base += kBuildLanes) {
+            const int j = base + sub;
+            const bool ok = j < e;
+            const int jj = ok ? j : b;
+            bool keep, flip;
+            if (Q && fixed) {
+                const int4 qj = a.qpos2[jj];
+                double r2;
+                if (CUBIC) {
+                    const double ux = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+                    const double uy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+                    const double uz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+                    r2 = ux * ux + uy * uy + uz * uz;
+                } else {
+                    double dx, dy, dz;
+                    qdelta(a.quant, qi, qj, dx, dy, dz);
+                    r2 = dx * dx + dy * dy + dz * dz;
+                }
+                flip = oi > qj.w;
+                hits += (ok && r2 >= lo && r2 < hi) ? 1u : 0u;
+                keep = ok && r2 < lo && j != t;
+            } else {
+                const int oj = a.orig2[jj];
+                flip = oi > oj;
+                double dx, dy, dz;
+                row_delta(a.box, pi, a.pos2[jj], flip, dx, dy, dz);
+                keep = ok && norm2_ref(dx, dy, dz) < rl2 && j != t;
+            }
+            keep_entry(keep, j, flip);
+        }
+    };
+    // the same 25 ranges in the same order as k_commit_build
+    auto stencil = [&](bool fixed) {
+        if (g.usable) {
+            const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
+            const int cx = ci % nx;
+            const int cy = (ci / nx) % ny;
+            const int cz = ci / (nx * ny);
+            const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            for (int oz = -2; oz <= 2; ++oz) {
+                int zz = cz + oz;
+                zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
+                for (int oy = -2; oy <= 2; ++oy) {
+                    int yy = cy + oy;
+                    yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
+                    const int rb = nx * (yy + ny * zz);
+                    range(a.cell_start[rb + xa], a.cell_start[rb + xb + 1], fixed);
+                    if (cx < 2)
+                        range(a.cell_start[rb + nx + cx - 2], a.cell_start[rb + nx], fixed);
+                    else if (cx + 2 >= nx)
+                        range(a.cell_start[rb], a.cell_start[rb + cx + 3 - nx], fixed);
+                }
+            }
+        } else {
+            range(0, static_cast<int>(n), fixed);
+        }
+    };
+    stencil(Q);
+    if (Q && (__ballot(hits != 0) & gmask) != 0) {  // a band hit in the group: the row exactly
+        cnt = 0;
+        stencil(false);
+    }
+    if (sub == 0) {
+        a.ncnt[t] = cnt;
+        if (cnt > cap) atomicOr(&a.sc->overflow, 1);
+    }
+}
+
 inline unsigned grid_capped(long n) {
     const long g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g < static_cast<long>(kGridCap) ? (g > 0 ? g : 1) : kGridCap);
@@ -396,7 +517,15 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
                        a.cell_fill, a.sc, a.flag);
     hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
     hipLaunchKernelGGL(k_sort_permute, gn, b, 0, s, a);
-    if (a.qpos2 != nullptr && a.quant.cubic)
+    if (n <= kBuildLanesMaxAtoms) {
+        const dim3 gl(static_cast<unsigned>((n * kBuildLanes + kBlock - 1) / kBlock));
+        if (a.qpos2 != nullptr && a.quant.cubic)
+            hipLaunchKernelGGL((k_commit_build_lanes<true, true>), gl, b, 0, s, a);
+        else if (a.qpos2 != nullptr)
+            hipLaunchKernelGGL((k_commit_build_lanes<true, false>), gl, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_commit_build_lanes<false, false>), gl, b, 0, s, a);
+    } else if (a.qpos2 != nullptr && a.quant.cubic)
         hipLaunchKernelGGL((k_commit_build<true, true>), dim3(xcd_grid(n)), b, 0, s, a);
     else if (a.qpos2 != nullptr)
         hipLaunchKernelGGL((k_commit_build<true, false>), dim3(xcd_grid(n)), b, 0, s, a);
