@@ -16,7 +16,8 @@
 // returns at once when it is 0, so a multi-step run launches the rebuild
 // unconditionally and keeps the decision on the device. The O(N) passes use
 // grid-stride loops over a capped grid so a skipped rebuild costs little;
-// commit+build runs one thread per slot in the XCD-aware block order.
+// commit+build runs one thread per slot in the XCD-aware block order (eight
+// lanes per slot for systems of at most 32,768 atoms).
 // Invariant: cell_count is all zero between rebuilds (k_scan clears it).
 #include "md_common.hpp"
 #include "md_kernels.hpp"
