@@ -550,18 +550,25 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
 // summed as integers, so the forces, energy and virial are bit for bit those
 // of one lane per atom). A group redoes its row exactly when any of its lanes
 // saw a band hit, which is the row's own condition.
+//
+// Above that size the mixed kernel keeps one lane per atom: it is bound by
+// VALU issue, and its octet-batched walk keeps eight gathers in flight per
+// thread. The FP64 kernel is bound by the gathers themselves (two L1
+// accesses per partner), and kFp64Lanes lanes per atom give it more waves
+// in flight.
 constexpr long kLanesMaxAtoms = 32768;
+constexpr int kFp64Lanes = 4;
 
-inline int lj_lanes(long n) {
-    if (n > kLanesMaxAtoms) return 1;
+inline int lj_lanes(long n, bool mixed) {
+    if (n > kLanesMaxAtoms) return mixed ? 1 : kFp64Lanes;
     int l = 16;
     while (l > 1 && static_cast<long>(l) * n > 262144) l >>= 1;
     return l;
 }
 
 // Workgroups (and partial-sum slots) of the pair kernel for n atoms.
-inline long lj_groups(long n) {
-    const int l = lj_lanes(n);
+inline long lj_groups(long n, bool mixed) {
+    const int l = lj_lanes(n, mixed);
     return (static_cast<long>(l) * n + kBlock - 1) / kBlock;
 }
 
@@ -677,8 +684,8 @@ __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
 
 template <bool LFS, bool MIXED, bool QP, bool KICK>
 void dispatch_lanes(const ForceArgs& a, hipStream_t s) {
-    const dim3 g(static_cast<unsigned>(lj_groups(a.n))), b(kBlock);
-    switch (lj_lanes(a.n)) {
+    const dim3 g(static_cast<unsigned>(lj_groups(a.n, MIXED))), b(kBlock);
+    switch (lj_lanes(a.n, MIXED)) {
         case 16: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 16>), g, b, 0, s, a); break;
         case 8: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 8>), g, b, 0, s, a); break;
         case 4: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 4>), g, b, 0, s, a); break;
@@ -688,7 +695,7 @@ void dispatch_lanes(const ForceArgs& a, hipStream_t s) {
 
 template <bool LFS, bool MIXED, bool QP>
 void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
-    if (lj_lanes(a.n) > 1) {
+    if (lj_lanes(a.n, MIXED) > 1) {
         if (a.kick)
             dispatch_lanes<LFS, MIXED, QP, true>(a, s);
         else
@@ -911,7 +918,7 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
     hipLaunchKernelGGL(k_chain_first, dim3(1), dim3(64), 0, s, chain, sc, delta);
 }
 
-long lj_partials(long n) { return lj_groups(n); }
+long lj_partials(long n) { return std::max(lj_groups(n, false), lj_groups(n, true)); }
 
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
     const dim3 g(xcd_grid(a.n));
@@ -930,7 +937,7 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
     // the pair kernel writes partials for its lj_groups(n) blocks (padding blocks of
     // the XCD-aware grid none)
-    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(lj_groups(a.n)));
+    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(lj_groups(a.n, a.mixed != 0)));
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

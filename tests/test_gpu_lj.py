@@ -402,10 +402,11 @@ def test_mixed_without_fixed_point_positions(ctx, orc):
 
 def test_lane_split_matches_one_lane_per_atom(gpu_ok, orc):
     """Systems of up to 32,768 atoms run the pair kernel with several lanes per
-    atom, larger ones with one. A crystallite of 32,768 atoms in a large box,
-    alone and with one isolated extra atom (32,769: one lane per atom), has the
-    same pair set: the fixed-point forces, energy and virial agree bit for bit,
-    the FP64 ones to rounding, and both FP64 results match the oracle."""
+    atom; larger ones run the mixed kernel with one lane per atom and the FP64
+    kernel with four. A crystallite of 32,768 atoms in a large box, alone and
+    with one isolated extra atom (32,769 atoms), has the same pair set: the
+    fixed-point forces, energy and virial agree bit for bit, the FP64 ones to
+    rounding, and both FP64 results match the oracle."""
     from moleculardynamics_amd import native
     from oracle.pyoracle import Frame
 
