@@ -347,8 +347,10 @@ def main() -> int:
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the workload would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path. A single large "
+              "system would shard by spatial domain decomposition (one brick of the box per "
+              "GPU, an rc + skin halo exchanged each step, atoms migrating at list rebuilds).",
+              file=sys.stderr)
         return 2
     if args.copy_calibration > 0:
         import torch
