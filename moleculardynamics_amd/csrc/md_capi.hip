@@ -298,6 +298,11 @@ mdk::ForceArgs force_args(md_ctx* c, int precision) {
     // the mixed kernel reads the quantised positions when the list keeps them
     a.qpos = (a.mixed && c->qpos != nullptr) ? c->qpos : nullptr;
     a.quant = c->quant;
+    // mixed-term constants: lengths in fixed-point units on a cubic box with
+    // fixed-point positions, real lengths otherwise
+    const double s = (c->qpos != nullptr && c->quant.cubic) ? c->quant.unit[0] : 1.0;
+    a.mq = mdk::mix_constants(c->ljc, s, c->qpos != nullptr ? c->quant.band_c : 0.0);
+    a.mx = mdk::mix_constants(c->ljc, 1.0, 0.0);
     a.part_d = c->part_d; a.part_l = c->part_l;
     a.sc = c->sc;
     return a;
@@ -348,6 +353,13 @@ md_status build_now(md_ctx* c) {
         if (st != MD_OK) return st;
     }
     return fail(c, MD_ERR_CAPACITY, "neighbour row capacity could not be satisfied");
+}
+
+// Clears the device flags a call reports (fixed-point saturation, row
+// overflow), stream-ordered, so that each call reports only its own.
+void clear_call_flags(md_ctx* c) {
+    mdk::launch_set_flag(&c->sc->fix_saturated, 0, c->stream);
+    mdk::launch_set_flag(&c->sc->overflow, 0, c->stream);
 }
 
 }  // namespace
@@ -427,7 +439,8 @@ md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, 
                            const double* z, const double* vx, const double* vy, const double* vz,
                            const double* m) {
     if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
-    if (n <= 0 || n >= 0x7fffffffL) return fail(c, MD_ERR_INVALID_ARGUMENT, "atom count out of range");
+    if (n <= 0 || n >= mdk::kMaxListAtoms)
+        return fail(c, MD_ERR_INVALID_ARGUMENT, "atom count out of range (1 .. 2^28 - 1)");
     if (L == nullptr || x == nullptr || y == nullptr || z == nullptr)
         return fail(c, MD_ERR_INVALID_ARGUMENT, "box and positions are required");
     for (int d = 0; d < 3; ++d)
@@ -449,7 +462,7 @@ md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, 
     std::vector<double4> p(static_cast<size_t>(n)), v(static_cast<size_t>(n)), f(static_cast<size_t>(n));
     std::vector<int> id(static_cast<size_t>(n));
     for (long i = 0; i < n; ++i) {
-        p[i] = make_double4(x[i], y[i], z[i], 0.0);
+        p[i] = make_double4(x[i], y[i], z[i], static_cast<double>(i));  // w: upload index
         v[i] = make_double4(vx ? vx[i] : 0.0, vy ? vy[i] : 0.0, vz ? vz[i] : 0.0, m ? m[i] : 1.0);
         f[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         id[i] = static_cast<int>(i);
@@ -499,7 +512,7 @@ md_status md_system_set_positions(md_ctx* c, const double* x, const double* y, c
     MD_HIP(c, hipMemcpyAsync(s, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + n, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + 2 * n, z, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->pos, 0, c->stream);
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->pos, mdk::kPermuteWIndex, c->stream);
     MD_HIP(c, hipGetLastError());
     MD_HIP(c, hipStreamSynchronize(c->stream));
     return MD_OK;
@@ -513,7 +526,7 @@ md_status md_system_set_velocities(md_ctx* c, const double* vx, const double* vy
     MD_HIP(c, hipMemcpyAsync(s, vx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + n, vy, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + 2 * n, vz, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->vel, 1, c->stream);
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->vel, mdk::kPermuteWKeep, c->stream);
     MD_HIP(c, hipGetLastError());
     MD_HIP(c, hipStreamSynchronize(c->stream));
     return MD_OK;
@@ -527,7 +540,7 @@ md_status md_system_set_forces(md_ctx* c, const double* fx, const double* fy, co
     MD_HIP(c, hipMemcpyAsync(s, fx, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + n, fy, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     MD_HIP(c, hipMemcpyAsync(s + 2 * n, fz, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->frc, 0, c->stream);
+    mdk::launch_permute_in(n, c->orig, s, s + n, s + 2 * n, c->frc, mdk::kPermuteWZero, c->stream);
     MD_HIP(c, hipGetLastError());
     MD_HIP(c, hipStreamSynchronize(c->stream));
     return MD_OK;
@@ -574,6 +587,7 @@ md_status md_lj_forces(md_ctx* c, int source, int precision, double* energy, dou
     md_status st = need_system(c);
     if (st != MD_OK) return st;
     if (!c->lj_set) return fail(c, MD_ERR_STATE, "md_lj_set has not been called");
+    clear_call_flags(c);
     if (source == MD_PAIRS_VERLET) {
         if (!c->list_built) return fail(c, MD_ERR_STATE, "the Verlet list has not been built");
         mdk::ForceArgs a = force_args(c, precision);
@@ -750,7 +764,7 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
         const int k = std::min(cnt[t], c->cap);
         for (int q = 0; q < k; ++q) {
             const uint32_t e = nb[static_cast<size_t>(mdk::oct_index(t, q, c->cap))];
-            const uint32_t j = e & mdk::kIndexMask;
+            const int j = mdk::entry_slot(e);
             const uint32_t a = static_cast<uint32_t>(orig[t]), b = static_cast<uint32_t>(orig[j]);
             if (a < b) keys.push_back((static_cast<uint64_t>(a) << 32) | b);
         }
@@ -812,6 +826,7 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     if (!c->lj_set || !c->integ_set || !c->list_built)
         return fail(c, MD_ERR_STATE, "md_run needs md_lj_set, md_integrator_set and a built list");
     if (nsteps <= 0) return MD_OK;
+    clear_call_flags(c);
     if (thermo && nsteps > c->thermo_alloc) {
         MD_HIP(c, alloc(c->thermo_dev, nsteps));
         c->thermo_alloc = nsteps;

@@ -7,10 +7,12 @@
 //   frc  double4  fx, fy, fz, (w unused)         32 B/atom
 //   ref  double4  positions at the last list build (displacement check)
 //   orig int32    upload index of slot t; slot int32 inverse map
-//   nbr  uint32   full Verlet list, column-major: nbr[k*npad + t], bit 31 set
-//                 when the upload index of t is larger than that of the
-//                 neighbour (the pair is oriented as the reference's i<j)
+//   nbr  uint32   full Verlet list in the octet layout (oct_index); an entry
+//                 is its partner's byte offset in the 16-byte fixed-point
+//                 position array, slot << 4 (entry_of / entry_slot)
 //   ncnt int32    row length of t
+// pos.w holds the atom's upload index (as a double): the orientation of a
+// pair (the reference's i < j) is read from the two positions themselves.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -166,6 +168,15 @@ __device__ __forceinline__ void pair_delta(const Box& b, const double4& pa, cons
     min_image(b, dx, dy, dz);
 }
 
+// Main-list entries: the partner's slot shifted left by 4, the byte offset of
+// its 16-byte fixed-point position, so the mixed pair kernel adds it to a
+// scalar base with no arithmetic. Slots stay below 2^28.
+constexpr long kMaxListAtoms = 1L << 28;
+__host__ __device__ __forceinline__ uint32_t entry_of(int slot) {
+    return static_cast<uint32_t>(slot) << 4;
+}
+__host__ __device__ __forceinline__ int entry_slot(uint32_t e) { return static_cast<int>(e >> 4); }
+
 // Row-oriented minimum image d = r_i - r_j with the bits of pair_delta's
 // result (negated when flip): when every component lies inside the fast band
 // (|v| <= 0.4999999 L) the fold is the identity, which is odd, so no
@@ -179,6 +190,26 @@ __device__ __forceinline__ void row_delta(const Box& b, const double4& pi, const
     dy = pi.y - pj.y;
     dz = pi.z - pj.z;
     if (!(fabs(dx) <= b.lo[0] && fabs(dy) <= b.lo[1] && fabs(dz) <= b.lo[2])) {
+        dx = flip ? -dx : dx;
+        dy = flip ? -dy : dy;
+        dz = flip ? -dz : dz;
+        min_image(b, dx, dy, dz);
+        dx = flip ? -dx : dx;
+        dy = flip ? -dy : dy;
+        dz = flip ? -dz : dz;
+    }
+}
+
+// row_delta with the orientation read from the upload indices the positions
+// carry in w (the pair is oriented i < j by upload index).
+__device__ __forceinline__ void row_delta_w(const Box& b, const double4& pi, const double4& pj,
+                                            double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+    dx = pi.x - pj.x;
+    dy = pi.y - pj.y;
+    dz = pi.z - pj.z;
+    if (!(fabs(dx) <= b.lo[0] && fabs(dy) <= b.lo[1] && fabs(dz) <= b.lo[2])) {
+        const bool flip = pi.w > pj.w;
         dx = flip ? -dx : dx;
         dy = flip ? -dy : dy;
         dz = flip ? -dz : dz;

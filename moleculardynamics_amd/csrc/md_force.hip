@@ -13,13 +13,15 @@
 // FP64: the minimum image and r2 are evaluated exactly as the reference does
 // (same cutoff decisions); s2 and w/r2 use one reciprocal (a last-bit
 // difference from the reference's two divisions, far inside 1e-12).
-// MIXED: d is formed in FP64 and rounded to FP32; r2, s2, s6, u, w and the
-// force components are FP32; per-atom force, energy and virial accumulate
-// in 64-bit fixed point (scale 2^32), so sums are exact integer additions.
-// When the list keeps periodic fixed-point positions (Quant), the mixed
-// kernel gathers those (16 bytes per partner instead of 32) and takes d from
-// them in FP64; a row with a cutoff decision too close to rc is evaluated
-// again from the FP64 positions (pair32q).
+// MIXED: the partner's periodic fixed-point position (Quant) gives d as exact
+// integer differences and r2 in FP64 from them; the LJ powers and c = w/r2
+// are FP32; the row sums c d, s^6 and s^12 in FP64 (AccMix), and each row's
+// force, energy and virial become 64-bit fixed point (2^-32), so every sum
+// of rows or lanes is an exact integer addition. A row with a cutoff decision
+// too close to rc for the fixed-point r2 is evaluated again from the FP64
+// positions with the reference's test (mix_exact), so the pairs inside the
+// cutoff are always the FP64 kernel's. Without fixed-point positions (boxes
+// longer than 128 rc) the mixed kernel reads the FP64 positions.
 //
 // E and W come out of a fixed reduction tree: wave shuffles, a 4-wave LDS
 // step, per-workgroup partials, and a one-workgroup finish kernel summing the
@@ -150,20 +152,37 @@ __device__ ChainStep chain_step(Chain* chain, double& ke, double delta, bool twi
 // Final reduction (one workgroup of kFinish threads, launched after the pair
 // kernel): the nb per-workgroup partials summed in a fixed order -- a strided
 // share per thread, wave shuffles, then the waves in index order. Thread 0
-// then advances the Nose-Hoover chain (chain_step).
+// then advances the Nose-Hoover chain (chain_step). The partials come from
+// every die's L2, so the loads of a thread's share are issued together,
+// kUnroll at a time, each into its own accumulator (combined in a fixed
+// order), rather than one dependent round trip per partial.
 constexpr int kFinish = 1024;
+constexpr int kUnroll = 8;
 
 __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
     __shared__ double red_d[3][kFinish / kWave];
     __shared__ long long red_l[3][kFinish / kWave];
     double sd[3] = {0.0, 0.0, 0.0};
     long long sl[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nb; b += kFinish) {
+    for (int b0 = threadIdx.x; b0 < nb; b0 += kFinish * kUnroll) {
+        double vd[kUnroll][3];
+        long long vl[kUnroll][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            sd[q] += a.part_d[3 * b + q];
-            sl[q] += a.part_l[3 * b + q];
+        for (int u = 0; u < kUnroll; ++u) {
+            const int b = b0 + u * kFinish;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                vd[u][q] = b < nb ? a.part_d[3 * b + q] : 0.0;
+                vl[u][q] = b < nb ? a.part_l[3 * b + q] : 0;
+            }
         }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                sd[q] += vd[u][q];
+                sl[q] += vl[u][q];
+            }
     }
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
@@ -231,11 +250,10 @@ struct Acc64 {
 };
 
 template <bool LFS>
-__device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, uint32_t jj,
-                                       const double4& pj, Acc64& q) {
-    const bool flip = (jj & kFlipBit) != 0;
+__device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, const double4& pj,
+                                       Acc64& q) {
     double dx, dy, dz;
-    row_delta(a.box, pi, pj, flip, dx, dy, dz);
+    row_delta_w(a.box, pi, pj, dx, dy, dz);
     const double r2 = norm2_ref(dx, dy, dz);
     if (r2 < a.lj.rc2) {
         // hardware reciprocal refined by two Newton steps (within an ulp of 1/r2)
@@ -263,118 +281,120 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, ui
     }
 }
 
-// Biased fixed-point accumulators (fix_biased_scaled / fix_value): n terms
-// each; r2min is the smallest r2 of an accumulated term, from which the end
-// of the row bounds every term's magnitude (fix_may_saturate).
-struct AccFix {
-    unsigned long long fx = 0, fy = 0, fz = 0, e = 0, w = 0;
-    unsigned int n = 0;  // biases added (at most the row capacity)
-    unsigned int k = 0;  // terms inside the cutoff
-    unsigned int r2min = 0x7f800000u;  // FP32 bits of the smallest r2 of a term
-    // pairs below the upper edge of the guard band (fixed-point walk); a row
-    // with kb != k had a cutoff decision inside the band
-    unsigned int kb = 0;
+// ---- mixed precision ---------------------------------------------------------
+//
+// Per entry: the coordinate differences are exact integers (fixed-point
+// positions, one wrapping 32-bit subtraction per axis), converted to FP64
+// once; r2 is formed in FP64 from them, so its only error is the fixed-point
+// quantisation (one unit per axis) and it decides the cutoff outside a narrow
+// band. The LJ powers are FP32: s2 = sigma^2 / r2 (hardware reciprocal and one
+// Newton step), s6 = s2^3, t = 2 s6^2 - s6 (one rounding); c = t / r2. The
+// force term c d is one FP64 FMA per axis into the row's sum, and the row
+// also sums s6 and t in FP64 (E = 2 eps (T - S6), W = 24 eps T per row). FP32 values have 24-bit mantissas, so these FP64 row sums are
+// exact to about 2^-53 relative and their order (list order) is fixed. Each
+// row's force, energy and virial then become 64-bit fixed point (2^-32
+// units), and every combination of rows or lanes -- multi-lane groups, the
+// workgroup and global E and W -- is an integer sum, independent of order.
+struct AccMix {
+    double fx = 0.0, fy = 0.0, fz = 0.0;  // sum of c d (scaled lengths)
+    double s6 = 0.0, t = 0.0;             // sums of s^6 and t = 2 s^12 - s^6
+    double r = 0.0;                       // LinearForceShift: sum of r (scaled)
+    int k = 0;                            // pairs inside the cutoff
 };
 
-// True when some term of a row whose pairs all have r2 >= r2min could reach
-// the fixed-point limit kFixLimitScaled (in units of 2^-32). With s6 =
-// (sigma^2 / r2)^3 falling in r2, every term is bounded at r2min:
-// |u| <= e4s (s6^2 + s6) [+ |u_rcs| + rc |dudr_rcs|],
-// |W| <= e24s (2 s6^2 + s6) [+ rc |dudr_rcs|], |f_x| <= |W| / r.
-// The check is conservative (bounds, 1e-6 relative margin for the FP32
-// rounding of the terms) and costs one minimum per pair.
-__device__ __forceinline__ bool fix_may_saturate(const LJConst& k, unsigned int r2min_bits) {
-    const float r2min = __uint_as_float(r2min_bits);
-    if (!(r2min < __builtin_huge_valf())) return false;
-    const double r2 = r2min;
-    const double s2 = static_cast<double>(k.sig2f) / r2;
-    const double s6 = s2 * s2 * s2;
-    double ub = fabs(static_cast<double>(k.e4s)) * (s6 * s6 + s6);
-    double wb = fabs(static_cast<double>(k.e24s)) * (2.0 * s6 * s6 + s6);
-    if (k.lfs) {
-        const double sh = static_cast<double>(k.rcf) * fabs(static_cast<double>(k.dudr_rcs));
-        ub += fabs(static_cast<double>(k.u_rcs)) + sh;
-        wb += sh;
-    }
-    const double tb = wb / sqrt(r2);
-    return fmax(ub, fmax(wb, tb)) * (1.0 + 1e-6) >= static_cast<double>(kFixLimitScaled);
-}
-
-// FP32 terms of a pair inside the cutoff, accumulated in fixed point. The
-// energy and force coefficients carry the fixed-point scale 2^32 (e4s, e24s,
-// u_rcs, dudr_rcs), so every FP32 term comes out multiplied by 2^32 exactly:
-// a power-of-two factor commutes with each rounding. The term's fixed-point
-// value round(v 2^32) is then one FP64 add of the bias.
-// IN: the pair is inside the cutoff. acc32 is also called branch-free with
-// IN false for a pair outside: its terms are then zeroed, adding only the
-// bias (and not counted), so a row's lanes need no divergent update.
+// The FP32 terms of one entry. A pair outside the cutoff (in false) enters
+// with s6 = 0 and adds nothing, so a wave's lanes need no divergent update.
 template <bool LFS>
-__device__ __forceinline__ void acc32(const ForceArgs& a, double r2d, double dx, double dy,
-                                      double dz, AccFix& q, bool in = true) {
-    const float r2 = static_cast<float>(r2d);
-    float inv = __builtin_amdgcn_rcpf(r2);
-    inv = __builtin_fmaf(inv, __builtin_fmaf(-r2, inv, 1.0f), inv);
-    const float s2 = a.lj.sig2f * inv;
+__device__ __forceinline__ void mix_term(const MixK& m, double dx, double dy, double dz, double r2,
+                                         bool in, AccMix& q) {
+    const float r2f = static_cast<float>(r2);
+    float y = __builtin_amdgcn_rcpf(r2f);
+    y = __builtin_fmaf(y, __builtin_fmaf(-r2f, y, 1.0f), y);
+    const float s2 = m.sig2s * y;
     float s6 = (s2 * s2) * s2;
-    if (!LFS) s6 = in ? s6 : 0.0f;  // zero s6 zeroes every Truncated term
-    float u = a.lj.e4s * __builtin_fmaf(s6, s6, -s6);
-    float ww = a.lj.e24s * __builtin_fmaf(2.0f * s6, s6, -s6);
+    s6 = in ? s6 : 0.0f;
+    // t = 2 s12 - s6 with one rounding: it cancels near the potential minimum
+    const float tt = __builtin_fmaf(2.0f * s6, s6, -s6);
+    float t = tt;
     if (LFS) {
-        const float r = sqrtf(r2);
-        u = u - a.lj.u_rcs - (r - a.lj.rcf) * a.lj.dudr_rcs;
-        ww = ww + r * a.lj.dudr_rcs;
-        u = in ? u : 0.0f;
-        ww = in ? ww : 0.0f;
+        const float rs = in ? __builtin_sqrtf(r2f) : 0.0f;
+        t = __builtin_fmaf(m.krs, rs, t);
+        q.r += static_cast<double>(rs);
     }
-    const float c = ww * inv;
-    const float tx = c * static_cast<float>(dx);
-    const float ty = c * static_cast<float>(dy);
-    const float tz = c * static_cast<float>(dz);
-    q.e += fix_biased_scaled(u);
-    q.w += fix_biased_scaled(ww);
-    q.fx += fix_biased_scaled(tx);
-    q.fy += fix_biased_scaled(ty);
-    q.fz += fix_biased_scaled(tz);
-    q.n += 1;  // every call adds one bias
-    q.k += in ? 1u : 0u;
-    // r2 > 0, so its bits order as unsigned integers. A pair outside the
-    // cutoff has r2 above every pair inside, so it changes r2min only in a
-    // row without terms, where the bound stays far below the limit.
-    q.r2min = min(q.r2min, __float_as_uint(r2));
+    const double c = static_cast<double>(t * y);
+    q.fx = fma(c, dx, q.fx);
+    q.fy = fma(c, dy, q.fy);
+    q.fz = fma(c, dz, q.fz);
+    q.s6 += static_cast<double>(s6);
+    q.t += static_cast<double>(tt);
+    q.k += in ? 1 : 0;
 }
 
-template <bool LFS>
-__device__ __forceinline__ void pair32(const ForceArgs& a, const double4& pi, uint32_t jj,
-                                       const double4& pj, AccFix& q) {
-    const bool flip = (jj & kFlipBit) != 0;
-    double dx, dy, dz;
-    row_delta(a.box, pi, pj, flip, dx, dy, dz);
-    const double r2d = norm2_ref(dx, dy, dz);
-    if (r2d < a.lj.rc2) acc32<LFS>(a, r2d, dx, dy, dz, q);
+// Entry from fixed-point positions. CUBIC: one unit on every axis, so the
+// integer differences are the scaled lengths (s = u); otherwise they are
+// scaled to real lengths per axis (s = 1). The two orientations of a pair
+// give integer differences of opposite sign, hence the same r2 bits and the
+// same decision in both rows. A decision inside the band (hit) sends the row
+// to the exact evaluation.
+template <bool LFS, bool CUBIC>
+__device__ __forceinline__ void mix_fixed(const ForceArgs& a, const int4& qi, const int4& qj,
+                                          AccMix& q, bool& hit) {
+    double dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+    double dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+    double dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+    if (!CUBIC) {
+        dx *= a.quant.unit[0];
+        dy *= a.quant.unit[1];
+        dz *= a.quant.unit[2];
+    }
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    const bool in = r2 < a.mq.lo;
+    hit = hit || (!in && r2 < a.mq.hi);
+    mix_term<LFS>(a.mq, dx, dy, dz, r2, in, q);
 }
 
-// Mixed pair from fixed-point positions: one 16-byte gather per partner. The
-// fixed-point r2 is within band_c (host-computed from the unit L / 2^32)
-// relative of the exact one, so it decides the cutoff outside that band
-// around rc2. A pair inside the band marks the row for redo: the row is then
-// evaluated again with pair32 (d and the cutoff test from the FP64
-// positions), so the set of pairs inside the cutoff is always the one the
-// FP64 kernel uses. Band hits are rare (about 1e-5 of the rows), and the redo
-// loop runs after the fixed-point one, so it adds no register pressure to it.
-// d is r_i - r_j in both orientations of an entry (within one unit of the
-// exact difference per axis), and the two rows of a pair see r2 bits that
-// are the same.
+// Entry from the FP64 positions with the reference's exact cutoff test (the
+// FP64 kernel's decision), in the units of m.
 template <bool LFS>
-__device__ __forceinline__ void pair32q(const ForceArgs& a, const int4& qi, const int4& qj,
-                                        AccFix& q) {
+__device__ __forceinline__ void mix_exact(const ForceArgs& a, const MixK& m, const double4& pi,
+                                          const double4& pj, AccMix& q) {
     double dx, dy, dz;
-    qdelta(a.quant, qi, qj, dx, dy, dz);
-    const double r2d = dx * dx + dy * dy + dz * dz;
-    const bool in = r2d < a.lj.rc2 * (1.0 - a.quant.band_c);
-    // no branch per pair: the band test only counts, and a pair outside the
-    // cutoff adds zero terms
-    q.kb += r2d < a.lj.rc2 * (1.0 + a.quant.band_c) ? 1u : 0u;
-    acc32<LFS>(a, r2d, dx, dy, dz, q, in);
+    row_delta_w(a.box, pi, pj, dx, dy, dz);
+    const double r2 = norm2_ref(dx, dy, dz);
+    const bool in = r2 < a.lj.rc2;
+    const double is = m.inv_s;
+    mix_term<LFS>(m, dx * is, dy * is, dz * is, r2 * (is * is), in, q);
+}
+
+// 64-bit fixed point (2^-32 units) of v, rounded to nearest. |v| < 2^30
+// holds for every physical force, energy and virial of a row; larger values
+// raise `sat` (the conversion would overflow near 2^31).
+__device__ __forceinline__ long long to_fix(double v, bool& sat) {
+    sat = sat || !(fabs(v) < 1073741824.0);
+    return __double2ll_rn(v * kFixScale);
+}
+
+struct FixRow {
+    long long fx, fy, fz, e, w;
+};
+
+// A row's (or a lane's share of a row's) sums as fixed point.
+template <bool LFS>
+__device__ __forceinline__ FixRow mix_out(const MixK& m, const AccMix& q, bool& sat) {
+    // E = 4 eps (S12 - S6) = 2 eps (T - S6), W = 24 eps T
+    double e = 0.5 * m.e4 * (q.t - q.s6);
+    double w = m.e24 * q.t;
+    if (LFS) {
+        e -= static_cast<double>(q.k) * m.ushift + m.urs * q.r;
+        w += m.urs * q.r;
+    }
+    FixRow r;
+    r.fx = to_fix(m.fscale * q.fx, sat);
+    r.fy = to_fix(m.fscale * q.fy, sat);
+    r.fz = to_fix(m.fscale * q.fz, sat);
+    r.e = to_fix(e, sat);
+    r.w = to_fix(w, sat);
+    return r;
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -423,36 +443,37 @@ __device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, Ga
     }
 }
 
-// Partners from the FP64 positions.
-template <class Acc, class PairFn>
-__device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt,
-                                             const double4& pi, Acc& q, PairFn pf) {
+// Partners from the FP64 positions: an entry is slot << 4, a double4 lies at
+// byte offset slot << 5.
+template <class PairFn>
+__device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt, PairFn pf) {
+    const char* pb = reinterpret_cast<const char*>(a.pos);
+    auto at = [&](uint32_t e) { return *reinterpret_cast<const double4*>(pb + (static_cast<size_t>(e) << 1)); };
     walk_row<double4>(
         a, t, cnt,
         [&](const u32x4& j, double4* p) {
-            p[0] = a.pos[j.x & kIndexMask];
-            p[1] = a.pos[j.y & kIndexMask];
-            p[2] = a.pos[j.z & kIndexMask];
-            p[3] = a.pos[j.w & kIndexMask];
+            p[0] = at(j.x);
+            p[1] = at(j.y);
+            p[2] = at(j.z);
+            p[3] = at(j.w);
         },
-        [&](const u32x4& j, const double4* p) {
-            pf(a, pi, j.x, p[0], q);
-            pf(a, pi, j.y, p[1], q);
-            pf(a, pi, j.z, p[2], q);
-            pf(a, pi, j.w, p[3], q);
+        [&](const u32x4&, const double4* p) {
+            pf(p[0]);
+            pf(p[1]);
+            pf(p[2]);
+            pf(p[3]);
         },
-        [&](uint32_t e) { pf(a, pi, e, a.pos[e & kIndexMask], q); });
+        [&](uint32_t e) { pf(at(e)); });
 }
 
-// Partners from the fixed-point positions (mixed kernel).
-template <bool LFS>
-__device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt,
-                                               const int4& qi, AccFix& q) {
-    // a partner's byte offset is its entry shifted left by 4 in 32 bits: the
-    // flip bit leaves the word, and the load takes the base from scalar
-    // registers and the offset from one vector register (n < 2^28 on this path)
+// Partners from the fixed-point positions (mixed kernel): an entry is the
+// partner's byte offset from the array base, so a gather is a scalar base
+// plus one vector register. Only x, y, z are read.
+template <bool LFS, bool CUBIC>
+__device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt, const int4& qi,
+                                               AccMix& q, bool& hit) {
     const char* qb = reinterpret_cast<const char*>(a.qpos);
-    auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + (e << 4)); };
+    auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + e); };
     walk_row<int4>(
         a, t, cnt,
         [&](const u32x4& j, int4* p) {
@@ -461,17 +482,17 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
             p[2] = at(j.z);
             p[3] = at(j.w);
         },
-        [&](const u32x4& j, const int4* p) {
-            pair32q<LFS>(a, qi, p[0], q);
-            pair32q<LFS>(a, qi, p[1], q);
-            pair32q<LFS>(a, qi, p[2], q);
-            pair32q<LFS>(a, qi, p[3], q);
+        [&](const u32x4&, const int4* p) {
+            mix_fixed<LFS, CUBIC>(a, qi, p[0], q, hit);
+            mix_fixed<LFS, CUBIC>(a, qi, p[1], q, hit);
+            mix_fixed<LFS, CUBIC>(a, qi, p[2], q, hit);
+            mix_fixed<LFS, CUBIC>(a, qi, p[3], q, hit);
         },
-        [&](uint32_t e) { pair32q<LFS>(a, qi, at(e), q); });
+        [&](uint32_t e) { mix_fixed<LFS, CUBIC>(a, qi, at(e), q, hit); });
 }
 
 // QP: the mixed kernel reads fixed-point positions (a.qpos) for the partners.
-template <bool LFS, bool MIXED, bool QP, bool KICK>
+template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
     __shared__ long long sm_l[kBlock / kWave];
@@ -485,7 +506,8 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         double fx, fy, fz;
         if (!MIXED) {
             Acc64 q;
-            walk_row_pos(a, t, cnt, a.pos[t], q, pair64<LFS>);
+            const double4 pi = a.pos[t];
+            walk_row_pos(a, t, cnt, [&](const double4& pj) { pair64<LFS>(a, pi, pj, q); });
             fx = q.fx;
             fy = q.fy;
             fz = q.fz;
@@ -493,22 +515,28 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             w = q.w;
             nt = q.n;
         } else {
-            AccFix q;
+            AccMix q;
+            const MixK& m = QP ? a.mq : a.mx;
             if (QP) {
-                walk_row_fixed<LFS>(a, t, cnt, a.qpos[t], q);
-                if (q.kb != q.k) {
-                    q = AccFix{};
-                    walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
+                bool hit = false;
+                walk_row_fixed<LFS, CUBIC>(a, t, cnt, a.qpos[t], q, hit);
+                if (hit) {  // rare: a decision inside the band, the row again exactly
+                    q = AccMix{};
+                    const double4 pi = a.pos[t];
+                    walk_row_pos(a, t, cnt, [&](const double4& pj) { mix_exact<LFS>(a, a.mq, pi, pj, q); });
                 }
             } else {
-                walk_row_pos(a, t, cnt, a.pos[t], q, pair32<LFS>);
+                const double4 pi = a.pos[t];
+                walk_row_pos(a, t, cnt, [&](const double4& pj) { mix_exact<LFS>(a, a.mx, pi, pj, q); });
             }
-            if (fix_may_saturate(a.lj, q.r2min)) atomicOr(&a.sc->fix_saturated, 1);
-            fx = static_cast<double>(fix_value(q.fx, q.n)) * kFixInv;
-            fy = static_cast<double>(fix_value(q.fy, q.n)) * kFixInv;
-            fz = static_cast<double>(fix_value(q.fz, q.n)) * kFixInv;
-            ef = fix_value(q.e, q.n);
-            wf = fix_value(q.w, q.n);
+            bool sat = false;
+            const FixRow r = mix_out<LFS>(m, q, sat);
+            if (sat) atomicOr(&a.sc->fix_saturated, 1);
+            fx = static_cast<double>(r.fx) * kFixInv;
+            fy = static_cast<double>(r.fy) * kFixInv;
+            fz = static_cast<double>(r.fz) * kFixInv;
+            ef = r.e;
+            wf = r.w;
             nt = q.k;
         }
         a.frc[t] = make_double4(fx, fy, fz, 0.0);
@@ -546,9 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
 // and every thread walks a dependent chain of ~80 gathers. LANES consecutive
 // lanes then share an atom: lane s takes entries s, s + LANES, ... of its row,
 // and the group's sums are combined by shuffles in a fixed butterfly (FP64:
-// deterministic; fixed point: the raw accumulators and bias counts are
-// summed as integers, so the forces, energy and virial are bit for bit those
-// of one lane per atom). A group redoes its row exactly when any of its lanes
+// deterministic; mixed: each lane's share becomes fixed point first and the
+// group adds integers). A group redoes its row exactly when any of its lanes
 // saw a band hit, which is the row's own condition.
 //
 // Above that size the mixed kernel keeps one lane per atom: it is bound by
@@ -578,12 +605,7 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
-template <int LANES>
-__device__ __forceinline__ unsigned long long group_sum_u64(unsigned long long v) {
-    return static_cast<unsigned long long>(group_sum<LANES>(static_cast<long long>(v)));
-}
-
-template <bool LFS, bool MIXED, bool QP, bool KICK, int LANES>
+template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK, int LANES>
 __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
     __shared__ long long sm_l[kBlock / kWave];
@@ -598,16 +620,15 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
     const int cnt = live ? min(a.ncnt[tt], a.cap) : 0;
     const uint32_t* row = a.nbr + oct_index(tt, 0, a.cap);
     auto entry = [&](int k) { return row[(k >> 3) * 512 + (k & 7)]; };
+    const char* pb = reinterpret_cast<const char*>(a.pos);
+    auto pos_of = [&](uint32_t e) { return *reinterpret_cast<const double4*>(pb + (static_cast<size_t>(e) << 1)); };
     double e = 0.0, w = 0.0, ke = 0.0;
     long long ef = 0, wf = 0, nt = 0;
     double fx, fy, fz;
     if (!MIXED) {
         Acc64 q;
         const double4 pi = a.pos[tt];
-        for (int k = sub; k < cnt; k += LANES) {
-            const uint32_t jj = entry(k);
-            pair64<LFS>(a, pi, jj, a.pos[jj & kIndexMask], q);
-        }
+        for (int k = sub; k < cnt; k += LANES) pair64<LFS>(a, pi, pos_of(entry(k)), q);
         fx = group_sum<LANES>(q.fx);
         fy = group_sum<LANES>(q.fy);
         fz = group_sum<LANES>(q.fz);
@@ -615,35 +636,33 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         w = group_sum<LANES>(q.w);
         nt = group_sum<LANES>(static_cast<long long>(q.n));
     } else {
-        AccFix q;
-        auto exact = [&]() {
+        AccMix q;
+        const MixK& m = QP ? a.mq : a.mx;
+        auto exact = [&](const MixK& mm) {
             const double4 pi = a.pos[tt];
-            for (int k = sub; k < cnt; k += LANES) {
-                const uint32_t jj = entry(k);
-                pair32<LFS>(a, pi, jj, a.pos[jj & kIndexMask], q);
-            }
+            for (int k = sub; k < cnt; k += LANES) mix_exact<LFS>(a, mm, pi, pos_of(entry(k)), q);
         };
         if (QP) {
             const int4 qi = a.qpos[tt];
             const char* qb = reinterpret_cast<const char*>(a.qpos);
+            bool hit = false;
             for (int k = sub; k < cnt; k += LANES)
-                pair32q<LFS>(a, qi, *reinterpret_cast<const int4*>(qb + (entry(k) << 4)), q);
-            if (group_sum<LANES>(static_cast<int>(q.kb != q.k)) != 0) {
-                q = AccFix{};
-                exact();
+                mix_fixed<LFS, CUBIC>(a, qi, *reinterpret_cast<const int4*>(qb + entry(k)), q, hit);
+            if (group_sum<LANES>(hit ? 1 : 0) != 0) {
+                q = AccMix{};
+                exact(a.mq);
             }
         } else {
-            exact();
+            exact(a.mx);
         }
-        unsigned int r2min = q.r2min;
-        for (int off = LANES / 2; off > 0; off >>= 1) r2min = min(r2min, static_cast<unsigned int>(__shfl_xor(static_cast<int>(r2min), off, kWave)));
-        const unsigned long long nb = group_sum_u64<LANES>(q.n);
-        if (live && sub == 0 && fix_may_saturate(a.lj, r2min)) atomicOr(&a.sc->fix_saturated, 1);
-        fx = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fx), nb)) * kFixInv;
-        fy = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fy), nb)) * kFixInv;
-        fz = static_cast<double>(fix_value(group_sum_u64<LANES>(q.fz), nb)) * kFixInv;
-        ef = fix_value(group_sum_u64<LANES>(q.e), nb);
-        wf = fix_value(group_sum_u64<LANES>(q.w), nb);
+        bool sat = false;
+        const FixRow r = mix_out<LFS>(m, q, sat);
+        if (live && sat) atomicOr(&a.sc->fix_saturated, 1);
+        fx = static_cast<double>(group_sum<LANES>(r.fx)) * kFixInv;
+        fy = static_cast<double>(group_sum<LANES>(r.fy)) * kFixInv;
+        fz = static_cast<double>(group_sum<LANES>(r.fz)) * kFixInv;
+        ef = group_sum<LANES>(r.e);
+        wf = group_sum<LANES>(r.w);
         nt = group_sum<LANES>(static_cast<long long>(q.k));
     }
     if (live && sub == 0) {
@@ -682,28 +701,28 @@ __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
-template <bool LFS, bool MIXED, bool QP, bool KICK>
+template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK>
 void dispatch_lanes(const ForceArgs& a, hipStream_t s) {
     const dim3 g(static_cast<unsigned>(lj_groups(a.n, MIXED))), b(kBlock);
     switch (lj_lanes(a.n, MIXED)) {
-        case 16: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 16>), g, b, 0, s, a); break;
-        case 8: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 8>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 4>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, KICK, 2>), g, b, 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 16>), g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 8>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 4>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 2>), g, b, 0, s, a); break;
     }
 }
 
-template <bool LFS, bool MIXED, bool QP>
+template <bool LFS, bool MIXED, bool QP, bool CUBIC>
 void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
     if (lj_lanes(a.n, MIXED) > 1) {
         if (a.kick)
-            dispatch_lanes<LFS, MIXED, QP, true>(a, s);
+            dispatch_lanes<LFS, MIXED, QP, CUBIC, true>(a, s);
         else
-            dispatch_lanes<LFS, MIXED, QP, false>(a, s);
+            dispatch_lanes<LFS, MIXED, QP, CUBIC, false>(a, s);
     } else if (a.kick) {
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, true>), g, dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, CUBIC, true>), g, dim3(kBlock), 0, s, a);
     } else {
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, false>), g, dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, CUBIC, false>), g, dim3(kBlock), 0, s, a);
     }
 }
 
@@ -923,14 +942,17 @@ long lj_partials(long n) { return std::max(lj_groups(n, false), lj_groups(n, tru
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
     const dim3 g(xcd_grid(a.n));
     const bool qp = a.qpos != nullptr;
+    const bool cubic = a.quant.cubic != 0;
     if (a.lj.lfs) {
-        if (!a.mixed) dispatch_kick<true, false, false>(a, g, s);
-        else if (qp) dispatch_kick<true, true, true>(a, g, s);
-        else dispatch_kick<true, true, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<true, false, false, false>(a, g, s);
+        else if (qp && cubic) dispatch_kick<true, true, true, true>(a, g, s);
+        else if (qp) dispatch_kick<true, true, true, false>(a, g, s);
+        else dispatch_kick<true, true, false, false>(a, g, s);
     } else {
-        if (!a.mixed) dispatch_kick<false, false, false>(a, g, s);
-        else if (qp) dispatch_kick<false, true, true>(a, g, s);
-        else dispatch_kick<false, true, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<false, false, false, false>(a, g, s);
+        else if (qp && cubic) dispatch_kick<false, true, true, true>(a, g, s);
+        else if (qp) dispatch_kick<false, true, true, false>(a, g, s);
+        else dispatch_kick<false, true, false, false>(a, g, s);
     }
 }
 
@@ -938,6 +960,24 @@ void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
     // the pair kernel writes partials for its lj_groups(n) blocks (padding blocks of
     // the XCD-aware grid none)
     hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(lj_groups(a.n, a.mixed != 0)));
+}
+
+MixK mix_constants(const LJConst& k, double s, double band) {
+    MixK m{};
+    m.inv_s = 1.0 / s;
+    m.sig2s = static_cast<float>(k.sig2 / (s * s));
+    const double rc2s = k.rc2 / (s * s);
+    m.lo = rc2s * (1.0 - band);
+    m.hi = rc2s * (1.0 + band);
+    m.e4 = 4.0 * k.eps;
+    m.e24 = 24.0 * k.eps;
+    m.fscale = m.e24 / s;
+    if (k.lfs) {
+        m.ushift = k.u_rc - k.rc * k.dudr_rc;
+        m.urs = k.dudr_rc * s;
+        m.krs = k.eps != 0.0 ? static_cast<float>(k.dudr_rc * s / m.e24) : 0.0f;
+    }
+    return m;
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

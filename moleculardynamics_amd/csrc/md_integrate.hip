@@ -80,14 +80,16 @@ __device__ __forceinline__ double displacement(const Box& b, const double4& p, c
     return sqrt(norm2_ref(dx, dy, dz));
 }
 
-// One workgroup of kFinish threads: strided shares, wave top-2 by shuffles,
-// then the waves' pairs merged by thread 0.
+// The decision from nb per-workgroup top-2 partials, by one workgroup
+// (blockDim.x threads: a multiple of the wave size, at most kFinish):
+// strided shares, wave top-2 by shuffles, then the waves' pairs merged by
+// thread 0.
 constexpr int kFinish = 1024;
 
 __device__ void finish_check(const StepArgs& a, int nb) {
     __shared__ double t1s[kFinish / kWave], t2s[kFinish / kWave];
     double t1 = 0.0, t2 = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kFinish)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
         top2_merge(t1, t2, a.part_d[2 * b], a.part_d[2 * b + 1]);
     wave_top2(t1, t2);
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -97,7 +99,7 @@ __device__ void finish_check(const StepArgs& a, int nb) {
     __syncthreads();
     if (threadIdx.x == 0) {
         double b1 = 0.0, b2 = 0.0;
-        for (int w = 0; w < kFinish / kWave; ++w) top2_merge(b1, b2, t1s[w], t2s[w]);
+        for (int w = 0; w < static_cast<int>(blockDim.x / kWave); ++w) top2_merge(b1, b2, t1s[w], t2s[w]);
         a.sc->top1 = b1;
         a.sc->top2 = b2;
         a.sc->rebuild_flag = (b1 + b2) > a.skin ? 1 : 0;
@@ -306,7 +308,7 @@ __global__ void k_unpermute(long n, const int* __restrict__ orig, const double4*
 
 __global__ void k_permute_in(long n, const int* __restrict__ orig, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             double4* __restrict__ a, int keep_w) {
+                             double4* __restrict__ a, int w_mode) {
     const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int o = orig[t];
@@ -314,7 +316,8 @@ __global__ void k_permute_in(long n, const int* __restrict__ orig, const double*
     v.x = x[o];
     v.y = y[o];
     v.z = z[o];
-    if (!keep_w) v.w = 0.0;
+    if (w_mode == kPermuteWZero) v.w = 0.0;
+    if (w_mode == kPermuteWIndex) v.w = static_cast<double>(o);
     a[t] = v;
 }
 
@@ -424,8 +427,8 @@ void launch_unpermute(long n, const int* orig, const double4* a, double* x, doub
 }
 
 void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
-                       double4* a, int keep_w, hipStream_t s) {
-    hipLaunchKernelGGL(k_permute_in, dim3(grid_of(n)), dim3(kBlock), 0, s, n, orig, x, y, z, a, keep_w);
+                       double4* a, int w_mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_in, dim3(grid_of(n)), dim3(kBlock), 0, s, n, orig, x, y, z, a, w_mode);
 }
 
 void launch_half_kick(long n, double4* vel, const double4* frc, double half_dt, hipStream_t s) {

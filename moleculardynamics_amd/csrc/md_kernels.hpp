@@ -92,10 +92,27 @@ struct LJConst {
     int lfs;
 };
 
+// Constants of the mixed pair terms in one length unit s: lengths in the
+// terms are lengths / s. On a cubic box with fixed-point positions s is the
+// fixed-point unit u, so an integer coordinate difference is the scaled
+// difference itself; otherwise s = 1. Host-computed (mix_constants).
+struct MixK {
+    float sig2s;     // sigma^2 / s^2
+    float krs;       // LinearForceShift: u'(rc) s / (24 eps) (0 when eps = 0)
+    double lo, hi;   // fixed-point cutoff band: rc^2 (1 -+ band) / s^2
+    double inv_s;    // 1 / s
+    double fscale;   // 24 eps / s: the row force is fscale * sum c d
+    double e4, e24;  // 4 eps, 24 eps
+    double ushift;   // LinearForceShift: u(rc) - rc u'(rc)
+    double urs;      // LinearForceShift: u'(rc) s
+};
+
 struct ForceArgs {
     long n;
     Box box;
     LJConst lj;
+    MixK mq;       // mixed terms from the fixed-point positions (s = u on a cubic box, else 1)
+    MixK mx;       // mixed terms from the FP64 positions (s = 1)
     const double4* pos;
     double4* vel;  // kicked when kick != 0
     double4* frc;
@@ -131,6 +148,8 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s);
 // One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
 // thermo row and NHC chain; launched right after launch_lj_force.
 void launch_lj_finish(const ForceArgs& a, hipStream_t s);
+// MixK for length unit s and relative cutoff band `band` (0: exact decisions).
+MixK mix_constants(const LJConst& k, double s, double band);
 // One chain propagation of length delta from sc->kinetic; the velocity
 // rescale is left pending in sc->vscale.
 void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s);
@@ -197,9 +216,12 @@ void launch_remove_com(long n, double4* vel, double* part_d, double* out4, hipSt
 // Upload-order scatter for download: out[orig[t]] = component.
 void launch_unpermute(long n, const int* orig, const double4* a, double* x, double* y, double* z,
                       hipStream_t s);
-// Upload-order gather for set_positions / set_velocities.
+// Upload-order gather for set_positions / set_velocities / set_forces; the
+// w component is zeroed, kept (velocities: the mass) or set to the upload
+// index (positions).
+constexpr int kPermuteWZero = 0, kPermuteWKeep = 1, kPermuteWIndex = 2;
 void launch_permute_in(long n, const int* orig, const double* x, const double* y, const double* z,
-                       double4* a, int keep_w, hipStream_t s);
+                       double4* a, int w_mode, hipStream_t s);
 void launch_set_flag(int* flag, int v, hipStream_t s);
 // Single integrator passes for host-driven steps.
 void launch_half_kick(long n, double4* vel, const double4* frc, double half_dt, hipStream_t s);

@@ -274,13 +274,13 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     auto drain = [&]() {
         if (cnt - done >= 8) write_octet();
     };
-    auto put = [&](int j, bool keep, bool flip) {
-        ring[cnt & 15][tid] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+    auto put = [&](int j, bool keep) {
+        ring[cnt & 15][tid] = entry_of(j);
         cnt += (keep && j != t) ? 1 : 0;
     };
-    auto exact = [&](const double4& q, bool flip) {
+    auto exact = [&](const double4& q) {
         double dx, dy, dz;
-        row_delta(box, pi, q, flip, dx, dy, dz);
+        row_delta_w(box, pi, q, dx, dy, dz);
         return norm2_ref(dx, dy, dz) < rl2;
     };
     const double u2 = CUBIC ? a.quant.unit[0] * a.quant.unit[0] : 1.0;
@@ -298,11 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
             qdelta(a.quant, qi, qj, dx, dy, dz);
             r2 = dx * dx + dy * dy + dz * dz;
         }
-        const bool flip = oi > qj.w;
         // no branch per candidate: a decision inside the band is only counted,
         // and the row is then built again with the exact test
         hits += (r2 >= lo && r2 < hi) ? 1u : 0u;
-        put(j, r2 < lo, flip);
+        put(j, r2 < lo);
     };
     // candidate loads with 32-bit byte offsets from a scalar base (n < 2^28
     // wherever fixed-point positions are kept)
@@ -328,16 +327,14 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
         } else {
             for (; j + 4 <= e; j += 4) {
                 const double4 q0 = pos2[j], q1 = pos2[j + 1], q2 = pos2[j + 2], q3 = pos2[j + 3];
-                const int o0 = orig2[j], o1 = orig2[j + 1], o2 = orig2[j + 2], o3 = orig2[j + 3];
-                put(j, exact(q0, oi > o0), oi > o0);
-                put(j + 1, exact(q1, oi > o1), oi > o1);
-                put(j + 2, exact(q2, oi > o2), oi > o2);
-                put(j + 3, exact(q3, oi > o3), oi > o3);
+                put(j, exact(q0));
+                put(j + 1, exact(q1));
+                put(j + 2, exact(q2));
+                put(j + 3, exact(q3));
                 drain();
             }
             for (; j < e; ++j) {
-                const int oj = orig2[j];
-                put(j, exact(pos2[j], oi > oj), oi > oj);
+                put(j, exact(pos2[j]));
                 drain();
             }
         }
@@ -423,11 +420,10 @@ __global__ __launch_bounds__(kBlock) void k_commit_build_lanes(NeighArgs a) {
     const double lo = rl2 * (1.0 - a.quant.band_l) / u2, hi = rl2 * (1.0 + a.quant.band_l) / u2;
     int cnt = 0;
     unsigned int hits = 0;
-    auto keep_entry = [&](bool keep, int j, bool flip) {
+    auto keep_entry = [&](bool keep, int j) {
         const unsigned long long m = __ballot(keep) & gmask;
         const int at = cnt + __popcll(m & below);
-        if (keep && at < cap)
-            a.nbr[oct_index(t, at, cap)] = static_cast<uint32_t>(j) | (flip ? kFlipBit : 0u);
+        if (keep && at < cap) a.nbr[oct_index(t, at, cap)] = entry_of(j);
         cnt += __popcll(m);
     };
     auto range = [&](int b, int e, bool fixed) {
@@ -435,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_build_lanes(NeighArgs a) {
             const int j = base + sub;
             const bool ok = j < e;
             const int jj = ok ? j : b;
-            bool keep, flip;
+            bool keep;
             if (Q && fixed) {
                 const int4 qj = a.qpos2[jj];
                 double r2;
@@ -449,17 +445,14 @@ __global__ __launch_bounds__(kBlock) void k_commit_build_lanes(NeighArgs a) {
                     qdelta(a.quant, qi, qj, dx, dy, dz);
                     r2 = dx * dx + dy * dy + dz * dz;
                 }
-                flip = oi > qj.w;
                 hits += (ok && r2 >= lo && r2 < hi) ? 1u : 0u;
                 keep = ok && r2 < lo && j != t;
             } else {
-                const int oj = a.orig2[jj];
-                flip = oi > oj;
                 double dx, dy, dz;
-                row_delta(a.box, pi, a.pos2[jj], flip, dx, dy, dz);
+                row_delta_w(a.box, pi, a.pos2[jj], dx, dy, dz);
                 keep = ok && norm2_ref(dx, dy, dz) < rl2 && j != t;
             }
-            keep_entry(keep, j, flip);
+            keep_entry(keep, j);
         }
     };
     // the same 25 ranges in the same order as k_commit_build

@@ -116,6 +116,33 @@ int orc_lj_forces(long n, const double* L, const double* x, const double* y, con
     });
 }
 
+// The pair terms of orc_lj_forces' linked-cell (mode 1) or brute-force
+// (mode 0) evaluation summed with compensated summation: out = {E, W, pairs
+// inside the cutoff}. A checker for systems of millions of pairs.
+int orc_lj_ev_compensated(long n, const double* L, const double* x, const double* y,
+                          const double* z, double sig, double eps, double rc, int lfs, int mode,
+                          double* out) {
+    return guarded([&] {
+        const State s = make_state(n, L, x, y, z);
+        const LJ lj(sig, eps, rc, lfs != 0);
+        EV comp;
+        long np = 0;
+        if (mode == 1) {
+            Cells c(s.L, rc);
+            c.build(s);
+            if (!c.usable()) throw std::invalid_argument("cell list is not usable");
+            lj_pair_sum(lj, s, nullptr, [&](const PairFn& fn) { c.pairs(fn); }, &comp, &np);
+        } else {
+            lj_pair_sum(lj, s, nullptr, [&](const PairFn& fn) { brute_pairs(s.n(), fn); }, &comp,
+                        &np);
+        }
+        out[0] = comp.energy;
+        out[1] = comp.virial;
+        out[2] = static_cast<double>(np);
+        return 0;
+    });
+}
+
 int orc_lj_lrc(long n, const double* L, double sig, double eps, double rc, int lfs, double* out) {
     State s;
     for (int d = 0; d < 3; ++d) s.L[d] = L[d];

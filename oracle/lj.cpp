@@ -9,7 +9,23 @@ namespace oracle {
 
 namespace {
 constexpr double kPi = 3.14159265358979323846;
-}
+
+// Neumaier's compensated summation (Neumaier 1974, "Rundungsfehleranalyse
+// einiger Verfahren zur Summation endlicher Summen", ZAMM 54:39-51): the
+// rounding error of every addition is carried in c.
+struct CompSum {
+    double s = 0.0, c = 0.0;
+    void add(double x) {
+        const double t = s + x;
+        if (std::fabs(s) >= std::fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    double value() const { return s + c; }
+};
+}  // namespace
 
 // u(rc) = 4 eps (s6^2 - s6), du/dr(rc) = -w(rc)/rc with s = sigma/rc
 // (lennard_jones.cpp:32-48).
@@ -33,8 +49,10 @@ void brute_pairs(std::size_t n, const PairFn& fn) {
 // u -= u(rc) + (r - rc) u'(rc) and w += r u'(rc); E += u, W += w; f = (w/r2) d
 // added to i and subtracted from j (lennard_jones.cpp:50-88).
 EV lj_pair_sum(const LJ& lj, const State& s, State* sink,
-               const std::function<void(const PairFn&)>& source) {
+               const std::function<void(const PairFn&)>& source, EV* compensated, long* pairs) {
     EV ev;
+    CompSum ce, cw;
+    long np = 0;
     const double sig2 = lj.sigma * lj.sigma;
     const double rc2 = lj.cutoff * lj.cutoff;
     source([&](std::size_t i, std::size_t j) {
@@ -53,6 +71,11 @@ EV lj_pair_sum(const LJ& lj, const State& s, State* sink,
         }
         ev.energy += u;
         ev.virial += w;
+        if (compensated != nullptr) {
+            ce.add(u);
+            cw.add(w);
+        }
+        ++np;
         if (sink != nullptr) {
             const double c = w / r2;
             const V3 f{c * d.x, c * d.y, c * d.z};
@@ -65,6 +88,11 @@ EV lj_pair_sum(const LJ& lj, const State& s, State* sink,
             sink->fz[j] += -1.0 * f.z;
         }
     });
+    if (compensated != nullptr) {
+        compensated->energy = ce.value();
+        compensated->virial = cw.value();
+    }
+    if (pairs != nullptr) *pairs = np;
     return ev;
 }
 

@@ -154,9 +154,15 @@ struct Verlet {
 };
 
 // Shared pair kernel (lennard_jones.cpp:50-88): one evaluation per offered
-// pair, cutoff test r2 < rc2, optional force accumulation.
+// pair, cutoff test r2 < rc2, optional force accumulation. When
+// `compensated` is given it also receives E and W summed with Neumaier's
+// compensated summation over the same pair terms (a checker for sums of
+// millions of terms, where the reference's sequential FP64 sum itself carries
+// rounding error of order n * 2^-53 * sum |u|); `pairs` (or null) counts the
+// pairs inside the cutoff.
 EV lj_pair_sum(const LJ& lj, const State& s, State* sink,
-               const std::function<void(const PairFn&)>& source);
+               const std::function<void(const PairFn&)>& source, EV* compensated = nullptr,
+               long* pairs = nullptr);
 EV lj_energy_virial(const LJ& lj, const State& s);                 // :90-92
 EV lj_forces_brute(const LJ& lj, State& s);                       // :94-97
 EV lj_forces_cells(const LJ& lj, State& s, const Cells& c);       // :99-114

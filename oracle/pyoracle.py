@@ -80,6 +80,9 @@ class Oracle:
         lib.orc_lj_forces.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         lib.orc_lj_lrc.argtypes = [C.c_long, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        if hasattr(lib, "orc_lj_ev_compensated"):
+            lib.orc_lj_ev_compensated.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double,
+                                                  C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         lib.orc_cells_usable.argtypes = [_dp, C.c_double]
         lib.orc_verlet_pairs.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                          _up, C.c_long]
@@ -154,6 +157,17 @@ class Oracle:
         if st != 0:
             raise RuntimeError(self.error())
         return np.stack(f, axis=1), float(ev[0]), float(ev[1])
+
+    def lj_ev_compensated(self, fr: Frame, sig=1.0, eps=1.0, rc=2.5, lfs=False, mode="cells"):
+        """(E, W, pairs inside rc) of the same pair terms as lj_forces, E and W
+        summed with compensated summation (oracle library only)."""
+        out = np.zeros(3)
+        st = self.lib.orc_lj_ev_compensated(fr.n, _ptr(fr.L), _ptr(fr.x), _ptr(fr.y), _ptr(fr.z),
+                                            sig, eps, rc, int(lfs), {"brute": 0, "cells": 1}[mode],
+                                            _ptr(out))
+        if st != 0:
+            raise RuntimeError(self.error())
+        return float(out[0]), float(out[1]), int(out[2])
 
     def lj_lrc(self, n: int, L, sig=1.0, eps=1.0, rc=2.5, lfs=False):
         out = np.zeros(2)

@@ -421,3 +421,56 @@ def test_spce_dynamics_matches_reference_build(orc, ref):
     pa_, pb_ = np.stack([wa.x, wa.y, wa.z], 1), np.stack([wb.x, wb.y, wb.z], 1)
     assert np.max(np.abs(pa_ - pb_)) < 1e-10
     assert _geometry_error(pa_, fr.L) < 1e-10
+
+
+def _sc_lattice_closed_form(m, a, rc):
+    """Simple-cubic lattice of m^3 atoms, spacing a, box m a: every pair
+    distance is a sqrt(k); E and W per atom from the shells by hand (the
+    shell counts of |n|^2 = k over integer vectors), summed exactly."""
+    import math
+
+    from oracle.pyoracle import Frame
+
+    g = (np.arange(m) - 0.5 * m) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    L = np.array([m * a] * 3)
+    fr = Frame(L, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+    kmax = int(np.floor((rc / a) ** 2 - 1e-12))
+    rng = range(-3, 4)
+    counts = {}
+    for i in rng:
+        for j in rng:
+            for k in rng:
+                s = i * i + j * j + k * k
+                if 0 < s <= kmax:
+                    counts[s] = counts.get(s, 0) + 1
+    eu, ew = [], []
+    for s, c in counts.items():
+        r2 = a * a * s
+        s6 = (1.0 / r2) ** 3
+        eu.append(c * 4.0 * (s6 * s6 - s6))
+        ew.append(c * 24.0 * (2.0 * s6 * s6 - s6))
+    n = m ** 3
+    return fr, 0.5 * n * math.fsum(eu), 0.5 * n * math.fsum(ew), n * sum(counts.values()) // 2
+
+
+def test_compensated_pair_sum_closed_form(orc):
+    """64,000 atoms, 1.28 M pairs inside rc: the compensated E and W equal the
+    shell closed form to 1e-14, the pair count exactly; the sequential
+    (reference-order) sums agree to their own rounding (1e-11)."""
+    fr, e_cf, w_cf, np_cf = _sc_lattice_closed_form(40, 1.1, 2.5)
+    e, w, npairs = orc.lj_ev_compensated(fr, rc=2.5, mode="cells")
+    assert npairs == np_cf
+    assert abs(e - e_cf) <= 1e-14 * abs(e_cf)
+    assert abs(w - w_cf) <= 1e-14 * abs(w_cf)
+    _, e_seq, w_seq = orc.lj_forces(fr, rc=2.5, mode="cells")
+    assert abs(e_seq - e_cf) <= 1e-11 * abs(e_cf)
+    assert abs(w_seq - w_cf) <= 1e-11 * abs(w_cf)
+
+
+def test_compensated_pair_sum_equals_plain_on_nist(orc):
+    for cfg in (1, 2):
+        fr = load_nist(nist_lj(cfg))
+        e, w, _ = orc.lj_ev_compensated(fr, rc=3.0, mode="brute")
+        e0, w0 = orc.lj_energy_virial(fr, rc=3.0)
+        assert abs(e - e0) <= 1e-13 * abs(e0) and abs(w - w0) <= 1e-13 * abs(w0)
