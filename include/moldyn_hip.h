@@ -171,6 +171,14 @@ md_status md_neigh_rebuild_count(const md_ctx* ctx, long* count);
 md_status md_neigh_export_pairs(md_ctx* ctx, uint32_t* ij, long cap, long* npairs);
 /* Neighbour row capacity (entries per atom) of the device list; 0 = auto. */
 md_status md_neigh_set_capacity(md_ctx* ctx, int per_atom);
+/* The automatic row capacity for n atoms in box L with the given cutoff and
+ * skin: ceil(2 * (4/3) pi (rc+skin)^3 rho + 32), at most n - 1, rounded up to
+ * a multiple of 8 (0 for invalid arguments). Host arithmetic. No reference
+ * counterpart: the reference's list is a growing std::vector
+ * (neighbor/verlet_list.cpp:8-42). A row that outgrows the capacity during
+ * md_run makes md_run restore its starting state, double the capacity and
+ * run again. */
+int md_neigh_auto_capacity(long n, const double* L, double cutoff, double skin);
 
 /* ---- integrators ---------------------------------------------------------- */
 

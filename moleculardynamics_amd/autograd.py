@@ -55,6 +55,9 @@ class MixSystem:
         self.run_id = 0
 
     def configure(self, eps: np.ndarray, sigma: np.ndarray, pos: np.ndarray, vel=None) -> None:
+        # any reconfiguration replaces the state the last run's checkpoints
+        # describe, so a pending trajectory adjoint must run its forward again
+        self.run_id += 1
         e = symmetric_table(eps, self.ntypes)
         s = symmetric_table(sigma, self.ntypes)
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)

@@ -396,27 +396,36 @@ md_status md_batch_run(md_batch* b, long nsteps, long segment, int use_graph, do
     const mdk::BatchArgs a = args_of(b, 1, th, th_steps);
     long done = 0;
     if (use_graph && nsteps >= segment) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        // the captured graph and its executable are destroyed on every return
+        // path, the early error returns of BHIP included (after the stream
+        // has drained, so no launch of them is still pending)
+        struct GraphGuard {
+            hipStream_t stream;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            ~GraphGuard() {
+                if (exec) {
+                    (void)hipStreamSynchronize(stream);
+                    (void)hipGraphExecDestroy(exec);
+                }
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+        } g{b->stream};
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         for (long k = 0; k < segment; ++k) {
             hipError_t e = mdk::launch_batch(a, b->block, b->stream);
             if (e != hipSuccess) {
-                hipGraph_t g2 = nullptr;
-                (void)hipStreamEndCapture(b->stream, &g2);
-                if (g2) (void)hipGraphDestroy(g2);
+                (void)hipStreamEndCapture(b->stream, &g.graph);
                 return bfail(b, MD_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
             }
         }
-        BHIP(b, hipStreamEndCapture(b->stream, &graph));
-        BHIP(b, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        BHIP(b, hipStreamEndCapture(b->stream, &g.graph));
+        BHIP(b, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
         while (done + segment <= nsteps) {
-            BHIP(b, hipGraphLaunch(exec, b->stream));
+            BHIP(b, hipGraphLaunch(g.exec, b->stream));
             done += segment;
         }
         BHIP(b, hipStreamSynchronize(b->stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
     }
     for (; done < nsteps; ++done) BHIP(b, mdk::launch_batch(a, b->block, b->stream));
     md_status st = check_flags(b);

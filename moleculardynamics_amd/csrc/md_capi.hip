@@ -75,6 +75,14 @@ struct md_ctx {
     md_thermo* thermo_dev = nullptr;
     long thermo_alloc = 0;
 
+    // md_run's step-boundary snapshot (restored after a neighbour-row overflow)
+    double4 *snap_pos = nullptr, *snap_vel = nullptr, *snap_frc = nullptr;
+    int *snap_orig = nullptr, *snap_slot = nullptr;
+    mdk::Scalars* snap_sc = nullptr;
+    mdk::Chain* snap_chain = nullptr;
+    long snap_alloc = 0;
+    uint64_t snap_step = 0;
+
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
     double ms[3] = {0, 0, 0};
@@ -146,6 +154,7 @@ mdk::Grid make_grid(const double L[3], double cutoff) {
         g.L[d] = L[d];
         g.nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / cutoff)));
         g.w[d] = L[d] / static_cast<double>(g.nc[d]);
+        g.inv_w[d] = static_cast<double>(g.nc[d]) / L[d];
     }
     g.usable = (g.nc[0] >= 3 && g.nc[1] >= 3 && g.nc[2] >= 3) ? 1 : 0;
     return g;
@@ -161,6 +170,7 @@ mdk::Grid make_list_grid(const double L[3], double rl) {
         g.L[d] = L[d];
         g.nc[d] = std::max(1, static_cast<int>(std::floor(2.0 * L[d] / (rl * (1.0 + 1e-9)))));
         g.w[d] = L[d] / static_cast<double>(g.nc[d]);
+        g.inv_w[d] = static_cast<double>(g.nc[d]) / L[d];
     }
     g.usable = (g.nc[0] >= 5 && g.nc[1] >= 5 && g.nc[2] >= 5) ? 1 : 0;
     return g;
@@ -209,15 +219,18 @@ md_status resolve_events(md_ctx* c) {
 }
 
 // Neighbour row capacity: twice the mean full-list row at the system's
-// density plus 32, capped at n - 1.
-int auto_capacity(const md_ctx* c) {
-    const double rl = c->rc + c->skin;
-    const double rho = static_cast<double>(c->n) / (c->L[0] * c->L[1] * c->L[2]);
+// density plus 32, capped at n - 1, rounded up to a multiple of 8 (the octet
+// layout): 192 at rho* = 0.8442 and rc + skin = 2.8 (mean row 77.6).
+int capacity_for(long n, const double* L, double rc, double skin) {
+    const double rl = rc + skin;
+    const double rho = static_cast<double>(n) / (L[0] * L[1] * L[2]);
     const double mean = 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho;
     long cap = static_cast<long>(std::ceil(2.0 * mean + 32.0));
-    cap = std::min(cap, std::max(1L, c->n - 1));
-    return static_cast<int>((cap + 7) / 8 * 8);  // the octet layout needs a multiple of 8
+    cap = std::min(cap, std::max(1L, n - 1));
+    return static_cast<int>((cap + 7) / 8 * 8);
 }
+
+int auto_capacity(const md_ctx* c) { return capacity_for(c->n, c->L, c->rc, c->skin); }
 
 md_status alloc_neigh(md_ctx* c) {
     c->grid = make_list_grid(c->L, c->rc + c->skin);
@@ -362,6 +375,44 @@ void clear_call_flags(md_ctx* c) {
     mdk::launch_set_flag(&c->sc->overflow, 0, c->stream);
 }
 
+// The state md_run starts from: positions, velocities, forces, the slot
+// permutation, the chain and the device scalars (stream-ordered copies).
+md_status save_snapshot(md_ctx* c) {
+    const long n = c->n;
+    if (c->snap_alloc < n) {
+        MD_HIP(c, alloc(c->snap_pos, n)); MD_HIP(c, alloc(c->snap_vel, n));
+        MD_HIP(c, alloc(c->snap_frc, n)); MD_HIP(c, alloc(c->snap_orig, n));
+        MD_HIP(c, alloc(c->snap_slot, n));
+        c->snap_alloc = n;
+    }
+    if (c->snap_sc == nullptr) MD_HIP(c, alloc(c->snap_sc, 1));
+    if (c->snap_chain == nullptr) MD_HIP(c, alloc(c->snap_chain, 1));
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    MD_HIP(c, hipMemcpyAsync(c->snap_pos, c->pos, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_vel, c->vel, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_frc, c->frc, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_orig, c->orig, n * sizeof(int), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_slot, c->slot, n * sizeof(int), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_sc, c->sc, sizeof(mdk::Scalars), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->snap_chain, c->chain, sizeof(mdk::Chain), dd, c->stream));
+    c->snap_step = c->step_counter;
+    return MD_OK;
+}
+
+md_status restore_snapshot(md_ctx* c) {
+    const long n = c->n;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    MD_HIP(c, hipMemcpyAsync(c->pos, c->snap_pos, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->vel, c->snap_vel, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->frc, c->snap_frc, n * sizeof(double4), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->orig, c->snap_orig, n * sizeof(int), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->slot, c->snap_slot, n * sizeof(int), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->sc, c->snap_sc, sizeof(mdk::Scalars), dd, c->stream));
+    MD_HIP(c, hipMemcpyAsync(c->chain, c->snap_chain, sizeof(mdk::Chain), dd, c->stream));
+    c->step_counter = c->snap_step;
+    return MD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -422,6 +473,8 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->ci_atoms); release(c->ci_tmp);
     release(c->part_d); release(c->part_l); release(c->sc); release(c->chain); release(c->com4);
     release(c->thermo_dev);
+    release(c->snap_pos); release(c->snap_vel); release(c->snap_frc);
+    release(c->snap_orig); release(c->snap_slot); release(c->snap_sc); release(c->snap_chain);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MD_OK;
@@ -820,13 +873,13 @@ md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
 constexpr long kGraphSegment = 32;
 constexpr long kGraphMaxAtoms = 262144;
 
-md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
-    md_status st = need_system(c);
-    if (st != MD_OK) return st;
-    if (!c->lj_set || !c->integ_set || !c->list_built)
-        return fail(c, MD_ERR_STATE, "md_run needs md_lj_set, md_integrator_set and a built list");
-    if (nsteps <= 0) return MD_OK;
-    clear_call_flags(c);
+// The steps of one md_run attempt; *overflowed reports a neighbour row
+// longer than the capacity at some build of the attempt (the state is then
+// not meaningful and md_run restores its snapshot).
+static md_status run_steps(md_ctx* c, long nsteps, int precision, md_thermo* thermo,
+                           bool* overflowed) {
+    md_status st = MD_OK;
+    *overflowed = false;
     if (thermo && nsteps > c->thermo_alloc) {
         MD_HIP(c, alloc(c->thermo_dev, nsteps));
         c->thermo_alloc = nsteps;
@@ -928,12 +981,51 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     st = read_scalars(c, &s);  // synchronises the stream
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (st != MD_OK) return st;
-    if (s.overflow) return fail(c, MD_ERR_CAPACITY, "neighbour row capacity exceeded during md_run");
+    if (s.overflow) {
+        *overflowed = true;
+        return MD_OK;
+    }
     if (s.fix_saturated)
         return fail(c, MD_ERR_RUNTIME, "fixed-point force accumulator saturated (|f| > 2^19)");
     if (thermo)
         MD_HIP(c, hipMemcpy(thermo, c->thermo_dev, nsteps * sizeof(md_thermo), hipMemcpyDeviceToHost));
     return MD_OK;
+}
+
+// A neighbour row longer than the capacity stops the work of the remaining
+// steps (the pre-force pass and the pair kernel return at once once the
+// device flag is set). md_run then restores the state it started from,
+// doubles the capacity, builds the list again and runs the whole call again
+// (at most three times), so a caller never sees a state advanced with a
+// truncated list. The run that completes holds one extra list build.
+md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
+    md_status st = need_system(c);
+    if (st != MD_OK) return st;
+    if (!c->lj_set || !c->integ_set || !c->list_built)
+        return fail(c, MD_ERR_STATE, "md_run needs md_lj_set, md_integrator_set and a built list");
+    if (nsteps <= 0) return MD_OK;
+    clear_call_flags(c);
+    st = save_snapshot(c);
+    if (st != MD_OK) return st;
+    for (int attempt = 0;; ++attempt) {
+        bool over = false;
+        st = run_steps(c, nsteps, precision, thermo, &over);
+        if (st != MD_OK || !over) return st;
+        if (attempt == 3 || c->cap >= c->n - 1)
+            return fail(c, MD_ERR_CAPACITY, "neighbour row capacity could not be satisfied in md_run");
+        st = restore_snapshot(c);
+        if (st != MD_OK) return st;
+        c->cap_user = static_cast<int>(std::min<long>(2L * c->cap, std::max(1L, c->n - 1)));
+        st = alloc_neigh(c);
+        if (st != MD_OK) return st;
+        st = build_now(c);
+        if (st != MD_OK) return st;
+    }
+}
+
+int md_neigh_auto_capacity(long n, const double* L, double cutoff, double skin) {
+    if (n <= 0 || L == nullptr || !(cutoff > 0.0) || !(skin >= 0.0)) return 0;
+    return capacity_for(n, L, cutoff, skin);
 }
 
 md_status md_half_kick(md_ctx* c, double dt) {

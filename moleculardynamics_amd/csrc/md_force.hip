@@ -498,6 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
     __shared__ long long sm_l[kBlock / kWave];
     const long blk = xcd_block();
     if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
+    if (a.sc->overflow) return;       // a truncated list was built: md_run restores and retries
     const long t = blk * kBlock + threadIdx.x;
     double e = 0.0, w = 0.0, ke = 0.0;
     long long ef = 0, wf = 0, nt = 0;
@@ -609,6 +610,7 @@ template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK, int LANES>
 __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
     __shared__ double sm_d[kBlock / kWave];
     __shared__ long long sm_l[kBlock / kWave];
+    if (a.sc->overflow) return;  // a truncated list was built: md_run restores and retries
     const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
     const long t = gt / LANES;
     const int sub = static_cast<int>(gt & (LANES - 1));

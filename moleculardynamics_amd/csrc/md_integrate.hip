@@ -110,6 +110,7 @@ __device__ void finish_check(const StepArgs& a, int nb) {
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
     __shared__ double sm1[kBlock / kWave], sm2[kBlock / kWave];
+    if (a.sc->overflow) return;  // a truncated list was built: md_run restores and retries
     const long t = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
     double d = 0.0;
     if (t < a.n) {
@@ -164,6 +165,10 @@ __global__ __launch_bounds__(kBlock) void k_pre(StepArgs a) {
 // One workgroup after k_pre: rebuild decision from the partials; the chain
 // rescale applied by k_pre is consumed.
 __global__ __launch_bounds__(kFinish) void k_pre_finish(StepArgs a, int nb) {
+    if (a.sc->overflow) {  // no further builds once a row has overflowed
+        if (threadIdx.x == 0) a.sc->rebuild_flag = 0;
+        return;
+    }
     if (a.check) {
         finish_check(a, nb);
     } else if (threadIdx.x == 0) {

@@ -17,6 +17,7 @@ struct Grid {
     int nc[3];
     double w[3];
     double L[3];
+    double inv_w[3];  // nc / L
     int usable;
 };
 

@@ -28,12 +28,15 @@ namespace {
 
 constexpr unsigned kGridCap = 2048;
 
-__device__ __forceinline__ int cell_coord(double r, double L, double w, int n) {
-    // floor((wrap(r) + L/2) / width) clamped to [0, n-1] (cell_list.cpp:31-38);
-    // wrap is the reference fold.
-#pragma clang fp contract(off)
-    const double f = fold_ref(r, L);
-    int k = static_cast<int>(floor((f + 0.5 * L) / w));
+// Cell coordinate of the folded position, floor((wrap(r) + L/2) nc / L)
+// clamped to [0, n-1] (the rule of cell_list.cpp:31-38 with the division
+// replaced by a product). The list grid's cells are at least
+// (rc + skin)/2 (1 + 1e-9) wide, a margin ten million times the rounding of
+// this product, so every pair within rc + skin still lies within two cells
+// per axis and the 5 x 5 x 5 stencil finds it. fold is the exact fast fold.
+__device__ __forceinline__ int cell_coord(double r, const Box& b, int d, double inv_w, int n) {
+    const double f = fold(r, b.L[d], b.lo[d], b.hi1[d], b.hi2[d]);
+    const int k = static_cast<int>(floor(fma(0.5, b.L[d], f) * inv_w));
     return k < 0 ? 0 : (k >= n ? n - 1 : k);
 }
 
@@ -69,8 +72,8 @@ __device__ __forceinline__ Run wave_run(int key, bool active) {
     return r;
 }
 
-__global__ void k_bin(const double4* __restrict__ pos, long n, Grid g, int* __restrict__ cell_of,
-                      int* __restrict__ cell_count, const int* flag) {
+__global__ void k_bin(const double4* __restrict__ pos, long n, Grid g, Box box,
+                      int* __restrict__ cell_of, int* __restrict__ cell_count, const int* flag) {
     if (*flag == 0) return;
     WAVE_STRIDE(base, n) {
         const long t = base + (threadIdx.x & (kWave - 1));
@@ -78,9 +81,9 @@ __global__ void k_bin(const double4* __restrict__ pos, long n, Grid g, int* __re
         int c = 0;
         if (active && g.usable) {
             const double4 p = pos[t];
-            const int cx = cell_coord(p.x, g.L[0], g.w[0], g.nc[0]);
-            const int cy = cell_coord(p.y, g.L[1], g.w[1], g.nc[1]);
-            const int cz = cell_coord(p.z, g.L[2], g.w[2], g.nc[2]);
+            const int cx = cell_coord(p.x, box, 0, g.inv_w[0], g.nc[0]);
+            const int cy = cell_coord(p.y, box, 1, g.inv_w[1], g.nc[1]);
+            const int cz = cell_coord(p.z, box, 2, g.inv_w[2], g.nc[2]);
             c = cx + g.nc[0] * (cy + g.nc[1] * cz);
         }
         const Run r = wave_run(c, active);
@@ -154,8 +157,10 @@ __global__ __launch_bounds__(kScanTile) void k_scan(int* __restrict__ cell_count
     }
 }
 
-__global__ void k_place(const int* __restrict__ cell_of, long n, int* __restrict__ cell_fill,
-                        int* __restrict__ order, const int* flag) {
+// Places every atom's upload index in its cell's range of `order` (in the
+// arrival order of the atomics; k_sort_permute ranks them).
+__global__ void k_place(const int* __restrict__ cell_of, const int* __restrict__ orig, long n,
+                        int* __restrict__ cell_fill, int* __restrict__ order, const int* flag) {
     if (*flag == 0) return;
     WAVE_STRIDE(base, n) {
         const long t = base + (threadIdx.x & (kWave - 1));
@@ -165,7 +170,7 @@ __global__ void k_place(const int* __restrict__ cell_of, long n, int* __restrict
         int at = 0;
         if (r.first) at = atomicAdd(&cell_fill[c], r.len);
         at = __shfl(at, r.head, kWave) + r.rank;
-        if (active) order[at] = static_cast<int>(t);
+        if (active) order[at] = orig[t];
     }
 }
 
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a) {
         const int c = a.cell_of[t];
         const int b = a.cell_start[c], e = a.cell_start[c + 1];
         int rank = 0;
-        for (int k = b; k < e; ++k) rank += (a.orig[a.order[k]] < key) ? 1 : 0;
+        for (int k = b; k < e; ++k) rank += (a.order[k] < key) ? 1 : 0;
         gather_slot(a, b + rank, t, key, c);
     }
 }
@@ -505,11 +510,11 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     // k_sort_permute takes one wave per cell
     const dim3 gn(grid_capped(n)), b(kBlock);
     const dim3 gt(scan_tiles(ncell)), bt(kScanTile);
-    hipLaunchKernelGGL(k_bin, gn, b, 0, s, a.pos, n, a.grid, a.cell_of, a.cell_count, a.flag);
+    hipLaunchKernelGGL(k_bin, gn, b, 0, s, a.pos, n, a.grid, a.box, a.cell_of, a.cell_count, a.flag);
     hipLaunchKernelGGL(k_tile_count, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.flag);
     hipLaunchKernelGGL(k_scan, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.cell_start,
                        a.cell_fill, a.sc, a.flag);
-    hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, n, a.cell_fill, a.order, a.flag);
+    hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, a.orig, n, a.cell_fill, a.order, a.flag);
     hipLaunchKernelGGL(k_sort_permute, gn, b, 0, s, a);
     if (n <= kBuildLanesMaxAtoms) {
         const dim3 gl(static_cast<unsigned>((n * kBuildLanes + kBlock - 1) / kBlock));

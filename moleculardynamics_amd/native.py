@@ -98,6 +98,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_neigh_rebuild_count.argtypes = [vp, C.POINTER(C.c_long)]
     lib.md_neigh_export_pairs.argtypes = [vp, _u32p, C.c_long, C.POINTER(C.c_long)]
     lib.md_neigh_set_capacity.argtypes = [vp, C.c_int]
+    lib.md_neigh_auto_capacity.argtypes = [C.c_long, _dp, C.c_double, C.c_double]
+    lib.md_neigh_auto_capacity.restype = C.c_int
     lib.md_integrator_set.argtypes = [vp, C.POINTER(Integrator)]
     lib.md_run.argtypes = [vp, C.c_long, C.c_int, C.POINTER(Thermo)]
     lib.md_nhc_state.argtypes = [vp, _dp, _dp, _dp]
@@ -328,6 +330,11 @@ class Context:
 
     def synchronize(self):
         self._chk(self.lib.md_ctx_synchronize(self.h))
+
+
+def auto_capacity(n, L, cutoff, skin) -> int:
+    """The automatic neighbour-row capacity (md_neigh_auto_capacity)."""
+    return int(load().md_neigh_auto_capacity(n, _p(_col(L)), cutoff, skin))
 
 
 def cells_usable(L, cutoff) -> bool:

@@ -81,3 +81,20 @@ def test_cells_usable_rule(orc):
 
     for L, rc in [([10, 10, 10], 3.0), ([5, 5, 5], 3.0), ([8.9, 9.1, 12], 3.0), ([9, 9, 9], 3.0)]:
         assert native.cells_usable(L, rc) == orc.cells_usable(L, rc)
+
+
+def test_auto_capacity_arithmetic(lib):
+    """ceil(2 (4/3) pi rl^3 rho + 32), at most n - 1, rounded up to 8: 192 at
+    the 1M-atom fluid's rho* = 0.8442 with rl = 2.8 (mean row 77.6)."""
+    from moleculardynamics_amd import native
+
+    n = 1048576
+    L = np.array([107.494156] * 3)
+    mean = 4.0 / 3.0 * np.pi * 2.8 ** 3 * n / np.prod(L)
+    assert abs(mean - 77.6) < 0.05
+    want = int(np.ceil(2.0 * mean + 32.0) + 7) // 8 * 8
+    assert native.auto_capacity(n, L, 2.5, 0.3) == want == 192
+    # small systems are capped at n - 1 (then rounded up to the octet size)
+    assert native.auto_capacity(10, np.array([2.0] * 3), 2.5, 0.3) == 16
+    assert native.auto_capacity(0, L, 2.5, 0.3) == 0
+    assert native.auto_capacity(n, L, -1.0, 0.3) == 0

@@ -140,3 +140,38 @@ def test_refused_calls(ctx):
     ctx.build_list()
     ctx.forces(native.VERLET, native.FP64)
     ctx.run(5, native.FP64, thermo=False)
+
+
+def test_row_overflow_during_run_recovers(gpu_ok, orc):
+    """A row capacity that fits the first build exactly overflows as the
+    fluid's rows fluctuate: md_run restores its starting state, doubles the
+    capacity, builds again and reruns the call. The result is the trajectory
+    of a run with ample capacity (the forces are the same pair sums; the extra
+    list build only changes summation order) and no MD_ERR_CAPACITY."""
+    from moleculardynamics_amd import native
+
+    fr = orc.fcc_lattice(6, 0.8442, 1.2, 77)
+    x, v = np.stack([fr.x, fr.y, fr.z], 1), np.stack([fr.vx, fr.vy, fr.vz], 1)
+    pairs = orc.verlet_pairs(fr, 2.5, 0.3)
+    rows = np.bincount(pairs.ravel(), minlength=fr.n)
+    tight = int(rows.max() + 7) // 8 * 8
+    out = []
+    for cap in (tight, 0):
+        c = native.Context()
+        c.upload(fr.L, x, v)
+        c.set_lj(1.0, 1.0, 2.5, False)
+        c.config_list(2.5, 0.3)
+        c.set_capacity(cap)
+        c.build_list()
+        c.forces()
+        c.set_integrator(kind=native.VV, dt=0.005)
+        b0 = c.rebuild_count()
+        th = c.run(300)
+        out.append((th, c.download(), c.rebuild_count() - b0))
+        c.close()
+    (th_t, st_t, nb_t), (th_a, st_a, nb_a) = out
+    assert nb_t >= nb_a + 1  # the recovery rebuilt at least once more
+    assert np.max(np.abs(th_t["pe"] - th_a["pe"]) / np.abs(th_a["pe"])) < 1e-8
+    d = st_t[0] - st_a[0]
+    d -= fr.L * np.floor(d / fr.L + 0.5)
+    assert np.max(np.abs(d)) < 1e-7

@@ -307,6 +307,36 @@ def test_autograd_function_wrappers(gpu_ok, small):
     sysm.close()
 
 
+def test_trajectory_adjoint_survives_interleaved_calls(gpu_ok, small):
+    """A MixEnergy call at other parameters between a MixTrajectory's forward
+    and backward reconfigures the shared handle; the backward then runs its
+    forward again from its own inputs, so the gradients equal those of an
+    uninterrupted forward/backward."""
+    import torch
+
+    from moleculardynamics_amd.autograd import MixEnergy, MixSystem, MixTrajectory
+
+    fr, t = small
+    sysm = MixSystem(fr.L, t, KA_RC, lfs=True)
+    grads = []
+    for interleave in (False, True):
+        eps = torch.tensor([KA_EPS[a][b] for a, b in CANON], dtype=torch.float64,
+                           requires_grad=True)
+        sig = torch.tensor([KA_SIGMA[a][b] for a, b in CANON], dtype=torch.float64,
+                           requires_grad=True)
+        x0 = torch.tensor(_pos(fr), dtype=torch.float64, requires_grad=True)
+        v0 = torch.tensor(_vel(fr), dtype=torch.float64, requires_grad=True)
+        em, wm, xT, vT = MixTrajectory.apply(eps, sig, x0, v0, sysm, 0.002, 20, 5)
+        if interleave:
+            with torch.no_grad():
+                MixEnergy.apply(1.3 * eps.detach(), 0.97 * sig.detach(), x0.detach() + 0.01, sysm)
+        (em + 0.1 * wm + 0.5 * (vT * vT).sum()).backward()
+        grads.append((eps.grad.clone(), sig.grad.clone(), x0.grad.clone(), v0.grad.clone()))
+    for a, b in zip(*grads):
+        assert torch.allclose(a, b, rtol=1e-12, atol=0), (a, b)
+    sysm.close()
+
+
 def test_binary_mixture_workload_gradients(gpu_ok):
     """The full differentiated workload: N = 32,000 (25,600 A + 6,400 B),
     L = (32000 / 1.2)^(1/3) = 29.876, seeded uniform positions relaxed by 500
