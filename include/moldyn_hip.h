@@ -36,7 +36,7 @@ typedef enum md_status {
 } md_status;
 
 /* Pair sources of LennardJones::computeForces overloads
- * (forcefield/lennard_jones.hpp:73-85). */
+ * (forcefield/lennard_jones.hpp:36-48). */
 typedef enum md_pair_source {
     MD_PAIRS_BRUTE = 0,  /* every i<j pair: computeForces(System&) */
     MD_PAIRS_CELLS = 1,  /* linked cells at the LJ cutoff: computeForces(System&, const CellList&) */
@@ -51,14 +51,14 @@ typedef enum md_precision { MD_FP64 = 0, MD_MIXED = 1 } md_precision;
 
 typedef enum md_integrator_kind {
     MD_VELOCITY_VERLET = 0, /* integrate/velocity_verlet.hpp:39-80 */
-    MD_LANGEVIN_BAOAB = 1,  /* integrate/langevin.hpp:122-190 */
+    MD_LANGEVIN_BAOAB = 1,  /* integrate/langevin.hpp:29-97 */
     MD_NOSE_HOOVER = 2      /* integrate/nose_hoover.hpp:26-67 */
 } md_integrator_kind;
 
 typedef struct md_ctx md_ctx;
 
 /* Lennard-Jones parameters: LennardJones(sigma, epsilon, cutoff, truncation)
- * (forcefield/lennard_jones.hpp:63; truncation enum :48). */
+ * (forcefield/lennard_jones.hpp:26; truncation enum :11). */
 typedef struct md_lj_params {
     double sigma;
     double epsilon;
@@ -67,7 +67,7 @@ typedef struct md_lj_params {
 } md_lj_params;
 
 /* Integrator parameters. VelocityVerlet(dt) (velocity_verlet.hpp:41);
- * LangevinBAOAB(dt, friction, temperature, seed) (langevin.hpp:124);
+ * LangevinBAOAB(dt, friction, temperature, seed) (langevin.hpp:31);
  * NoseHooverChain(dt, temperature, tau, chainLength, dof) (nose_hoover.hpp:31). */
 typedef struct md_integrator {
     int kind;          /* md_integrator_kind */
@@ -125,7 +125,7 @@ md_status md_system_kinetic_energy(md_ctx* ctx, double* ke);
 md_status md_system_remove_com_motion(md_ctx* ctx);
 md_status md_system_size(const md_ctx* ctx, long* n);
 
-/* ---- Lennard-Jones (forcefield/lennard_jones.{hpp:61-135,cpp:32-151}) ---- */
+/* ---- Lennard-Jones (forcefield/lennard_jones.{hpp:24-98,cpp:32-151}) ---- */
 
 /* Sets the pair potential; precomputes u(rc) and du/dr(rc)
  * (lennard_jones.cpp:32-48). */
@@ -214,7 +214,7 @@ md_status md_scale_velocities(md_ctx* ctx, double s);
  * the 27 cells around each atom on a grid of floor(L/cell_cutoff) cells per
  * side; MD_ERR_INVALID_ARGUMENT when fewer than 3 fit. */
 md_status md_lj_forces_cells(md_ctx* ctx, double cell_cutoff, double* energy, double* virial);
-/* NoseHooverChain::conservedQuantityContribution (nose_hoover.cpp:52-61)
+/* NoseHooverChain::conservedQuantityContribution (nose_hoover.cpp:45-54)
  * and the chain arrays (length chain_length each; NULL to skip). */
 md_status md_nhc_state(md_ctx* ctx, double* conserved, double* xi, double* p_xi);
 
