@@ -214,6 +214,34 @@ md_status md_scale_velocities(md_ctx* ctx, double s);
  * the 27 cells around each atom on a grid of floor(L/cell_cutoff) cells per
  * side; MD_ERR_INVALID_ARGUMENT when fewer than 3 fit. */
 md_status md_lj_forces_cells(md_ctx* ctx, double cell_cutoff, double* energy, double* virial);
+
+/* ---- allocation-free force evaluation ------------------------------------- */
+
+#define MD_NOALLOC_MAX_ATOMS (1L << 20) /* atoms per md_lj_forces_noalloc call */
+#define MD_NOALLOC_MAX_CELLS (1L << 20) /* cells of its CellList grid */
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): md_lj_forces with MD_PAIRS_BRUTE or MD_PAIRS_CELLS, at
+ * FP64, on caller-owned device arrays and without a context. It calls no
+ * hipMalloc or hipFree: every intermediate buffer lives in one fixed device
+ * array of the library (about 60 MiB per device, held from the first call on
+ * that device until the process ends).
+ *   x, y, z:    device arrays of n positions (box of lengths L[3], host array)
+ *   fx, fy, fz: device arrays receiving the n forces (all NULL: E and W only)
+ *   energy, virial: host pointers (may be NULL) for E and the raw virial W
+ *   hip_stream: the hipStream_t to order the work on (NULL: the null stream);
+ *               the call returns when that stream has finished it.
+ * n > MD_NOALLOC_MAX_ATOMS or a grid of more than MD_NOALLOC_MAX_CELLS cells
+ * is refused with MD_ERR_CAPACITY; an unusable CellList with
+ * MD_ERR_INVALID_ARGUMENT. md_last_error(NULL) describes the calling
+ * thread's last failure. Calls on one device share the fixed array: calls
+ * that overlap on one GPU (other threads or streams) race. Results equal
+ * md_lj_forces on a context holding the same atoms in the same order. */
+md_status md_lj_forces_noalloc(int pair_source, const md_lj_params* p, long n, const double* L,
+                               const double* x, const double* y, const double* z, double* fx,
+                               double* fy, double* fz, double* energy, double* virial,
+                               void* hip_stream);
 /* NoseHooverChain::conservedQuantityContribution (nose_hoover.cpp:45-54)
  * and the chain arrays (length chain_length each; NULL to skip). */
 md_status md_nhc_state(md_ctx* ctx, double* conserved, double* xi, double* p_xi);

@@ -123,42 +123,10 @@ long blocks_of(long n) { return (n + mdk::kBlock - 1) / mdk::kBlock; }
 
 void set_lj_const(md_ctx* c) {
     const md_lj_params& p = c->lj;
-    mdk::LJConst& k = c->ljc;
-    // u(rc) and du/dr(rc) as lennard_jones.cpp:32-48 computes them
-    const double s2 = (p.sigma * p.sigma) / (p.cutoff * p.cutoff);
-    const double s6 = s2 * s2 * s2;
-    const double u_rc = 4.0 * p.epsilon * (s6 * s6 - s6);
-    const double w_rc = 24.0 * p.epsilon * (2.0 * s6 * s6 - s6);
-    k.sig2 = p.sigma * p.sigma;
-    k.eps = p.epsilon;
-    k.rc = p.cutoff;
-    k.rc2 = p.cutoff * p.cutoff;
-    k.u_rc = u_rc;
-    k.dudr_rc = -w_rc / p.cutoff;
-    k.sig2f = static_cast<float>(k.sig2);
-    k.epsf = static_cast<float>(k.eps);
-    k.rc2f = static_cast<float>(k.rc2);
-    k.rcf = static_cast<float>(k.rc);
-    k.u_rcf = static_cast<float>(k.u_rc);
-    k.dudr_rcf = static_cast<float>(k.dudr_rc);
-    k.e4s = (4.0f * k.epsf) * 4294967296.0f;
-    k.e24s = (24.0f * k.epsf) * 4294967296.0f;
-    k.u_rcs = k.u_rcf * 4294967296.0f;
-    k.dudr_rcs = k.dudr_rcf * 4294967296.0f;
-    k.lfs = p.linear_force_shift ? 1 : 0;
+    c->ljc = mdk::lj_constants(p.sigma, p.epsilon, p.cutoff, p.linear_force_shift);
 }
 
-mdk::Grid make_grid(const double L[3], double cutoff) {
-    mdk::Grid g{};
-    for (int d = 0; d < 3; ++d) {
-        g.L[d] = L[d];
-        g.nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / cutoff)));
-        g.w[d] = L[d] / static_cast<double>(g.nc[d]);
-        g.inv_w[d] = static_cast<double>(g.nc[d]) / L[d];
-    }
-    g.usable = (g.nc[0] >= 3 && g.nc[1] >= 3 && g.nc[2] >= 3) ? 1 : 0;
-    return g;
-}
+mdk::Grid make_grid(const double L[3], double cutoff) { return mdk::cell_grid(L, cutoff); }
 
 // The Verlet rebuild's grid: cells of at least rl/2 (a 1e-9 relative margin
 // keeps rounding from putting a pair within rl three cells apart), searched
@@ -480,7 +448,11 @@ md_status md_ctx_destroy(md_ctx* c) {
     return MD_OK;
 }
 
-const char* md_last_error(const md_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* md_last_error(const md_ctx* c) {
+    if (c != nullptr) return c->err.c_str();
+    const char* e = mdk::noalloc_error();
+    return e != nullptr ? e : "null context";
+}
 
 md_status md_ctx_synchronize(md_ctx* c) {
     if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;

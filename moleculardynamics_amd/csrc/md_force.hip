@@ -819,6 +819,11 @@ __global__ __launch_bounds__(kBlock) void k_lj_direct(DirectArgs a) {
                     }
         }
         if (a.frc != nullptr) a.frc[t] = make_double4(fx, fy, fz, 0.0);
+        if (a.fx_out != nullptr) {
+            a.fx_out[oi] = fx;
+            a.fy_out[oi] = fy;
+            a.fz_out[oi] = fz;
+        }
     }
     const double be = block_reduce(e, sm_d, wave_sum);
     const double bw = block_reduce(w, sm_d, wave_sum);
@@ -980,6 +985,45 @@ MixK mix_constants(const LJConst& k, double s, double band) {
         m.krs = k.eps != 0.0 ? static_cast<float>(k.dudr_rc * s / m.e24) : 0.0f;
     }
     return m;
+}
+
+LJConst lj_constants(double sigma, double epsilon, double cutoff, int linear_force_shift) {
+    LJConst k{};
+    const double s2 = (sigma * sigma) / (cutoff * cutoff);
+    const double s6 = s2 * s2 * s2;
+    const double u_rc = 4.0 * epsilon * (s6 * s6 - s6);
+    const double w_rc = 24.0 * epsilon * (2.0 * s6 * s6 - s6);
+    k.sig2 = sigma * sigma;
+    k.eps = epsilon;
+    k.rc = cutoff;
+    k.rc2 = cutoff * cutoff;
+    k.u_rc = u_rc;
+    k.dudr_rc = -w_rc / cutoff;
+    k.sig2f = static_cast<float>(k.sig2);
+    k.epsf = static_cast<float>(k.eps);
+    k.rc2f = static_cast<float>(k.rc2);
+    k.rcf = static_cast<float>(k.rc);
+    k.u_rcf = static_cast<float>(k.u_rc);
+    k.dudr_rcf = static_cast<float>(k.dudr_rc);
+    k.e4s = (4.0f * k.epsf) * 4294967296.0f;
+    k.e24s = (24.0f * k.epsf) * 4294967296.0f;
+    k.u_rcs = k.u_rcf * 4294967296.0f;
+    k.dudr_rcs = k.dudr_rcf * 4294967296.0f;
+    k.lfs = linear_force_shift ? 1 : 0;
+    return k;
+}
+
+Grid cell_grid(const double L[3], double cutoff) {
+    Grid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = L[d];
+        const int k = static_cast<int>(floor(L[d] / cutoff));
+        g.nc[d] = k > 1 ? k : 1;
+        g.w[d] = L[d] / static_cast<double>(g.nc[d]);
+        g.inv_w[d] = static_cast<double>(g.nc[d]) / L[d];
+    }
+    g.usable = (g.nc[0] >= 3 && g.nc[1] >= 3 && g.nc[2] >= 3) ? 1 : 0;
+    return g;
 }
 
 void launch_lj_direct(const DirectArgs& a, hipStream_t s) {

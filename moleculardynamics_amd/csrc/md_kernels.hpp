@@ -149,6 +149,13 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s);
 // One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
 // thermo row and NHC chain; launched right after launch_lj_force.
 void launch_lj_finish(const ForceArgs& a, hipStream_t s);
+// LennardJones(sigma, epsilon, cutoff, truncation) constants: u(rc) and
+// du/dr(rc) as lennard_jones.cpp:32-48 computes them, and their FP32 and
+// fixed-point-scaled forms.
+LJConst lj_constants(double sigma, double epsilon, double cutoff, int linear_force_shift);
+// The CellList grid at `cutoff` (cell_list.cpp:8-14): floor(L_d / cutoff)
+// cells per axis (at least 1), usable with 3 or more on every axis.
+Grid cell_grid(const double L[3], double cutoff);
 // MixK for length unit s and relative cutoff band `band` (0: exact decisions).
 MixK mix_constants(const LJConst& k, double s, double band);
 // One chain propagation of length delta from sc->kinetic; the velocity
@@ -163,6 +170,10 @@ struct DirectArgs {
     LJConst lj;
     const double4* pos;
     double4* frc;  // null: energy/virial only
+    // or, when fx_out is set, forces written as three arrays in upload order
+    double* fx_out;
+    double* fy_out;
+    double* fz_out;
     const int* orig;
     // cells (null cell_atoms: brute force)
     Grid grid;
@@ -174,6 +185,8 @@ struct DirectArgs {
     Scalars* sc;
 };
 void launch_lj_direct(const DirectArgs& a, hipStream_t s);
+// The calling thread's last md_lj_forces_noalloc failure, or null.
+const char* noalloc_error();
 // Bins atoms into cells without permuting them: cell_atoms holds slot
 // indices grouped by cell (ordered by slot within a cell).
 // flag (or null): a device rebuild decision; 0 skips the build.

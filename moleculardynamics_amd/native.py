@@ -114,6 +114,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_langevin_o.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64]
     lib.md_scale_velocities.argtypes = [vp, C.c_double]
     lib.md_lj_forces_cells.argtypes = [vp, C.c_double, _dp, _dp]
+    lib.md_lj_forces_noalloc.argtypes = ([C.c_int, C.POINTER(LJParams), C.c_long, _dp] +
+                                         [vp] * 6 + [_dp, _dp, vp])
     _ip = C.POINTER(C.c_int)
     _lp = C.POINTER(C.c_long)
     _u64p = C.POINTER(C.c_uint64)
@@ -164,6 +166,26 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_mix_random_state.argtypes = [C.c_long, C.c_double, C.c_uint64, C.c_double] + [_dp] * 7
     _lib = lib
     return lib
+
+
+NOALLOC_MAX_ATOMS = 1 << 20
+
+
+def lj_forces_noalloc(source, sigma, epsilon, cutoff, lfs, n, L, x, y, z, fx=0, fy=0, fz=0,
+                      stream=None):
+    """md_lj_forces_noalloc on device arrays given as integer addresses (for
+    example torch tensors' data_ptr()); fx = fy = fz = 0 asks for E and W
+    only. Returns (E, W); raises MDError with md_last_error(NULL)'s text."""
+    lib = load()
+    p = LJParams(sigma, epsilon, cutoff, int(lfs))
+    Lb = _col(L)
+    e, w = C.c_double(), C.c_double()
+    ptr = [C.c_void_p(int(v)) if v else None for v in (x, y, z, fx, fy, fz)]
+    st = lib.md_lj_forces_noalloc(source, C.byref(p), n, _p(Lb), *ptr, C.byref(e), C.byref(w),
+                                  stream)
+    if st != MD_OK:
+        raise MDError(st, lib.md_last_error(None).decode())
+    return e.value, w.value
 
 
 def declared_symbols(header: str = HEADER) -> list[str]:
