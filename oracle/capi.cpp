@@ -438,4 +438,54 @@ int orc_mix_vv(long n, const double* L, double* x, double* y, double* z, double*
     return 0;
 }
 
+// Trajectory analyses over `frames` frames. Frame f holds box lengths
+// L[3f..3f+2] and, for each of x, y, z, n values at xyz[(3f + d) n ...].
+// orc_rdf writes g, the bin centres and the raw pair counts (hist, oracle
+// only); orc_msd / orc_vacf write the max_lag + 1 correlation values and D.
+int orc_rdf(long frames, long n, const double* L, const double* xyz, double r_max, long bins,
+            double* g, double* centres, double* hist) {
+    return guarded([&] {
+        oracle::Rdf rdf(r_max, bins);
+        for (long f = 0; f < frames; ++f) {
+            const double* b = xyz + 3 * f * n;
+            rdf.accumulate(make_state(n, L + 3 * f, b, b + n, b + 2 * n));
+        }
+        copy_out(rdf.g(), g);
+        copy_out(rdf.centres(), centres);
+        copy_out(rdf.hist, hist);
+        return 0;
+    });
+}
+
+int orc_msd(long frames, long n, long max_lag, const double* L, const double* xyz, double dt,
+            long fit_from, long fit_to, double* msd, double* D) {
+    return guarded([&] {
+        Corr c(n, max_lag, true);
+        for (long f = 0; f < frames; ++f) {
+            const double* b = xyz + 3 * f * n;
+            c.push(make_state(n, L + 3 * f, b, b + n, b + 2 * n));
+        }
+        copy_out(c.mean(), msd);
+        if (D) *D = c.msd_diffusion(dt, fit_from, fit_to);
+        return 0;
+    });
+}
+
+int orc_vacf(long frames, long n, long max_lag, const double* vxyz, double dt, double* vacf,
+             double* D) {
+    return guarded([&] {
+        Corr c(n, max_lag, false);
+        const double L[3] = {1.0, 1.0, 1.0};
+        for (long f = 0; f < frames; ++f) {
+            const double* b = vxyz + 3 * f * n;
+            State s = make_state(n, L, b, b + n, b + 2 * n);
+            s.vx = s.x; s.vy = s.y; s.vz = s.z;
+            c.push(s);
+        }
+        copy_out(c.normalised(), vacf);
+        if (D) *D = c.vacf_diffusion(dt);
+        return 0;
+    });
+}
+
 }  // extern "C"

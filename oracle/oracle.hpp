@@ -257,4 +257,34 @@ std::vector<double> mix_minimize(const MixLJ& p, State& s, const std::vector<int
 void mix_vv(const MixLJ& p, State& s, const std::vector<int>& type, double dt, int nsteps,
             std::vector<double>& energy, std::vector<double>& virial);
 
+// ---- trajectory analysis (analyze/rdf.cpp:16-71, msd.cpp:12-92,
+// vacf.cpp:11-85) -------------------------------------------------------------
+struct Rdf {
+    double r_max;
+    long bins;
+    double width;
+    std::vector<double> hist;  // pair counts, exact integers in FP64
+    long frames = 0, natoms = 0;
+    double sum_rho = 0.0;
+    Rdf(double rmax, long nbins);
+    void accumulate(const State& s);  // throws std::invalid_argument
+    std::vector<double> centres() const;
+    std::vector<double> g() const;
+};
+// One ring-buffered time correlation: the MSD (displacement = true) or the
+// VACF. Per-atom vectors are packed x,y,z per atom.
+struct Corr {
+    long n, max_lag, period;
+    bool msd;
+    long pushed = 0;
+    std::vector<double> prev, cur, ring, sum;
+    std::vector<long> count;
+    Corr(long natoms, long maxlag, bool displacement);
+    void push(const State& s);
+    std::vector<double> mean() const;        // msd() / the unnormalised VACF
+    std::vector<double> normalised() const;  // vacf()
+    double msd_diffusion(double dt, long from, long to) const;
+    double vacf_diffusion(double dt) const;
+};
+
 }  // namespace oracle

@@ -84,6 +84,10 @@ class Oracle:
             lib.orc_lj_ev_compensated.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double,
                                                   C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         lib.orc_cells_usable.argtypes = [_dp, C.c_double]
+        lib.orc_rdf.argtypes = [C.c_long, C.c_long, _dp, _dp, C.c_double, C.c_long, _dp, _dp, _dp]
+        lib.orc_msd.argtypes = [C.c_long, C.c_long, C.c_long, _dp, _dp, C.c_double, C.c_long,
+                                C.c_long, _dp, _dp]
+        lib.orc_vacf.argtypes = [C.c_long, C.c_long, C.c_long, _dp, C.c_double, _dp, _dp]
         lib.orc_verlet_pairs.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                          _up, C.c_long]
         lib.orc_verlet_pairs.restype = C.c_long
@@ -173,6 +177,45 @@ class Oracle:
         out = np.zeros(2)
         self.lib.orc_lj_lrc(n, _ptr(np.asarray(L, dtype=float)), sig, eps, rc, int(lfs), _ptr(out))
         return float(out[0]), float(out[1])
+
+    # ---- trajectory analysis -------------------------------------------------
+    @staticmethod
+    def _frames(L, xyz):
+        """L: (frames, 3) or (3,); xyz: (frames, n, 3) -> packed (frames, 3, n)."""
+        xyz = np.asarray(xyz, dtype=np.float64)
+        nf, n = xyz.shape[0], xyz.shape[1]
+        Lf = np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.float64), (nf, 3)))
+        packed = np.ascontiguousarray(np.transpose(xyz, (0, 2, 1)))
+        return nf, n, Lf, packed
+
+    def rdf(self, L, xyz, r_max: float, bins: int):
+        """(g, centres, hist) of Rdf(r_max, bins) accumulated over the frames
+        (hist is all zero from the reference build, whose Rdf keeps it private)."""
+        nf, n, Lf, packed = self._frames(L, xyz)
+        g, c, h = np.zeros(bins), np.zeros(bins), np.zeros(bins)
+        st = self.lib.orc_rdf(nf, n, _ptr(Lf), _ptr(packed), r_max, bins, _ptr(g), _ptr(c), _ptr(h))
+        if st != 0:
+            raise (ValueError if st == -1 else RuntimeError)(self.error())
+        return g, c, h
+
+    def msd(self, L, xyz, max_lag: int, dt=1.0, fit_from=0, fit_to=0):
+        """(msd per lag, D) of Msd(n, max_lag) pushed once per frame."""
+        nf, n, Lf, packed = self._frames(L, xyz)
+        m, D = np.zeros(max_lag + 1), np.zeros(1)
+        st = self.lib.orc_msd(nf, n, max_lag, _ptr(Lf), _ptr(packed), dt, fit_from, fit_to,
+                              _ptr(m), _ptr(D))
+        if st != 0:
+            raise RuntimeError(self.error())
+        return m, float(D[0])
+
+    def vacf(self, vxyz, max_lag: int, dt=1.0):
+        """(normalised vacf per lag, D) of Vacf(n, max_lag) pushed once per frame."""
+        nf, n, _, packed = self._frames(np.ones(3), vxyz)
+        c, D = np.zeros(max_lag + 1), np.zeros(1)
+        st = self.lib.orc_vacf(nf, n, max_lag, _ptr(packed), dt, _ptr(c), _ptr(D))
+        if st != 0:
+            raise RuntimeError(self.error())
+        return c, float(D[0])
 
     def cells_usable(self, L, cutoff: float) -> bool:
         return bool(self.lib.orc_cells_usable(_ptr(np.asarray(L, dtype=float)), cutoff))

@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include "moldyn/analyze/msd.hpp"
+#include "moldyn/analyze/rdf.hpp"
+#include "moldyn/analyze/vacf.hpp"
 #include "moldyn/constrain/rigid_water.hpp"
 #include "moldyn/core/box.hpp"
 #include "moldyn/core/system.hpp"
@@ -360,6 +363,56 @@ int orc_water_run(long n, const double* L, double* x, double* y, double* z, doub
             x[i] = r.x; y[i] = r.y; z[i] = r.z;
             vx[i] = v.x; vy[i] = v.y; vz[i] = v.z;
         }
+        return 0;
+    });
+}
+
+int orc_rdf(long frames, long n, const double* L, const double* xyz, double r_max, long bins,
+            double* g, double* centres, double* /*hist: not exposed by Rdf*/) {
+    return guarded([&] {
+        Rdf rdf(r_max, static_cast<std::size_t>(bins));
+        for (long f = 0; f < frames; ++f) {
+            const double* b = xyz + 3 * f * n;
+            rdf.accumulate(make_system(n, L + 3 * f, b, b + n, b + 2 * n));
+        }
+        const std::vector<double> gv = rdf.g(), cv = rdf.binCentres();
+        if (g) std::memcpy(g, gv.data(), gv.size() * sizeof(double));
+        if (centres) std::memcpy(centres, cv.data(), cv.size() * sizeof(double));
+        return 0;
+    });
+}
+
+int orc_msd(long frames, long n, long max_lag, const double* L, const double* xyz, double dt,
+            long fit_from, long fit_to, double* msd, double* D) {
+    return guarded([&] {
+        Msd m(static_cast<std::size_t>(n), static_cast<std::size_t>(max_lag));
+        for (long f = 0; f < frames; ++f) {
+            const double* b = xyz + 3 * f * n;
+            m.push(make_system(n, L + 3 * f, b, b + n, b + 2 * n));
+        }
+        const std::vector<double> v = m.msd();
+        if (msd) std::memcpy(msd, v.data(), v.size() * sizeof(double));
+        if (D)
+            *D = m.diffusionCoefficient(dt, static_cast<std::size_t>(fit_from),
+                                        static_cast<std::size_t>(fit_to));
+        return 0;
+    });
+}
+
+int orc_vacf(long frames, long n, long max_lag, const double* vxyz, double dt, double* vacf,
+             double* D) {
+    return guarded([&] {
+        Vacf c(static_cast<std::size_t>(n), static_cast<std::size_t>(max_lag));
+        for (long f = 0; f < frames; ++f) {
+            const double* b = vxyz + 3 * f * n;
+            System s(Box(Vec3{1.0, 1.0, 1.0}));
+            for (long i = 0; i < n; ++i)
+                s.addAtom(Vec3{0, 0, 0}, Vec3{b[i], b[n + i], b[2 * n + i]}, 1.0);
+            c.push(s);
+        }
+        const std::vector<double> v = c.vacf();
+        if (vacf) std::memcpy(vacf, v.data(), v.size() * sizeof(double));
+        if (D) *D = c.diffusionCoefficient(dt);
         return 0;
     });
 }
