@@ -215,6 +215,62 @@ md_status md_scale_velocities(md_ctx* ctx, double s);
  * side; MD_ERR_INVALID_ARGUMENT when fewer than 3 fit. */
 md_status md_lj_forces_cells(md_ctx* ctx, double cell_cutoff, double* energy, double* virial);
 
+/* ---- trajectory analysis (analyze/rdf.hpp:16-34, msd.hpp:19-42,
+ * vacf.hpp:151-174) -----------------------------------------------------------
+ * Device-resident accumulators fed from a context's current positions or
+ * velocities (its device copy; no host transfer). Each handle lives on the
+ * device current at creation and takes contexts on that device; work is
+ * queued on the context's stream; the *_result / *_diffusion calls wait for
+ * the device. */
+typedef struct md_rdf md_rdf;
+typedef struct md_msd md_msd;
+typedef struct md_vacf md_vacf;
+
+/* Pair sources of md_rdf_accumulate. */
+#define MD_RDF_AUTO 0      /* cells when 3 or more of width >= rMax fit per side, else all pairs */
+#define MD_RDF_ALL_PAIRS 1 /* every i<j pair (the reference's loop) */
+#define MD_RDF_CELLS 2     /* 27 cells of width >= rMax around each atom */
+
+/* Rdf(rMax, bins) (rdf.hpp:18, rdf.cpp:16-17): bin width rMax / bins. */
+md_status md_rdf_create(double r_max, long bins, md_rdf** out);
+md_status md_rdf_destroy(md_rdf* rdf);
+const char* md_rdf_last_error(const md_rdf* rdf);
+/* Rdf::accumulate (rdf.cpp:19-46): counts every unordered pair of the
+ * context's system with r = |minimum image| < rMax in bin floor(r / width);
+ * MD_ERR_INVALID_ARGUMENT when rMax exceeds half the smallest box length
+ * (the reference throws std::invalid_argument). Counts are exact integers,
+ * the same for every pair source. */
+md_status md_rdf_accumulate(md_rdf* rdf, md_ctx* ctx, int source);
+/* Rdf::binCentres, g, frames (rdf.cpp:48-75): bins values each (any pointer
+ * may be NULL); hist receives the raw pair counts. */
+md_status md_rdf_result(md_rdf* rdf, double* centres, double* g, uint64_t* hist, long* frames);
+
+/* Msd(natoms, maxLag) (msd.hpp:21, msd.cpp:12-20). */
+md_status md_msd_create(long natoms, long max_lag, md_msd** out);
+md_status md_msd_destroy(md_msd* msd);
+const char* md_msd_last_error(const md_msd* msd);
+/* Msd::push (msd.cpp:22-52): unwraps incrementally with the minimum image of
+ * the step from the previous push, by atom upload index, and adds every lag
+ * up to min(pushes, maxLag). MD_ERR_INVALID_ARGUMENT when the context's atom
+ * count differs from natoms. */
+md_status md_msd_push(md_msd* msd, md_ctx* ctx);
+/* Msd::msd (msd.cpp:54-62): max_lag + 1 values. */
+md_status md_msd_result(md_msd* msd, double* out);
+/* Msd::diffusionCoefficient (msd.cpp:64-92). */
+md_status md_msd_diffusion(md_msd* msd, double dt, long fit_from, long fit_to, double* D);
+
+/* Vacf(natoms, maxLag) (vacf.hpp:153, vacf.cpp:11-17). */
+md_status md_vacf_create(long natoms, long max_lag, md_vacf** out);
+md_status md_vacf_destroy(md_vacf* vacf);
+const char* md_vacf_last_error(const md_vacf* vacf);
+/* Vacf::push (vacf.cpp:19-40). */
+md_status md_vacf_push(md_vacf* vacf, md_ctx* ctx);
+/* Vacf::vacf (vacf.cpp:52-63), max_lag + 1 values normalised to lag 0; the
+ * unnormalised correlation (vacf.cpp:42-50) when `normalised` is 0. */
+md_status md_vacf_result(md_vacf* vacf, int normalised, double* out);
+/* Vacf::diffusionCoefficient (vacf.cpp:65-85): Green-Kubo, trapezoid rule. */
+md_status md_vacf_diffusion(md_vacf* vacf, double dt, double* D);
+
 /* ---- allocation-free force evaluation ------------------------------------- */
 
 #define MD_NOALLOC_MAX_ATOMS (1L << 20) /* atoms per md_lj_forces_noalloc call */

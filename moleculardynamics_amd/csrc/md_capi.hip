@@ -383,6 +383,22 @@ md_status restore_snapshot(md_ctx* c) {
 
 }  // namespace
 
+namespace mdk {
+bool ctx_view(md_ctx* c, CtxView* v) {
+    if (c == nullptr || c->n <= 0) return false;
+    v->n = c->n;
+    v->box = c->box;
+    for (int d = 0; d < 3; ++d) v->L[d] = c->L[d];
+    v->pos = c->pos;
+    v->vel = c->vel;
+    v->orig = c->orig;
+    v->slot = c->slot;
+    v->stream = c->stream;
+    v->device = c->device;
+    return true;
+}
+}  // namespace mdk
+
 extern "C" {
 
 md_status md_ctx_create(void* hip_stream, md_ctx** out) {

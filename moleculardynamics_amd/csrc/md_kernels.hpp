@@ -8,6 +8,8 @@
 
 #include "md_common.hpp"
 
+struct md_ctx;
+
 namespace mdk {
 
 // Cell grid of the list build: nc = max(1, floor(L/cutoff)) per axis
@@ -185,6 +187,21 @@ struct DirectArgs {
     Scalars* sc;
 };
 void launch_lj_direct(const DirectArgs& a, hipStream_t s);
+// A context's device system, for the analysis entry points (positions and
+// velocities in device slot order; slot[upload index] = slot).
+struct CtxView {
+    long n;
+    Box box;
+    double L[3];
+    const double4* pos;
+    const double4* vel;
+    const int* orig;
+    const int* slot;
+    hipStream_t stream;
+    int device;
+};
+// False for a null context or one without a system.
+bool ctx_view(md_ctx* c, CtxView* v);
 // The calling thread's last md_lj_forces_noalloc failure, or null.
 const char* noalloc_error();
 // Bins atoms into cells without permuting them: cell_atoms holds slot

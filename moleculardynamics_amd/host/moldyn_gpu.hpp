@@ -5,13 +5,13 @@
 // Drop-in shape: the class names, constructor arguments, member functions,
 // argument meanings and exception types are those of the reference
 // (VishnuR23/MolecularDynamics src/moldyn/{core,forcefield,neighbor,
-// integrate,constrain,util}), so code written against
+// integrate,constrain,analyze,util}), so code written against
 //   moldyn::System, moldyn::Box, moldyn::ForceField, moldyn::LennardJones,
 //   moldyn::CellList, moldyn::VerletList, moldyn::VelocityVerlet,
 //   moldyn::LangevinBAOAB, moldyn::NoseHooverChain, moldyn::Rng,
 //   moldyn::units, moldyn::Ewald, moldyn::SpceForceField,
 //   moldyn::buildSpceSystem, moldyn::RigidWater,
-//   moldyn::ConstrainedVelocityVerlet
+//   moldyn::ConstrainedVelocityVerlet, moldyn::Rdf, moldyn::Msd, moldyn::Vacf
 // compiles against moldyn::gpu::* with the namespace changed. The System
 // keeps host arrays and a device copy, synchronised lazily: device work
 // (forces, list builds, integrator passes) uploads host edits first and
@@ -544,6 +544,129 @@ private:
     std::size_t dof_;
     VelocityVerlet core_;
     std::vector<double> Q_, xi_, p_;
+};
+
+namespace detail {
+// Status of an analysis handle call, with the handle's own message.
+template <class H>
+inline void check_handle(md_status st, const H* h, const char* (*last)(const H*), const char* what) {
+    if (st == MD_OK) return;
+    const std::string msg = std::string(what) + ": " + (h ? last(h) : "");
+    if (st == MD_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+}  // namespace detail
+
+// analyze/rdf.hpp:16-34: g(r) accumulated on the device from the System's
+// device copy (md_rdf_*). Pair counts are exact, so g equals the
+// reference's bit for bit on the same frames.
+class Rdf {
+public:
+    Rdf(double rMax, std::size_t bins) : bins_(bins) {
+        md_rdf* h = nullptr;
+        detail::check_handle<md_rdf>(md_rdf_create(rMax, static_cast<long>(bins), &h), nullptr,
+                                     md_rdf_last_error, "Rdf");
+        h_.reset(h);
+    }
+    // rdf.cpp:19-46; throws std::invalid_argument when rMax exceeds half the
+    // smallest box length
+    void accumulate(const System& sys) {
+        detail::check_handle<md_rdf>(md_rdf_accumulate(h_.get(), sys.device(), MD_RDF_AUTO), h_.get(),
+                                     md_rdf_last_error, "Rdf::accumulate");
+    }
+    std::vector<double> binCentres() const { return fetch(true); }
+    std::vector<double> g() const { return fetch(false); }
+    std::size_t frames() const {
+        long f = 0;
+        detail::check_handle<md_rdf>(md_rdf_result(h_.get(), nullptr, nullptr, nullptr, &f), h_.get(),
+                                     md_rdf_last_error, "Rdf::frames");
+        return static_cast<std::size_t>(f);
+    }
+
+private:
+    std::vector<double> fetch(bool centres) const {
+        std::vector<double> v(bins_);
+        detail::check_handle<md_rdf>(md_rdf_result(h_.get(), centres ? v.data() : nullptr,
+                                                   centres ? nullptr : v.data(), nullptr, nullptr),
+                                     h_.get(), md_rdf_last_error, "Rdf::g");
+        return v;
+    }
+    struct Del {
+        void operator()(md_rdf* h) const { md_rdf_destroy(h); }
+    };
+    std::size_t bins_;
+    std::unique_ptr<md_rdf, Del> h_;
+};
+
+// analyze/msd.hpp:19-42: ring buffer and unwrapping on the device (md_msd_*).
+class Msd {
+public:
+    Msd(std::size_t natoms, std::size_t maxLag) {
+        md_msd* h = nullptr;
+        detail::check_handle<md_msd>(md_msd_create(static_cast<long>(natoms), static_cast<long>(maxLag), &h),
+                                     nullptr, md_msd_last_error, "Msd");
+        h_.reset(h);
+        lags_ = maxLag + 1;
+    }
+    void push(const System& sys) {
+        detail::check_handle<md_msd>(md_msd_push(h_.get(), sys.device()), h_.get(), md_msd_last_error,
+                                     "Msd::push");
+    }
+    std::vector<double> msd() const {
+        std::vector<double> v(lags_);
+        detail::check_handle<md_msd>(md_msd_result(h_.get(), v.data()), h_.get(), md_msd_last_error,
+                                     "Msd::msd");
+        return v;
+    }
+    double diffusionCoefficient(double dt, std::size_t fitFrom, std::size_t fitTo) const {
+        double d = 0.0;
+        detail::check_handle<md_msd>(md_msd_diffusion(h_.get(), dt, static_cast<long>(fitFrom),
+                                                      static_cast<long>(fitTo), &d),
+                                     h_.get(), md_msd_last_error, "Msd::diffusionCoefficient");
+        return d;
+    }
+
+private:
+    struct Del {
+        void operator()(md_msd* h) const { md_msd_destroy(h); }
+    };
+    std::size_t lags_ = 0;
+    std::unique_ptr<md_msd, Del> h_;
+};
+
+// analyze/vacf.hpp:151-174 on the device (md_vacf_*).
+class Vacf {
+public:
+    Vacf(std::size_t natoms, std::size_t maxLag) {
+        md_vacf* h = nullptr;
+        detail::check_handle<md_vacf>(md_vacf_create(static_cast<long>(natoms), static_cast<long>(maxLag), &h),
+                                      nullptr, md_vacf_last_error, "Vacf");
+        h_.reset(h);
+        lags_ = maxLag + 1;
+    }
+    void push(const System& sys) {
+        detail::check_handle<md_vacf>(md_vacf_push(h_.get(), sys.device()), h_.get(), md_vacf_last_error,
+                                      "Vacf::push");
+    }
+    std::vector<double> vacf() const {
+        std::vector<double> v(lags_);
+        detail::check_handle<md_vacf>(md_vacf_result(h_.get(), 1, v.data()), h_.get(),
+                                      md_vacf_last_error, "Vacf::vacf");
+        return v;
+    }
+    double diffusionCoefficient(double dt) const {
+        double d = 0.0;
+        detail::check_handle<md_vacf>(md_vacf_diffusion(h_.get(), dt, &d), h_.get(), md_vacf_last_error,
+                                      "Vacf::diffusionCoefficient");
+        return d;
+    }
+
+private:
+    struct Del {
+        void operator()(md_vacf* h) const { md_vacf_destroy(h); }
+    };
+    std::size_t lags_ = 0;
+    std::unique_ptr<md_vacf, Del> h_;
 };
 
 // util/lattice.cpp:15-64 through md_fcc_lattice (bit-identical inputs).

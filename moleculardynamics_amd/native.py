@@ -21,6 +21,9 @@ VV, BAOAB, NHC = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
 
 
 class LJParams(C.Structure):
@@ -114,11 +117,22 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_langevin_o.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64]
     lib.md_scale_velocities.argtypes = [vp, C.c_double]
     lib.md_lj_forces_cells.argtypes = [vp, C.c_double, _dp, _dp]
+    for kind in ("rdf", "msd", "vacf"):
+        getattr(lib, f"md_{kind}_destroy").argtypes = [vp]
+        getattr(lib, f"md_{kind}_last_error").argtypes = [vp]
+        getattr(lib, f"md_{kind}_last_error").restype = C.c_char_p
+        if kind != "rdf":
+            getattr(lib, f"md_{kind}_create").argtypes = [C.c_long, C.c_long, C.POINTER(vp)]
+            getattr(lib, f"md_{kind}_push").argtypes = [vp, vp]
+    lib.md_rdf_create.argtypes = [C.c_double, C.c_long, C.POINTER(vp)]
+    lib.md_rdf_accumulate.argtypes = [vp, vp, C.c_int]
+    lib.md_rdf_result.argtypes = [vp, _dp, _dp, _u64p, C.POINTER(C.c_long)]
+    lib.md_msd_result.argtypes = [vp, _dp]
+    lib.md_msd_diffusion.argtypes = [vp, C.c_double, C.c_long, C.c_long, _dp]
+    lib.md_vacf_result.argtypes = [vp, C.c_int, _dp]
+    lib.md_vacf_diffusion.argtypes = [vp, C.c_double, _dp]
     lib.md_lj_forces_noalloc.argtypes = ([C.c_int, C.POINTER(LJParams), C.c_long, _dp] +
                                          [vp] * 6 + [_dp, _dp, vp])
-    _ip = C.POINTER(C.c_int)
-    _lp = C.POINTER(C.c_long)
-    _u64p = C.POINTER(C.c_uint64)
     lib.md_batch_draw.argtypes = [C.c_int, C.c_uint64, _ip, _dp, _dp, _u64p]
     lib.md_batch_create.argtypes = ([vp, C.c_int, _lp, _dp] + [_dp] * 6 +
                                     [_dp, _u64p, C.POINTER(LJParams), C.c_double, C.c_double,
@@ -166,6 +180,100 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_mix_random_state.argtypes = [C.c_long, C.c_double, C.c_uint64, C.c_double] + [_dp] * 7
     _lib = lib
     return lib
+
+
+RDF_AUTO, RDF_ALL_PAIRS, RDF_CELLS = 0, 1, 2
+
+
+class _Handle:
+    """An md_rdf / md_msd / md_vacf handle on the current device."""
+
+    kind = ""
+
+    def _make(self, *args):
+        self.lib = load()
+        h = C.c_void_p()
+        st = getattr(self.lib, f"md_{self.kind}_create")(*args, C.byref(h))
+        if st != MD_OK:
+            raise MDError(st, f"md_{self.kind}_create failed")
+        self.h = h
+
+    def _chk(self, st):
+        if st != MD_OK:
+            msg = getattr(self.lib, f"md_{self.kind}_last_error")(self.h).decode()
+            raise MDError(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, f"md_{self.kind}_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Rdf(_Handle):
+    """Rdf(r_max, bins) accumulated from a Context's device system."""
+
+    kind = "rdf"
+
+    def __init__(self, r_max, bins):
+        self.bins = int(bins)
+        self._make(C.c_double(r_max), C.c_long(self.bins))
+
+    def accumulate(self, ctx, source=RDF_AUTO):
+        self._chk(self.lib.md_rdf_accumulate(self.h, ctx.h, source))
+
+    def result(self):
+        """(centres, g, hist, frames)."""
+        c, g = np.zeros(self.bins), np.zeros(self.bins)
+        h = np.zeros(self.bins, dtype=np.uint64)
+        f = C.c_long()
+        self._chk(self.lib.md_rdf_result(self.h, _p(c), _p(g), h.ctypes.data_as(_u64p),
+                                         C.byref(f)))
+        return c, g, h, f.value
+
+
+class Msd(_Handle):
+    kind = "msd"
+
+    def __init__(self, natoms, max_lag):
+        self.max_lag = int(max_lag)
+        self._make(C.c_long(natoms), C.c_long(self.max_lag))
+
+    def push(self, ctx):
+        self._chk(self.lib.md_msd_push(self.h, ctx.h))
+
+    def result(self):
+        m = np.zeros(self.max_lag + 1)
+        self._chk(self.lib.md_msd_result(self.h, _p(m)))
+        return m
+
+    def diffusion(self, dt, fit_from, fit_to):
+        d = C.c_double()
+        self._chk(self.lib.md_msd_diffusion(self.h, dt, fit_from, fit_to, C.byref(d)))
+        return d.value
+
+
+class Vacf(_Handle):
+    kind = "vacf"
+
+    def __init__(self, natoms, max_lag):
+        self.max_lag = int(max_lag)
+        self._make(C.c_long(natoms), C.c_long(self.max_lag))
+
+    def push(self, ctx):
+        self._chk(self.lib.md_vacf_push(self.h, ctx.h))
+
+    def result(self, normalised=True):
+        c = np.zeros(self.max_lag + 1)
+        self._chk(self.lib.md_vacf_result(self.h, int(normalised), _p(c)))
+        return c
+
+    def diffusion(self, dt):
+        d = C.c_double()
+        self._chk(self.lib.md_vacf_diffusion(self.h, dt, C.byref(d)))
+        return d.value
 
 
 NOALLOC_MAX_ATOMS = 1 << 20

@@ -1,6 +1,6 @@
 // The reference's own test cases (tests/test_neighbor.cpp, test_forces.cpp,
-// test_integrator.cpp, test_thermostat.cpp, test_nist_spce.cpp and
-// test_rigid_water.cpp of VishnuR23/MolecularDynamics),
+// test_integrator.cpp, test_thermostat.cpp, test_nist_spce.cpp,
+// test_rigid_water.cpp and test_analysis.cpp of VishnuR23/MolecularDynamics),
 // restated against the GPU drop-in classes of moleculardynamics_amd/host/
 // moldyn_gpu.hpp: same calls, same tolerances, same exception types. Runs
 // on a GPU; pytest drives it (tests/test_gpu_mirror.py).
@@ -31,6 +31,9 @@ static int g_fail = 0, g_pass = 0;
         }                                                                               \
     } while (0)
 #define APPROX(a, b, rel) (std::abs((a) - (b)) <= (rel) * std::max(std::abs(b), 1e-300))
+// doctest::Approx(b).epsilon(eps), the reference tests' comparison
+#define DAPPROX(a, b, eps) \
+    (std::abs((a) - (b)) < (eps) * (1.0 + std::max(std::abs(a), std::abs(b))))
 
 static std::string g_data;
 
@@ -576,6 +579,140 @@ int main(int argc, char** argv) {
         const double slope = num / den;
         std::printf("  constrained conservation slope %.4f\n", slope);
         CHECK(slope > 1.85 && slope < 2.15);
+    });
+
+    // ---- tests/test_analysis.cpp:19-157, 228-307 ----
+    run("g(r) of an ideal gas is 1 everywhere", [] {
+        Box box(Vec3{20, 20, 20});
+        Rng rng(4242);
+        Rdf rdf(8.0, 80);
+        for (int frame = 0; frame < 40; ++frame) {
+            System sys(box);
+            for (int i = 0; i < 1500; ++i)
+                sys.addAtom(Vec3{(rng.uniform() - 0.5) * 20.0, (rng.uniform() - 0.5) * 20.0,
+                                 (rng.uniform() - 0.5) * 20.0},
+                            Vec3{0, 0, 0}, 1.0);
+            rdf.accumulate(sys);
+        }
+        auto g = rdf.g();
+        bool ok = true;
+        for (std::size_t k = 5; k < g.size(); ++k) ok = ok && DAPPROX(g[k], 1.0, 0.06);
+        CHECK(ok);
+        CHECK(rdf.frames() == 40);
+    });
+
+    run("g(r) of two atoms puts all weight in one bin", [] {
+        Box box(Vec3{20, 20, 20});
+        Rdf rdf(8.0, 80);
+        System sys(box);
+        sys.addAtom(Vec3{0.0, 0.0, 0.0}, Vec3{0, 0, 0}, 1.0);
+        sys.addAtom(Vec3{3.05, 0.0, 0.0}, Vec3{0, 0, 0}, 1.0);
+        rdf.accumulate(sys);
+        auto g = rdf.g();
+        const std::size_t expected = static_cast<std::size_t>(3.05 / (8.0 / 80.0));
+        bool ok = true;
+        for (std::size_t k = 0; k < g.size(); ++k)
+            ok = ok && (k == expected ? g[k] > 0.0 : DAPPROX(g[k], 0.0, 1e-12));
+        CHECK(ok);
+        auto r = rdf.binCentres();
+        CHECK(APPROX(r[expected], (expected + 0.5) * 0.1, 1e-15));
+        bool threw = false;
+        try {
+            Rdf(10.5, 10).accumulate(sys);
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        CHECK(threw);
+    });
+
+    run("Msd ballistic limit: msd(t) = <v^2> t^2 for force-free atoms", [] {
+        Box box(Vec3{1.0e6, 1.0e6, 1.0e6});
+        const std::vector<Vec3> v = {Vec3{1.0, 0.0, 0.0}, Vec3{0.0, 2.0, 0.0}, Vec3{0.5, 0.5, 0.5}};
+        System sys(box);
+        for (std::size_t i = 0; i < 3; ++i) sys.addAtom(Vec3{0.0, 0.0, 0.0}, v[i], 1.0);
+        double vSq = 0.0;
+        for (const Vec3& a : v) vSq += norm2(a);
+        vSq /= 3.0;
+        Msd msd(3, 50);
+        for (std::size_t frame = 0; frame <= 50; ++frame) {
+            for (std::size_t i = 0; i < 3; ++i) sys.setPosition(i, v[i] * (frame * 0.1));
+            msd.push(sys);
+        }
+        const auto m = msd.msd();
+        bool ok = true;
+        for (std::size_t lag = 0; lag <= 50; ++lag) {
+            const double t = lag * 0.1;
+            ok = ok && DAPPROX(m[lag], vSq * t * t, 1e-10);
+        }
+        CHECK(ok);
+    });
+
+    run("Msd unwrapping: an atom crossing the periodic boundary keeps accumulating", [] {
+        Box box(Vec3{4.0, 4.0, 4.0});
+        System sys(box);
+        sys.addAtom(Vec3{0.0, 0.0, 0.0}, Vec3{1.7, 0.0, 0.0}, 1.0);
+        Msd msd(1, 1500);
+        for (std::size_t frame = 0; frame < 2000; ++frame) {
+            sys.setPosition(0, box.wrap(Vec3{1.7 * (frame * 0.05), 0.0, 0.0}));
+            msd.push(sys);
+        }
+        const auto m = msd.msd();
+        bool ok = true;
+        for (std::size_t lag = 0; lag <= 1500; lag += 100) {
+            const double t = lag * 0.05;
+            ok = ok && DAPPROX(m[lag], 1.7 * 1.7 * t * t, 1e-8);
+        }
+        CHECK(ok);
+        CHECK(m[1500] > 16.0);
+    });
+
+    run("Vacf of free particles is exactly 1 at every lag", [] {
+        Box box(Vec3{100.0, 100.0, 100.0});
+        System sys(box);
+        sys.addAtom(Vec3{0, 0, 0}, Vec3{1.0, 0.0, 0.0}, 1.0);
+        sys.addAtom(Vec3{0, 0, 0}, Vec3{0.0, -2.0, 0.5}, 1.0);
+        sys.addAtom(Vec3{0, 0, 0}, Vec3{0.3, 0.3, 0.3}, 1.0);
+        sys.addAtom(Vec3{0, 0, 0}, Vec3{-1.5, 0.0, 2.0}, 1.0);
+        Vacf vacf(4, 40);
+        for (std::size_t frame = 0; frame <= 45; ++frame) vacf.push(sys);
+        const auto c = vacf.vacf();
+        bool ok = true;
+        for (std::size_t lag = 0; lag <= 40; ++lag) ok = ok && DAPPROX(c[lag], 1.0, 1e-10);
+        CHECK(ok);
+    });
+
+    run("Diffusion coefficient of a free Ornstein-Uhlenbeck particle matches D = kT/(m*friction)", [] {
+        const std::size_t natoms = 6000, maxLag = 100;
+        const double dt = 0.2;
+        Box box(Vec3{1000.0, 1000.0, 1000.0});
+        System sys(box);
+        for (std::size_t i = 0; i < natoms; ++i) sys.addAtom(Vec3{0.0, 0.0, 0.0}, Vec3{0.0, 0.0, 0.0}, 1.0);
+        auto forceFn = [](System& s) {
+            s.zeroForces();
+            return 0.0;
+        };
+        sys.zeroForces();
+        LangevinBAOAB thermostat(dt, 1.0, 1.0, 20260921);
+        for (int i = 0; i < 100; ++i) thermostat.step(sys, forceFn);
+        Msd msd(natoms, maxLag);
+        Vacf vacf(natoms, maxLag);
+        msd.push(sys);
+        vacf.push(sys);
+        for (int i = 0; i < 750; ++i) {
+            thermostat.step(sys, forceFn);
+            msd.push(sys);
+            vacf.push(sys);
+        }
+        const double dMsd = msd.diffusionCoefficient(dt, static_cast<std::size_t>(10.0 / dt), maxLag + 1);
+        const double dVacf = vacf.diffusionCoefficient(dt);
+        std::printf("  D(msd) %.5f  D(vacf) %.5f\n", dMsd, dVacf);
+        CHECK(DAPPROX(dMsd, 1.0, 0.03));
+        CHECK(DAPPROX(dVacf, 1.0, 0.03));
+        CHECK(std::fabs(dMsd - dVacf) < 0.03);
+        const auto c = vacf.vacf();
+        bool ok = true;
+        for (std::size_t lag = 0; lag <= maxLag; lag += 10) ok = ok && DAPPROX(c[lag], std::exp(-(lag * dt)), 0.05);
+        CHECK(ok);
     });
 
     std::printf("%d checks passed, %d failed\n", g_pass, g_fail);
