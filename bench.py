@@ -12,9 +12,11 @@ path and the mixed path (FP32 pair arithmetic, 64-bit fixed-point
 accumulation, FP64 state). `value` is the mixed path's timesteps/s; the FP64
 numbers, pair-interactions/s, the mixed-vs-FP64 force error and the roofline
 of the dominant kernel (the LJ pair kernel, timed with HIP events on the
-stream it runs on) are in the same JSON line, next to the CPU baseline: the
-reference compiled from its own sources (oracle/_ref) when present, else the
-oracle restatement, single-threaded, on a bounded sample of the same workload.
+stream it runs on, against the device's peaks measured in the same run), a
+2000-step NVE drift of both arithmetic paths, the mixed-Ewald error on the
+SPC/E configuration, and the CPU baseline: the oracle restatement of the
+reference CPU path (-O3, one thread) on 20 NHC steps of the same workload,
+with the host CPU model and core count.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -34,9 +36,10 @@ sys.path.insert(0, ROOT)
 CELLS, RHO, TEMP, SEED = 64, 0.8442, 0.722, 20260001
 RC, SKIN, DT, TAU, CHAIN = 2.5, 0.3, 0.005, 0.5, 3
 F_PAIR = 31.0              # flops per pair-interaction (fixed convention)
-PEAK_FP64 = 78.6e12        # MI355X vector FP64, FLOP/s
-PEAK_FP32 = 157.3e12       # MI355X vector FP32, FLOP/s
-PEAK_HBM = 8.0e12          # B/s
+# AMD-published MI355X figures, reported beside the peaks measured in the run
+PUB_FP64 = 78.6e12         # vector FP64, FLOP/s
+PUB_FP32 = 157.3e12        # vector FP32 (packed), FLOP/s
+PUB_HBM = 8.0e12           # B/s
 
 
 def log(msg: str) -> None:
@@ -115,12 +118,80 @@ def force_error(L, pos, vel):
                 virial_rel=abs(w32 - w64) / abs(w64))
 
 
-def cpu_baseline(timed_steps: int = 10, warm_steps: int = 2):
+def host_description():
+    """CPU model and the cores this process may use (and the machine's count)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count()}
+
+
+def nve_drift(L, pos, vel, steps: int):
+    """max_t |E(t) - E(0)| / |E(0)| of velocity-Verlet NVE on the LinearForceShift
+    potential (continuous energy, so cutoff crossings do not mask precision),
+    FP64 and mixed legs from the same state, plus the largest relative gap
+    between the two legs' total energies."""
+    from moleculardynamics_amd import native
+
+    out = {}
+    h = {}
+    for name, prec in [("fp64", native.FP64), ("mixed", native.MIXED)]:
+        ctx = native.Context()
+        ctx.upload(L, pos, vel)
+        ctx.set_lj(1.0, 1.0, RC, True)
+        ctx.config_list(RC, SKIN)
+        ctx.build_list()
+        ctx.forces(native.VERLET, prec)
+        ctx.set_integrator(kind=native.VV, dt=DT)
+        th = ctx.run(steps, prec, thermo=True)
+        ctx.close()
+        h[name] = th["pe"] + th["ke"]
+        out[name] = float(np.max(np.abs(h[name] - h[name][0])) / abs(h[name][0]))
+    out["legs_max_rel_gap"] = float(np.max(np.abs(h["mixed"] - h["fp64"])) / abs(h["fp64"][0]))
+    out.update(steps=steps, potential="LinearForceShift rc=2.5", integrator="VV dt*=0.005")
+    return out
+
+
+def water_mixed_error():
+    """Mixed (FP32 real-space erfc) against FP64 SPC/E forces on the 4096-molecule
+    configuration (12,288 sites, rc 9 A, alpha/kmax tuned for 1e-5)."""
+    from moleculardynamics_amd import native
+
+    L = 49.71853
+    pos, vel = native.water_lattice(16, L, 2026, 300.0)
+    q2 = 4096 * (0.8476 ** 2 + 2 * 0.4238 ** 2)
+    alpha, kmax = native.ewald_parameters(12288, [L] * 3, q2, 9.0, 1e-5)
+    w = native.Water([L] * 3, pos, vel, params=native.water_params(cutoff=9.0, kmax=kmax,
+                                                                    alpha=alpha))
+    t64, pe64 = w.forces(native.FP64)
+    _, _, f64 = w.download()
+    t32, pe32 = w.forces(native.MIXED)
+    _, _, f32 = w.download()
+    w.close()
+    d = f32 - f64
+    return {"sites": 12288, "alpha": alpha, "kmax": kmax,
+            "rms_rel": float(np.sqrt(np.sum(d * d) / np.sum(f64 * f64))),
+            "max_rel": float(np.max(np.linalg.norm(d, axis=1))
+                             / np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))),
+            "energy_rel": abs(pe32 - pe64) / abs(pe64),
+            "real_space_energy_rel": abs(t32["real"] - t64["real"]) / abs(t64["real"]),
+            "pressure": "not computed: the water path has no virial (as the reference's)"}
+
+
+def cpu_baseline(timed_steps: int = 20, warm_steps: int = 2):
     from oracle import pyoracle
 
-    path = pyoracle.REF_SO if os.path.exists(pyoracle.REF_SO) else pyoracle.ORACLE_SO
-    kind = "reference" if path == pyoracle.REF_SO else "port"
-    o = pyoracle.Oracle(path)
+    o = pyoracle.Oracle(pyoracle.ORACLE_SO)
     fr = o.fcc_lattice(CELLS, RHO, TEMP, SEED)
     n = fr.n
     t0 = time.perf_counter()
@@ -131,11 +202,13 @@ def cpu_baseline(timed_steps: int = 10, warm_steps: int = 2):
     t0 = time.perf_counter()
     sim.run(timed_steps)
     dt = time.perf_counter() - t0
-    return dict(value=timed_steps / dt, unit="timesteps/s", cores=1, kind=kind,
+    host = host_description()
+    return dict(value=timed_steps / dt, unit="timesteps/s", cores=1, kind="port",
                 sample=(f"N={n} NHC steps through the Verlet list, {warm_steps} warm-up + "
                         f"{timed_steps} timed steps after the initial build "
-                        f"(setup {t_setup:.1f} s), single thread"),
-                s_per_step=dt / timed_steps)
+                        f"(setup {t_setup:.1f} s), one thread of {host['cpu_model']} "
+                        f"({host['nproc']} usable cores); oracle restatement built -O3 -DNDEBUG"),
+                s_per_step=dt / timed_steps, **host)
 
 
 def _line(metric, value, unit, steps, warmup, ms, dtype, workload, extra):
@@ -174,16 +247,15 @@ def bench_lj864(args) -> int:
     if not args.no_cpu:
         from oracle import pyoracle
 
-        path = pyoracle.REF_SO if os.path.exists(pyoracle.REF_SO) else pyoracle.ORACLE_SO
-        o = pyoracle.Oracle(path)
+        o = pyoracle.Oracle(pyoracle.ORACLE_SO)
         fr = o.fcc_lattice(6, RHO, TEMP, 12345)
         sim = o.sim(fr, kind="vv", dt=DT, rc=RC, skin=SKIN)
         c0 = time.perf_counter()
         sim.run(1000)
         cw = time.perf_counter() - c0
         extra["cpu_baseline"] = {"value": 1000 / cw, "unit": "timesteps/s", "cores": 1,
-                                 "kind": "reference" if path == pyoracle.REF_SO else "port",
-                                 "sample": "all 1000 NVE steps, single thread"}
+                                 "kind": "port", "sample": "all 1000 NVE steps, single thread",
+                                 **host_description()}
     return _line("timesteps/s, 864-atom LJ NVE, FP64, 1 MI355X", 1000 / wall, "timesteps/s", 1000,
                  args.warmup, 1e3 * wall / 1000, "f64",
                  "LJ fluid N=864 (fcc 6^3x4), rho*=0.8442, T*=0.722, rc=2.5, skin=0.3, VV dt*=0.005",
@@ -215,8 +287,9 @@ def bench_batch(args) -> int:
     batch_rate = 256 * steps / wall  # replica-steps per second
     # the plain path, one replica at a time, 100 steps each
     plain_steps = 100
-    t0 = time.perf_counter()
+    plain_wall = plain_setup = 0.0
     for k, (L, x, v) in enumerate(systems):
+        t0 = time.perf_counter()
         ctx = native.Context()
         ctx.upload(L, x, v)
         ctx.set_lj(1.0, 1.0, RC, False)
@@ -225,18 +298,25 @@ def bench_batch(args) -> int:
         ctx.forces(native.VERLET, native.MIXED)
         ctx.set_integrator(kind=native.BAOAB, dt=DT, friction=1.0, temperature=float(T[k]),
                            seed=int(seeds[k]))
+        ctx.run(10, native.MIXED, thermo=False)  # graph capture and first launches, untimed
+        ctx.synchronize()
+        t1 = time.perf_counter()
         ctx.run(plain_steps, native.MIXED, thermo=False)
         ctx.synchronize()
+        t2 = time.perf_counter()
         ctx.close()
-    plain_wall = time.perf_counter() - t0
+        plain_setup += t1 - t0
+        plain_wall += t2 - t1
     plain_rate = 256 * plain_steps / plain_wall
     extra = {"replicas": 256, "atoms_total": atoms, "block": int(block),
              "tpa_classes": sorted(set(int(t) for t in tpa)),
              "workgroups_per_step": int(sum(int(p) for p in pieces)),
              "plain_path_replica_steps_per_s": plain_rate,
              "batch_vs_plain": batch_rate / plain_rate,
-             "plain_path_sample": f"256 replicas x {plain_steps} steps, one md_ctx each, "
-                                  "setup included"}
+             "plain_path_setup_s": plain_setup,
+             "plain_path_sample": f"256 replicas x {plain_steps} timed steps, one md_ctx each, "
+                                  "run one after another; context setup, upload, list build and "
+                                  "10 warm-up steps per replica timed separately"}
     return _line("replica-steps/s, 256 uneven LJ replicas, BAOAB, mixed, 1 MI355X", batch_rate,
                  "replica-steps/s", steps, 100, 1e3 * wall / steps,
                  "f32 pair / i64 fixed-point accumulate / f64 state",
@@ -337,7 +417,8 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true",
                     help="batch workload: plain launches instead of HIP-graph segments "
                          "(for kernel traces)")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--nve-steps", type=int, default=2000)
     ap.add_argument("--workload", default="lj1m",
                     choices=["lj1m", "lj864", "water", "batch", "mixture"],
                     help="lj1m (default): the 1M-atom LJ line; the others are the remaining "
@@ -370,6 +451,10 @@ def main() -> int:
 
     from moleculardynamics_amd import native
 
+    log("measuring device peaks ...")
+    peak_hbm, peak_fp64, peak_fp32 = native.device_peaks()
+    log(f"  HBM {peak_hbm / 1e12:.3f} TB/s, FP64 {peak_fp64 / 1e12:.2f} TF, "
+        f"FP32 {peak_fp32 / 1e12:.2f} TF")
     L, pos, vel = native.fcc_lattice(CELLS, RHO, TEMP, SEED)
     n = pos.shape[0]
     log(f"workload: N={n} L={L[0]:.6f}")
@@ -389,6 +474,13 @@ def main() -> int:
     h32 = th32["pe"] + th32["ke"] + th32["conserved"]
     drift64 = float(np.max(np.abs(h64 - h64[0])) / abs(h64[0]))
     drift32 = float(np.max(np.abs(h32 - h32[0])) / abs(h32[0]))
+
+    log(f"NVE drift, {args.nve_steps} VV steps per leg ...")
+    drift = nve_drift(L, pos, vel, args.nve_steps)
+    log(f"  {drift}")
+    log("mixed-Ewald error (SPC/E, 12,288 sites) ...")
+    werr = water_mixed_error()
+    log(f"  {werr}")
 
     m = res["mixed"]
     d = res["fp64"]
@@ -422,17 +514,26 @@ def main() -> int:
         "fp64": {"timesteps_per_s": tps64, "ms_per_step": d["ms_per_step"],
                  "pair_interactions_per_s": d["pairs_per_step"] * tps64,
                  "force_kernel_ms": d["force_ms"],
-                 "roofline_frac_valu_fp64": F_PAIR * d["pairs_per_step"] / (d["force_ms"] * 1e-3) / PEAK_FP64,
+                 "roofline_frac_valu_fp64": F_PAIR * d["pairs_per_step"] / (d["force_ms"] * 1e-3) / peak_fp64,
                  "nhc_drift": drift64},
         "mixed_vs_fp64_speedup": tps / tps64,
         "mixed_error": {**err, "nhc_drift": drift32, "nhc_drift_fp64": drift64},
+        "nve_drift": drift,
+        "water_mixed_error": werr,
         "rebuilds": {"fp64": d["rebuilds"], "mixed": m["rebuilds"]},
+        "peaks_measured": {"hbm_TB_s": peak_hbm / 1e12, "fp64_TFLOP_s": peak_fp64 / 1e12,
+                           "fp32_TFLOP_s": peak_fp32 / 1e12},
+        "peaks_published": {"hbm_TB_s": PUB_HBM / 1e12, "fp64_TFLOP_s": PUB_FP64 / 1e12,
+                            "fp32_TFLOP_s": PUB_FP32 / 1e12},
         "roofline": {"bound": "valu", "kernel": "k_lj (LJ pair kernel, mixed)",
-                     "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32,
+                     "achieved": achieved / 1e12, "peak": peak_fp32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak_fp32,
+                     "frac_of_published": achieved / PUB_FP32,
                      "traffic": profiled_traffic("traffic_bytes_per_launch_mixed"),
+                     "traffic_source": "profiles/k_lj_lj1m_nhc_traffic.md (rocprofv3 --pmc "
+                                       "FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
                      "launch_ms": m["force_ms"],
-                     "step_frac": (max(168.0 * n / PEAK_HBM, F_PAIR * pairs / PEAK_FP32)
+                     "step_frac": (max(168.0 * n / peak_hbm, F_PAIR * pairs / peak_fp32)
                                    / (m["ms_per_step"] * 1e-3))},
         "cpu_baseline": cpu,
     }

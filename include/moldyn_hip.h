@@ -271,6 +271,15 @@ md_status md_vacf_result(md_vacf* vacf, int normalised, double* out);
 /* Vacf::diffusionCoefficient (vacf.cpp:65-85): Green-Kubo, trapezoid rule. */
 md_status md_vacf_diffusion(md_vacf* vacf, double dt, double* D);
 
+/* ---- device peaks ------------------------------------------------------------ */
+
+/* No reference counterpart. Measures the current device's attainable HBM copy
+ * bandwidth (bytes read + written per second, 1 GiB buffers) and FP64 / FP32
+ * vector FMA rates (flops per second, FMA = 2, FP32 without packed forms),
+ * the peaks that roofline fractions are quoted against. Any pointer may be
+ * NULL to skip that measurement. Allocates 2 GiB for the duration of the call. */
+md_status md_device_peaks(double* hbm_bytes_per_s, double* fp64_flops, double* fp32_flops);
+
 /* ---- allocation-free force evaluation ------------------------------------- */
 
 #define MD_NOALLOC_MAX_ATOMS (1L << 20) /* atoms per md_lj_forces_noalloc call */

@@ -131,6 +131,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_msd_diffusion.argtypes = [vp, C.c_double, C.c_long, C.c_long, _dp]
     lib.md_vacf_result.argtypes = [vp, C.c_int, _dp]
     lib.md_vacf_diffusion.argtypes = [vp, C.c_double, _dp]
+    lib.md_device_peaks.argtypes = [_dp, _dp, _dp]
     lib.md_lj_forces_noalloc.argtypes = ([C.c_int, C.POINTER(LJParams), C.c_long, _dp] +
                                          [vp] * 6 + [_dp, _dp, vp])
     lib.md_batch_draw.argtypes = [C.c_int, C.c_uint64, _ip, _dp, _dp, _u64p]
@@ -183,6 +184,16 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 
 RDF_AUTO, RDF_ALL_PAIRS, RDF_CELLS = 0, 1, 2
+
+
+def device_peaks():
+    """Measured (HBM bytes/s, FP64 flops/s, FP32 flops/s) of the current device."""
+    lib = load()
+    h, d, f = C.c_double(), C.c_double(), C.c_double()
+    st = lib.md_device_peaks(C.byref(h), C.byref(d), C.byref(f))
+    if st != MD_OK:
+        raise MDError(st, "md_device_peaks failed")
+    return h.value, d.value, f.value
 
 
 class _Handle:
