@@ -61,6 +61,9 @@ struct md_ctx {
     mdk::Scalars* sc = nullptr;
     mdk::Chain* chain = nullptr;
     double* com4 = nullptr;
+    // the LJ finish kernel's partials (3 doubles + 3 int64 per finish
+    // workgroup) and its arrival ticket, one block
+    void* fin = nullptr;
 
     bool lj_set = false;
     md_lj_params lj{};
@@ -120,6 +123,8 @@ void release(T*& p) {
 }
 
 long blocks_of(long n) { return (n + mdk::kBlock - 1) / mdk::kBlock; }
+
+constexpr size_t kFinBytes = 6 * mdk::kFinishGroups * sizeof(double) + 256;
 
 void set_lj_const(md_ctx* c) {
     const md_lj_params& p = c->lj;
@@ -285,6 +290,9 @@ mdk::ForceArgs force_args(md_ctx* c, int precision) {
     a.mq = mdk::mix_constants(c->ljc, s, c->qpos != nullptr ? c->quant.band_c : 0.0);
     a.mx = mdk::mix_constants(c->ljc, 1.0, 0.0);
     a.part_d = c->part_d; a.part_l = c->part_l;
+    a.fin_d = static_cast<double*>(c->fin);
+    a.fin_l = reinterpret_cast<long long*>(a.fin_d + 3 * mdk::kFinishGroups);
+    a.fin_count = reinterpret_cast<unsigned int*>(a.fin_l + 3 * mdk::kFinishGroups);
     a.sc = c->sc;
     return a;
 }
@@ -421,7 +429,8 @@ md_status md_ctx_create(void* hip_stream, md_ctx** out) {
     }
     if (hipMalloc(reinterpret_cast<void**>(&c->sc), sizeof(mdk::Scalars)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&c->chain), sizeof(mdk::Chain)) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&c->com4), 8 * sizeof(double)) != hipSuccess) {
+        hipMalloc(reinterpret_cast<void**>(&c->com4), 8 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->fin, kFinBytes) != hipSuccess) {
         md_ctx_destroy(c);
         return MD_ERR_HIP;
     }
@@ -429,7 +438,8 @@ md_status md_ctx_create(void* hip_stream, md_ctx** out) {
     s.vscale = 1.0;
     double zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (hipMemcpy(c->sc, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->com4, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->com4, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->fin, 0, kFinBytes) != hipSuccess) {
         md_ctx_destroy(c);
         return MD_ERR_HIP;
     }
@@ -456,6 +466,8 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
     release(c->ci_atoms); release(c->ci_tmp);
     release(c->part_d); release(c->part_l); release(c->sc); release(c->chain); release(c->com4);
+    if (c->fin) (void)hipFree(c->fin);
+    c->fin = nullptr;
     release(c->thermo_dev);
     release(c->snap_pos); release(c->snap_vel); release(c->snap_frc);
     release(c->snap_orig); release(c->snap_slot); release(c->snap_sc); release(c->snap_chain);
@@ -845,6 +857,11 @@ md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
         for (int k = 0; k < ch.M; ++k) ch.Q[k] = it->temperature * it->tau * it->tau;
         ch.Q[0] = static_cast<double>(it->dof) * it->temperature * it->tau * it->tau;
         for (int k = 0; k < ch.M; ++k) ch.iQ[k] = 1.0 / ch.Q[k];
+        // w1 = 1 / (2 - 2^(1/3)) with the host's cbrt, as nose_hoover.cpp:15-17 forms it
+        const double w1 = 1.0 / (2.0 - std::cbrt(2.0));
+        ch.sy[0] = w1;
+        ch.sy[1] = 1.0 - 2.0 * w1;
+        ch.sy[2] = w1;
         MD_HIP(c, hipMemcpy(c->chain, &ch, sizeof(ch), hipMemcpyHostToDevice));
     } else if (it->kind != MD_VELOCITY_VERLET && it->kind != MD_LANGEVIN_BAOAB) {
         return fail(c, MD_ERR_INVALID_ARGUMENT, "unknown integrator kind");

@@ -68,6 +68,7 @@ struct Chain {
     double dof, T;
     double Q[kMaxChain], xi[kMaxChain], p[kMaxChain];
     double iQ[kMaxChain];  // 1 / Q
+    double sy[3];          // Suzuki-Yoshida weights w1, 1 - 2 w1, w1 (host std::cbrt)
 };
 
 // Calls f(std::integral_constant<int, j>) for j = N-1 .. 0 / 0 .. N-1, so

@@ -36,102 +36,113 @@ namespace mdk {
 
 namespace {
 
-// Working copy of the chain for thread 0 of a finish kernel: the first MM
-// links in registers (every index a compile-time constant after unrolling),
-// so the chain's long dependent sequence of FP64 operations does no memory
-// access. Divisions by Q use the reciprocals iQ computed once on the host.
-template <int MM>
-struct ChainRegs {
+// Working copy of the chain for the one thread that advances it, in LDS:
+// the links are updated by runtime loops over M, so the code is one link
+// body, not an unrolled copy per link (a long straight-line sequence runs
+// out of the instruction cache). Divisions by Q use the reciprocals iQ
+// computed once on the host.
+struct ChainWork {
     int M;
     double dof, T;
-    double p[MM], xi[MM], iQ[MM];
+    double sy[3];
+    double p[kMaxChain], xi[kMaxChain], iQ[kMaxChain];
 
     __device__ void load(const Chain& c) {
         M = c.M;
         dof = c.dof;
         T = c.T;
-#pragma unroll
-        for (int k = 0; k < MM; ++k) {
-            p[k] = k < M ? c.p[k] : 0.0;
-            xi[k] = k < M ? c.xi[k] : 0.0;
-            iQ[k] = k < M ? c.iQ[k] : 0.0;
+        sy[0] = c.sy[0];
+        sy[1] = c.sy[1];
+        sy[2] = c.sy[2];
+        for (int k = 0; k < M; ++k) {
+            p[k] = c.p[k];
+            xi[k] = c.xi[k];
+            iQ[k] = c.iQ[k];
         }
     }
     __device__ void store(Chain& c) const {
-#pragma unroll
-        for (int k = 0; k < MM; ++k) {
-            if (k < M) {
-                c.p[k] = p[k];
-                c.xi[k] = xi[k];
-            }
+        for (int k = 0; k < M; ++k) {
+            c.p[k] = p[k];
+            c.xi[k] = xi[k];
         }
     }
 };
+
+// e^x for the chain's damping and rescale factors. Their arguments are
+// -dt/4 .. -dt/2 times a thermostat rate, far below 1/8 in magnitude for any
+// stable step, where the Taylor series to x^13 (Horner form, 13 dependent
+// FMAs) is exact to the last bit (the first omitted term, x^14/14!, is below
+// 1e-23); larger arguments take the library exp. The chain runs on one
+// thread, so its latency is the step's: this saves most of an exp's.
+__device__ __forceinline__ double chain_exp(double x) {
+    if (!(fabs(x) <= 0.125)) return exp(x);
+    double p = 1.0 / 6227020800.0;  // 1/13!
+    p = fma(p, x, 1.0 / 479001600.0);
+    p = fma(p, x, 1.0 / 39916800.0);
+    p = fma(p, x, 1.0 / 3628800.0);
+    p = fma(p, x, 1.0 / 362880.0);
+    p = fma(p, x, 1.0 / 40320.0);
+    p = fma(p, x, 1.0 / 5040.0);
+    p = fma(p, x, 1.0 / 720.0);
+    p = fma(p, x, 1.0 / 120.0);
+    p = fma(p, x, 1.0 / 24.0);
+    p = fma(p, x, 1.0 / 6.0);
+    p = fma(p, x, 0.5);
+    p = fma(p, x, 1.0);
+    return fma(p, x, 1.0);
+}
+
+// Link j of one Suzuki-Yoshida sub-step of length d: the thermostat force g
+// on p_j (from the kinetic energy for j = 0, else from p_{j-1}) with the
+// damping by p_{j+1} on both sides (p_{j+1} is not touched in between, so
+// one exp serves both factors).
+__device__ __forceinline__ void chain_link(ChainWork& c, int j, double ke, double d2, double d4) {
+    const int M = c.M;
+    double damp = 1.0;
+    if (j + 1 < M) damp = chain_exp(-d4 * c.p[j + 1] * c.iQ[j + 1]);
+    const double g = j == 0 ? 2.0 * ke - c.dof * c.T : c.p[j - 1] * c.p[j - 1] * c.iQ[j - 1] - c.T;
+    c.p[j] = (c.p[j] * damp + d2 * g) * damp;
+}
 
 // One MTK96 chain propagation of length delta with Suzuki-Yoshida weights
 // (integrate/nose_hoover.cpp:63-124). The chain rescales every velocity by a
 // common factor, so the kinetic energy after a rescale is KE * f^2; it is
 // tracked that way instead of re-summing the velocities. Returns the product
 // of the three rescale factors and updates ke.
-template <int MM>
-__device__ double chain_half(ChainRegs<MM>& c, double& ke, double delta) {
-    const double w1 = 1.0 / (2.0 - cbrt(2.0));
-    const double ws[3] = {w1, 1.0 - 2.0 * w1, w1};
+__device__ __noinline__ double chain_half(ChainWork& c, double& ke, double delta) {
     const int M = c.M;
     double scale = 1.0;
+#pragma unroll 1
     for (int s = 0; s < 3; ++s) {
-        const double d = ws[s] * delta;
+        const double d = c.sy[s] * delta;
         const double d2 = 0.5 * d, d4 = 0.25 * d;
-        auto link = [&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            if (j >= M) return;
-            double damp = 1.0;
-            if constexpr (j + 1 < MM) {
-                // p[j+1] is not touched in between: one exp serves both damping factors
-                if (j + 1 < M) damp = exp(-d4 * c.p[j + 1] * c.iQ[j + 1]);
-            }
-            double g;
-            if constexpr (j == 0) {
-                g = 2.0 * ke - c.dof * c.T;
-            } else {
-                g = c.p[j - 1] * c.p[j - 1] * c.iQ[j - 1] - c.T;
-            }
-            c.p[j] = (c.p[j] * damp + d2 * g) * damp;
-        };
-        unroll_down<MM>(link);
-#pragma unroll
-        for (int k = 0; k < MM; ++k)
-            if (k < M) c.xi[k] += d * c.p[k] * c.iQ[k];
-        const double f = exp(-d * c.p[0] * c.iQ[0]);
+#pragma unroll 1
+        for (int j = M - 1; j >= 0; --j) chain_link(c, j, ke, d2, d4);
+#pragma unroll 1
+        for (int k = 0; k < M; ++k) c.xi[k] += d * c.p[k] * c.iQ[k];
+        const double f = chain_exp(-d * c.p[0] * c.iQ[0]);
         scale *= f;
         ke *= f * f;
-        unroll_up<MM>(link);
+#pragma unroll 1
+        for (int j = 0; j < M; ++j) chain_link(c, j, ke, d2, d4);
     }
     return scale;
 }
 
-template <int MM>
-__device__ double chain_conserved(const ChainRegs<MM>& c) {
+__device__ double chain_conserved(const ChainWork& c) {
     double acc = c.dof * c.T * c.xi[0];
-#pragma unroll
-    for (int k = 1; k < MM; ++k)
-        if (k < c.M) acc += c.T * c.xi[k];
-#pragma unroll
-    for (int k = 0; k < MM; ++k)
-        if (k < c.M) acc += 0.5 * c.p[k] * c.p[k] * c.iQ[k];
+    for (int k = 1; k < c.M; ++k) acc += c.T * c.xi[k];
+    for (int k = 0; k < c.M; ++k) acc += 0.5 * c.p[k] * c.p[k] * c.iQ[k];
     return acc;
 }
 
-// The chain's part of a step on thread 0: one or two half-chains, the
-// conserved contribution, the rescale factor. Short chains (the usual M = 3)
-// run in registers; longer ones in a 16-link copy.
+// The chain's part of a step (one thread): one or two half-chains, the
+// conserved contribution, the rescale factor.
 struct ChainStep {
     double scale, conserved;
 };
 
-template <int MM>
-__device__ ChainStep chain_step_m(Chain* chain, double& ke, double delta, bool twice) {
-    ChainRegs<MM> c;
+__device__ ChainStep chain_step(Chain* chain, ChainWork& c, double& ke, double delta, bool twice) {
     c.load(*chain);
     ChainStep r;
     r.scale = chain_half(c, ke, delta);
@@ -144,27 +155,31 @@ __device__ ChainStep chain_step_m(Chain* chain, double& ke, double delta, bool t
     return r;
 }
 
-__device__ ChainStep chain_step(Chain* chain, double& ke, double delta, bool twice) {
-    if (chain->M <= 4) return chain_step_m<4>(chain, ke, delta, twice);
-    return chain_step_m<kMaxChain>(chain, ke, delta, twice);
-}
-
-// Final reduction (one workgroup of kFinish threads, launched after the pair
-// kernel): the nb per-workgroup partials summed in a fixed order -- a strided
-// share per thread, wave shuffles, then the waves in index order. Thread 0
-// then advances the Nose-Hoover chain (chain_step). The partials come from
-// every die's L2, so the loads of a thread's share are issued together,
-// kUnroll at a time, each into its own accumulator (combined in a fixed
-// order), rather than one dependent round trip per partial.
-constexpr int kFinish = 1024;
+// Final reduction, launched after the pair kernel: kFinishGroups workgroups
+// each sum a fixed contiguous share of the pair kernel's nb per-workgroup
+// partials (a strided share per thread, kUnroll loads in flight, wave
+// shuffles, the waves in index order) into one partial of their own; the
+// workgroup that finishes last (an atomic ticket, after a release fence)
+// sums those kFinishGroups partials with one wave's shuffle tree and
+// finishes the step on one thread: E, W, the pair count, the Nose-Hoover
+// chain and the thermo row. Every sum has a fixed order, so the result does
+// not depend on which workgroup arrives last; the partials are read from
+// every die's L2 by many CUs at once instead of one.
+constexpr int kFinish = 256;
 constexpr int kUnroll = 8;
+static_assert(kFinishGroups == kWave, "the last workgroup reduces one partial per lane");
 
 __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
     __shared__ double red_d[3][kFinish / kWave];
     __shared__ long long red_l[3][kFinish / kWave];
+    __shared__ int last;
+    __shared__ ChainWork cw;
+    const int g = blockIdx.x;
+    const int chunk = (nb + kFinishGroups - 1) / kFinishGroups;
+    const int b1 = min(nb, (g + 1) * chunk);
     double sd[3] = {0.0, 0.0, 0.0};
     long long sl[3] = {0, 0, 0};
-    for (int b0 = threadIdx.x; b0 < nb; b0 += kFinish * kUnroll) {
+    for (int b0 = g * chunk + threadIdx.x; b0 < b1; b0 += kFinish * kUnroll) {
         double vd[kUnroll][3];
         long long vl[kUnroll][3];
 #pragma unroll
@@ -172,8 +187,8 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
             const int b = b0 + u * kFinish;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                vd[u][q] = b < nb ? a.part_d[3 * b + q] : 0.0;
-                vl[u][q] = b < nb ? a.part_l[3 * b + q] : 0;
+                vd[u][q] = b < b1 ? a.part_d[3 * b + q] : 0.0;
+                vl[u][q] = b < b1 ? a.part_l[3 * b + q] : 0;
             }
         }
 #pragma unroll
@@ -195,16 +210,34 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
         }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double tot_d[3] = {0.0, 0.0, 0.0};
-    long long tot_l[3] = {0, 0, 0};
-    for (int w = 0; w < kFinish / kWave; ++w) {
-#pragma unroll
+    if (threadIdx.x == 0) {
         for (int q = 0; q < 3; ++q) {
-            tot_d[q] += red_d[q][w];
-            tot_l[q] += red_l[q][w];
+            double td = 0.0;
+            long long tl = 0;
+            for (int w = 0; w < kFinish / kWave; ++w) {
+                td += red_d[q][w];
+                tl += red_l[q][w];
+            }
+            a.fin_d[3 * g + q] = td;
+            a.fin_l[3 * g + q] = tl;
         }
+        __threadfence();
+        last = atomicAdd(a.fin_count, 1u) == kFinishGroups - 1 ? 1 : 0;
     }
+    __syncthreads();
+    if (!last || threadIdx.x >= kWave) return;
+    __threadfence();
+    const volatile double* fd = a.fin_d;
+    const volatile long long* fl = a.fin_l;
+    double tot_d[3];
+    long long tot_l[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        tot_d[q] = wave_sum(fd[3 * lane + q]);
+        tot_l[q] = wave_sum_ll(fl[3 * lane + q]);
+    }
+    if (lane != 0) return;
+    *a.fin_count = 0u;  // the next launch's tickets
     double E, W;
     if (mixed) {
         E = 0.5 * (static_cast<double>(tot_l[0]) * kFixInv);
@@ -222,7 +255,7 @@ __device__ void finish_step(const ForceArgs& a, bool mixed, int nb) {
     double conserved = 0.0;
     if (a.kick) {
         if (a.chain != nullptr) {
-            const ChainStep r = chain_step(a.chain, ke, a.chain_delta, a.chain_twice != 0);
+            const ChainStep r = chain_step(a.chain, cw, ke, a.chain_delta, a.chain_twice != 0);
             conserved = r.conserved;
             sc->vscale = r.scale;
         }
@@ -491,11 +524,46 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
         [&](uint32_t e) { mix_fixed<LFS, CUBIC>(a, qi, at(e), q, hit); });
 }
 
+// The workgroup's partials of a pair kernel in one LDS exchange: each wave's
+// shuffle sums, then the waves in index order on thread 0 (the order of
+// block_reduce, so the same bits), for the values the precision uses (FP64:
+// E, W, KE and the term count; mixed: the fixed-point E, W, the term count
+// and KE); the unused slots are written as 0.
+template <bool MIXED>
+__device__ __forceinline__ void write_partials(const ForceArgs& a, long b, double e, double w,
+                                               double ke, long long ef, long long wf,
+                                               long long nt) {
+    __shared__ double xd[3][kBlock / kWave];
+    __shared__ long long xl[3][kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const double v0 = MIXED ? 0.0 : wave_sum(e);
+    const double v1 = MIXED ? 0.0 : wave_sum(w);
+    const double v2 = wave_sum(ke);
+    const long long l0 = MIXED ? wave_sum_ll(ef) : 0;
+    const long long l1 = MIXED ? wave_sum_ll(wf) : 0;
+    const long long l2 = wave_sum_ll(nt);
+    if (lane == 0) {
+        xd[0][wid] = v0; xd[1][wid] = v1; xd[2][wid] = v2;
+        xl[0][wid] = l0; xl[1][wid] = l1; xl[2][wid] = l2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 3; ++q) {
+            double td = 0.0;
+            long long tl = 0;
+            for (int k = 0; k < kBlock / kWave; ++k) {
+                td += xd[q][k];
+                tl += xl[q][k];
+            }
+            a.part_d[3 * b + q] = td;
+            a.part_l[3 * b + q] = tl;
+        }
+    }
+}
+
 // QP: the mixed kernel reads fixed-point positions (a.qpos) for the partners.
 template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
-    __shared__ double sm_d[kBlock / kWave];
-    __shared__ long long sm_l[kBlock / kWave];
     const long blk = xcd_block();
     if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
     if (a.sc->overflow) return;       // a truncated list was built: md_run restores and retries
@@ -552,21 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
         }
     }
     // workgroup partials (the in-cutoff term count is twice the pair count)
-    const double be = block_reduce(e, sm_d, wave_sum);
-    const double bw = block_reduce(w, sm_d, wave_sum);
-    const double bk = block_reduce(ke, sm_d, wave_sum);
-    const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
-    const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
-    const long long bnt = block_reduce(nt, sm_l, wave_sum_ll);
-    if (threadIdx.x == 0) {
-        const long b = blk;
-        a.part_d[3 * b + 0] = be;
-        a.part_d[3 * b + 1] = bw;
-        a.part_d[3 * b + 2] = bk;
-        a.part_l[3 * b + 0] = bef;
-        a.part_l[3 * b + 1] = bwf;
-        a.part_l[3 * b + 2] = bnt;
-    }
+    write_partials<MIXED>(a, blk, e, w, ke, ef, wf, nt);
 }
 
 // ---- small systems: several lanes per atom ----
@@ -608,8 +662,6 @@ __device__ __forceinline__ T group_sum(T v) {
 
 template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK, int LANES>
 __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
-    __shared__ double sm_d[kBlock / kWave];
-    __shared__ long long sm_l[kBlock / kWave];
     if (a.sc->overflow) return;  // a truncated list was built: md_run restores and retries
     const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
     const long t = gt / LANES;
@@ -682,21 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         e = w = 0.0;
         ef = wf = nt = 0;
     }
-    const double be = block_reduce(e, sm_d, wave_sum);
-    const double bw = block_reduce(w, sm_d, wave_sum);
-    const double bk = block_reduce(ke, sm_d, wave_sum);
-    const long long bef = block_reduce(ef, sm_l, wave_sum_ll);
-    const long long bwf = block_reduce(wf, sm_l, wave_sum_ll);
-    const long long bnt = block_reduce(nt, sm_l, wave_sum_ll);
-    if (threadIdx.x == 0) {
-        const long b = blockIdx.x;
-        a.part_d[3 * b + 0] = be;
-        a.part_d[3 * b + 1] = bw;
-        a.part_d[3 * b + 2] = bk;
-        a.part_l[3 * b + 0] = bef;
-        a.part_l[3 * b + 1] = bwf;
-        a.part_l[3 * b + 2] = bnt;
-    }
+    write_partials<MIXED>(a, blockIdx.x, e, w, ke, ef, wf, nt);
 }
 
 __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
@@ -931,9 +969,10 @@ __global__ __launch_bounds__(kBlock) void k_rank_index(const int* __restrict__ s
 }
 
 __global__ void k_chain_first(Chain* chain, Scalars* sc, double delta) {
+    __shared__ ChainWork cw;
     if (threadIdx.x != 0) return;
     double ke = sc->kinetic;
-    sc->vscale = chain_step(chain, ke, delta, false).scale;
+    sc->vscale = chain_step(chain, cw, ke, delta, false).scale;
 }
 
 inline unsigned grid_of(long n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
@@ -966,7 +1005,8 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
     // the pair kernel writes partials for its lj_groups(n) blocks (padding blocks of
     // the XCD-aware grid none)
-    hipLaunchKernelGGL(k_lj_finish, dim3(1), dim3(kFinish), 0, s, a, static_cast<int>(lj_groups(a.n, a.mixed != 0)));
+    hipLaunchKernelGGL(k_lj_finish, dim3(kFinishGroups), dim3(kFinish), 0, s, a,
+                       static_cast<int>(lj_groups(a.n, a.mixed != 0)));
 }
 
 MixK mix_constants(const LJConst& k, double s, double band) {

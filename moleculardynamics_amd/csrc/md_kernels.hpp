@@ -132,8 +132,13 @@ struct ForceArgs {
     // reductions
     double* part_d;     // per-workgroup FP64 partials, 3 per group (E, W, KE)
     long long* part_l;  // per-workgroup int64 partials, 3 per group (E, W, pairs)
+    // the finish kernel's own partials (3 per finish workgroup each) and its
+    // arrival ticket (0 between launches)
+    double* fin_d;
+    long long* fin_l;
+    unsigned int* fin_count;
     Scalars* sc;
-    Chain* chain;       // NHC: chain update in the last workgroup (null otherwise)
+    Chain* chain;       // NHC: chain update on one thread of the finish (null otherwise)
     double chain_delta; // dt/2 of the chain half-steps
     int chain_twice;    // 1: also propagate the next step's first half-chain
     void* thermo;       // md_thermo* record for this step, or null
@@ -148,7 +153,9 @@ struct ForceArgs {
 // systems use several lanes per atom, hence more workgroups).
 long lj_partials(long n);
 void launch_lj_force(const ForceArgs& a, hipStream_t s);
-// One-workgroup reduction of the pair kernel's partials (E, W, KE, pairs),
+// Workgroups of the finish kernel (each owns 3 + 3 fin partials).
+constexpr int kFinishGroups = 64;
+// Fixed-order reduction of the pair kernel's partials (E, W, KE, pairs),
 // thermo row and NHC chain; launched right after launch_lj_force.
 void launch_lj_finish(const ForceArgs& a, hipStream_t s);
 // LennardJones(sigma, epsilon, cutoff, truncation) constants: u(rc) and
