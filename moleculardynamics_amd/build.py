@@ -62,6 +62,32 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+CLI = os.path.join(HERE, "bin", "moldyn_run_gpu")
+
+
+def build_cli(verbose: bool = True) -> str:
+    """Compiles apps/moldyn_run_gpu.cpp (the reference driver's command line
+    on the device path) against the library, into moleculardynamics_amd/bin."""
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    sha = "unknown"
+    try:
+        r = subprocess.run(["git", "-C", os.path.dirname(HERE), "rev-parse", "--short", "HEAD"],
+                           capture_output=True, text=True, timeout=10)
+        if r.returncode == 0 and r.stdout.strip():
+            sha = r.stdout.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-DMOLDYN_GPU_GIT_SHA=\"{sha}\"", "-o", CLI,
+           os.path.join(HERE, "apps", "moldyn_run_gpu.cpp"), "-L" + LIBDIR, "-lmoldyn_hip",
+           "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"moldyn_run_gpu build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {CLI}")
+    return CLI
+
+
 def build_mirror_test(verbose: bool = True) -> str:
     """Compiles tests/cpp/test_mirror.cpp (the reference's test cases against
     the C++ drop-in classes of host/moldyn_gpu.hpp), linked to the library."""
@@ -83,4 +109,5 @@ def build_mirror_test(verbose: bool = True) -> str:
 if __name__ == "__main__":
     build(verbose=True)
     build_mirror_test(verbose=True)
+    build_cli(verbose=True)
     sys.exit(0)
