@@ -88,6 +88,54 @@ def build_cli(verbose: bool = True) -> str:
     return CLI
 
 
+SAN_DIR = os.path.join(OBJDIR, "sanitize")
+SAN_EXE = os.path.join(SAN_DIR, "test_sanitize")
+# host-only code with AddressSanitizer and UndefinedBehaviorSanitizer; each
+# sanitizer flag applies to the host compilation only
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=all"]
+
+
+def build_sanitized(verbose: bool = True) -> str:
+    """Builds tests/cpp/test_sanitize: the library with ASan + UBSan on its
+    host code (device code as in the product build) and the CPU oracle with
+    both sanitizers, linked into one driver that runs without a GPU."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    root = os.path.dirname(HERE)
+    flags = FLAGS + ["-Xarch_host", "-g"] + SAN_HOST
+
+    def one(src):
+        obj = os.path.join(SAN_DIR, os.path.splitext(src)[0] + ".o")
+        r = subprocess.run([HIPCC, *flags, "-c", os.path.join(CSRC, src), "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"sanitized hipcc failed for {src}:\n{r.stderr}")
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(one, SOURCES))
+    lib = os.path.join(SAN_DIR, "libmoldyn_hip_san.so")
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *SAN_HOST, "-o", lib,
+                        *objs], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitized link failed:\n{r.stderr}")
+    clang = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "bin", "clang++")
+    if not os.path.exists(clang):
+        clang = "/opt/rocm/lib/llvm/bin/clang++"
+    osrc = [os.path.join(root, "oracle", f) for f in ("core.cpp", "lj.cpp", "integrate.cpp",
+                                                       "ewald.cpp", "analysis.cpp")]
+    cmd = [clang, "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address,undefined",
+           "-fno-omit-frame-pointer", "-fno-sanitize-recover=all", "-o", SAN_EXE,
+           os.path.join(root, "tests", "cpp", "test_sanitize.cpp"), *osrc, lib,
+           "-Wl,-rpath," + SAN_DIR, "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"test_sanitize build failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {SAN_EXE}")
+    return SAN_EXE
+
+
 def build_mirror_test(verbose: bool = True) -> str:
     """Compiles tests/cpp/test_mirror.cpp (the reference's test cases against
     the C++ drop-in classes of host/moldyn_gpu.hpp), linked to the library."""

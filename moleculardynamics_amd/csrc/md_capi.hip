@@ -198,8 +198,10 @@ int capacity_for(long n, const double* L, double rc, double skin) {
     const double rl = rc + skin;
     const double rho = static_cast<double>(n) / (L[0] * L[1] * L[2]);
     const double mean = 4.0 / 3.0 * 3.14159265358979323846 * rl * rl * rl * rho;
-    long cap = static_cast<long>(std::ceil(2.0 * mean + 32.0));
-    cap = std::min(cap, std::max(1L, n - 1));
+    const double want = std::ceil(2.0 * mean + 32.0);
+    const long most = std::max(1L, n - 1);
+    // compared in FP64 first: a huge density would overflow the conversion
+    const long cap = want < static_cast<double>(most) ? static_cast<long>(want) : most;
     return static_cast<int>((cap + 7) / 8 * 8);
 }
 
@@ -1030,6 +1032,8 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
 
 int md_neigh_auto_capacity(long n, const double* L, double cutoff, double skin) {
     if (n <= 0 || L == nullptr || !(cutoff > 0.0) || !(skin >= 0.0)) return 0;
+    for (int d = 0; d < 3; ++d)
+        if (!(L[d] > 0.0) || !std::isfinite(L[d])) return 0;
     return capacity_for(n, L, cutoff, skin);
 }
 
