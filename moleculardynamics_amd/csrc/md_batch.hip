@@ -36,35 +36,6 @@ namespace mdk {
 
 namespace {
 
-struct U4 {
-    uint32_t x, y, z, w;
-};
-
-__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-        k0 += W0;
-        k1 += W1;
-    }
-    return c;
-}
-
-__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
-    const uint64_t v = (static_cast<uint64_t>(a) << 21) ^ static_cast<uint64_t>(b >> 11);
-    return (static_cast<double>(v & ((1ULL << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-__device__ __forceinline__ double4 drift(const Box& b, double4 p, const double4& v, double h) {
-    p.x = fold(fma(h, v.x, p.x), b.L[0], b.lo[0], b.hi1[0], b.hi2[0]);
-    p.y = fold(fma(h, v.y, p.y), b.L[1], b.lo[1], b.hi1[1], b.hi2[1]);
-    p.z = fold(fma(h, v.z, p.z), b.L[2], b.lo[2], b.hi1[2], b.hi2[2]);
-    return p;
-}
 
 // Cell of a coordinate: floor((wrap(r) + L/2) / width) clamped to [0, n-1]
 // (neighbor/cell_list.cpp:31-38), wrap the reference fold.
@@ -266,19 +237,12 @@ __global__ __launch_bounds__(256) void k_batch_pre(BatchArgs a) {
         v.y = fma(c, f.y, v.y);
         v.z = fma(c, f.z, v.z);
         p = drift(box, p, v, h);
-        const U4 q0 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
-                                  static_cast<uint32_t>(step >> 32), 0u}, k0, k1);
-        const U4 q1 = philox10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(step),
-                                  static_cast<uint32_t>(step >> 32), 1u}, k0, k1);
-        const double r1 = sqrt(-2.0 * log(u53(q0.x, q0.y)));
-        const double r2 = sqrt(-2.0 * log(u53(q1.x, q1.y)));
-        double s1, cs1, s2, cs2;
-        sincospi(2.0 * u53(q0.z, q0.w), &s1, &cs1);
-        sincospi(2.0 * u53(q1.z, q1.w), &s2, &cs2);
+        double g0, g1, g2;
+        normals3(k0, k1, step, static_cast<uint32_t>(i), g0, g1, g2);
         const double k = c2 * sqrt(T / v.w);
-        v.x = c1 * v.x + (r1 * cs1) * k;
-        v.y = c1 * v.y + (r1 * s1) * k;
-        v.z = c1 * v.z + (r2 * cs2) * k;
+        v.x = c1 * v.x + g0 * k;
+        v.y = c1 * v.y + g1 * k;
+        v.z = c1 * v.z + g2 * k;
         p = drift(box, p, v, h);
         a.pos[g] = p;
         a.vel[g] = v;

@@ -26,52 +26,6 @@ namespace mdk {
 
 namespace {
 
-struct U4 {
-    uint32_t x, y, z, w;
-};
-
-__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-        k0 += W0;
-        k1 += W1;
-    }
-    return c;
-}
-
-__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
-    const uint64_t v = (static_cast<uint64_t>(a) << 21) ^ static_cast<uint64_t>(b >> 11);
-    return (static_cast<double>(v & ((1ULL << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-__device__ __forceinline__ void normals3(uint64_t seed, uint64_t step, uint32_t id, double& g0,
-                                         double& g1, double& g2) {
-    const uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
-    const U4 a = philox10(U4{id, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32), 0u}, k0, k1);
-    const U4 b = philox10(U4{id, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32), 1u}, k0, k1);
-    const double u1 = u53(a.x, a.y), u2 = u53(a.z, a.w);
-    const double u3 = u53(b.x, b.y), u4 = u53(b.z, b.w);
-    const double r1 = sqrt(-2.0 * log(u1));
-    const double r2 = sqrt(-2.0 * log(u3));
-    double s1, c1, s2, c2;
-    sincospi(2.0 * u2, &s1, &c1);
-    sincospi(2.0 * u4, &s2, &c2);
-    g0 = r1 * c1;
-    g1 = r1 * s1;
-    g2 = r2 * c2;
-}
-
-__device__ __forceinline__ double4 drift(const Box& b, double4 p, const double4& v, double h) {
-    p.x = fold(fma(h, v.x, p.x), b.L[0], b.lo[0], b.hi1[0], b.hi2[0]);
-    p.y = fold(fma(h, v.y, p.y), b.L[1], b.lo[1], b.hi1[1], b.hi2[1]);
-    p.z = fold(fma(h, v.z, p.z), b.L[2], b.lo[2], b.hi1[2], b.hi2[2]);
-    return p;
-}
 
 __device__ __forceinline__ double displacement(const Box& b, const double4& p, const double4& r) {
 #pragma clang fp contract(off)
