@@ -344,8 +344,21 @@ def bench_water(args) -> int:
     pe, ke = w.run(500, dt)
     wall = time.perf_counter() - t0
     w.close()
-    extra = {"alpha": alpha, "kmax": kmax, "sites": 12288,
-             "energy_drift": float(np.max(np.abs((pe + ke) - (pe[0] + ke[0]))) / abs(pe[0] + ke[0]))}
+    drift = lambda p, k: float(np.max(np.abs((p + k) - (p[0] + k[0]))) / abs(p[0] + k[0]))  # noqa: E731
+    extra = {"alpha": alpha, "kmax": kmax, "sites": 12288, "constraint": "SHAKE/RATTLE (RigidWater)",
+             "energy_drift": drift(pe, ke)}
+    # the same run with SETTLE, the closed-form constraint (same fixed points)
+    ws = native.Water([L] * 3, pos, vel, params=prm)
+    ws.set_constraint(native.RIGID_SETTLE)
+    ws.forces()
+    if args.warmup:
+        ws.run(args.warmup, dt)
+    t0 = time.perf_counter()
+    pes, kes = ws.run(500, dt)
+    wall_s = time.perf_counter() - t0
+    ws.close()
+    extra["settle"] = {"timesteps_per_s": 500 / wall_s, "ms_per_step": 1e3 * wall_s / 500,
+                       "energy_drift": drift(pes, kes), "speedup_vs_shake": wall / wall_s}
     if not args.no_cpu:
         from oracle import pyoracle
         from oracle.pyoracle import Frame

@@ -426,6 +426,22 @@ md_status md_water_shake(md_water* w, const double* rx, const double* ry, const 
                          int* iterations);
 /* RigidWater::constrainVelocities. */
 md_status md_water_rattle(md_water* w, int* iterations);
+/* Closed-form rigid constraints (SETTLE, Miyamoto & Kollman 1992), the
+ * replacement of RigidWater's SHAKE/RATTLE that the reference plans
+ * (constrain/rigid_water.hpp:20-24; absent from its sources): the same fixed
+ * points solved exactly, one pass per molecule. md_water_settle is
+ * RigidWater::constrainPositions against reference positions,
+ * md_water_settle_velocities RigidWater::constrainVelocities;
+ * MD_ERR_RUNTIME when a molecule's step has no solution (a reference
+ * molecule that is degenerate, or turned by 90 degrees or more). The two
+ * hydrogen masses of a molecule must be equal. */
+md_status md_water_settle(md_water* w, const double* rx, const double* ry, const double* rz);
+md_status md_water_settle_velocities(md_water* w);
+/* Constraint algorithm of md_water_run: RigidWater's SHAKE/RATTLE (default)
+ * or SETTLE, whose step takes two per-molecule kernels in place of five. */
+#define MD_RIGID_SHAKE 0
+#define MD_RIGID_SETTLE 1
+md_status md_water_set_constraint(md_water* w, int method);
 /* nsteps of ConstrainedVelocityVerlet::step with the SPC/E force field;
  * pe/ke (or NULL) receive per-step values in K. */
 md_status md_water_run(md_water* w, long nsteps, double dt, int precision, double* pe, double* ke);

@@ -123,7 +123,7 @@ def build_sanitized(verbose: bool = True) -> str:
     if not os.path.exists(clang):
         clang = "/opt/rocm/lib/llvm/bin/clang++"
     osrc = [os.path.join(root, "oracle", f) for f in ("core.cpp", "lj.cpp", "integrate.cpp",
-                                                       "ewald.cpp", "analysis.cpp")]
+                                                       "ewald.cpp", "settle.cpp", "analysis.cpp")]
     cmd = [clang, "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address,undefined",
            "-fno-omit-frame-pointer", "-fno-sanitize-recover=all", "-o", SAN_EXE,
            os.path.join(root, "tests", "cpp", "test_sanitize.cpp"), *osrc, lib,

@@ -507,6 +507,225 @@ __global__ void k_rattle(WaterArgs a, int* __restrict__ status) {
     }
 }
 
+// ---- SETTLE: closed-form rigid constraints, one thread per molecule ---------
+// The fixed points of SHAKE and RATTLE above, solved exactly (Miyamoto &
+// Kollman, J. Comput. Chem. 13, 952 (1992); the replacement the reference
+// plans, constrain/rigid_water.hpp:20-24). Positions: in the frame of the
+// reference molecule (ez its normal, ex along H1->H2), the constrained
+// molecule is the canonical triangle O (0, ra), H1 (-rc, -rb), H2 (rc, -rb)
+// at the unconstrained centre of mass, rotated by R = Rz(theta) M: M is the
+// Rodrigues rotation taking ez to R's third row (s1, s2, s3), which keeps
+// every site's out-of-plane coordinate (constraint forces lie along the
+// reference bonds, so in the reference plane), and theta makes the in-plane
+// angular momentum of the displacement zero (bond forces exert no torque):
+// alpha sin + beta cos = T. Velocities: the 3 x 3 linear system that RATTLE
+// iterates on, A mu = (r_l . v_l), solved by Cramer's rule. Arithmetic in
+// the operation order of the CPU restatement, without contraction.
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3(const double4& p) { return D3{p.x, p.y, p.z}; }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) {
+#pragma clang fp contract(off)
+    return D3{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+__device__ __forceinline__ D3 dadd(D3 a, D3 b) {
+#pragma clang fp contract(off)
+    return D3{a.x + b.x, a.y + b.y, a.z + b.z};
+}
+__device__ __forceinline__ D3 dscale(D3 a, double s) {
+#pragma clang fp contract(off)
+    return D3{a.x * s, a.y * s, a.z * s};
+}
+__device__ __forceinline__ D3 dcross(D3 a, D3 b) {
+#pragma clang fp contract(off)
+    return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return norm2_dot(a.x, a.y, a.z, b.x, b.y, b.z); }
+__device__ __forceinline__ D3 dimage(const Box& b, D3 d) {
+    min_image(b, d.x, d.y, d.z);
+    return d;
+}
+
+// Displacements D[3] that take the unconstrained sites r[3] onto the rigid
+// molecule, against the reference sites r0[3]; false when the step has no
+// solution (a molecule turned out of its plane by 90 degrees or more, or a
+// degenerate reference).
+__device__ __forceinline__ bool settle_shift(const WaterArgs& a, const D3 r0[3], const D3 r[3],
+                                             double mo, double mh, D3 D[3]) {
+#pragma clang fp contract(off)
+    const double rc = a.settle_rc, h = a.settle_h;
+    const double mt = mo + 2.0 * mh;
+    const double ra = 2.0 * mh * h / mt, rb = h - ra;
+    const D3 b0 = dimage(a.box, dsub(r0[1], r0[0]));
+    const D3 c0 = dimage(a.box, dsub(r0[2], r0[0]));
+    const D3 nz = dcross(b0, c0);
+    const double nn = ddot(nz, nz);
+    const D3 ez = dscale(nz, 1.0 / sqrt(nn));
+    const D3 hx = dsub(c0, b0);
+    const D3 ex = dscale(hx, 1.0 / sqrt(ddot(hx, hx)));
+    const D3 ey = dcross(ez, ex);
+    const D3 b1 = dimage(a.box, dsub(r[1], r[0]));
+    const D3 c1 = dimage(a.box, dsub(r[2], r[0]));
+    const D3 d = dscale(dadd(dscale(b1, mh), dscale(c1, mh)), 1.0 / mt);
+    const D3 u[3] = {dscale(d, -1.0), dsub(b1, d), dsub(c1, d)};
+    double ux[3], uy[3], uz[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ux[k] = ddot(u[k], ex);
+        uy[k] = ddot(u[k], ey);
+        uz[k] = ddot(u[k], ez);
+    }
+    const double s2 = uz[0] / ra, s1 = (uz[2] - uz[1]) / (2.0 * rc);
+    const double q = 1.0 - s1 * s1 - s2 * s2;
+    const double s3 = sqrt(q);
+    const double k = 1.0 / (1.0 + s3);
+    const double mxx = 1.0 - s1 * s1 * k, myy = 1.0 - s2 * s2 * k, mxy = -s1 * s2 * k;
+    const double sxx = 2.0 * mh * rc * rc, syy = mo * ra * ra + 2.0 * mh * rb * rb;
+    const double T = -mo * ra * ux[0] + mh * rc * (uy[2] - uy[1]) + mh * rb * (ux[1] + ux[2]);
+    const double al = sxx * mxx + syy * myy, be = (sxx - syy) * mxy;
+    const double rho2 = al * al + be * be;
+    const double disc = rho2 - T * T;
+    const double Q = sqrt(disc);
+    const double st = (T * al - Q * be) / rho2, ct = (Q * al + T * be) / rho2;
+    const double r1x = ct * mxx - st * mxy, r1y = ct * mxy - st * myy;
+    const double r2x = st * mxx + ct * mxy, r2y = st * mxy + ct * myy;
+    const double px[3] = {0.0, -rc, rc}, py[3] = {ra, -rb, -rb};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double nx = r1x * px[i] + r1y * py[i];
+        const double ny = r2x * px[i] + r2y * py[i];
+        const double nzc = s1 * px[i] + s2 * py[i];
+        const double dx = nx - ux[i], dy = ny - uy[i], dz = nzc - uz[i];
+        D[i] = dadd(dadd(dscale(ex, dx), dscale(ey, dy)), dscale(ez, dz));
+    }
+    return nn > 0.0 && q > 0.0 && disc >= 0.0;
+}
+
+// Velocities v[3] (in/out) made rigid at the sites r[3]; false when the
+// 3 x 3 system is singular.
+__device__ __forceinline__ bool settle_velocities(const WaterArgs& a, const D3 r[3], D3 v[3],
+                                                  double mo, double mh) {
+#pragma clang fp contract(off)
+    const D3 r01 = dimage(a.box, dsub(r[0], r[1]));
+    const D3 r02 = dimage(a.box, dsub(r[0], r[2]));
+    const D3 r12 = dimage(a.box, dsub(r[1], r[2]));
+    const double ioh = 1.0 / mo + 1.0 / mh;
+    const double a00 = ioh * ddot(r01, r01), a11 = ioh * ddot(r02, r02),
+                 a22 = (2.0 / mh) * ddot(r12, r12);
+    const double a01 = ddot(r01, r02) / mo, a02 = -ddot(r01, r12) / mh, a12 = ddot(r02, r12) / mh;
+    const double b0 = ddot(r01, dsub(v[0], v[1])), b1 = ddot(r02, dsub(v[0], v[2])),
+                 b2 = ddot(r12, dsub(v[1], v[2]));
+    const double c00 = a11 * a22 - a12 * a12, c01 = a02 * a12 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    const double c11 = a00 * a22 - a02 * a02, c12 = a01 * a02 - a00 * a12, c22 = a00 * a11 - a01 * a01;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    const double inv = 1.0 / det;
+    const double mu0 = (c00 * b0 + c01 * b1 + c02 * b2) * inv;
+    const double mu1 = (c01 * b0 + c11 * b1 + c12 * b2) * inv;
+    const double mu2 = (c02 * b0 + c12 * b1 + c22 * b2) * inv;
+    const D3 g01 = dscale(r01, mu0), g02 = dscale(r02, mu1), g12 = dscale(r12, mu2);
+    v[0] = dsub(v[0], dscale(dadd(g01, g02), 1.0 / mo));
+    v[1] = dadd(v[1], dscale(dsub(g01, g12), 1.0 / mh));
+    v[2] = dadd(v[2], dscale(dadd(g02, g12), 1.0 / mh));
+    return fabs(det) > 0.0;
+}
+
+__device__ __forceinline__ void settle_fail(const WaterArgs& a, int* status, long m, bool ok, int slot) {
+    status[m] = ok ? 1 : -2;
+    if (!ok && a.fail != nullptr) atomicMin(&a.fail[slot], -2);
+}
+
+// RigidWater::constrainPositions by SETTLE: positions against refp.
+__global__ void k_settle(WaterArgs a, const double4* __restrict__ refp, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+        const long b = 3 * m;
+        const D3 r0[3] = {d3(refp[b]), d3(refp[b + 1]), d3(refp[b + 2])};
+        const double4 p[3] = {a.pos[b], a.pos[b + 1], a.pos[b + 2]};
+        const D3 r[3] = {d3(p[0]), d3(p[1]), d3(p[2])};
+        D3 D[3];
+        const bool ok = settle_shift(a, r0, r, a.vel[b].w, a.vel[b + 1].w, D);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const D3 q = dadd(r[i], D[i]);
+            a.pos[b + i] = make_double4(q.x, q.y, q.z, p[i].w);
+        }
+        settle_fail(a, status, m, ok, 0);
+    }
+}
+
+// RigidWater::constrainVelocities by SETTLE.
+__global__ void k_settle_vel(WaterArgs a, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+        const long b = 3 * m;
+        const D3 r[3] = {d3(a.pos[b]), d3(a.pos[b + 1]), d3(a.pos[b + 2])};
+        const double4 w[3] = {a.vel[b], a.vel[b + 1], a.vel[b + 2]};
+        D3 v[3] = {d3(w[0]), d3(w[1]), d3(w[2])};
+        const bool ok = settle_velocities(a, r, v, w[0].w, w[1].w);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.vel[b + i] = make_double4(v[i].x, v[i].y, v[i].z, w[i].w);
+        settle_fail(a, status, m, ok, 1);
+    }
+}
+
+// ConstrainedVelocityVerlet::step up to the forces, per molecule with
+// SETTLE: kick v + f (dt/2/m), drift without wrap, the closed-form position
+// constraint against the start-of-step sites, and v += minimumImage(r -
+// r_free) / dt (constrained_velocity_verlet.hpp:33-47), in the arithmetic of
+// k_cvv_first, k_shake's replacement k_settle and k_cvv_correct.
+__global__ void k_cvv_settle_first(WaterArgs a, double dt, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+#pragma clang fp contract(off)
+        const long b = 3 * m;
+        D3 r0[3], r[3], v[3];
+        double w[3], mass[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double4 p = a.pos[b + i], u = a.vel[b + i], f = a.frc[b + i];
+            r0[i] = d3(p);
+            w[i] = p.w;
+            mass[i] = u.w;
+            const double c = 0.5 * dt / u.w;
+            v[i] = D3{u.x + f.x * c, u.y + f.y * c, u.z + f.z * c};
+            r[i] = D3{p.x + v[i].x * dt, p.y + v[i].y * dt, p.z + v[i].z * dt};
+        }
+        D3 D[3];
+        const bool ok = settle_shift(a, r0, r, mass[0], mass[1], D);
+        const double inv = 1.0 / dt;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const D3 q = dadd(r[i], D[i]);
+            const D3 sh = dimage(a.box, dsub(q, r[i]));
+            a.pos[b + i] = make_double4(q.x, q.y, q.z, w[i]);
+            a.vel[b + i] = make_double4(v[i].x + sh.x * inv, v[i].y + sh.y * inv, v[i].z + sh.z * inv,
+                                        mass[i]);
+        }
+        settle_fail(a, status, m, ok, 0);
+    }
+}
+
+// The step's end with SETTLE: kick v + f (dt/2/m), then the velocity
+// constraint (constrained_velocity_verlet.hpp:51-55).
+__global__ void k_cvv_settle_second(WaterArgs a, double dt, int* __restrict__ status) {
+    GRID_STRIDE(m, a.n / 3) {
+#pragma clang fp contract(off)
+        const long b = 3 * m;
+        D3 r[3], v[3];
+        double mass[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double4 u = a.vel[b + i], f = a.frc[b + i];
+            r[i] = d3(a.pos[b + i]);
+            mass[i] = u.w;
+            const double c = 0.5 * dt / u.w;
+            v[i] = D3{u.x + f.x * c, u.y + f.y * c, u.z + f.z * c};
+        }
+        const bool ok = settle_velocities(a, r, v, mass[0], mass[1]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.vel[b + i] = make_double4(v[i].x, v[i].y, v[i].z, mass[i]);
+        settle_fail(a, status, m, ok, 1);
+    }
+}
+
 // Constrained velocity Verlet, first half: save the start positions, kick
 // v + f (dt/2/m) (plain multiply and add), drift without wrap, keep the
 // unconstrained positions (constrained_velocity_verlet.hpp:33-42).
@@ -667,6 +886,22 @@ void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hi
 
 void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s) {
     hipLaunchKernelGGL(k_rattle, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, status);
+}
+
+void water_launch_settle(const WaterArgs& a, const double4* refp, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_settle, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, refp, status);
+}
+
+void water_launch_settle_vel(const WaterArgs& a, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_settle_vel, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, status);
+}
+
+void water_launch_cvv_settle_first(const WaterArgs& a, double dt, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_cvv_settle_first, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, dt, status);
+}
+
+void water_launch_cvv_settle_second(const WaterArgs& a, double dt, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_cvv_settle_second, dim3(wave_groups(a.n / 3)), dim3(kWave), 0, s, a, dt, status);
 }
 
 void water_launch_cvv_first(const WaterArgs& a, double dt, double4* start, double4* free_pos,

@@ -54,6 +54,9 @@ struct WaterArgs {
     int* overflow;
     double oh, hh, tol;
     int max_iter;
+    // SETTLE geometry: half the H-H distance and the distance from O to the
+    // H-H midpoint, sqrt(oh^2 - settle_rc^2)
+    double settle_rc, settle_h;
 };
 
 void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
@@ -65,6 +68,15 @@ long water_real_blocks(long n);
 void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s);
 void water_launch_shake(const WaterArgs& a, const double4* refp, int* status, hipStream_t s);
 void water_launch_rattle(const WaterArgs& a, int* status, hipStream_t s);
+// SETTLE in place of SHAKE / RATTLE: status[m] = 1, or -2 when the molecule's
+// step has no solution (atomic minimum into fail[0] / fail[1] when set).
+void water_launch_settle(const WaterArgs& a, const double4* refp, int* status, hipStream_t s);
+void water_launch_settle_vel(const WaterArgs& a, int* status, hipStream_t s);
+// The constrained step with SETTLE in two per-molecule kernels: kick, drift,
+// position constraint and velocity correction; then kick and velocity
+// constraint.
+void water_launch_cvv_settle_first(const WaterArgs& a, double dt, int* status, hipStream_t s);
+void water_launch_cvv_settle_second(const WaterArgs& a, double dt, int* status, hipStream_t s);
 void water_launch_cvv_first(const WaterArgs& a, double dt, double4* start, double4* free_pos,
                             hipStream_t s);
 void water_launch_cvv_correct(const WaterArgs& a, double dt, const double4* free_pos, hipStream_t s);

@@ -57,6 +57,7 @@ struct md_water {
     bool list_built = false;
     double* eterm = nullptr;  // per-step device sums of md_water_run: 5 energy sums + KE
     long eterm_steps = 0;
+    int constraint = MD_RIGID_SHAKE;  // constraint algorithm of md_water_run
 };
 
 namespace {
@@ -143,6 +144,21 @@ void terms_of(const md_water* w, const double* sums, double* terms) {
     if (terms) {
         terms[0] = lj; terms[1] = lrc; terms[2] = real; terms[3] = recip; terms[4] = self; terms[5] = intra;
     }
+}
+
+// A SETTLE status: -2 for a molecule whose step has no closed-form solution.
+md_status settle_check(md_water* w, const char* what) {
+    const long nm = w->n / 3;
+    std::vector<int> st(static_cast<size_t>(nm));
+    WHIP(w, hipStreamSynchronize(w->stream));
+    WHIP(w, hipMemcpy(st.data(), w->status, nm * sizeof(int), hipMemcpyDeviceToHost));
+    for (long m = 0; m < nm; ++m)
+        if (st[m] < 0)
+            return wfail(w, MD_ERR_RUNTIME,
+                         std::string(what) + ": molecule " + std::to_string(m) +
+                             " has no rigid solution (degenerate, or rotated by 90 degrees or more in "
+                             "one step); the timestep is far too large");
+    return MD_OK;
 }
 
 md_status status_check(md_water* w, const char* what, int* iters) {
@@ -431,6 +447,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     a.hh = 2.0 * p->r_oh * std::sin(0.5 * p->angle_hoh * kPi / 180.0);
     a.tol = p->tolerance;
     a.max_iter = p->max_iterations;
+    a.settle_rc = 0.5 * a.hh;
+    a.settle_h = std::sqrt(a.oh * a.oh - a.settle_rc * a.settle_rc);
     return MD_OK;
 }
 
@@ -498,6 +516,31 @@ md_status md_water_rattle(md_water* w, int* iterations) {
     return status_check(w, "RigidWater: RATTLE", iterations);
 }
 
+md_status md_water_set_constraint(md_water* w, int method) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    if (method != MD_RIGID_SHAKE && method != MD_RIGID_SETTLE)
+        return wfail(w, MD_ERR_INVALID_ARGUMENT, "constraint must be MD_RIGID_SHAKE or MD_RIGID_SETTLE");
+    w->constraint = method;
+    return MD_OK;
+}
+
+md_status md_water_settle(md_water* w, const double* rx, const double* ry, const double* rz) {
+    if (w == nullptr || rx == nullptr || ry == nullptr || rz == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    std::vector<double4> r(static_cast<size_t>(w->n));
+    for (long i = 0; i < w->n; ++i) r[i] = make_double4(rx[i], ry[i], rz[i], 0.0);
+    WHIP(w, hipMemcpy(w->start, r.data(), w->n * sizeof(double4), hipMemcpyHostToDevice));
+    mdk::water_launch_settle(w->a, w->start, w->status, w->stream);
+    WHIP(w, hipGetLastError());
+    return settle_check(w, "SETTLE positions");
+}
+
+md_status md_water_settle_velocities(md_water* w) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::water_launch_settle_vel(w->a, w->status, w->stream);
+    WHIP(w, hipGetLastError());
+    return settle_check(w, "SETTLE velocities");
+}
+
 // nsteps of constrained velocity Verlet with the SPC/E force field; forces
 // must be current on entry (md_water_forces once). The Verlet list is
 // rebuilt when the two largest displacements exceed the skin.
@@ -523,10 +566,15 @@ md_status md_water_run(md_water* w, long nsteps, double dt, int precision, doubl
     a.build_flag = w->flags + 1;
     WHIP(w, hipMemsetAsync(w->flags + 2, 0, 2 * sizeof(int), w->stream));
     const mdk::Grid& g = w->grid;
+    const bool settle = w->constraint == MD_RIGID_SETTLE;
     for (long k = 0; k < nsteps; ++k) {
-        mdk::water_launch_cvv_first(a, dt, w->start, w->freep, w->stream);
-        mdk::water_launch_shake(a, w->start, w->status, w->stream);
-        mdk::water_launch_cvv_correct(a, dt, w->freep, w->stream);
+        if (settle) {
+            mdk::water_launch_cvv_settle_first(a, dt, w->status, w->stream);
+        } else {
+            mdk::water_launch_cvv_first(a, dt, w->start, w->freep, w->stream);
+            mdk::water_launch_shake(a, w->start, w->status, w->stream);
+            mdk::water_launch_cvv_correct(a, dt, w->freep, w->stream);
+        }
         mdk::water_launch_check(a, w->flags + 1, w->stream);
         if (g.usable)
             mdk::launch_bin_index(w->n, w->pos, g, w->cell_of, w->cell_count, w->cell_start,
@@ -534,8 +582,12 @@ md_status md_water_run(md_water* w, long nsteps, double dt, int precision, doubl
         mdk::water_launch_list(a, g, w->cell_of, w->cell_start, w->cell_atoms, w->stream);
         mdk::water_launch_forces(a, mixed, w->stream);
         mdk::water_launch_energy_sum(a, w->eterm + 6 * k, w->stream);
-        mdk::water_launch_cvv_kick(a, dt, w->stream);
-        mdk::water_launch_rattle(a, w->status, w->stream);
+        if (settle) {
+            mdk::water_launch_cvv_settle_second(a, dt, w->status, w->stream);
+        } else {
+            mdk::water_launch_cvv_kick(a, dt, w->stream);
+            mdk::water_launch_rattle(a, w->status, w->stream);
+        }
         if (ke) {
             mdk::water_launch_kinetic(a, w->stream);
             mdk::water_launch_kinetic_sum(a, w->eterm + 6 * k + 5, w->stream);
@@ -551,6 +603,12 @@ md_status md_water_run(md_water* w, long nsteps, double dt, int precision, doubl
                                w->stream));
     }
     WHIP(w, hipStreamSynchronize(w->stream));
+    for (int c = 0; c < 2 && settle; ++c)
+        if (f[2 + c] < 0)
+            return wfail(w, MD_ERR_RUNTIME,
+                         std::string(c == 0 ? "SETTLE positions" : "SETTLE velocities") +
+                             ": a molecule has no rigid solution (rotated by 90 degrees or more in one "
+                             "step); the timestep is far too large");
     for (int c = 0; c < 2; ++c) {
         const char* what = c == 0 ? "RigidWater: SHAKE" : "RigidWater: RATTLE";
         if (f[2 + c] == -2) return wfail(w, MD_ERR_RUNTIME, std::string(what) + ": constraint became singular");

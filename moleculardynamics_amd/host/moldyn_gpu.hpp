@@ -1074,14 +1074,75 @@ private:
     detail::WaterBinding water_;
 };
 
+// SETTLE (Miyamoto & Kollman 1992): the closed-form replacement of
+// RigidWater that the reference plans (constrain/rigid_water.hpp:20-24; no
+// class in its sources). Same interface and the same fixed points as
+// RigidWater's SHAKE and RATTLE, solved exactly in one pass per molecule on
+// the device (both calls return 1); std::runtime_error when a molecule's step
+// has no solution. The two hydrogen masses of a molecule must be equal.
+class Settle {
+public:
+    Settle(double ohDistance, double hohAngleDegrees) : oh_(ohDistance), angle_(hohAngleDegrees) {
+        if (!(ohDistance > 0.0)) throw std::invalid_argument("Settle: ohDistance must be positive");
+        if (!(hohAngleDegrees > 0.0) || !(hohAngleDegrees < 180.0))
+            throw std::invalid_argument("Settle: hohAngleDegrees must lie in (0, 180)");
+        hh_ = 2.0 * oh_ * std::sin(0.5 * hohAngleDegrees * 3.14159265358979323846 / 180.0);
+    }
+    Settle(const Settle& o) : oh_(o.oh_), angle_(o.angle_), hh_(o.hh_) {}
+
+    int constrainPositions(System& sys, const std::vector<Vec3>& reference) const {
+        md_water* w = water_.bind(sys, params());
+        const std::size_t n = sys.size();
+        if (reference.size() != n) throw std::invalid_argument("Settle: reference size differs from the system");
+        std::vector<double> rx(n), ry(n), rz(n);
+        for (std::size_t i = 0; i < n; ++i) {
+            rx[i] = reference[i].x;
+            ry[i] = reference[i].y;
+            rz[i] = reference[i].z;
+        }
+        detail::wcheck(md_water_settle(w, rx.data(), ry.data(), rz.data()), w, "Settle::constrainPositions");
+        std::vector<Vec3> r;
+        water_.download(0, r);
+        for (std::size_t i = 0; i < n; ++i) sys.setPosition(i, r[i]);
+        return 1;
+    }
+    int constrainVelocities(System& sys) const {
+        md_water* w = water_.bind(sys, params());
+        detail::wcheck(md_water_settle_velocities(w), w, "Settle::constrainVelocities");
+        std::vector<Vec3> v;
+        water_.download(1, v);
+        for (std::size_t i = 0; i < v.size(); ++i) sys.setVelocity(i, v[i]);
+        return 1;
+    }
+    double ohDistance() const { return oh_; }
+    double hhDistance() const { return hh_; }
+    static std::size_t degreesOfFreedom(std::size_t molecules, bool comRemoved) {
+        return RigidWater::degreesOfFreedom(molecules, comRemoved);
+    }
+
+private:
+    md_water_params params() const {
+        md_water_params p = detail::water_params(10.0, 5);
+        p.r_oh = oh_;
+        p.angle_hoh = angle_;
+        return p;
+    }
+    double oh_, angle_;
+    double hh_ = 0.0;
+    detail::WaterBinding water_;
+};
+
 // integrate/constrained_velocity_verlet.hpp:22-68: kick, drift (no wrap),
 // SHAKE against the step's starting positions, the velocity corrected by the
 // SHAKE displacement over dt, forces, kick, RATTLE. The kicks are the
 // reference's plain multiply-add v + f (dt / 2 / m). Forces must be current
-// on entry.
+// on entry. The constraint is RigidWater, as in the reference, or Settle
+// (class template argument deduction picks it from the constructor, so
+// `ConstrainedVelocityVerlet integrator(dt, rigid);` reads as the reference).
+template <class Constraint = RigidWater>
 class ConstrainedVelocityVerlet {
 public:
-    ConstrainedVelocityVerlet(double dt, RigidWater constraint) : dt_(dt), constraint_(constraint) {}
+    ConstrainedVelocityVerlet(double dt, Constraint constraint) : dt_(dt), constraint_(constraint) {}
     template <class ForceFn>
     void step(System& sys, ForceFn&& computeForces) {
         const std::size_t n = sys.size();
@@ -1105,11 +1166,11 @@ public:
         constraint_.constrainVelocities(sys);
     }
     double dt() const { return dt_; }
-    const RigidWater& constraint() const { return constraint_; }
+    const Constraint& constraint() const { return constraint_; }
 
 private:
     double dt_;
-    RigidWater constraint_;
+    Constraint constraint_;
     std::vector<Vec3> reference_, unconstrained_;
 };
 

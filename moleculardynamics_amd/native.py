@@ -160,6 +160,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_water_build_list.argtypes = [vp]
     lib.md_water_shake.argtypes = [vp, _dp, _dp, _dp, _ip]
     lib.md_water_rattle.argtypes = [vp, _ip]
+    lib.md_water_settle.argtypes = [vp, _dp, _dp, _dp]
+    lib.md_water_settle_velocities.argtypes = [vp]
+    lib.md_water_set_constraint.argtypes = [vp, C.c_int]
     lib.md_water_run.argtypes = [vp, C.c_long, C.c_double, C.c_int, _dp, _dp]
     lib.md_water_download.argtypes = [vp] + [_dp] * 9
     lib.md_water_set_state.argtypes = [vp] + [_dp] * 6
@@ -184,6 +187,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 
 RDF_AUTO, RDF_ALL_PAIRS, RDF_CELLS = 0, 1, 2
+RIGID_SHAKE, RIGID_SETTLE = 0, 1
 
 
 def device_peaks():
@@ -647,6 +651,20 @@ class Water:
         it = C.c_int()
         self._chk(self.lib.md_water_rattle(self.h, C.byref(it)))
         return it.value
+
+    def settle(self, ref_pos):
+        """Closed-form position constraint against ref_pos (SETTLE)."""
+        cols = [_col(np.asarray(ref_pos)[:, d]) for d in range(3)]
+        self._chk(self.lib.md_water_settle(self.h, *[_p(c) for c in cols]))
+        return 1
+
+    def settle_velocities(self):
+        self._chk(self.lib.md_water_settle_velocities(self.h))
+        return 1
+
+    def set_constraint(self, method):
+        """RIGID_SHAKE (default) or RIGID_SETTLE for run()."""
+        self._chk(self.lib.md_water_set_constraint(self.h, method))
 
     def run(self, nsteps, dt, precision=FP64):
         pe, ke = np.zeros(nsteps), np.zeros(nsteps)

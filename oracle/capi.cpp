@@ -335,18 +335,52 @@ int orc_rigid_rattle(long n, const double* L, const double* x, const double* y, 
     });
 }
 
-// Rigid SPC/E dynamics: constrained velocity Verlet with the full SPC/E
-// force field; state updated in place; PE (K) and KE per step.
-int orc_water_run(long n, const double* L, double* x, double* y, double* z, double* vx, double* vy,
-                  double* vz, double rc, int kmax, double alpha, double dt, long nsteps, double* pe,
-                  double* ke) {
+// SETTLE positions (x,y,z updated in place) against reference positions;
+// returns 1 or a negative status. No reference symbol (the reference plans
+// SETTLE, constrain/rigid_water.hpp:20-24).
+int orc_settle_positions(long n, const double* L, double* x, double* y, double* z, const double* m,
+                         const double* rx, const double* ry, const double* rz, double oh,
+                         double angle) {
+    return guarded([&] {
+        State s = make_state(n, L, x, y, z);
+        for (long i = 0; i < n; ++i) s.m[i] = m[i];
+        std::vector<V3> ref(static_cast<std::size_t>(n));
+        for (long i = 0; i < n; ++i) ref[i] = V3{rx[i], ry[i], rz[i]};
+        const int it = Settle(oh, angle).positions(s, ref);
+        copy_out(s.x, x); copy_out(s.y, y); copy_out(s.z, z);
+        return it;
+    });
+}
+
+int orc_settle_velocities(long n, const double* L, const double* x, const double* y,
+                          const double* z, const double* m, double* vx, double* vy, double* vz,
+                          double oh, double angle) {
+    return guarded([&] {
+        State s = make_state(n, L, x, y, z);
+        for (long i = 0; i < n; ++i) {
+            s.m[i] = m[i]; s.vx[i] = vx[i]; s.vy[i] = vy[i]; s.vz[i] = vz[i];
+        }
+        const int it = Settle(oh, angle).velocities(s);
+        copy_out(s.vx, vx); copy_out(s.vy, vy); copy_out(s.vz, vz);
+        return it;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+// Rigid SPC/E dynamics with constraint C (Rigid or Settle).
+template <class Constraint>
+int water_run(long n, const double* L, double* x, double* y, double* z, double* vx, double* vy,
+              double* vz, double rc, int kmax, double alpha, double dt, long nsteps, double* pe,
+              double* ke) {
     return guarded([&] {
         State s = water_state(n, L, x, y, z);
         for (long i = 0; i < n; ++i) {
             s.vx[i] = vx[i]; s.vy[i] = vy[i]; s.vz[i] = vz[i];
         }
         const Spce p;
-        const Rigid rw(p.rOH, p.angleHOH);
+        const Constraint c(p.rOH, p.angleHOH);
         double last = 0.0;
         ForceFn fn = [&](State& st) {
             last = spce_forces(p, rc, kmax, alpha, st);
@@ -354,7 +388,7 @@ int orc_water_run(long n, const double* L, double* x, double* y, double* z, doub
         };
         fn(s);
         for (long k = 0; k < nsteps; ++k) {
-            cvv_step(s, dt, rw, fn);
+            cvv_step(s, dt, c, fn);
             if (pe) pe[k] = last;
             if (ke) ke[k] = kinetic_energy(s);
         }
@@ -362,6 +396,24 @@ int orc_water_run(long n, const double* L, double* x, double* y, double* z, doub
         copy_out(s.vx, vx); copy_out(s.vy, vy); copy_out(s.vz, vz);
         return 0;
     });
+}
+}  // namespace
+
+extern "C" {
+
+// Rigid SPC/E dynamics: constrained velocity Verlet with the full SPC/E
+// force field; state updated in place; PE (K) and KE per step.
+int orc_water_run(long n, const double* L, double* x, double* y, double* z, double* vx, double* vy,
+                  double* vz, double rc, int kmax, double alpha, double dt, long nsteps, double* pe,
+                  double* ke) {
+    return water_run<Rigid>(n, L, x, y, z, vx, vy, vz, rc, kmax, alpha, dt, nsteps, pe, ke);
+}
+
+// The same with SETTLE in place of SHAKE / RATTLE (no reference symbol).
+int orc_water_run_settle(long n, const double* L, double* x, double* y, double* z, double* vx,
+                         double* vy, double* vz, double rc, int kmax, double alpha, double dt,
+                         long nsteps, double* pe, double* ke) {
+    return water_run<Settle>(n, L, x, y, z, vx, vy, vz, rc, kmax, alpha, dt, nsteps, pe, ke);
 }
 
 // Per-type-pair LJ; sig/eps/rc are ntypes*ntypes row-major symmetric tables;

@@ -123,7 +123,10 @@ void NoseHoover::step(State& s, const ForceFn& forces) {
 // drift without wrap, SHAKE against the start-of-step positions, velocity
 // correction by the minimum-imaged SHAKE shift over dt, forces, kick, RATTLE
 // (integrate/constrained_velocity_verlet.hpp:27-57).
-void cvv_step(State& s, double dt, const Rigid& rw, const ForceFn& forces) {
+namespace {
+template <class PosFn, class VelFn>
+void cvv_impl(State& s, double dt, const PosFn& constrain_positions, const VelFn& constrain_velocities,
+              const ForceFn& forces) {
     const std::size_t n = s.n();
     std::vector<V3> start(n), free_pos(n);
     for (std::size_t i = 0; i < n; ++i) start[i] = s.r(i);
@@ -139,7 +142,7 @@ void cvv_step(State& s, double dt, const Rigid& rw, const ForceFn& forces) {
         s.z[i] = s.z[i] + s.vz[i] * dt;
         free_pos[i] = s.r(i);
     }
-    rw.shake(s, start);
+    constrain_positions(s, start);
     const double inv_dt = 1.0 / dt;
     for (std::size_t i = 0; i < n; ++i) {
         const V3 r = s.r(i);
@@ -155,7 +158,21 @@ void cvv_step(State& s, double dt, const Rigid& rw, const ForceFn& forces) {
         s.vy[i] = s.vy[i] + s.fy[i] * c;
         s.vz[i] = s.vz[i] + s.fz[i] * c;
     }
-    rw.rattle(s);
+    constrain_velocities(s);
+}
+}  // namespace
+
+void cvv_step(State& s, double dt, const Rigid& rw, const ForceFn& forces) {
+    cvv_impl(
+        s, dt, [&](State& st, const std::vector<V3>& ref) { rw.shake(st, ref); },
+        [&](State& st) { rw.rattle(st); }, forces);
+}
+
+// The same step with SETTLE in place of SHAKE and RATTLE.
+void cvv_step(State& s, double dt, const Settle& c, const ForceFn& forces) {
+    cvv_impl(
+        s, dt, [&](State& st, const std::vector<V3>& ref) { c.positions(st, ref); },
+        [&](State& st) { c.velocities(st); }, forces);
 }
 
 }  // namespace oracle

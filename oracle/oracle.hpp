@@ -227,9 +227,22 @@ struct Rigid {
     int rattle(State& s) const;
     static std::size_t dof(std::size_t molecules, bool com_removed);
 };
+// Closed-form rigid three-site constraints (SETTLE, Miyamoto & Kollman 1992),
+// the replacement of Rigid's SHAKE/RATTLE that the reference plans
+// (constrain/rigid_water.hpp:20-24): the same fixed points, solved exactly
+// (settle.cpp). Molecules are O, H, H triples with equal hydrogen masses.
+// Both return 1 (one pass); degenerate or over-large steps throw.
+struct Settle {
+    double oh, hh;
+    Settle(double oh_dist, double hoh_deg);
+    int positions(State& s, const std::vector<V3>& reference) const;
+    int velocities(State& s) const;
+};
 // RATTLE-form constrained velocity Verlet
-// (integrate/constrained_velocity_verlet.hpp:27-57).
+// (integrate/constrained_velocity_verlet.hpp:27-57), with SHAKE/RATTLE or
+// with SETTLE as the constraint.
 void cvv_step(State& s, double dt, const Rigid& rw, const ForceFn& forces);
+void cvv_step(State& s, double dt, const Settle& st, const ForceFn& forces);
 
 // ---- per-type-pair LJ (no reference symbol; built from the single-type
 // kernel, lennard_jones.cpp:50-88, by giving each unordered type pair its own

@@ -117,6 +117,12 @@ class Oracle:
         lib.orc_water_run.argtypes = ([C.c_long, _dp] + [_dp] * 6 +
                                       [C.c_double, C.c_int, C.c_double, C.c_double, C.c_long,
                                        _dp, _dp])
+        if hasattr(lib, "orc_settle_positions"):
+            lib.orc_settle_positions.argtypes = [C.c_long, _dp] + [_dp] * 7 + [C.c_double,
+                                                                               C.c_double]
+            lib.orc_settle_velocities.argtypes = [C.c_long, _dp] + [_dp] * 7 + [C.c_double,
+                                                                                C.c_double]
+            lib.orc_water_run_settle.argtypes = lib.orc_water_run.argtypes
         if hasattr(lib, "orc_mix_lj"):
             lib.orc_mix_lj.argtypes = ([C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp,
                                         C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
@@ -291,13 +297,38 @@ class Oracle:
             raise RuntimeError(self.error())
         return it, np.stack([vx, vy, vz], axis=1)
 
-    def water_run(self, fr: Frame, rc=10.0, kmax=5, alpha=0.0, dt=0.00182367, nsteps=1):
+    def settle_positions(self, fr: Frame, m, ref: Frame, oh=1.0, angle=109.47):
+        """Closed-form constraint of the positions against ``ref`` (oracle only:
+        the reference has no SETTLE)."""
+        x, y, z = fr.x.copy(), fr.y.copy(), fr.z.copy()
+        m = np.ascontiguousarray(m, dtype=float)
+        it = self.lib.orc_settle_positions(fr.n, _ptr(fr.L), _ptr(x), _ptr(y), _ptr(z), _ptr(m),
+                                           _ptr(ref.x), _ptr(ref.y), _ptr(ref.z), oh, angle)
+        if it == -1:
+            raise ValueError(self.error())
+        if it < 0:
+            raise RuntimeError(self.error())
+        return it, np.stack([x, y, z], axis=1)
+
+    def settle_velocities(self, fr: Frame, m, oh=1.0, angle=109.47):
+        vx, vy, vz = fr.vx.copy(), fr.vy.copy(), fr.vz.copy()
+        m = np.ascontiguousarray(m, dtype=float)
+        it = self.lib.orc_settle_velocities(fr.n, _ptr(fr.L), _ptr(fr.x), _ptr(fr.y), _ptr(fr.z),
+                                            _ptr(m), _ptr(vx), _ptr(vy), _ptr(vz), oh, angle)
+        if it == -1:
+            raise ValueError(self.error())
+        if it < 0:
+            raise RuntimeError(self.error())
+        return it, np.stack([vx, vy, vz], axis=1)
+
+    def water_run(self, fr: Frame, rc=10.0, kmax=5, alpha=0.0, dt=0.00182367, nsteps=1,
+                  settle=False):
         w = fr.copy()
         pe = np.zeros(nsteps)
         ke = np.zeros(nsteps)
-        st = self.lib.orc_water_run(w.n, _ptr(w.L), _ptr(w.x), _ptr(w.y), _ptr(w.z), _ptr(w.vx),
-                                    _ptr(w.vy), _ptr(w.vz), rc, kmax, alpha, dt, nsteps, _ptr(pe),
-                                    _ptr(ke))
+        fn = self.lib.orc_water_run_settle if settle else self.lib.orc_water_run
+        st = fn(w.n, _ptr(w.L), _ptr(w.x), _ptr(w.y), _ptr(w.z), _ptr(w.vx), _ptr(w.vy),
+                _ptr(w.vz), rc, kmax, alpha, dt, nsteps, _ptr(pe), _ptr(ke))
         if st != 0:
             raise RuntimeError(self.error())
         return w, pe, ke

@@ -581,6 +581,144 @@ int main(int argc, char** argv) {
         CHECK(slope > 1.85 && slope < 2.15);
     });
 
+    // ---- SETTLE, the closed form that constrain/rigid_water.hpp:20-24 plans:
+    // it must reproduce RigidWater to machine precision; the planned tests
+    // (geometry held over a trajectory, the dt^2 slope with constraints
+    // active) run through it as through RigidWater ----
+    auto makeSettle = []() {
+        const SpceParameters p;
+        return Settle(p.rOH, p.angleHOH);
+    };
+    auto positionsOf = [](const System& sys) {
+        std::vector<Vec3> r(sys.size());
+        for (std::size_t i = 0; i < sys.size(); ++i) r[i] = sys.position(i);
+        return r;
+    };
+
+    run("SETTLE reproduces converged SHAKE and RATTLE to machine precision", [&] {
+        auto w = loadWater();
+        const SpceParameters p;
+        const RigidWater tight(p.rOH, p.angleHOH, 1e-14, 5000);
+        const Settle settle = makeSettle();
+        const std::size_t n = w.full.size();
+        tight.constrainPositions(w.full, positionsOf(w.full));  // the file holds 6 digits
+        const std::vector<Vec3> reference = positionsOf(w.full);
+        Rng rng(77);
+        std::vector<Vec3> moved(n);
+        for (std::size_t i = 0; i < n; ++i)
+            moved[i] = reference[i] + Vec3{0.03 * rng.normal(), 0.03 * rng.normal(), 0.03 * rng.normal()};
+        for (std::size_t i = 0; i < n; ++i) w.full.setPosition(i, moved[i]);
+        CHECK(settle.constrainPositions(w.full, reference) == 1);
+        const std::vector<Vec3> bySettle = positionsOf(w.full);
+        for (std::size_t i = 0; i < n; ++i) w.full.setPosition(i, moved[i]);
+        CHECK(tight.constrainPositions(w.full, reference) > 1);
+        double worst = 0.0;
+        for (std::size_t i = 0; i < n; ++i) worst = std::max(worst, norm(w.full.position(i) - bySettle[i]));
+        CHECK(worst < 1e-12);
+        CHECK(worstGeometryError(w.full, tight) < 1e-13);
+        // velocities
+        for (std::size_t i = 0; i < n; ++i) w.full.setPosition(i, bySettle[i]);
+        std::vector<Vec3> v0(n);
+        for (std::size_t i = 0; i < n; ++i) {
+            const double sp = std::sqrt(300.0 / w.full.mass(i));
+            v0[i] = Vec3{sp * rng.normal(), sp * rng.normal(), sp * rng.normal()};
+            w.full.setVelocity(i, v0[i]);
+        }
+        CHECK(settle.constrainVelocities(w.full) == 1);
+        std::vector<Vec3> vSettle(n);
+        for (std::size_t i = 0; i < n; ++i) vSettle[i] = w.full.velocity(i);
+        for (std::size_t i = 0; i < n; ++i) w.full.setVelocity(i, v0[i]);
+        CHECK(tight.constrainVelocities(w.full) > 1);
+        double worstV = 0.0, scale = 0.0;
+        for (std::size_t i = 0; i < n; ++i) {
+            worstV = std::max(worstV, norm(w.full.velocity(i) - vSettle[i]));
+            scale = std::max(scale, norm(v0[i]));
+        }
+        CHECK(worstV < 1e-13 * scale);
+    });
+
+    run("SETTLE: constraining an already-rigid molecule changes nothing", [&] {
+        auto w = loadWater();
+        const Settle settle = makeSettle();
+        const std::vector<Vec3> before = positionsOf(w.full);
+        settle.constrainPositions(w.full, before);
+        double worst = 0.0;
+        for (std::size_t i = 0; i < w.full.size(); ++i) worst = std::max(worst, norm(w.full.position(i) - before[i]));
+        CHECK(worst < 1e-5);  // the 6-digit file is rigid to ~1e-6; the result is rigid to rounding
+        CHECK(worstGeometryError(w.full, makeConstraint()) < 1e-14);
+    });
+
+    run("SETTLE: geometry is held over a trajectory under real forces", [&] {
+        auto w = loadWater();
+        const Settle settle = makeSettle();
+        const SpceForceField ff;
+        auto forces = [&](System& s) { ff.computeForces(s); };
+        forces(w.full);
+        ConstrainedVelocityVerlet integrator(units::femtoseconds(2.0), settle);
+        double worst = 0.0;
+        for (int step = 0; step < 50; ++step) {
+            integrator.step(w.full, forces);
+            worst = std::max(worst, worstGeometryError(w.full, makeConstraint()));
+        }
+        CHECK(worst < 1e-12);
+    });
+
+    run("SETTLE: energy error still scales as dt^2 with constraints active", [&] {
+        const std::vector<double> stepsFs = {0.5, 1.0, 2.0, 4.0};
+        const double totalTimeFs = 100.0;
+        std::vector<double> logDt, logErr;
+        for (double fs : stepsFs) {
+            auto w = loadWater();
+            const Settle settle = makeSettle();
+            const SpceParameters params;
+            const LennardJones lj(params.sigmaO, params.epsilonO, 9.0, Truncation::LinearForceShift);
+            auto forces = [&](System& sys) {
+                System ox(sys.box());
+                ox.reserve(sys.size() / 3);
+                for (std::size_t i = 0; i < sys.size(); i += 3) ox.addAtom(sys.position(i), Vec3{0, 0, 0}, params.massOxygen);
+                const double e = lj.computeForces(ox).energy;
+                sys.zeroForces();
+                for (std::size_t m = 0, o = 0; m < sys.size(); m += 3, ++o) sys.addForce(m, ox.force(o));
+                return e;
+            };
+            Rng rng(2026);
+            for (std::size_t i = 0; i < w.full.size(); ++i) {
+                const double sp = std::sqrt(298.0 / w.full.mass(i));
+                const double gx = rng.normal(), gy = rng.normal(), gz = rng.normal();
+                w.full.setVelocity(i, Vec3{sp * gx, sp * gy, sp * gz});
+            }
+            w.full.removeCenterOfMassMotion();
+            settle.constrainVelocities(w.full);
+            ConstrainedVelocityVerlet integrator(units::femtoseconds(fs), settle);
+            double pe = forces(w.full);
+            const double e0 = pe + w.full.kineticEnergy();
+            double maxDev = 0.0;
+            const int steps = static_cast<int>(std::lround(totalTimeFs / fs));
+            for (int k = 0; k < steps; ++k) {
+                integrator.step(w.full, forces);
+                pe = forces(w.full);
+                maxDev = std::max(maxDev, std::abs(pe + w.full.kineticEnergy() - e0));
+            }
+            logDt.push_back(std::log(fs));
+            logErr.push_back(std::log(maxDev / std::abs(e0)));
+        }
+        double mx = 0.0, my = 0.0;
+        for (std::size_t i = 0; i < logDt.size(); ++i) {
+            mx += logDt[i];
+            my += logErr[i];
+        }
+        mx /= logDt.size();
+        my /= logDt.size();
+        double num = 0.0, den = 0.0;
+        for (std::size_t i = 0; i < logDt.size(); ++i) {
+            num += (logDt[i] - mx) * (logErr[i] - my);
+            den += (logDt[i] - mx) * (logDt[i] - mx);
+        }
+        const double slope = num / den;
+        std::printf("  constrained conservation slope with SETTLE %.4f\n", slope);
+        CHECK(slope > 1.85 && slope < 2.15);
+    });
+
     // ---- tests/test_analysis.cpp:19-157, 228-307 ----
     run("g(r) of an ideal gas is 1 everywhere", [] {
         Box box(Vec3{20, 20, 20});

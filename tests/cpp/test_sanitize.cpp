@@ -117,6 +117,11 @@ static void water_passes() {
     EXPECT(rw.shake(s, ref) >= 0);
     EXPECT(rw.rattle(s) >= 0);
     cvv_step(s, 0.001, rw, [&](State& st) { return spce_forces(p, 9.0, 3, 0.0, st); });
+    Settle sw(1.0, 109.47);
+    for (int i = 0; i < 6; ++i) ref[i] = s.r(i);
+    EXPECT(sw.positions(s, ref) == 1);
+    EXPECT(sw.velocities(s) == 1);
+    cvv_step(s, 0.001, sw, [&](State& st) { return spce_forces(p, 9.0, 3, 0.0, st); });
 }
 
 static void library_host_paths() {
@@ -171,6 +176,9 @@ static void library_host_paths() {
     EXPECT(md_neigh_export_pairs(nullptr, nullptr, 0, nullptr) != MD_OK);
     EXPECT(md_batch_run(nullptr, 10, 10, 0, nullptr) != MD_OK);
     EXPECT(md_water_forces(nullptr, MD_FP64, nullptr, nullptr) != MD_OK);
+    EXPECT(md_water_settle(nullptr, x.data(), y.data(), z.data()) != MD_OK);
+    EXPECT(md_water_settle_velocities(nullptr) != MD_OK);
+    EXPECT(md_water_set_constraint(nullptr, MD_RIGID_SETTLE) != MD_OK);
     EXPECT(md_mix_forces(nullptr, nullptr, nullptr, nullptr) != MD_OK);
     EXPECT(md_rdf_accumulate(nullptr, nullptr, MD_RDF_AUTO) != MD_OK);
     EXPECT(md_msd_push(nullptr, nullptr) != MD_OK);

@@ -390,3 +390,40 @@ def test_water_config3_dynamics(gpu_ok, orc, c3):
     d = p - np.stack([wo.x, wo.y, wo.z], 1)
     d -= L * np.floor(d / L + 0.5)
     assert np.max(np.abs(d)) < 1e-10
+
+
+def test_water_config3_settle_follows_shake(gpu_ok, c3):
+    """Config 3 under SETTLE and under SHAKE/RATTLE on the device: the same
+    trajectory to the SHAKE tolerance over 20 steps, and the bonds held to
+    1e-12 after 200 SETTLE steps."""
+    from moleculardynamics_amd import native
+
+    L, pos, vel, alpha, kmax = c3
+    dt = 0.0018236739
+    out = {}
+    for method in (native.RIGID_SHAKE, native.RIGID_SETTLE):
+        w = native.Water([L] * 3, pos, vel,
+                         params=native.water_params(cutoff=9.0, kmax=kmax, alpha=alpha))
+        try:
+            w.set_constraint(method)
+            w.forces()
+            print("case: config 3, 20 steps, constraint", method, flush=True)
+            out[method] = w.run(20, dt) + (w.download()[0],)
+            if method == native.RIGID_SETTLE:
+                w.run(180, dt)
+                out["long"] = w.download()[0]
+        finally:
+            w.close()
+    (pa, ka, xa), (pb, kb, xb) = out[native.RIGID_SHAKE], out[native.RIGID_SETTLE]
+    assert np.max(np.abs(pa - pb) / np.abs(pa)) < 1e-9
+    assert np.max(np.abs(ka - kb) / np.abs(ka)) < 1e-9
+    d = xa - xb
+    d -= L * np.floor(d / L + 0.5)
+    assert np.max(np.abs(d)) < 1e-9
+    hh = 2.0 * np.sin(0.5 * np.deg2rad(109.47))
+    p = out["long"]
+    for a, b, t in ((0, 1, 1.0), (0, 2, 1.0), (1, 2, hh)):
+        e = p[a::3] - p[b::3]
+        e -= L * np.floor(e / L + 0.5)
+        assert np.max(np.abs(np.linalg.norm(e, axis=1) - t)) < 1e-12
+

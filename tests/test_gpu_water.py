@@ -1,6 +1,7 @@
 """Rigid SPC/E water with Ewald on the GPU (md_water_*) against NIST and the
 oracle: the 24 NIST energy components, Ewald and SPC/E forces, SHAKE/RATTLE,
-constrained dynamics, the config-3 builder and the Ewald parameter choice."""
+SETTLE (against the oracle's and against SHAKE/RATTLE), constrained dynamics
+with either, the config-3 builder and the Ewald parameter choice."""
 import json
 import os
 
@@ -130,6 +131,105 @@ def test_constrained_dynamics_match_oracle(gpu_ok, orc):
     p, _, _ = w.download()
     assert np.max(np.abs(p - np.stack([wo.x, wo.y, wo.z], 1))) < 1e-10
     w.close()
+
+
+def _rigid_nist1(orc):
+    fr = load_nist(nist_spce(1))
+    m = _masses(fr.n)
+    _, p = orc.rigid_shake(fr, m, fr, tol=1e-14, maxit=2000)
+    fr.x, fr.y, fr.z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+    return fr, m
+
+
+def test_settle_matches_oracle_and_shake(gpu_ok, orc):
+    """md_water_settle / md_water_settle_velocities (closed form) against the
+    oracle's SETTLE, and against the GPU's own SHAKE / RATTLE at their 1e-12
+    tolerance."""
+    from moleculardynamics_amd import native
+
+    fr, m = _rigid_nist1(orc)
+    rng = np.random.default_rng(21)
+    moved = fr.copy()
+    pm = _pos(fr) + rng.normal(size=(fr.n, 3)) * 0.05
+    moved.x, moved.y, moved.z = pm[:, 0].copy(), pm[:, 1].copy(), pm[:, 2].copy()
+    w = native.Water(moved.L, pm, params=native.water_params(cutoff=10.0, kmax=5))
+    try:
+        print("case: md_water_settle, NIST SPC/E 1, steps of 0.05 A", flush=True)
+        assert w.settle(_pos(fr)) == 1
+        p, _, _ = w.download()
+        _, p0 = orc.settle_positions(moved, m, fr)
+        assert np.max(np.abs(p - p0)) < 1e-12
+        w.set_state(pos=pm)
+        w.shake(_pos(fr))
+        ps, _, _ = w.download()
+        assert np.max(np.abs(p - ps)) < 1e-10
+        v = rng.normal(size=(fr.n, 3)) * np.sqrt(300.0 / m)[:, None]
+        w.set_state(pos=p0, vel=v)
+        print("case: md_water_settle_velocities", flush=True)
+        assert w.settle_velocities() == 1
+        _, vg, _ = w.download()
+        after = moved.copy()
+        after.x, after.y, after.z = p0[:, 0].copy(), p0[:, 1].copy(), p0[:, 2].copy()
+        after.vx, after.vy, after.vz = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+        _, v0 = orc.settle_velocities(after, m)
+        assert np.max(np.abs(vg - v0)) < 1e-13 * np.max(np.abs(v))
+        w.set_state(vel=v)
+        w.rattle()
+        _, vr, _ = w.download()
+        assert np.max(np.abs(vg - vr)) < 1e-10 * np.max(np.abs(v))
+    finally:
+        w.close()
+
+
+def test_settle_unsolvable_step_is_an_error(gpu_ok, orc):
+    from moleculardynamics_amd import native
+
+    fr, m = _rigid_nist1(orc)
+    p0 = _pos(fr)
+    d1, d2 = p0[1] - p0[0], p0[2] - p0[0]
+    nrm = np.cross(d1, d2)
+    p = p0.copy()
+    p[1] += 3.0 * nrm / np.linalg.norm(nrm)
+    w = native.Water(fr.L, p, params=native.water_params(cutoff=10.0, kmax=5))
+    try:
+        with pytest.raises(native.MDError) as ei:
+            w.settle(p0)
+        assert ei.value.status == native.MD_ERR_RUNTIME
+        assert "molecule 0" in str(ei.value)
+        with pytest.raises(native.MDError) as ei:
+            w.set_constraint(7)
+        assert ei.value.status == native.MD_ERR_INVALID_ARGUMENT
+    finally:
+        w.close()
+
+
+def test_settle_dynamics_match_oracle(gpu_ok, orc):
+    """md_water_run with SETTLE (two per-molecule kernels per step) against
+    the oracle's constrained velocity Verlet with SETTLE and with SHAKE."""
+    from moleculardynamics_amd import native
+
+    fr, m = _rigid_nist1(orc)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(fr.n, 3)) * np.sqrt(298.0 / m)[:, None]
+    v -= (v * m[:, None]).sum(0) / m.sum()
+    fr.vx, fr.vy, fr.vz = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    _, vr = orc.settle_velocities(fr, m)
+    fr.vx, fr.vy, fr.vz = vr[:, 0].copy(), vr[:, 1].copy(), vr[:, 2].copy()
+    dt = 0.0018236739  # 2 fs in units of 1.0966875353 ps
+    wo, pe0, ke0 = orc.water_run(fr, dt=dt, nsteps=10, settle=True)
+    _, pe1, _ = orc.water_run(fr, dt=dt, nsteps=10, settle=False)
+    w = native.Water(fr.L, _pos(fr), vr, params=native.water_params(cutoff=10.0, kmax=5))
+    try:
+        w.set_constraint(native.RIGID_SETTLE)
+        w.forces()
+        pe, ke = w.run(10, dt)
+        p, _, _ = w.download()
+    finally:
+        w.close()
+    assert np.max(np.abs(pe - pe0) / np.abs(pe0)) < 1e-10
+    assert np.max(np.abs(ke - ke0) / np.abs(ke0)) < 1e-10
+    assert np.max(np.abs(pe - pe1) / np.abs(pe1)) < 1e-10
+    assert np.max(np.abs(p - np.stack([wo.x, wo.y, wo.z], 1))) < 1e-10
 
 
 def test_config3_builder_and_ewald_parameters(gpu_ok):
