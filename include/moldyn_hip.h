@@ -179,6 +179,18 @@ md_status md_neigh_set_capacity(md_ctx* ctx, int per_atom);
  * md_run makes md_run restore its starting state, double the capacity and
  * run again. */
 int md_neigh_auto_capacity(long n, const double* L, double cutoff, double skin);
+/* Bricks of list cells for the LDS-staged pair kernels: cells per brick side
+ * (1 .. 8), -1 automatic (4), 0 off (the default). With bricks on, each
+ * pair-kernel workgroup stages the positions of one brick's two-cell halo
+ * in LDS and the list rows index that halo. They apply to systems above
+ * 32,768 atoms with fixed-point positions whose grid has at least the brick
+ * width plus 4 cells per axis; the pair set, the forces, the mixed-path
+ * energies and virials are the same bits with and without them. No
+ * reference counterpart (a device data layout); the list must be built
+ * again afterwards. */
+md_status md_neigh_set_bricks(md_ctx* ctx, int cells);
+/* Number of bricks of the current list layout (0 when bricks are off). */
+md_status md_neigh_bricks(const md_ctx* ctx, int* count);
 
 /* ---- integrators ---------------------------------------------------------- */
 

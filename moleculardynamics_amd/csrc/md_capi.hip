@@ -49,15 +49,24 @@ struct md_ctx {
     long npad = 0;
     int cap = 0, cap_user = 0;
     long nbr_alloc = 0;
+    // bricks of list cells for the LDS-staged pair kernels (mdk::Bricks):
+    // brick_width 0 off (the default), -1 automatic, else cells per brick
+    // side; halo_boost grows the LDS rows after a build found a halo above them
+    int brick_width = 0;
+    double halo_boost = 1.0;
+    mdk::Bricks bk{};
+    int *brick_off = nullptr, *brick_tab = nullptr;
+    long brick_off_alloc = 0, brick_tab_alloc = 0;
 
     // cells at the LJ cutoff (MD_PAIRS_CELLS)
     int *ci_cell_of = nullptr, *ci_count = nullptr, *ci_start = nullptr, *ci_fill = nullptr,
         *ci_atoms = nullptr, *ci_tmp = nullptr;
     long ci_ncell_alloc = 0;
 
-    // reductions and device scalars
+    // reductions and device scalars (partial slots for part_alloc workgroups)
     double* part_d = nullptr;
     long long* part_l = nullptr;
+    long part_alloc = 0;
     mdk::Scalars* sc = nullptr;
     mdk::Chain* chain = nullptr;
     double* com4 = nullptr;
@@ -207,6 +216,73 @@ int capacity_for(long n, const double* L, double rc, double skin) {
 
 int auto_capacity(const md_ctx* c) { return capacity_for(c->n, c->L, c->rc, c->skin); }
 
+// Bricks of list cells for the LDS-staged pair kernels (mdk::Bricks), when
+// requested with md_neigh_set_bricks. Used for systems above 32,768 atoms
+// (below, the pair kernels run several lanes per atom over the whole grid)
+// with fixed-point positions, when the grid takes bricks of at most the
+// chosen width (4 cells for automatic) with their two-cell halo inside the
+// box on every axis (width + 4 <= cells per axis). The LDS rows hold 1.15
+// times the expected halo (times halo_boost) plus 64 atoms, at most 1,904
+// (the FP64 kernel's 32-byte positions and its static arrays within 64 KB of
+// LDS); denser grids keep slot entries. Waves per workgroup: the expected
+// own atoms (times 4 lanes for FP64) over 64, plus 2 %.
+constexpr long kBrickMinAtoms = 32769;
+constexpr int kBrickAutoWidth = 4;
+constexpr double kBrickMaxHalo = 1904.0;
+
+md_status setup_bricks(md_ctx* c, bool fixed_point) {
+    c->bk = mdk::Bricks{};
+    if (!fixed_point || c->n < kBrickMinAtoms || c->brick_width == 0) return MD_OK;
+    const int want = c->brick_width > 0 ? c->brick_width : kBrickAutoWidth;
+    mdk::Bricks k{};
+    int wmax[3];
+    for (int d = 0; d < 3; ++d) {
+        const int nc = c->grid.nc[d];
+        k.kb[d] = std::max(1, (nc + want - 1) / want);  // no brick wider than want
+        wmax[d] = (nc + k.kb[d] - 1) / k.kb[d];
+        if (wmax[d] + 4 > nc) return MD_OK;  // the halo would wrap onto itself
+    }
+    if (wmax[1] * wmax[2] > 64) return MD_OK;  // own rows of one wave
+    if ((wmax[1] + 4) * (wmax[2] + 4) > 160) return MD_OK;  // halo rows of the copy plan
+    const double per_cell = static_cast<double>(c->n) / static_cast<double>(c->ncell);
+    const double halo = per_cell * (wmax[0] + 4) * (wmax[1] + 4) * (wmax[2] + 4);
+    const double rows = std::ceil(1.15 * c->halo_boost * halo) + 64.0;
+    if (rows > kBrickMaxHalo) return MD_OK;
+    k.halo_cap = static_cast<int>(rows);
+    const double own = per_cell * wmax[0] * wmax[1] * wmax[2];
+    k.waves_mixed = std::min(8, std::max(1, static_cast<int>(std::ceil(1.02 * own / 64.0))));
+    k.waves_fp64 = std::min(16, std::max(1, static_cast<int>(std::ceil(1.02 * 4.0 * own / 64.0))));
+    k.nb = k.kb[0] * k.kb[1] * k.kb[2];
+    std::vector<int> off(static_cast<size_t>(k.nb) + 1);
+    long tot = 0;
+    for (int b = 0; b < k.nb; ++b) {
+        const mdk::BrickGeo o = mdk::brick_geo(k, c->grid, b);
+        off[b] = static_cast<int>(tot);
+        tot += static_cast<long>(o.ex + 4) * (o.ey + 4) * (o.ez + 4) + 1;
+        if (tot >= (1L << 31)) return MD_OK;
+    }
+    off[k.nb] = static_cast<int>(tot);
+    if (k.nb + 1 > c->brick_off_alloc) {
+        MD_HIP(c, alloc(c->brick_off, k.nb + 1));
+        c->brick_off_alloc = k.nb + 1;
+    }
+    if (tot > c->brick_tab_alloc) {
+        MD_HIP(c, alloc(c->brick_tab, tot));
+        c->brick_tab_alloc = tot;
+    }
+    MD_HIP(c, hipMemcpy(c->brick_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (k.nb > c->part_alloc) {  // one partial slot per brick workgroup
+        MD_HIP(c, alloc(c->part_d, 4L * k.nb + 8));
+        MD_HIP(c, alloc(c->part_l, 3L * k.nb + 8));
+        c->part_alloc = k.nb;
+    }
+    k.tab_off = c->brick_off;
+    k.tab = c->brick_tab;
+    k.on = 1;
+    c->bk = k;
+    return MD_OK;
+}
+
 md_status alloc_neigh(md_ctx* c) {
     c->grid = make_list_grid(c->L, c->rc + c->skin);
     c->ncell = c->grid.usable ? static_cast<long>(c->grid.nc[0]) * c->grid.nc[1] * c->grid.nc[2] : 1;
@@ -246,6 +322,8 @@ md_status alloc_neigh(md_ctx* c) {
         release(c->qpos2);
         c->quant = mdk::Quant{};
     }
+    md_status bst = setup_bricks(c, fixed_point);
+    if (bst != MD_OK) return bst;
     c->npad = (c->n + 63) / 64 * 64;
     c->cap = c->cap_user > 0 ? (c->cap_user + 7) / 8 * 8 : auto_capacity(c);
     const long need = static_cast<long>(c->cap) * c->npad;
@@ -260,6 +338,7 @@ mdk::NeighArgs neigh_args(md_ctx* c) {
     mdk::NeighArgs a{};
     a.n = c->n;
     a.grid = c->grid;
+    a.bk = c->bk;
     a.box = c->box;
     a.rl2 = (c->rc + c->skin) * (c->rc + c->skin);
     a.pos = c->pos; a.vel = c->vel; a.frc = c->frc; a.ref = c->ref;
@@ -286,6 +365,9 @@ mdk::ForceArgs force_args(md_ctx* c, int precision) {
     // the mixed kernel reads the quantised positions when the list keeps them
     a.qpos = (a.mixed && c->qpos != nullptr) ? c->qpos : nullptr;
     a.quant = c->quant;
+    a.grid = c->grid;
+    a.bk = c->bk;
+    a.cell_start = c->cell_start;
     // mixed-term constants: lengths in fixed-point units on a cubic box with
     // fixed-point positions, real lengths otherwise
     const double s = (c->qpos != nullptr && c->quant.cubic) ? c->quant.unit[0] : 1.0;
@@ -334,12 +416,14 @@ md_status build_now(md_ctx* c) {
             c->list_built = true;
             return MD_OK;
         }
-        // grow and retry (the build counter is restored so one build counts once)
+        // grow and retry (the build counter is restored so one build counts once):
+        // bit 1 a row longer than the capacity, bit 2 a brick halo above the LDS rows
         int zero = 0;
         long long builds = s.builds - 1;
         MD_HIP(c, hipMemcpyAsync(&c->sc->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
         MD_HIP(c, hipMemcpyAsync(&c->sc->builds, &builds, sizeof(long long), hipMemcpyHostToDevice, c->stream));
-        c->cap_user = std::min<long>(2L * c->cap, std::max(1L, c->n - 1));
+        if (s.overflow & 1) c->cap_user = std::min<long>(2L * c->cap, std::max(1L, c->n - 1));
+        if (s.overflow & 2) c->halo_boost *= 1.5;
         st = alloc_neigh(c);
         if (st != MD_OK) return st;
     }
@@ -465,6 +549,7 @@ md_status md_ctx_destroy(md_ctx* c) {
     release(c->tile_sum);
     release(c->qpos); release(c->qpos2);
     release(c->nbr); release(c->ncnt);
+    release(c->brick_off); release(c->brick_tab);
     release(c->ci_cell_of); release(c->ci_count); release(c->ci_start); release(c->ci_fill);
     release(c->ci_atoms); release(c->ci_tmp);
     release(c->part_d); release(c->part_l); release(c->sc); release(c->chain); release(c->com4);
@@ -513,6 +598,7 @@ md_status md_system_upload(md_ctx* c, long n, const double* L, const double* x, 
     const long nb = std::max(blocks_of(n), mdk::lj_partials(n));
     MD_HIP(c, alloc(c->part_d, 4 * nb + 8));
     MD_HIP(c, alloc(c->part_l, 3 * nb + 8));
+    c->part_alloc = nb;
 
     std::vector<double4> p(static_cast<size_t>(n)), v(static_cast<size_t>(n)), f(static_cast<size_t>(n));
     std::vector<int> id(static_cast<size_t>(n));
@@ -812,7 +898,20 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
     MD_HIP(c, hipStreamSynchronize(c->stream));
     MD_HIP(c, hipMemcpy(cnt.data(), c->ncnt, n * sizeof(int), hipMemcpyDeviceToHost));
     MD_HIP(c, hipMemcpy(orig.data(), c->orig, n * sizeof(int), hipMemcpyDeviceToHost));
-    MD_HIP(c, hipMemcpy(nb.data(), c->nbr, nb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (c->bk.on) {
+        // brick entries are halo indices: translated to slot entries on the device
+        uint32_t* tr = nullptr;
+        MD_HIP(c, hipMalloc(reinterpret_cast<void**>(&tr), nb.size() * sizeof(uint32_t)));
+        mdk::launch_brick_export(force_args(c, MD_FP64), tr, c->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy(nb.data(), tr, nb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        (void)hipFree(tr);
+        MD_HIP(c, e);
+    } else {
+        MD_HIP(c, hipMemcpy(nb.data(), c->nbr, nb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
     std::vector<uint64_t> keys;
     keys.reserve(static_cast<size_t>(n) * 40);
     for (long t = 0; t < n; ++t) {
@@ -832,6 +931,21 @@ md_status md_neigh_export_pairs(md_ctx* c, uint32_t* ij, long cap, long* npairs)
         ij[2 * k] = static_cast<uint32_t>(keys[k] >> 32);
         ij[2 * k + 1] = static_cast<uint32_t>(keys[k] & 0xffffffffu);
     }
+    return MD_OK;
+}
+
+md_status md_neigh_set_bricks(md_ctx* c, int cells) {
+    if (c == nullptr || cells < -1 || cells > 8) return MD_ERR_INVALID_ARGUMENT;
+    c->brick_width = cells;
+    c->halo_boost = 1.0;
+    c->list_built = false;
+    if (c->n > 0 && c->neigh_configured) return alloc_neigh(c);
+    return MD_OK;
+}
+
+md_status md_neigh_bricks(const md_ctx* c, int* count) {
+    if (c == nullptr || count == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    *count = c->bk.on ? c->bk.nb : 0;
     return MD_OK;
 }
 
@@ -880,13 +994,14 @@ md_status md_integrator_set(md_ctx* c, const md_integrator* it) {
 constexpr long kGraphSegment = 32;
 constexpr long kGraphMaxAtoms = 262144;
 
-// The steps of one md_run attempt; *overflowed reports a neighbour row
-// longer than the capacity at some build of the attempt (the state is then
-// not meaningful and md_run restores its snapshot).
+// The steps of one md_run attempt; *overflowed reports the overflow bits of
+// the attempt's builds (1: a neighbour row longer than the capacity, 2: a
+// brick halo above the LDS rows; the state is then not meaningful and md_run
+// restores its snapshot).
 static md_status run_steps(md_ctx* c, long nsteps, int precision, md_thermo* thermo,
-                           bool* overflowed) {
+                           int* overflowed) {
     md_status st = MD_OK;
-    *overflowed = false;
+    *overflowed = 0;
     if (thermo && nsteps > c->thermo_alloc) {
         MD_HIP(c, alloc(c->thermo_dev, nsteps));
         c->thermo_alloc = nsteps;
@@ -989,7 +1104,7 @@ static md_status run_steps(md_ctx* c, long nsteps, int precision, md_thermo* the
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (st != MD_OK) return st;
     if (s.overflow) {
-        *overflowed = true;
+        *overflowed = s.overflow;
         return MD_OK;
     }
     if (s.fix_saturated)
@@ -999,12 +1114,14 @@ static md_status run_steps(md_ctx* c, long nsteps, int precision, md_thermo* the
     return MD_OK;
 }
 
-// A neighbour row longer than the capacity stops the work of the remaining
-// steps (the pre-force pass and the pair kernel return at once once the
-// device flag is set). md_run then restores the state it started from,
-// doubles the capacity, builds the list again and runs the whole call again
-// (at most three times), so a caller never sees a state advanced with a
-// truncated list. The run that completes holds one extra list build.
+// A neighbour row longer than the capacity (or a brick halo above the LDS
+// rows) stops the work of the remaining steps (the pre-force pass and the
+// pair kernel return at once once the device flag is set). md_run then
+// restores the state it started from, doubles the capacity (or grows the
+// LDS rows, turning the bricks off past 64 KB), builds the list again and
+// runs the whole call again (at most three times), so a caller never sees a
+// state advanced with a truncated list. The run that completes holds one
+// extra list build.
 md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     md_status st = need_system(c);
     if (st != MD_OK) return st;
@@ -1015,14 +1132,15 @@ md_status md_run(md_ctx* c, long nsteps, int precision, md_thermo* thermo) {
     st = save_snapshot(c);
     if (st != MD_OK) return st;
     for (int attempt = 0;; ++attempt) {
-        bool over = false;
+        int over = 0;
         st = run_steps(c, nsteps, precision, thermo, &over);
         if (st != MD_OK || !over) return st;
-        if (attempt == 3 || c->cap >= c->n - 1)
+        if (attempt == 3 || ((over & 1) && c->cap >= c->n - 1))
             return fail(c, MD_ERR_CAPACITY, "neighbour row capacity could not be satisfied in md_run");
         st = restore_snapshot(c);
         if (st != MD_OK) return st;
-        c->cap_user = static_cast<int>(std::min<long>(2L * c->cap, std::max(1L, c->n - 1)));
+        if (over & 1) c->cap_user = static_cast<int>(std::min<long>(2L * c->cap, std::max(1L, c->n - 1)));
+        if (over & 2) c->halo_boost *= 1.5;
         st = alloc_neigh(c);
         if (st != MD_OK) return st;
         st = build_now(c);

@@ -386,6 +386,27 @@ __device__ __forceinline__ void mix_fixed(const ForceArgs& a, const int4& qi, co
     mix_term<LFS>(a.mq, dx, dy, dz, r2, in, q);
 }
 
+// mix_fixed for an entry of a row's last octet, which may lie past the row
+// end (live false: it enters as outside the cutoff and outside the band).
+template <bool LFS, bool CUBIC>
+__device__ __forceinline__ void mix_fixed_masked(const ForceArgs& a, const int4& qi, const int4& qj,
+                                                 bool live, AccMix& q, bool& hit) {
+    double dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+    double dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+    double dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+    if (!CUBIC) {
+        dx *= a.quant.unit[0];
+        dy *= a.quant.unit[1];
+        dz *= a.quant.unit[2];
+    }
+    // a dead entry may read the row's own atom (r2 = 0): it enters with
+    // r2 = 1 so that its zero terms stay finite
+    const double r2 = live ? fma(dz, dz, fma(dy, dy, dx * dx)) : 1.0;
+    const bool in = live && r2 < a.mq.lo;
+    hit = hit || (live && !in && r2 < a.mq.hi);
+    mix_term<LFS>(a.mq, dx, dy, dz, r2, in, q);
+}
+
 // Entry from the FP64 positions with the reference's exact cutoff test (the
 // FP64 kernel's decision), in the units of m.
 template <bool LFS>
@@ -529,12 +550,12 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
 // block_reduce, so the same bits), for the values the precision uses (FP64:
 // E, W, KE and the term count; mixed: the fixed-point E, W, the term count
 // and KE); the unused slots are written as 0.
-template <bool MIXED>
+template <bool MIXED, int WAVES = kBlock / kWave>
 __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, double e, double w,
                                                double ke, long long ef, long long wf,
                                                long long nt) {
-    __shared__ double xd[3][kBlock / kWave];
-    __shared__ long long xl[3][kBlock / kWave];
+    __shared__ double xd[3][WAVES];
+    __shared__ long long xl[3][WAVES];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const double v0 = MIXED ? 0.0 : wave_sum(e);
     const double v1 = MIXED ? 0.0 : wave_sum(w);
@@ -551,7 +572,7 @@ __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, doubl
         for (int q = 0; q < 3; ++q) {
             double td = 0.0;
             long long tl = 0;
-            for (int k = 0; k < kBlock / kWave; ++k) {
+            for (int k = 0; k < static_cast<int>(blockDim.x / kWave); ++k) {
                 td += xd[q][k];
                 tl += xl[q][k];
             }
@@ -735,6 +756,330 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         ef = wf = nt = 0;
     }
     write_partials<MIXED>(a, blockIdx.x, e, w, ke, ef, wf, nt);
+}
+
+
+// ---- bricks: partners staged in LDS ----
+//
+// One workgroup per brick of list cells (Bricks). It first copies its halo
+// -- every atom within two cells of the brick, in the halo order the list
+// build indexed -- from HBM into LDS, then walks the rows of its own atoms,
+// whose entries are halo indices: every partner read is an LDS read instead
+// of a gather through the vector L1. The halo rows are contiguous slot
+// ranges (two where a row wraps in x), copied by one wave each with
+// consecutive lanes on consecutive slots. The rows, their order and the
+// arithmetic are those of k_lj (mixed) and k_lj_lanes (FP64), so forces,
+// energies and virials are bit for bit the same (FP64 E and W: the same up
+// to the grouping of the workgroup sums).
+constexpr int kBrickMaxWaves = 8;
+
+__device__ __forceinline__ int wave_incl_sum(int x, int lane) {
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Halo copy plan: per halo row, its first halo index, its first slot and
+// length before a wrap in x, and its first slot after the wrap.
+constexpr int kMaxHaloRows = 160;
+struct HaloRows {
+    int dst[kMaxHaloRows + 1];
+    int a0[kMaxHaloRows];
+    int la[kMaxHaloRows];
+    int b0[kMaxHaloRows];
+};
+
+// Copies the brick's halo into LDS: store(k, load(g)) for halo index k and
+// its slot g. The row plans are formed by all threads at once; then every
+// thread copies halo indices tid, tid + blockDim, ... (consecutive lanes on
+// consecutive slots of a row), four loads in flight before the stores. Ends
+// with a barrier.
+template <class T, class Load, class Store>
+__device__ __forceinline__ void brick_stage(const ForceArgs& a, const BrickGeo& o, const int* tb,
+                                            HaloRows& hr, Load load, Store store) {
+    const int nx = a.grid.nc[0], ny = a.grid.nc[1], nz = a.grid.nc[2];
+    const int* __restrict__ cs = a.cell_start;
+    const int hxn = o.ex + 4, hyn = o.ey + 4, rows = hyn * (o.ez + 4);
+    const int xs = o.x0 - 2, xe = o.x0 + o.ex + 2;  // unwrapped cells [xs, xe) of each row
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        int y = o.y0 - 2 + r % hyn, z = o.z0 - 2 + r / hyn;
+        y += y < 0 ? ny : (y >= ny ? -ny : 0);
+        z += z < 0 ? nz : (z >= nz ? -nz : 0);
+        const int rb = nx * (y + ny * z);
+        // the row's first piece, then (when it wraps) the rest from cell 0
+        const int a0 = cs[xs < 0 ? rb + nx + xs : rb + xs];
+        hr.a0[r] = a0;
+        hr.la[r] = cs[xs < 0 ? rb + nx : rb + min(xe, nx)] - a0;
+        hr.b0[r] = cs[rb];
+        hr.dst[r] = tb[r * hxn];
+    }
+    if (threadIdx.x == 0) hr.dst[rows] = tb[rows * hxn];
+    __syncthreads();
+    const int total = hr.dst[rows];
+    auto slot_of = [&](int k) {
+        int lo = 0, hi = rows - 1;  // the last row r with dst[r] <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (hr.dst[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int off = k - hr.dst[lo];
+        return off < hr.la[lo] ? hr.a0[lo] + off : hr.b0[lo] + (off - hr.la[lo]);
+    };
+    const int step = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < total; k0 += 4 * step) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * step;
+            if (k < total) v[u] = load(slot_of(k));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * step;
+            if (k < total) store(k, v[u]);
+        }
+    }
+    __syncthreads();
+}
+
+// The brick's own rows (cells x0 .. x0+ex-1 of each (y, z) row, never
+// wrapped): own atom k lies in row r with pre[r] <= k < pre[r + 1], at slot
+// g0[r] + k - pre[r] and halo index l0[r] + k - pre[r]. Wave 0 fills it.
+struct OwnRows {
+    int pre[kWave + 1];
+    int g0[kWave];
+    int l0[kWave];
+};
+
+__device__ __forceinline__ void brick_own(const ForceArgs& a, const BrickGeo& o, const int* tb,
+                                          OwnRows& s) {
+    if (threadIdx.x >= kWave) return;
+    const int lane = threadIdx.x;
+    const int nr = o.ey * o.ez;  // at most 64 (brick widths of at most 8 cells)
+    int cnt = 0;
+    if (lane < nr) {
+        const int oy = lane % o.ey, oz = lane / o.ey;
+        const int h = ((oz + 2) * (o.ey + 4) + oy + 2) * (o.ex + 4) + 2;
+        const int l0 = tb[h];
+        cnt = tb[h + o.ex] - l0;
+        s.l0[lane] = l0;
+        s.g0[lane] = a.cell_start[a.grid.nc[0] * ((o.y0 + oy) + a.grid.nc[1] * (o.z0 + oz)) + o.x0];
+    }
+    const int incl = wave_incl_sum(cnt, lane);
+    if (lane < nr) s.pre[lane] = incl - cnt;
+    if (lane == nr - 1) s.pre[nr] = incl;
+}
+
+__device__ __forceinline__ int own_row(const OwnRows& s, int nr, int k) {
+    int lo = 0, hi = nr - 1;  // the last row r with pre[r] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.pre[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Mixed: the halo holds (X, Y, Z, slot) per atom, 16 bytes; one lane per
+// own atom walks its row as k_lj does (same entries in the same order, so
+// the same bits). Entries of a row's last octet past its end read halo atom
+// 0 and enter as outside the cutoff.
+template <bool LFS, bool CUBIC, bool KICK>
+__global__ __launch_bounds__(kBrickMaxWaves * kWave) void k_lj_brick(ForceArgs a) {
+    extern __shared__ int4 halo[];
+    __shared__ OwnRows own;
+    __shared__ HaloRows hr;
+    const long b = xcd_block();
+    if (b >= a.bk.nb) return;      // padding block of the XCD-aware grid (whole block)
+    if (a.sc->overflow) return;    // a truncated list was built: md_run restores and retries
+    const BrickGeo o = brick_geo(a.bk, a.grid, static_cast<int>(b));
+    const int* tb = a.bk.tab + a.bk.tab_off[b];
+    brick_own(a, o, tb, own);
+    brick_stage<int4>(
+        a, o, tb, hr,
+        [&](int g) {
+            const int4 q = a.qpos[g];
+            return make_int4(q.x, q.y, q.z, g);
+        },
+        [&](int k, const int4& v) { halo[k] = v; });
+    const int nr = o.ey * o.ez;
+    const int nown = own.pre[nr];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const MixK& m = a.mq;
+    double ke = 0.0;
+    long long ef = 0, wf = 0, nt = 0;
+    for (int c = wv; c * kWave < nown; c += nw) {
+        const int k = c * kWave + lane;
+        if (k >= nown) break;
+        const int r = own_row(own, nr, k);
+        const int i = own.g0[r] + (k - own.pre[r]);
+        const int4 qi = halo[own.l0[r] + (k - own.pre[r])];
+        const int cnt = min(a.ncnt[i], a.cap);
+        AccMix q;
+        bool hit = false;
+        // octets with the next one's indices in flight, partners four at a
+        // time from LDS
+        const uint32_t* row = a.nbr + oct_index(i, 0, a.cap);
+        const int noct = (cnt + 7) >> 3;
+        u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
+        if (noct > 0) load_octet(row, lo, hi);
+#pragma unroll 1
+        for (int g = 0; g + 1 < noct; ++g) {
+            u32x4 nlo, nhi;
+            load_octet(row + (g + 1) * 512, nlo, nhi);
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                const u32x4 j = h ? hi : lo;
+                const int4 p0 = halo[j.x], p1 = halo[j.y], p2 = halo[j.z], p3 = halo[j.w];
+                mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
+                mix_fixed<LFS, CUBIC>(a, qi, p1, q, hit);
+                mix_fixed<LFS, CUBIC>(a, qi, p2, q, hit);
+                mix_fixed<LFS, CUBIC>(a, qi, p3, q, hit);
+            }
+            lo = nlo;
+            hi = nhi;
+        }
+        if (noct > 0) {  // the last octet, past-the-end entries masked
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                const u32x4 j = h ? hi : lo;
+                const int base = 8 * (noct - 1) + 4 * h;
+                const uint32_t e0 = base < cnt ? j.x : 0u;
+                const uint32_t e1 = base + 1 < cnt ? j.y : 0u;
+                const uint32_t e2 = base + 2 < cnt ? j.z : 0u;
+                const uint32_t e3 = base + 3 < cnt ? j.w : 0u;
+                const int4 p0 = halo[e0], p1 = halo[e1], p2 = halo[e2], p3 = halo[e3];
+                mix_fixed_masked<LFS, CUBIC>(a, qi, p0, base < cnt, q, hit);
+                mix_fixed_masked<LFS, CUBIC>(a, qi, p1, base + 1 < cnt, q, hit);
+                mix_fixed_masked<LFS, CUBIC>(a, qi, p2, base + 2 < cnt, q, hit);
+                mix_fixed_masked<LFS, CUBIC>(a, qi, p3, base + 3 < cnt, q, hit);
+            }
+        }
+        if (hit) {  // rare: a decision inside the band, the row again exactly
+            q = AccMix{};
+            const double4 pi = a.pos[i];
+#pragma unroll 1
+            for (int k2 = 0; k2 < cnt; ++k2) {
+                const uint32_t e = row[(k2 >> 3) * 512 + (k2 & 7)];
+                mix_exact<LFS>(a, m, pi, a.pos[halo[e].w], q);
+            }
+        }
+        bool sat = false;
+        const FixRow fr = mix_out<LFS>(m, q, sat);
+        if (sat) atomicOr(&a.sc->fix_saturated, 1);
+        const double fx = static_cast<double>(fr.fx) * kFixInv;
+        const double fy = static_cast<double>(fr.fy) * kFixInv;
+        const double fz = static_cast<double>(fr.fz) * kFixInv;
+        ef += fr.e;
+        wf += fr.w;
+        nt += q.k;
+        a.frc[i] = make_double4(fx, fy, fz, 0.0);
+        if (KICK) {
+            double4 v = a.vel[i];
+            const double ch = a.half_dt * (1.0 / v.w);
+            v.x = fma(ch, fx, v.x);
+            v.y = fma(ch, fy, v.y);
+            v.z = fma(ch, fz, v.z);
+            a.vel[i] = v;
+            ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+        }
+    }
+    write_partials<true, kBrickMaxWaves>(a, b, 0.0, 0.0, ke, ef, wf, nt);
+}
+
+// FP64: the halo holds each atom's FP64 position (double4 with the upload
+// index in w, 32 bytes); kFp64Lanes lanes per own atom as k_lj_lanes, so the
+// forces are k_lj_lanes' bit for bit.
+constexpr int kBrick64MaxWaves = 16;
+
+template <bool LFS, bool KICK>
+__global__ __launch_bounds__(kBrick64MaxWaves * kWave) void k_lj_brick64(ForceArgs a) {
+    constexpr int LANES = kFp64Lanes;
+    extern __shared__ double4 hpos[];
+    __shared__ OwnRows own;
+    __shared__ HaloRows hr;
+    const long b = xcd_block();
+    if (b >= a.bk.nb) return;
+    if (a.sc->overflow) return;
+    const BrickGeo o = brick_geo(a.bk, a.grid, static_cast<int>(b));
+    const int* tb = a.bk.tab + a.bk.tab_off[b];
+    brick_own(a, o, tb, own);
+    brick_stage<double4>(
+        a, o, tb, hr, [&](int g) { return a.pos[g]; },
+        [&](int k, const double4& v) { hpos[k] = v; });
+    const int nr = o.ey * o.ez;
+    const int nown = own.pre[nr];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const int sub = lane & (LANES - 1);
+    constexpr int kPer = kWave / LANES;  // own atoms per wave chunk
+    double e = 0.0, w = 0.0, ke = 0.0;
+    long long nt = 0;
+    for (int c = wv; c * kPer < nown; c += nw) {
+        const int k = c * kPer + lane / LANES;
+        const bool live = k < nown;
+        const int kk = live ? k : 0;
+        const int r = own_row(own, nr, kk);
+        const int i = own.g0[r] + (kk - own.pre[r]);
+        const int cnt = live ? min(a.ncnt[i], a.cap) : 0;
+        const uint32_t* row = a.nbr + oct_index(i, 0, a.cap);
+        Acc64 q;
+        const double4 pi = hpos[own.l0[r] + (kk - own.pre[r])];
+        for (int j = sub; j < cnt; j += LANES)
+            pair64<LFS>(a, pi, hpos[row[(j >> 3) * 512 + (j & 7)]], q);
+        const double fx = group_sum<LANES>(q.fx);
+        const double fy = group_sum<LANES>(q.fy);
+        const double fz = group_sum<LANES>(q.fz);
+        const double qe = group_sum<LANES>(q.e);
+        const double qw = group_sum<LANES>(q.w);
+        const long long qn = group_sum<LANES>(static_cast<long long>(q.n));
+        if (live && sub == 0) {
+            e += qe;
+            w += qw;
+            nt += qn;
+            a.frc[i] = make_double4(fx, fy, fz, 0.0);
+            if (KICK) {
+                double4 v = a.vel[i];
+                const double ch = a.half_dt * (1.0 / v.w);
+                v.x = fma(ch, fx, v.x);
+                v.y = fma(ch, fy, v.y);
+                v.z = fma(ch, fz, v.z);
+                a.vel[i] = v;
+                ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+            }
+        }
+    }
+    write_partials<false, kBrick64MaxWaves>(a, b, e, w, ke, 0, 0, nt);
+}
+
+// Translation of a brick list to slot entries (entry_of), same layout, for
+// md_neigh_export_pairs.
+__global__ __launch_bounds__(kBrickMaxWaves * kWave) void k_brick_export(ForceArgs a, uint32_t* out) {
+    extern __shared__ int hslot[];
+    __shared__ OwnRows own;
+    __shared__ HaloRows hr;
+    const long b = xcd_block();
+    if (b >= a.bk.nb) return;
+    const BrickGeo o = brick_geo(a.bk, a.grid, static_cast<int>(b));
+    const int* tb = a.bk.tab + a.bk.tab_off[b];
+    brick_own(a, o, tb, own);
+    brick_stage<int>(
+        a, o, tb, hr, [&](int g) { return g; }, [&](int k, int g) { hslot[k] = g; });
+    const int nr = o.ey * o.ez;
+    const int nown = own.pre[nr];
+    for (int k = threadIdx.x; k < nown; k += blockDim.x) {
+        const int r = own_row(own, nr, k);
+        const int i = own.g0[r] + (k - own.pre[r]);
+        const int cnt = min(a.ncnt[i], a.cap);
+        for (int j = 0; j < cnt; ++j) {
+            const long at = oct_index(i, j, a.cap);
+            out[at] = entry_of(hslot[a.nbr[at]]);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
@@ -985,7 +1330,44 @@ void launch_chain_first(Chain* chain, Scalars* sc, double delta, hipStream_t s) 
 
 long lj_partials(long n) { return std::max(lj_groups(n, false), lj_groups(n, true)); }
 
+inline unsigned brick_grid(const Bricks& k) {
+    return static_cast<unsigned>((k.nb + kXcd - 1) / kXcd * kXcd);
+}
+
+template <bool LFS>
+void launch_brick(const ForceArgs& a, hipStream_t s) {
+    const Bricks& k = a.bk;
+    const dim3 g(brick_grid(k));
+    if (a.mixed) {
+        const dim3 b(static_cast<unsigned>(k.waves_mixed * kWave));
+        const size_t lds = static_cast<size_t>(k.halo_cap) * sizeof(int4);
+        if (a.quant.cubic) {
+            if (a.kick) hipLaunchKernelGGL((k_lj_brick<LFS, true, true>), g, b, lds, s, a);
+            else hipLaunchKernelGGL((k_lj_brick<LFS, true, false>), g, b, lds, s, a);
+        } else {
+            if (a.kick) hipLaunchKernelGGL((k_lj_brick<LFS, false, true>), g, b, lds, s, a);
+            else hipLaunchKernelGGL((k_lj_brick<LFS, false, false>), g, b, lds, s, a);
+        }
+    } else {
+        const dim3 b(static_cast<unsigned>(k.waves_fp64 * kWave));
+        const size_t lds = static_cast<size_t>(k.halo_cap) * sizeof(double4);
+        if (a.kick) hipLaunchKernelGGL((k_lj_brick64<LFS, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((k_lj_brick64<LFS, false>), g, b, lds, s, a);
+    }
+}
+
+void launch_brick_export(const ForceArgs& a, uint32_t* out, hipStream_t s) {
+    const Bricks& k = a.bk;
+    hipLaunchKernelGGL(k_brick_export, dim3(brick_grid(k)), dim3(256),
+                       static_cast<size_t>(k.halo_cap) * sizeof(int), s, a, out);
+}
+
 void launch_lj_force(const ForceArgs& a, hipStream_t s) {
+    if (a.bk.on) {
+        if (a.lj.lfs) launch_brick<true>(a, s);
+        else launch_brick<false>(a, s);
+        return;
+    }
     const dim3 g(xcd_grid(a.n));
     const bool qp = a.qpos != nullptr;
     const bool cubic = a.quant.cubic != 0;
@@ -1005,8 +1387,9 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
 void launch_lj_finish(const ForceArgs& a, hipStream_t s) {
     // the pair kernel writes partials for its lj_groups(n) blocks (padding blocks of
     // the XCD-aware grid none)
+    const long nb = a.bk.on ? a.bk.nb : lj_groups(a.n, a.mixed != 0);
     hipLaunchKernelGGL(k_lj_finish, dim3(kFinishGroups), dim3(kFinish), 0, s, a,
-                       static_cast<int>(lj_groups(a.n, a.mixed != 0)));
+                       static_cast<int>(nb));
 }
 
 MixK mix_constants(const LJConst& k, double s, double band) {

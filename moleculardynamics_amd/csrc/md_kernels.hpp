@@ -59,9 +59,56 @@ __device__ __forceinline__ void qdelta(const Quant& q, const int4& a, const int4
     dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(a.z) - static_cast<uint32_t>(b.z))) * q.unit[2];
 }
 
+// Bricks of list cells. The cell grid (nc per axis) is cut into kb bricks
+// per axis, brick i spanning cells [floor(i nc / kb), floor((i+1) nc / kb)).
+// A brick's halo is the box of cells within two cells of it (every list
+// partner of an atom lies in the 5 x 5 x 5 cells around its own), taken in
+// the order z, y, x (x fastest). When bricks are on, one pair-kernel
+// workgroup stages its brick's halo into LDS in that order, and a list
+// entry is the partner's index in its row's brick halo instead of a slot:
+// tab[tab_off[B] + h] is the number of halo atoms before halo cell h of brick
+// B (an exclusive prefix, with the halo total at index H_B), rebuilt with
+// each list build.
+struct Bricks {
+    int on;
+    int kb[3];
+    int nb;             // bricks (kb[0] kb[1] kb[2])
+    int halo_cap;       // halo atoms one workgroup stages (LDS rows)
+    int waves_mixed;    // waves per workgroup of the mixed brick kernel
+    int waves_fp64;     // waves per workgroup of the FP64 brick kernel
+    const int* tab_off; // nb + 1 entries
+    int* tab;           // tab_off[nb] entries
+};
+
+// First cell of brick i along an axis of n cells cut into kb bricks, and the
+// brick of cell c (the i with lo(i) <= c < lo(i + 1)).
+__host__ __device__ __forceinline__ int brick_lo(int i, int n, int kb) {
+    return static_cast<int>(static_cast<long>(i) * n / kb);
+}
+__host__ __device__ __forceinline__ int brick_of(int c, int n, int kb) {
+    return static_cast<int>((static_cast<long>(c + 1) * kb - 1) / n);
+}
+
+// Origin (first cell) and extent in cells of brick b.
+struct BrickGeo {
+    int x0, y0, z0, ex, ey, ez;
+};
+__host__ __device__ __forceinline__ BrickGeo brick_geo(const Bricks& k, const Grid& g, int b) {
+    const int bx = b % k.kb[0], by = (b / k.kb[0]) % k.kb[1], bz = b / (k.kb[0] * k.kb[1]);
+    BrickGeo o;
+    o.x0 = brick_lo(bx, g.nc[0], k.kb[0]);
+    o.y0 = brick_lo(by, g.nc[1], k.kb[1]);
+    o.z0 = brick_lo(bz, g.nc[2], k.kb[2]);
+    o.ex = brick_lo(bx + 1, g.nc[0], k.kb[0]) - o.x0;
+    o.ey = brick_lo(by + 1, g.nc[1], k.kb[1]) - o.y0;
+    o.ez = brick_lo(bz + 1, g.nc[2], k.kb[2]) - o.z0;
+    return o;
+}
+
 struct NeighArgs {
     long n;
     Grid grid;
+    Bricks bk;
     Box box;
     double rl2;  // (rc + skin)^2 as the reference computes it
     double4 *pos, *vel, *frc, *ref;
@@ -126,6 +173,10 @@ struct ForceArgs {
     int cap;         // row capacity: rows longer than this overflowed and are clamped
     const int4* qpos;  // fixed-point positions for the mixed kernel (or null)
     Quant quant;
+    // bricks (bk.on: list entries are brick-halo indices, the brick kernels run)
+    Grid grid;
+    Bricks bk;
+    const int* cell_start;
     double half_dt;  // kick coefficient h (v += h/m f) when kick != 0
     int kick;
     int mixed;
@@ -158,6 +209,9 @@ constexpr int kFinishGroups = 64;
 // Fixed-order reduction of the pair kernel's partials (E, W, KE, pairs),
 // thermo row and NHC chain; launched right after launch_lj_force.
 void launch_lj_finish(const ForceArgs& a, hipStream_t s);
+// Rewrites a brick list (a.bk.on) as slot entries (entry_of) into out, in
+// the same octet layout (md_neigh_export_pairs).
+void launch_brick_export(const ForceArgs& a, uint32_t* out, hipStream_t s);
 // LennardJones(sigma, epsilon, cutoff, truncation) constants: u(rc) and
 // du/dr(rc) as lennard_jones.cpp:32-48 computes them, and their FP32 and
 // fixed-point-scaled forms.

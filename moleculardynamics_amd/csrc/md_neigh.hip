@@ -157,12 +157,65 @@ __global__ __launch_bounds__(kScanTile) void k_scan(int* __restrict__ cell_count
     }
 }
 
+// Halo table of brick b (one workgroup): the exclusive prefix of the atom
+// counts of its halo cells in z, y, x order, and the total, into
+// tab[tab_off[b] ..]. A halo larger than the workgroup's LDS rows sets bit 2
+// of the overflow flag (md_run and the list build then grow the rows or
+// turn the bricks off and build again).
+__device__ void brick_table(const Grid& g, const Bricks& k, const int* __restrict__ cell_start,
+                            int b, Scalars* sc) {
+    __shared__ int ws[kBlock / kWave];
+    const BrickGeo o = brick_geo(k, g, b);
+    const int hx = o.ex + 4, hy = o.ey + 4, hz = o.ez + 4;
+    const int H = hx * hy * hz;
+    const int per = (H + kBlock - 1) / kBlock;
+    const int h0 = min(H, static_cast<int>(threadIdx.x) * per), h1 = min(H, h0 + per);
+    auto cell_at = [&](int h) {
+        const int ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
+        int x = o.x0 - 2 + ix, y = o.y0 - 2 + iy, z = o.z0 - 2 + iz;
+        x += x < 0 ? g.nc[0] : (x >= g.nc[0] ? -g.nc[0] : 0);
+        y += y < 0 ? g.nc[1] : (y >= g.nc[1] ? -g.nc[1] : 0);
+        z += z < 0 ? g.nc[2] : (z >= g.nc[2] ? -g.nc[2] : 0);
+        return x + g.nc[0] * (y + g.nc[1] * z);
+    };
+    int s = 0;
+    for (int h = h0; h < h1; ++h) {
+        const int c = cell_at(h);
+        s += cell_start[c + 1] - cell_start[c];
+    }
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int incl = wave_prefix(s, lane);
+    if (lane == kWave - 1) ws[wv] = incl;
+    __syncthreads();
+    int run = incl - s;
+    for (int w = 0; w < wv; ++w) run += ws[w];
+    int* t = k.tab + k.tab_off[b];
+    for (int h = h0; h < h1; ++h) {
+        t[h] = run;
+        const int c = cell_at(h);
+        run += cell_start[c + 1] - cell_start[c];
+    }
+    if (threadIdx.x == kBlock - 1) {
+        t[H] = run;
+        if (run > k.halo_cap) atomicOr(&sc->overflow, 2);
+    }
+}
+
 // Places every atom's upload index in its cell's range of `order` (in the
-// arrival order of the atomics; k_sort_permute ranks them).
+// arrival order of the atomics; k_sort_permute ranks them). With bricks on,
+// workgroups from gridDim.x - nb on build the bricks' halo tables instead
+// (they need only the cell starts).
 __global__ void k_place(const int* __restrict__ cell_of, const int* __restrict__ orig, long n,
-                        int* __restrict__ cell_fill, int* __restrict__ order, const int* flag) {
+                        int* __restrict__ cell_fill, int* __restrict__ order, const int* flag,
+                        Grid grid, Bricks bk, const int* __restrict__ cell_start, Scalars* sc) {
     if (*flag == 0) return;
-    WAVE_STRIDE(base, n) {
+    const unsigned place_groups = gridDim.x - (bk.on ? bk.nb : 0);
+    if (blockIdx.x >= place_groups) {
+        brick_table(grid, bk, cell_start, static_cast<int>(blockIdx.x - place_groups), sc);
+        return;
+    }
+    for (long base = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) & ~(kWave - 1L);
+         base < n; base += static_cast<long>(place_groups) * blockDim.x) {
         const long t = base + (threadIdx.x & (kWave - 1));
         const bool active = t < n;
         const int c = active ? cell_of[t] : 0;
@@ -228,7 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_sort_permute(NeighArgs a) {
 // the FP64 positions. The pair set is therefore the same. CUBIC: the box has one
 // unit on every axis, so r2 is compared in units squared (the integer
 // differences converted exactly, no scaling per axis).
-template <bool Q, bool CUBIC>
+//
+// BRICK: entries are the partner's index in the halo of the row atom's brick
+// (the halo's LDS order, see Bricks). Each candidate range is a run of
+// consecutive cells in one halo row, so its halo indices are consecutive
+// too: entry = j + (halo index of the range's first atom - j of it).
+template <bool Q, bool CUBIC, bool BRICK>
 __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     if (*a.flag == 0) return;
     const long n = a.n;
@@ -279,8 +337,9 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
     auto drain = [&]() {
         if (cnt - done >= 8) write_octet();
     };
+    int delta = 0;  // BRICK: halo index minus slot over the current range
     auto put = [&](int j, bool keep) {
-        ring[cnt & 15][tid] = entry_of(j);
+        ring[cnt & 15][tid] = BRICK ? static_cast<uint32_t>(j + delta) : entry_of(j);
         cnt += (keep && j != t) ? 1 : 0;
     };
     auto exact = [&](const double4& q) {
@@ -351,6 +410,24 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
             const int cy = (ci / nx) % ny;
             const int cz = ci / (nx * ny);
             const int xa = cx >= 2 ? cx - 2 : 0, xb = cx + 2 < nx ? cx + 2 : nx - 1;
+            // BRICK: the row atom's brick, and the halo x of the pieces' first
+            // cells (global cell gx lies at halo x = gx - x0 + 2, wrapped ones
+            // at gx -+ nx - x0 + 2)
+            const int* tb = nullptr;
+            int hxn = 0, hyn = 0, hy0 = 0, hz0 = 0, hxa = 0, hxw = 0;
+            if (BRICK) {
+                const Bricks& k = a.bk;
+                const int b = brick_of(cx, nx, k.kb[0]) +
+                              k.kb[0] * (brick_of(cy, ny, k.kb[1]) + k.kb[1] * brick_of(cz, nz, k.kb[2]));
+                const BrickGeo o = brick_geo(k, g, b);
+                tb = k.tab + k.tab_off[b];
+                hxn = o.ex + 4;
+                hyn = o.ey + 4;
+                hy0 = cy - o.y0 + 2;
+                hz0 = cz - o.z0 + 2;
+                hxa = xa - o.x0 + 2;
+                hxw = cx < 2 ? cx - o.x0 : nx - o.x0 + 2;
+            }
             for (int oz = -2; oz <= 2; ++oz) {
                 int zz = cz + oz;
                 zz += zz < 0 ? nz : (zz >= nz ? -nz : 0);
@@ -358,11 +435,15 @@ __global__ __launch_bounds__(kBlock) void k_commit_build(NeighArgs a) {
                     int yy = cy + oy;
                     yy += yy < 0 ? ny : (yy >= ny ? -ny : 0);
                     const int rb = nx * (yy + ny * zz);
-                    range(cell_start[rb + xa], cell_start[rb + xb + 1], fixed);
-                    if (cx < 2)
-                        range(cell_start[rb + nx + cx - 2], cell_start[rb + nx], fixed);
-                    else if (cx + 2 >= nx)
-                        range(cell_start[rb], cell_start[rb + cx + 3 - nx], fixed);
+                    const int hrow = BRICK ? ((hz0 + oz) * hyn + (hy0 + oy)) * hxn : 0;
+                    int s0 = cell_start[rb + xa];
+                    if (BRICK) delta = tb[hrow + hxa] - s0;
+                    range(s0, cell_start[rb + xb + 1], fixed);
+                    if (cx < 2 || cx + 2 >= nx) {
+                        s0 = cell_start[cx < 2 ? rb + nx + cx - 2 : rb];
+                        if (BRICK) delta = tb[hrow + hxw] - s0;
+                        range(s0, cell_start[cx < 2 ? rb + nx : rb + cx + 3 - nx], fixed);
+                    }
                 }
             }
         } else {
@@ -514,7 +595,9 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_count, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.flag);
     hipLaunchKernelGGL(k_scan, gt, bt, 0, s, a.cell_count, ncell, a.tile_sum, a.cell_start,
                        a.cell_fill, a.sc, a.flag);
-    hipLaunchKernelGGL(k_place, gn, b, 0, s, a.cell_of, a.orig, n, a.cell_fill, a.order, a.flag);
+    const dim3 gp(gn.x + (a.bk.on ? static_cast<unsigned>(a.bk.nb) : 0u));
+    hipLaunchKernelGGL(k_place, gp, b, 0, s, a.cell_of, a.orig, n, a.cell_fill, a.order, a.flag,
+                       a.grid, a.bk, a.cell_start, a.sc);
     hipLaunchKernelGGL(k_sort_permute, gn, b, 0, s, a);
     if (n <= kBuildLanesMaxAtoms) {
         const dim3 gl(static_cast<unsigned>((n * kBuildLanes + kBlock - 1) / kBlock));
@@ -524,12 +607,16 @@ void launch_rebuild(const NeighArgs& a, hipStream_t s) {
             hipLaunchKernelGGL((k_commit_build_lanes<true, false>), gl, b, 0, s, a);
         else
             hipLaunchKernelGGL((k_commit_build_lanes<false, false>), gl, b, 0, s, a);
-    } else if (a.qpos2 != nullptr && a.quant.cubic)
-        hipLaunchKernelGGL((k_commit_build<true, true>), dim3(xcd_grid(n)), b, 0, s, a);
+    } else if (a.bk.on && a.quant.cubic)
+        hipLaunchKernelGGL((k_commit_build<true, true, true>), dim3(xcd_grid(n)), b, 0, s, a);
+    else if (a.bk.on)
+        hipLaunchKernelGGL((k_commit_build<true, false, true>), dim3(xcd_grid(n)), b, 0, s, a);
+    else if (a.qpos2 != nullptr && a.quant.cubic)
+        hipLaunchKernelGGL((k_commit_build<true, true, false>), dim3(xcd_grid(n)), b, 0, s, a);
     else if (a.qpos2 != nullptr)
-        hipLaunchKernelGGL((k_commit_build<true, false>), dim3(xcd_grid(n)), b, 0, s, a);
+        hipLaunchKernelGGL((k_commit_build<true, false, false>), dim3(xcd_grid(n)), b, 0, s, a);
     else
-        hipLaunchKernelGGL((k_commit_build<false, false>), dim3(xcd_grid(n)), b, 0, s, a);
+        hipLaunchKernelGGL((k_commit_build<false, false, false>), dim3(xcd_grid(n)), b, 0, s, a);
 }
 
 long scan_tiles(long ncell) { return (ncell + kScanTile - 1) / kScanTile; }

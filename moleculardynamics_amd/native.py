@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_neigh_rebuild_count.argtypes = [vp, C.POINTER(C.c_long)]
     lib.md_neigh_export_pairs.argtypes = [vp, _u32p, C.c_long, C.POINTER(C.c_long)]
     lib.md_neigh_set_capacity.argtypes = [vp, C.c_int]
+    lib.md_neigh_set_bricks.argtypes = [vp, C.c_int]
+    lib.md_neigh_bricks.argtypes = [vp, C.POINTER(C.c_int)]
     lib.md_neigh_auto_capacity.argtypes = [C.c_long, _dp, C.c_double, C.c_double]
     lib.md_neigh_auto_capacity.restype = C.c_int
     lib.md_integrator_set.argtypes = [vp, C.POINTER(Integrator)]
@@ -424,6 +426,17 @@ class Context:
 
     def set_capacity(self, per_atom: int):
         self._chk(self.lib.md_neigh_set_capacity(self.h, per_atom))
+
+    def set_bricks(self, cells: int):
+        """Cells per brick side of the LDS-staged pair kernels (-1 automatic,
+        0 off, the default); the list must be built again afterwards."""
+        self._chk(self.lib.md_neigh_set_bricks(self.h, cells))
+
+    def bricks(self) -> int:
+        """Bricks of the current list layout (0: bricks off)."""
+        v = C.c_int(0)
+        self._chk(self.lib.md_neigh_bricks(self.h, C.byref(v)))
+        return int(v.value)
 
     def pairs(self) -> np.ndarray:
         npairs = C.c_long()
