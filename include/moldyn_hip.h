@@ -454,6 +454,16 @@ md_status md_water_settle_velocities(md_water* w);
 #define MD_RIGID_SHAKE 0
 #define MD_RIGID_SETTLE 1
 md_status md_water_set_constraint(md_water* w, int method);
+/* Reciprocal-space method of md_water_forces / md_water_run: the
+ * reference's k-vector sums (MD_KSPACE_EWALD, the default;
+ * forcefield/ewald.cpp:112-172) or smooth particle-mesh Ewald (MD_KSPACE_PME,
+ * Essmann et al. 1995) over the same k-vector set (0 < |n|^2 < n2max), with a
+ * grid of `grid` points per axis (0: 4.2 kmax rounded up to even; at least
+ * 2 kmax + 1) and B-spline order `order` (3 .. 10; 0: 8). The reference names
+ * PME as future work for the O(K N) sum and has no counterpart. */
+#define MD_KSPACE_EWALD 0
+#define MD_KSPACE_PME 1
+md_status md_water_set_kspace(md_water* w, int method, int grid, int order);
 /* nsteps of ConstrainedVelocityVerlet::step with the SPC/E force field;
  * pe/ke (or NULL) receive per-step values in K. */
 md_status md_water_run(md_water* w, long nsteps, double dt, int precision, double* pe, double* ke);

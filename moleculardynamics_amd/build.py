@@ -22,7 +22,7 @@ LIB = os.path.join(LIBDIR, "libmoldyn_hip.so")
 
 SOURCES = ["md_capi.hip", "md_neigh.hip", "md_force.hip", "md_integrate.hip", "md_batch.hip",
            "md_batch_capi.hip", "md_water.hip", "md_water_capi.hip", "md_mix.hip",
-           "md_mix_capi.hip", "md_noalloc.hip", "md_analysis.hip", "md_peaks.hip"]
+           "md_mix_capi.hip", "md_noalloc.hip", "md_analysis.hip", "md_peaks.hip", "md_pme.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [

@@ -866,6 +866,10 @@ void water_launch_forces(const WaterArgs& a, bool mixed, hipStream_t s) {
         hipLaunchKernelGGL(k_wreal<true>, gr, dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_wreal<false>, gr, dim3(kBlock), 0, s, a);
+    if (a.pme.on) {
+        water_launch_pme(a, s);
+        return;
+    }
     hipLaunchKernelGGL(k_weik, dim3(gridn(a.n)), dim3(kBlock), 0, s, a);
     const int groups = water_site_groups(a.n);
     const int spg = static_cast<int>((a.n + groups - 1) / groups);
@@ -923,7 +927,7 @@ void water_launch_check(const WaterArgs& a, int* flag, hipStream_t s) {
 
 void water_launch_energy_sum(const WaterArgs& a, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_wenergy_sum, dim3(1), dim3(kBlock), 0, s, a,
-                       static_cast<int>(water_real_blocks(a.n)), static_cast<int>(gridn(a.nk)), out);
+                       static_cast<int>(water_real_blocks(a.n)), a.nkpart, out);
 }
 
 void water_launch_kinetic_sum(const WaterArgs& a, double* out, hipStream_t s) {

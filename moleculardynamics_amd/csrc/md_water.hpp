@@ -11,6 +11,31 @@
 namespace mdk {
 
 constexpr int kMaxK = 24;  // largest supported kmax per axis
+constexpr int kPmeMaxOrder = 10;  // largest B-spline order of the PME path
+
+// Smooth particle-mesh Ewald of the reciprocal term (md_pme.hip): grid
+// K[0] x K[1] x K[2] (x fastest), B-spline order p, the reference's k-set
+// restricted to |n_d| <= kmax, nb = 2 kmax + 1 frequencies per axis.
+struct PmeArgs {
+    int on;
+    int K[3];
+    int p;
+    int kmax, nb;
+    long ng;               // K[0] K[1] K[2]
+    double* q;             // [ng] the spread charges
+    double* W;             // [n][3][p] spline weights M_p(w + j) per site and axis
+    double* DW;            // [n][3][p] their derivatives
+    int* base;             // [n][3] grid index floor(u) per site and axis
+    double2* A;            // [K2][K1][nb] after the x transform
+    double2* B;            // [K2][nb][nb] after the y transform
+    double2* P;            // [nb][nb][nb] G conj(F[Q])
+    double2* Cz;           // [K2][nb][nb]
+    double2* Dy;           // [K2][K1][nb]
+    double* phi;           // [ng] dE/dQ
+    const double2* tw[3];  // [nb][K_d]: e^{2 pi i (c - kmax) k / K_d}
+    const double2* twT[3]; // the same, transposed: [K_d][nb]
+    const double* G;       // [nb][nb][nb] influence function (0 off the k-set)
+};
 
 struct WaterArgs {
     long n;  // sites (3 per molecule, O H H)
@@ -57,6 +82,10 @@ struct WaterArgs {
     // SETTLE geometry: half the H-H distance and the distance from O to the
     // H-H midpoint, sqrt(oh^2 - settle_rc^2)
     double settle_rc, settle_h;
+    // reciprocal term by PME (pme.on) instead of the k-vector sums; nkpart is
+    // the number of k-space energy partials in part2
+    PmeArgs pme;
+    int nkpart;
 };
 
 void water_launch_list(const WaterArgs& a, const Grid& g, const int* cell_of, const int* cell_start,
@@ -89,5 +118,13 @@ void water_launch_energy_sum(const WaterArgs& a, double* out, hipStream_t s);
 // One workgroup: the kinetic partials of water_launch_kinetic summed into *out.
 void water_launch_kinetic_sum(const WaterArgs& a, double* out, hipStream_t s);
 int water_site_groups(long n);
+// The PME reciprocal term: charges spread, transformed, convolved and
+// interpolated back; forces added to frc (after the real-space kernel), the
+// energy partials (pme_energy_blocks of them) to part2.
+void water_launch_pme(const WaterArgs& a, hipStream_t s);
+long pme_energy_blocks(const PmeArgs& p);
+// Grid rows per workgroup of the charge spread (its LDS region is that many
+// rows of one plane).
+int pme_spread_rows(const PmeArgs& p);
 
 }  // namespace mdk

@@ -58,6 +58,15 @@ struct md_water {
     double* eterm = nullptr;  // per-step device sums of md_water_run: 5 energy sums + KE
     long eterm_steps = 0;
     int constraint = MD_RIGID_SHAKE;  // constraint algorithm of md_water_run
+    // reciprocal term: k-vector sums (MD_KSPACE_EWALD) or PME
+    int kspace = MD_KSPACE_EWALD;
+    int n2max = 0;
+    long part2_alloc = 0;
+    double* pme_q = nullptr;
+    double2 *pme_A = nullptr, *pme_B = nullptr, *pme_P = nullptr, *pme_C = nullptr, *pme_D = nullptr,
+            *pme_tw = nullptr;
+    double *pme_phi = nullptr, *pme_G = nullptr, *pme_W = nullptr, *pme_DW = nullptr;
+    int* pme_base = nullptr;
 };
 
 namespace {
@@ -390,6 +399,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     WHIP(w, walloc(w->cell_start, ncell + 1)); WHIP(w, walloc(w->cell_fill, ncell));
     WHIP(w, walloc(w->cell_atoms, n)); WHIP(w, walloc(w->cell_tmp, n)); WHIP(w, walloc(w->status, n / 3)); WHIP(w, walloc(w->flags, 4));
     WHIP(w, walloc(w->part, 4 * mdk::water_real_blocks(n) + 8)); WHIP(w, walloc(w->part2, wblocks(std::max<long>(n, nk)) + 8));
+    w->part2_alloc = wblocks(std::max<long>(n, nk)) + 8;
+    w->n2max = n2max;
     WHIP(w, walloc(w->kvec, std::max(1, nk))); WHIP(w, walloc(w->kfac, std::max(1, nk)));
     WHIP(w, walloc(w->eik, 3L * (km + 1) * n));
     WHIP(w, walloc(w->skpart, static_cast<long>(groups) * std::max(1, nk)));
@@ -442,6 +453,8 @@ md_status md_water_create(void* hip_stream, long n, const double* L, const doubl
     a.kcol = w->kcol; a.kcol0 = w->kcol0; a.ncol = ncol; a.nchunk = nk > 0 ? nchunk : 0;
     a.kseg = w->kseg; a.kseg0 = w->kseg0; a.nseg = nk > 0 ? w->nseg : 0;
     a.fpart = w->fpart;
+    a.nkpart = static_cast<int>(wblocks(nk));
+    a.pme = mdk::PmeArgs{};
     a.overflow = w->flags;
     a.oh = p->r_oh;
     a.hh = 2.0 * p->r_oh * std::sin(0.5 * p->angle_hoh * kPi / 180.0);
@@ -466,7 +479,12 @@ md_status md_water_destroy(md_water* w) {
                     static_cast<void*>(w->eik), static_cast<void*>(w->skpart), static_cast<void*>(w->sk),
                     static_cast<void*>(w->skf), static_cast<void*>(w->kcol), static_cast<void*>(w->kcol0),
                     static_cast<void*>(w->kseg), static_cast<void*>(w->kseg0),
-                    static_cast<void*>(w->fpart), static_cast<void*>(w->eterm)})
+                    static_cast<void*>(w->fpart), static_cast<void*>(w->eterm),
+                    static_cast<void*>(w->pme_q), static_cast<void*>(w->pme_A), static_cast<void*>(w->pme_B),
+                    static_cast<void*>(w->pme_P), static_cast<void*>(w->pme_C), static_cast<void*>(w->pme_D),
+                    static_cast<void*>(w->pme_tw), static_cast<void*>(w->pme_phi), static_cast<void*>(w->pme_G),
+                    static_cast<void*>(w->pme_W), static_cast<void*>(w->pme_DW),
+                    static_cast<void*>(w->pme_base)})
         if (q) (void)hipFree(q);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
@@ -483,6 +501,137 @@ md_status md_water_forces(md_water* w, int precision, double* terms, double* pe)
     if (terms)
         for (int k = 0; k < 6; ++k) terms[k] = t[k];
     if (pe) *pe = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
+    return MD_OK;
+}
+
+namespace {
+// Cardinal B-spline M_p at the integer points 1 .. p-1 (M_p(t) = (t M_{p-1}(t)
+// + (p - t) M_{p-1}(t - 1)) / (p - 1) from M_2 = 1 - |t - 1|), and the PME
+// spline correction of one axis, |b(m)|^2 = 1 / |sum_{j=0}^{p-2} M_p(j+1)
+// e^{2 pi i m j / K}|^2 (Essmann et al. 1995, eq. 4.4).
+std::vector<double> bspline_integers(int p) {
+    std::vector<double> c(static_cast<size_t>(p) + 1, 0.0);  // M_n(j), j = 0 .. p
+    c[1] = 1.0;                                              // M_2(1) = 1
+    for (int n = 3; n <= p; ++n) {
+        std::vector<double> d(c.size(), 0.0);
+        for (int j = 1; j < n; ++j) d[j] = (j * c[j] + (n - j) * c[j - 1]) / (n - 1);
+        c = d;
+    }
+    return c;
+}
+
+double pme_bsq(const std::vector<double>& mp, int p, int K, int m) {
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j <= p - 2; ++j) {
+        long r = (static_cast<long>(m) * j) % K;
+        if (r < 0) r += K;
+        const double ang = 2.0 * kPi * static_cast<double>(r) / K;
+        re += mp[j + 1] * std::cos(ang);
+        im += mp[j + 1] * std::sin(ang);
+    }
+    return 1.0 / (re * re + im * im);
+}
+}  // namespace
+
+md_status md_water_set_kspace(md_water* w, int method, int grid, int order) {
+    if (w == nullptr) return MD_ERR_INVALID_ARGUMENT;
+    mdk::WaterArgs& a = w->a;
+    if (method == MD_KSPACE_EWALD) {
+        a.pme = mdk::PmeArgs{};
+        a.nkpart = static_cast<int>(wblocks(a.nk));
+        w->kspace = method;
+        return MD_OK;
+    }
+    if (method != MD_KSPACE_PME) return wfail(w, MD_ERR_INVALID_ARGUMENT, "unknown k-space method");
+    const int km = w->p.kmax;
+    const int nb = 2 * km + 1;
+    if (order == 0) order = 8;
+    if (order < 3 || order > mdk::kPmeMaxOrder)
+        return wfail(w, MD_ERR_INVALID_ARGUMENT, "PME spline order must be 3 .. 10");
+    int K = grid;
+    if (K == 0) K = std::max(nb, 2 * ((21 * km + 9) / 10));  // 4.2 kmax rounded up to even
+    if (K < nb || K < order || K > 1024)
+        return wfail(w, MD_ERR_INVALID_ARGUMENT,
+                     "PME grid must hold max(2 kmax + 1, order) .. 1024 points per axis");
+    mdk::PmeArgs pm{};
+    pm.on = 1;
+    pm.K[0] = pm.K[1] = pm.K[2] = K;
+    pm.p = order;
+    pm.kmax = km;
+    pm.nb = nb;
+    pm.ng = static_cast<long>(K) * K * K;
+    // tables: twiddles per axis and the influence function on the k-set
+    const std::vector<double> mp = bspline_integers(order);
+    std::vector<double2> tw(3L * nb * K), twT(3L * nb * K);
+    std::vector<double> bsq(3L * nb);
+    for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < nb; ++c) {
+            const long m = c - km;
+            for (int k = 0; k < K; ++k) {
+                long r = (m * k) % K;
+                if (r < 0) r += K;
+                const double ang = 2.0 * kPi * static_cast<double>(r) / K;
+                tw[(static_cast<long>(d) * nb + c) * K + k] = make_double2(std::cos(ang), std::sin(ang));
+                twT[(static_cast<long>(d) * K + k) * nb + c] = tw[(static_cast<long>(d) * nb + c) * K + k];
+            }
+            bsq[d * nb + c] = pme_bsq(mp, order, K, static_cast<int>(m));
+        }
+    std::vector<double> G(static_cast<size_t>(nb) * nb * nb, 0.0);
+    const double V = w->L[0] * w->L[1] * w->L[2];
+    const double C = coulomb_prefactor();
+    const double k0[3] = {2.0 * kPi / w->L[0], 2.0 * kPi / w->L[1], 2.0 * kPi / w->L[2]};
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j)
+            for (int l = 0; l < nb; ++l) {
+                const int m0 = i - km, m1 = j - km, m2 = l - km;
+                const int n2 = m0 * m0 + m1 * m1 + m2 * m2;
+                if (n2 == 0 || n2 >= w->n2max) continue;
+                const double kx = k0[0] * m0, ky = k0[1] * m1, kz = k0[2] * m2;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                G[(static_cast<size_t>(i) * nb + j) * nb + l] =
+                    C * (2.0 * kPi / V) * std::exp(-k2 / (4.0 * w->alpha * w->alpha)) / k2 *
+                    bsq[0 * nb + i] * bsq[1 * nb + j] * bsq[2 * nb + l];
+            }
+    WHIP(w, walloc(w->pme_q, pm.ng));
+    WHIP(w, walloc(w->pme_A, static_cast<long>(K) * K * nb));
+    WHIP(w, walloc(w->pme_B, static_cast<long>(K) * nb * nb));
+    WHIP(w, walloc(w->pme_P, static_cast<long>(nb) * nb * nb));
+    WHIP(w, walloc(w->pme_C, static_cast<long>(K) * nb * nb));
+    WHIP(w, walloc(w->pme_D, static_cast<long>(K) * K * nb));
+    WHIP(w, walloc(w->pme_phi, pm.ng));
+    WHIP(w, walloc(w->pme_tw, 6L * nb * K));
+    WHIP(w, walloc(w->pme_G, static_cast<long>(nb) * nb * nb));
+    WHIP(w, walloc(w->pme_W, 3L * order * w->n));
+    WHIP(w, walloc(w->pme_DW, 3L * order * w->n));
+    WHIP(w, walloc(w->pme_base, 3L * w->n));
+    WHIP(w, hipMemcpy(w->pme_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    WHIP(w, hipMemcpy(w->pme_tw + tw.size(), twT.data(), twT.size() * sizeof(double2),
+                      hipMemcpyHostToDevice));
+    WHIP(w, hipMemcpy(w->pme_G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+    pm.q = w->pme_q;
+    pm.A = w->pme_A;
+    pm.B = w->pme_B;
+    pm.P = w->pme_P;
+    pm.Cz = w->pme_C;
+    pm.Dy = w->pme_D;
+    pm.phi = w->pme_phi;
+    for (int d = 0; d < 3; ++d) {
+        pm.tw[d] = w->pme_tw + static_cast<long>(d) * nb * K;
+        pm.twT[d] = w->pme_tw + 3L * nb * K + static_cast<long>(d) * nb * K;
+    }
+    pm.G = w->pme_G;
+    pm.W = w->pme_W;
+    pm.DW = w->pme_DW;
+    pm.base = w->pme_base;
+    const long nkp = mdk::pme_energy_blocks(pm);
+    if (nkp + 8 > w->part2_alloc) {
+        WHIP(w, walloc(w->part2, nkp + 8));
+        w->part2_alloc = nkp + 8;
+        a.part2 = w->part2;
+    }
+    a.pme = pm;
+    a.nkpart = static_cast<int>(nkp);
+    w->kspace = method;
     return MD_OK;
 }
 

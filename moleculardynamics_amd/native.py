@@ -165,6 +165,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_water_settle.argtypes = [vp, _dp, _dp, _dp]
     lib.md_water_settle_velocities.argtypes = [vp]
     lib.md_water_set_constraint.argtypes = [vp, C.c_int]
+    lib.md_water_set_kspace.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.md_water_run.argtypes = [vp, C.c_long, C.c_double, C.c_int, _dp, _dp]
     lib.md_water_download.argtypes = [vp] + [_dp] * 9
     lib.md_water_set_state.argtypes = [vp] + [_dp] * 6
@@ -190,6 +191,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 RDF_AUTO, RDF_ALL_PAIRS, RDF_CELLS = 0, 1, 2
 RIGID_SHAKE, RIGID_SETTLE = 0, 1
+KSPACE_EWALD, KSPACE_PME = 0, 1
 
 
 def device_peaks():
@@ -678,6 +680,11 @@ class Water:
     def set_constraint(self, method):
         """RIGID_SHAKE (default) or RIGID_SETTLE for run()."""
         self._chk(self.lib.md_water_set_constraint(self.h, method))
+
+    def set_kspace(self, method, grid=0, order=0):
+        """KSPACE_EWALD (default) or KSPACE_PME with `grid` points per axis
+        (0: automatic) and B-spline `order` (0: 8) for forces() and run()."""
+        self._chk(self.lib.md_water_set_kspace(self.h, method, grid, order))
 
     def run(self, nsteps, dt, precision=FP64):
         pe, ke = np.zeros(nsteps), np.zeros(nsteps)

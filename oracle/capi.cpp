@@ -279,6 +279,23 @@ int orc_ewald(long n, const double* L, const double* x, const double* y, const d
     return 0;
 }
 
+// PME reciprocal energy (and forces, when asked) over the Ewald k-set.
+int orc_pme(long n, const double* L, const double* x, const double* y, const double* z,
+            const double* q, int kmax, double alpha, const int* K, int order, int forces,
+            double* fx, double* fy, double* fz, double* recip) {
+    return guarded([&] {
+        State s = make_state(n, L, x, y, z);
+        for (long i = 0; i < n; ++i) s.q[i] = q[i];
+        *recip = pme_recip(s, kmax, alpha, K, order, forces != 0, &s);
+        if (forces) {
+            copy_out(s.fx, fx); copy_out(s.fy, fy); copy_out(s.fz, fz);
+        }
+        return 0;
+    });
+}
+
+double orc_pme_bsq(int order, int K, int m) { return pme_bsq(order, K, m); }
+
 namespace {
 State water_state(long n, const double* L, const double* x, const double* y, const double* z) {
     const Spce p;

@@ -210,6 +210,15 @@ struct EwaldTerms {
 EwaldTerms ewald(const State& s, double rc, int kmax, double alpha, bool accumulate_forces,
                  State* sink);
 
+// Smooth particle-mesh Ewald (Essmann et al. 1995) for the reciprocal term
+// over the same k-vector set as ewald() (pme.cpp): K[d] grid points per
+// axis (at least 2 kmax + 1), B-spline order p (3 .. 12). Returns the
+// reciprocal energy; forces are added to sink when asked. pme_bsq is the
+// spline correction |b_d(m)|^2 of one axis.
+double pme_recip(const State& s, int kmax, double alpha, const int K[3], int p,
+                 bool accumulate_forces, State* sink);
+double pme_bsq(int p, int K, int m);
+
 struct Spce {
     double qH = 0.42380, sigmaO = 3.16555789, epsilonO = 78.19743111;
     double rOH = 1.0, angleHOH = 109.47, mO = 15.9994, mH = 1.008;

@@ -110,6 +110,11 @@ class Oracle:
                                   C.c_double, C.c_int, _dp, _dp, _dp, _dp]
         lib.orc_spce_forces.argtypes = [C.c_long, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
                                         C.c_double, _dp, _dp, _dp, _dp]
+        if hasattr(lib, "orc_pme"):  # the oracle's own (the reference has no PME)
+            lib.orc_pme.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                    C.POINTER(C.c_int), C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+            lib.orc_pme_bsq.argtypes = [C.c_int, C.c_int, C.c_int]
+            lib.orc_pme_bsq.restype = C.c_double
         lib.orc_rigid_shake.argtypes = [C.c_long, _dp] + [_dp] * 7 + [C.c_double, C.c_double,
                                                                       C.c_double, C.c_int]
         lib.orc_rigid_rattle.argtypes = [C.c_long, _dp] + [_dp] * 7 + [C.c_double, C.c_double,
@@ -264,6 +269,22 @@ class Oracle:
         if st != 0:
             raise RuntimeError(self.error())
         return dict(real=t[0], recip=t[1], self=t[2], intra=t[3]), np.stack(f, axis=1)
+
+    def pme(self, fr: Frame, q, kmax=5, alpha=0.0, grid=(32, 32, 32), order=6, forces=False):
+        """Smooth PME reciprocal energy (and forces) over the Ewald k-set."""
+        n = fr.n
+        q = np.ascontiguousarray(q, dtype=float)
+        K = (C.c_int * 3)(*[int(g) for g in grid])
+        f = [np.zeros(n) for _ in range(3)]
+        e = np.zeros(1)
+        st = self.lib.orc_pme(n, _ptr(fr.L), _ptr(fr.x), _ptr(fr.y), _ptr(fr.z), _ptr(q), kmax,
+                              alpha, K, order, int(forces), *[_ptr(v) for v in f], _ptr(e))
+        if st != 0:
+            raise RuntimeError(self.error())
+        return float(e[0]), np.stack(f, axis=1)
+
+    def pme_bsq(self, order, K, m):
+        return float(self.lib.orc_pme_bsq(order, K, m))
 
     def spce_forces(self, fr: Frame, rc=10.0, kmax=5, alpha=0.0):
         n = fr.n
