@@ -325,6 +325,40 @@ def bench_batch(args) -> int:
                  + ("100-step graph segments" if graph else "plain launches"), extra)
 
 
+def pme_scaling(sides=(16, 20, 24), calls=20):
+    """Force evaluations (SPC/E, rc 9 A, alpha and kmax for 1e-5 at each size) with the
+    k-vector sums and with PME, at m^3 molecules on the water density's lattice: the
+    sums grow as N kmax^3 (kmax grows with L), PME as N plus its grid."""
+    from moleculardynamics_amd import native
+
+    out = []
+    for m in sides:
+        L = 49.71853 * m / 16.0
+        n = 3 * m ** 3
+        pos, vel = native.water_lattice(m, L, 2026, 300.0)
+        q2 = m ** 3 * (0.8476 ** 2 + 2 * 0.4238 ** 2)
+        alpha, kmax = native.ewald_parameters(n, [L] * 3, q2, 9.0, 1e-5)
+        row = {"sites": n, "L": L, "kmax": kmax}
+        w = native.Water([L] * 3, pos, vel, params=native.water_params(cutoff=9.0, kmax=kmax, alpha=alpha))
+        try:
+            for name, method in (("ewald", native.KSPACE_EWALD), ("pme", native.KSPACE_PME)):
+                w.set_kspace(method)
+                t, _ = w.forces()
+                t0 = time.perf_counter()
+                for _ in range(calls):
+                    w.forces()
+                row[name + "_ms_per_force_call"] = 1e3 * (time.perf_counter() - t0) / calls
+                row[name + "_recip"] = t["recip"]
+        finally:
+            w.close()
+        row["pme_speedup"] = row["ewald_ms_per_force_call"] / row["pme_ms_per_force_call"]
+        row["recip_rel_diff"] = abs(row["pme_recip"] - row["ewald_recip"]) / abs(row["ewald_recip"])
+        log(f"  PME scaling: {n} sites, kmax {kmax}: Ewald {row['ewald_ms_per_force_call']:.3f} ms, "
+            f"PME {row['pme_ms_per_force_call']:.3f} ms per force call ({row['pme_speedup']:.2f}x)")
+        out.append(row)
+    return out
+
+
 def bench_water(args) -> int:
     """4096 SPC/E molecules (12,288 sites), L = 49.71853 A, rc 9 A, alpha and kmax for a
     1e-5 relative force error, 500 constrained velocity-Verlet steps of 2 fs at 300 K."""
@@ -384,6 +418,7 @@ def bench_water(args) -> int:
     log(f"  SETTLE + PME: {500 / wall_p:.1f} steps/s ({wall_s / wall_p:.2f}x SETTLE + Ewald), "
         f"recip error {extra['settle_pme']['recip_energy_rel_error']:.2e}, "
         f"force RMS error {extra['settle_pme']['force_rms_rel_error']:.2e}")
+    extra["pme_scaling"] = pme_scaling()
     if not args.no_cpu:
         from oracle import pyoracle
         from oracle.pyoracle import Frame

@@ -35,6 +35,16 @@ namespace {
 constexpr double kPmeFix = 281474976710656.0;  // 2^48
 constexpr double kPmeFixInv = 1.0 / 281474976710656.0;
 
+// round(v) to nearest even as int64 for |v| < 2^51: v + 1.5 2^52 keeps the
+// integer in its low mantissa bits, and subtracting the constant's bit
+// pattern as an integer leaves it (two's complement for negative v). The
+// same result as __double2ll_rn in three instructions.
+__device__ __forceinline__ long long fix_round(double v) {
+#pragma clang fp contract(off)
+    constexpr double kMagic52 = 6755399441055744.0;  // 1.5 2^52
+    return __double_as_longlong(v + kMagic52) - __double_as_longlong(kMagic52);
+}
+
 inline unsigned blocks(long n) {
     const long g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g > 0 ? g : 1);
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_pme_spread(WaterArgs a, int hy) {
                 for (int j0 = 0; j0 < p; ++j0) {
                     int k0 = b0 - j0;
                     k0 += k0 < 0 ? K0 : 0;
-                    atomicAdd(row + k0, static_cast<unsigned long long>(__double2ll_rn(qw * W[j0] * kPmeFix)));
+                    atomicAdd(row + k0, static_cast<unsigned long long>(fix_round(qw * W[j0] * kPmeFix)));
                 }
             }
         }
