@@ -523,26 +523,59 @@ __device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt
 // Partners from the fixed-point positions (mixed kernel): an entry is the
 // partner's byte offset from the array base, so a gather is a scalar base
 // plus one vector register. Only x, y, z are read.
+//
+// The row's last octet, full or not, is evaluated as a whole octet: its
+// slots past the row end are replaced by the row atom's own entry before
+// the gathers (an in-bounds read of a position already in L1) and enter as
+// dead entries, which add exactly nothing. A wave then runs as many octets
+// as its longest row needs, with eight gathers in flight in each, instead
+// of the full octets of its longest row followed by up to seven single
+// entries, each waiting for its own gather.
 template <bool LFS, bool CUBIC>
 __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt, const int4& qi,
                                                AccMix& q, bool& hit) {
     const char* qb = reinterpret_cast<const char*>(a.qpos);
     auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + e); };
-    walk_row<int4>(
-        a, t, cnt,
-        [&](const u32x4& j, int4* p) {
-            p[0] = at(j.x);
-            p[1] = at(j.y);
-            p[2] = at(j.z);
-            p[3] = at(j.w);
-        },
-        [&](const u32x4&, const int4* p) {
-            mix_fixed<LFS, CUBIC>(a, qi, p[0], q, hit);
-            mix_fixed<LFS, CUBIC>(a, qi, p[1], q, hit);
-            mix_fixed<LFS, CUBIC>(a, qi, p[2], q, hit);
-            mix_fixed<LFS, CUBIC>(a, qi, p[3], q, hit);
-        },
-        [&](uint32_t e) { mix_fixed<LFS, CUBIC>(a, qi, at(e), q, hit); });
+    const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
+    const int noct = (cnt + 7) >> 3;
+    u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
+    if (noct > 0) load_octet(row, lo, hi);
+    for (int g = 0; g + 1 < noct; ++g) {
+        const int4 p0 = at(lo.x), p1 = at(lo.y), p2 = at(lo.z), p3 = at(lo.w);
+        const int4 p4 = at(hi.x), p5 = at(hi.y), p6 = at(hi.z), p7 = at(hi.w);
+        u32x4 nlo, nhi;
+        load_octet(row + (g + 1) * 512, nlo, nhi);
+        mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p1, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p2, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p3, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p4, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p5, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p6, q, hit);
+        mix_fixed<LFS, CUBIC>(a, qi, p7, q, hit);
+        lo = nlo;
+        hi = nhi;
+    }
+    if (noct > 0) {
+        const int live = cnt - 8 * (noct - 1);  // 1..8 entries of the last octet
+        const uint32_t self = entry_of(static_cast<int>(t));
+        const int4 p0 = at(lo.x);               // live >= 1
+        const int4 p1 = at(live > 1 ? lo.y : self);
+        const int4 p2 = at(live > 2 ? lo.z : self);
+        const int4 p3 = at(live > 3 ? lo.w : self);
+        const int4 p4 = at(live > 4 ? hi.x : self);
+        const int4 p5 = at(live > 5 ? hi.y : self);
+        const int4 p6 = at(live > 6 ? hi.z : self);
+        const int4 p7 = at(live > 7 ? hi.w : self);
+        mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p1, live > 1, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p2, live > 2, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p3, live > 3, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p4, live > 4, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p5, live > 5, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p6, live > 6, q, hit);
+        mix_fixed_masked<LFS, CUBIC>(a, qi, p7, live > 7, q, hit);
+    }
 }
 
 // The workgroup's partials of a pair kernel in one LDS exchange: each wave's
