@@ -531,13 +531,48 @@ __device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt
 // as its longest row needs, with eight gathers in flight in each, instead
 // of the full octets of its longest row followed by up to seven single
 // entries, each waiting for its own gather.
+//
+// Band hits are found by counting: q.k counts the entries with r2 < lo (in
+// the cutoff) and the walk returns the count of entries with r2 < hi, so the
+// row saw a decision inside the band exactly when the two differ. Both
+// counts are per-lane carry adds, which keeps the walk free of per-entry
+// boolean chains.
+template <bool CUBIC>
+__device__ __forceinline__ void fixed_delta(const ForceArgs& a, const int4& qi, const int4& qj,
+                                            double& dx, double& dy, double& dz) {
+    dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+    dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+    dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+    if (!CUBIC) {
+        dx *= a.quant.unit[0];
+        dy *= a.quant.unit[1];
+        dz *= a.quant.unit[2];
+    }
+}
+
+// mix_fixed / mix_fixed_masked counting the entries below the band's upper
+// bound in kb. MASKED: live false enters as a dead entry with r2 = 4 hi,
+// beyond both bounds, so it adds to neither count and its terms are exact
+// zeros.
+template <bool LFS, bool CUBIC, bool MASKED>
+__device__ __forceinline__ void mix_fixed_kb(const ForceArgs& a, const int4& qi, const int4& qj,
+                                             bool live, AccMix& q, int& kb) {
+    double dx, dy, dz;
+    fixed_delta<CUBIC>(a, qi, qj, dx, dy, dz);
+    const double r2f = fma(dz, dz, fma(dy, dy, dx * dx));
+    const double r2 = MASKED ? (live ? r2f : 4.0 * a.mq.hi) : r2f;
+    kb += r2 < a.mq.hi ? 1 : 0;
+    mix_term<LFS>(a.mq, dx, dy, dz, r2, r2 < a.mq.lo, q);
+}
+
 template <bool LFS, bool CUBIC>
-__device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int cnt, const int4& qi,
-                                               AccMix& q, bool& hit) {
+__device__ __forceinline__ int walk_row_fixed(const ForceArgs& a, long t, int cnt, const int4& qi,
+                                              AccMix& q) {
     const char* qb = reinterpret_cast<const char*>(a.qpos);
     auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + e); };
     const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
     const int noct = (cnt + 7) >> 3;
+    int kb = 0;
     u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
     if (noct > 0) load_octet(row, lo, hi);
     for (int g = 0; g + 1 < noct; ++g) {
@@ -545,14 +580,14 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
         const int4 p4 = at(hi.x), p5 = at(hi.y), p6 = at(hi.z), p7 = at(hi.w);
         u32x4 nlo, nhi;
         load_octet(row + (g + 1) * 512, nlo, nhi);
-        mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p1, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p2, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p3, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p4, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p5, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p6, q, hit);
-        mix_fixed<LFS, CUBIC>(a, qi, p7, q, hit);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p0, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p1, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p2, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p3, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p4, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p5, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p6, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p7, true, q, kb);
         lo = nlo;
         hi = nhi;
     }
@@ -567,15 +602,16 @@ __device__ __forceinline__ void walk_row_fixed(const ForceArgs& a, long t, int c
         const int4 p5 = at(live > 5 ? hi.y : self);
         const int4 p6 = at(live > 6 ? hi.z : self);
         const int4 p7 = at(live > 7 ? hi.w : self);
-        mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p1, live > 1, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p2, live > 2, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p3, live > 3, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p4, live > 4, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p5, live > 5, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p6, live > 6, q, hit);
-        mix_fixed_masked<LFS, CUBIC>(a, qi, p7, live > 7, q, hit);
+        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p0, true, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p1, live > 1, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p2, live > 2, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p3, live > 3, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p4, live > 4, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p5, live > 5, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p6, live > 6, q, kb);
+        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p7, live > 7, q, kb);
     }
+    return kb;
 }
 
 // The workgroup's partials of a pair kernel in one LDS exchange: each wave's
@@ -641,9 +677,9 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
             AccMix q;
             const MixK& m = QP ? a.mq : a.mx;
             if (QP) {
-                bool hit = false;
-                walk_row_fixed<LFS, CUBIC>(a, t, cnt, a.qpos[t], q, hit);
-                if (hit) {  // rare: a decision inside the band, the row again exactly
+                const int4 qi = a.qpos[t];
+                const int kb = walk_row_fixed<LFS, CUBIC>(a, t, cnt, qi, q);
+                if (kb != q.k) {  // rare: a decision inside the band, the row again exactly
                     q = AccMix{};
                     const double4 pi = a.pos[t];
                     walk_row_pos(a, t, cnt, [&](const double4& pj) { mix_exact<LFS>(a, a.mq, pi, pj, q); });
