@@ -284,11 +284,11 @@ struct Acc64 {
 
 template <bool LFS>
 __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, const double4& pj,
-                                       Acc64& q) {
+                                       Acc64& q, bool live = true) {
     double dx, dy, dz;
     row_delta_w(a.box, pi, pj, dx, dy, dz);
     const double r2 = norm2_ref(dx, dy, dz);
-    if (r2 < a.lj.rc2) {
+    if (live && r2 < a.lj.rc2) {
         // hardware reciprocal refined by two Newton steps (within an ulp of 1/r2)
         double inv = __builtin_amdgcn_rcp(r2);
         double e = fma(-r2, inv, 1.0);
@@ -723,13 +723,15 @@ __global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
 // group adds integers). A group redoes its row exactly when any of its lanes
 // saw a band hit, which is the row's own condition.
 //
-// Above that size the mixed kernel keeps one lane per atom: it is bound by
-// VALU issue, and its octet-batched walk keeps eight gathers in flight per
-// thread. The FP64 kernel is bound by the gathers themselves (two L1
-// accesses per partner), and kFp64Lanes lanes per atom give it more waves
-// in flight.
+// Above that size the mixed kernel keeps one lane per atom: its
+// octet-batched walk keeps eight 12-byte gathers in flight per thread. The
+// FP64 kernel gathers 32 bytes per partner and keeps its registers low
+// (about 80, six waves per SIMD) with kFp64Lanes lanes per atom, each taking
+// its entries kFp64Batch at a time (walk_lane64). At 1M atoms, measured per
+// launch: 8 lanes x 2 entries 212 us, 4 x 4 214 us, 8 x 4 229 us, 16 x 2
+// 296 us, and 4 lanes one entry at a time 264 us.
 constexpr long kLanesMaxAtoms = 32768;
-constexpr int kFp64Lanes = 4;
+constexpr int kFp64Lanes = 8;
 
 inline int lj_lanes(long n, bool mixed) {
     if (n > kLanesMaxAtoms) return mixed ? 1 : kFp64Lanes;
@@ -742,6 +744,38 @@ inline int lj_lanes(long n, bool mixed) {
 inline long lj_groups(long n, bool mixed) {
     const int l = lj_lanes(n, mixed);
     return (static_cast<long>(l) * n + kBlock - 1) / kBlock;
+}
+
+// A lane's share of an FP64 row (entries sub, sub + LANES, ...) BATCH entries
+// at a time: the batch's partner gathers are issued together, then the next
+// batch's indices, then the batch's terms, so a lane keeps several memory
+// accesses in flight instead of one dependent index-gather-term chain per
+// entry. The last batch is masked: its slots past the row end read the
+// lane's first entry again (a valid partner) and add nothing. The terms are
+// added in entry order, as one at a time would add them.
+constexpr int kFp64Batch = 2;
+
+template <bool LFS, int LANES, int BATCH, class EntryFn, class PosFn>
+__device__ __forceinline__ void walk_lane64(const ForceArgs& a, const double4& pi, int sub, int cnt,
+                                            EntryFn entry, PosFn pos_of, Acc64& q) {
+    constexpr int kStep = BATCH * LANES;
+    uint32_t e[BATCH];
+    auto indices = [&](int k0) {  // k0 < cnt
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int k = k0 + b * LANES;
+            e[b] = entry(k < cnt ? k : sub);
+        }
+    };
+    if (sub < cnt) indices(sub);
+    for (int k = sub; k < cnt; k += kStep) {
+        double4 p[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) p[b] = pos_of(e[b]);
+        indices(k + kStep < cnt ? k + kStep : k);  // the next batch's (this one's after the last)
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) pair64<LFS>(a, pi, p[b], q, k + b * LANES < cnt);
+    }
 }
 
 template <int LANES, class T>
@@ -772,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
     if (!MIXED) {
         Acc64 q;
         const double4 pi = a.pos[tt];
-        for (int k = sub; k < cnt; k += LANES) pair64<LFS>(a, pi, pos_of(entry(k)), q);
+        walk_lane64<LFS, LANES, kFp64Batch>(a, pi, sub, cnt, entry, pos_of, q);
         fx = group_sum<LANES>(q.fx);
         fy = group_sum<LANES>(q.fy);
         fz = group_sum<LANES>(q.fz);
