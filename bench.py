@@ -604,7 +604,7 @@ def main() -> int:
                            "fp32_TFLOP_s": peak_fp32 / 1e12},
         "peaks_published": {"hbm_TB_s": PUB_HBM / 1e12, "fp64_TFLOP_s": PUB_FP64 / 1e12,
                             "fp32_TFLOP_s": PUB_FP32 / 1e12},
-        "roofline": {"bound": "valu", "kernel": "k_lj (LJ pair kernel, mixed)",
+        "roofline": {"bound": "valu", "kernel": "k_lj_lanes (LJ pair kernel, mixed, 4 lanes per atom)",
                      "achieved": achieved / 1e12, "peak": peak_fp32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak_fp32,
                      "frac_of_published": achieved / PUB_FP32,

@@ -363,48 +363,42 @@ __device__ __forceinline__ void mix_term(const MixK& m, double dx, double dy, do
     q.k += in ? 1 : 0;
 }
 
-// Entry from fixed-point positions. CUBIC: one unit on every axis, so the
-// integer differences are the scaled lengths (s = u); otherwise they are
-// scaled to real lengths per axis (s = 1). The two orientations of a pair
-// give integer differences of opposite sign, hence the same r2 bits and the
-// same decision in both rows. A decision inside the band (hit) sends the row
-// to the exact evaluation.
-template <bool LFS, bool CUBIC>
-__device__ __forceinline__ void mix_fixed(const ForceArgs& a, const int4& qi, const int4& qj,
-                                          AccMix& q, bool& hit) {
-    double dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
-    double dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
-    double dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
+// Entry from fixed-point positions (mixed kernels; an entry is the partner's
+// byte offset from the array base, so a gather is a scalar base plus one
+// vector register, and only x, y, z are read). CUBIC: one unit on every
+// axis, so the integer differences are the scaled lengths (s = u); otherwise
+// they are scaled to real lengths per axis (s = 1). The two orientations of
+// a pair give integer differences of opposite sign, hence the same r2 bits
+// and the same decision in both rows.
+template <bool CUBIC>
+__device__ __forceinline__ void fixed_delta(const ForceArgs& a, const int4& qi, const int4& qj,
+                                            double& dx, double& dy, double& dz) {
+    dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
+    dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
+    dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
     if (!CUBIC) {
         dx *= a.quant.unit[0];
         dy *= a.quant.unit[1];
         dz *= a.quant.unit[2];
     }
-    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const bool in = r2 < a.mq.lo;
-    hit = hit || (!in && r2 < a.mq.hi);
-    mix_term<LFS>(a.mq, dx, dy, dz, r2, in, q);
 }
 
-// mix_fixed for an entry of a row's last octet, which may lie past the row
-// end (live false: it enters as outside the cutoff and outside the band).
-template <bool LFS, bool CUBIC>
-__device__ __forceinline__ void mix_fixed_masked(const ForceArgs& a, const int4& qi, const int4& qj,
-                                                 bool live, AccMix& q, bool& hit) {
-    double dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
-    double dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
-    double dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
-    if (!CUBIC) {
-        dx *= a.quant.unit[0];
-        dy *= a.quant.unit[1];
-        dz *= a.quant.unit[2];
-    }
-    // a dead entry may read the row's own atom (r2 = 0): it enters with
-    // r2 = 1 so that its zero terms stay finite
-    const double r2 = live ? fma(dz, dz, fma(dy, dy, dx * dx)) : 1.0;
-    const bool in = live && r2 < a.mq.lo;
-    hit = hit || (live && !in && r2 < a.mq.hi);
-    mix_term<LFS>(a.mq, dx, dy, dz, r2, in, q);
+// Band hits are found by counting: q.k counts the entries with r2 < lo (in
+// the cutoff) and mix_fixed_kb counts those with r2 < hi in kb, so a row saw
+// a decision inside the band, and goes to the exact evaluation, exactly when
+// the two differ. Both counts are per-lane carry adds, which keeps the walk
+// free of per-entry boolean chains. MASKED: live false enters as a dead
+// entry with r2 = 4 hi, beyond both bounds, so it adds to neither count and
+// its terms are exact zeros.
+template <bool LFS, bool CUBIC, bool MASKED>
+__device__ __forceinline__ void mix_fixed_kb(const ForceArgs& a, const int4& qi, const int4& qj,
+                                             bool live, AccMix& q, int& kb) {
+    double dx, dy, dz;
+    fixed_delta<CUBIC>(a, qi, qj, dx, dy, dz);
+    const double r2f = fma(dz, dz, fma(dy, dy, dx * dx));
+    const double r2 = MASKED ? (live ? r2f : 4.0 * a.mq.hi) : r2f;
+    kb += r2 < a.mq.hi ? 1 : 0;
+    mix_term<LFS>(a.mq, dx, dy, dz, r2, r2 < a.mq.lo, q);
 }
 
 // Entry from the FP64 positions with the reference's exact cutoff test (the
@@ -462,158 +456,6 @@ __device__ __forceinline__ void load_octet(const uint32_t* p, u32x4& lo, u32x4& 
     hi = __builtin_nontemporal_load(v + 1);
 }
 
-// Row walk over the octet layout. Per octet: the eight partner gathers are
-// issued first (gather), then the next octet's indices (loads return in
-// issue order, so waiting for a gather never waits for the index stream,
-// which comes from HBM), then the eight pair terms are evaluated in row order
-// (eval). The last octet's indices are requested once more in place of a
-// next one, so the loop has no branch around its loads. one(e) handles an
-// entry of the last, partial octet.
-template <class P, class GatherFn, class EvalFn, class OneFn>
-__device__ __forceinline__ void walk_row(const ForceArgs& a, long t, int cnt, GatherFn gather,
-                                         EvalFn eval, OneFn one) {
-    const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
-    const int full = cnt >> 3;
-    u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
-    if (full > 0) load_octet(row, lo, hi);
-    for (int g = 0; g < full; ++g) {
-        P p[8];
-        gather(lo, p);
-        gather(hi, p + 4);
-        u32x4 nlo, nhi;
-        load_octet(row + min(g + 1, full - 1) * 512, nlo, nhi);
-        eval(lo, p);
-        eval(hi, p + 4);
-        lo = nlo;
-        hi = nhi;
-    }
-    const int rem = cnt & 7;
-    if (rem) {
-        load_octet(row + full * 512, lo, hi);
-        const uint32_t e[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-#pragma unroll
-        for (int r = 0; r < 7; ++r)
-            if (r < rem) one(e[r]);
-    }
-}
-
-// Partners from the FP64 positions: an entry is slot << 4, a double4 lies at
-// byte offset slot << 5.
-template <class PairFn>
-__device__ __forceinline__ void walk_row_pos(const ForceArgs& a, long t, int cnt, PairFn pf) {
-    const char* pb = reinterpret_cast<const char*>(a.pos);
-    auto at = [&](uint32_t e) { return *reinterpret_cast<const double4*>(pb + (static_cast<size_t>(e) << 1)); };
-    walk_row<double4>(
-        a, t, cnt,
-        [&](const u32x4& j, double4* p) {
-            p[0] = at(j.x);
-            p[1] = at(j.y);
-            p[2] = at(j.z);
-            p[3] = at(j.w);
-        },
-        [&](const u32x4&, const double4* p) {
-            pf(p[0]);
-            pf(p[1]);
-            pf(p[2]);
-            pf(p[3]);
-        },
-        [&](uint32_t e) { pf(at(e)); });
-}
-
-// Partners from the fixed-point positions (mixed kernel): an entry is the
-// partner's byte offset from the array base, so a gather is a scalar base
-// plus one vector register. Only x, y, z are read.
-//
-// The row's last octet, full or not, is evaluated as a whole octet: its
-// slots past the row end are replaced by the row atom's own entry before
-// the gathers (an in-bounds read of a position already in L1) and enter as
-// dead entries, which add exactly nothing. A wave then runs as many octets
-// as its longest row needs, with eight gathers in flight in each, instead
-// of the full octets of its longest row followed by up to seven single
-// entries, each waiting for its own gather.
-//
-// Band hits are found by counting: q.k counts the entries with r2 < lo (in
-// the cutoff) and the walk returns the count of entries with r2 < hi, so the
-// row saw a decision inside the band exactly when the two differ. Both
-// counts are per-lane carry adds, which keeps the walk free of per-entry
-// boolean chains.
-template <bool CUBIC>
-__device__ __forceinline__ void fixed_delta(const ForceArgs& a, const int4& qi, const int4& qj,
-                                            double& dx, double& dy, double& dz) {
-    dx = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.x) - static_cast<uint32_t>(qj.x)));
-    dy = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.y) - static_cast<uint32_t>(qj.y)));
-    dz = static_cast<double>(static_cast<int>(static_cast<uint32_t>(qi.z) - static_cast<uint32_t>(qj.z)));
-    if (!CUBIC) {
-        dx *= a.quant.unit[0];
-        dy *= a.quant.unit[1];
-        dz *= a.quant.unit[2];
-    }
-}
-
-// mix_fixed / mix_fixed_masked counting the entries below the band's upper
-// bound in kb. MASKED: live false enters as a dead entry with r2 = 4 hi,
-// beyond both bounds, so it adds to neither count and its terms are exact
-// zeros.
-template <bool LFS, bool CUBIC, bool MASKED>
-__device__ __forceinline__ void mix_fixed_kb(const ForceArgs& a, const int4& qi, const int4& qj,
-                                             bool live, AccMix& q, int& kb) {
-    double dx, dy, dz;
-    fixed_delta<CUBIC>(a, qi, qj, dx, dy, dz);
-    const double r2f = fma(dz, dz, fma(dy, dy, dx * dx));
-    const double r2 = MASKED ? (live ? r2f : 4.0 * a.mq.hi) : r2f;
-    kb += r2 < a.mq.hi ? 1 : 0;
-    mix_term<LFS>(a.mq, dx, dy, dz, r2, r2 < a.mq.lo, q);
-}
-
-template <bool LFS, bool CUBIC>
-__device__ __forceinline__ int walk_row_fixed(const ForceArgs& a, long t, int cnt, const int4& qi,
-                                              AccMix& q) {
-    const char* qb = reinterpret_cast<const char*>(a.qpos);
-    auto at = [&](uint32_t e) { return *reinterpret_cast<const int4*>(qb + e); };
-    const uint32_t* row = a.nbr + oct_index(t, 0, a.cap);
-    const int noct = (cnt + 7) >> 3;
-    int kb = 0;
-    u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
-    if (noct > 0) load_octet(row, lo, hi);
-    for (int g = 0; g + 1 < noct; ++g) {
-        const int4 p0 = at(lo.x), p1 = at(lo.y), p2 = at(lo.z), p3 = at(lo.w);
-        const int4 p4 = at(hi.x), p5 = at(hi.y), p6 = at(hi.z), p7 = at(hi.w);
-        u32x4 nlo, nhi;
-        load_octet(row + (g + 1) * 512, nlo, nhi);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p0, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p1, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p2, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p3, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p4, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p5, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p6, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p7, true, q, kb);
-        lo = nlo;
-        hi = nhi;
-    }
-    if (noct > 0) {
-        const int live = cnt - 8 * (noct - 1);  // 1..8 entries of the last octet
-        const uint32_t self = entry_of(static_cast<int>(t));
-        const int4 p0 = at(lo.x);               // live >= 1
-        const int4 p1 = at(live > 1 ? lo.y : self);
-        const int4 p2 = at(live > 2 ? lo.z : self);
-        const int4 p3 = at(live > 3 ? lo.w : self);
-        const int4 p4 = at(live > 4 ? hi.x : self);
-        const int4 p5 = at(live > 5 ? hi.y : self);
-        const int4 p6 = at(live > 6 ? hi.z : self);
-        const int4 p7 = at(live > 7 ? hi.w : self);
-        mix_fixed_kb<LFS, CUBIC, false>(a, qi, p0, true, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p1, live > 1, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p2, live > 2, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p3, live > 3, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p4, live > 4, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p5, live > 5, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p6, live > 6, q, kb);
-        mix_fixed_kb<LFS, CUBIC, true>(a, qi, p7, live > 7, q, kb);
-    }
-    return kb;
-}
-
 // The workgroup's partials of a pair kernel in one LDS exchange: each wave's
 // shuffle sums, then the waves in index order on thread 0 (the order of
 // block_reduce, so the same bits), for the values the precision uses (FP64:
@@ -651,90 +493,31 @@ __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, doubl
     }
 }
 
-// QP: the mixed kernel reads fixed-point positions (a.qpos) for the partners.
-template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK>
-__global__ __launch_bounds__(kBlock) void k_lj(ForceArgs a) {
-    const long blk = xcd_block();
-    if (blk * kBlock >= a.n) return;  // padding block of the XCD-aware grid (whole block)
-    if (a.sc->overflow) return;       // a truncated list was built: md_run restores and retries
-    const long t = blk * kBlock + threadIdx.x;
-    double e = 0.0, w = 0.0, ke = 0.0;
-    long long ef = 0, wf = 0, nt = 0;
-    if (t < a.n) {
-        const int cnt = min(a.ncnt[t], a.cap);
-        double fx, fy, fz;
-        if (!MIXED) {
-            Acc64 q;
-            const double4 pi = a.pos[t];
-            walk_row_pos(a, t, cnt, [&](const double4& pj) { pair64<LFS>(a, pi, pj, q); });
-            fx = q.fx;
-            fy = q.fy;
-            fz = q.fz;
-            e = q.e;
-            w = q.w;
-            nt = q.n;
-        } else {
-            AccMix q;
-            const MixK& m = QP ? a.mq : a.mx;
-            if (QP) {
-                const int4 qi = a.qpos[t];
-                const int kb = walk_row_fixed<LFS, CUBIC>(a, t, cnt, qi, q);
-                if (kb != q.k) {  // rare: a decision inside the band, the row again exactly
-                    q = AccMix{};
-                    const double4 pi = a.pos[t];
-                    walk_row_pos(a, t, cnt, [&](const double4& pj) { mix_exact<LFS>(a, a.mq, pi, pj, q); });
-                }
-            } else {
-                const double4 pi = a.pos[t];
-                walk_row_pos(a, t, cnt, [&](const double4& pj) { mix_exact<LFS>(a, a.mx, pi, pj, q); });
-            }
-            bool sat = false;
-            const FixRow r = mix_out<LFS>(m, q, sat);
-            if (sat) atomicOr(&a.sc->fix_saturated, 1);
-            fx = static_cast<double>(r.fx) * kFixInv;
-            fy = static_cast<double>(r.fy) * kFixInv;
-            fz = static_cast<double>(r.fz) * kFixInv;
-            ef = r.e;
-            wf = r.w;
-            nt = q.k;
-        }
-        a.frc[t] = make_double4(fx, fy, fz, 0.0);
-        if (KICK) {
-            double4 v = a.vel[t];
-            const double ch = a.half_dt * (1.0 / v.w);
-            v.x = fma(ch, fx, v.x);
-            v.y = fma(ch, fy, v.y);
-            v.z = fma(ch, fz, v.z);
-            a.vel[t] = v;
-            ke = (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
-        }
-    }
-    // workgroup partials (the in-cutoff term count is twice the pair count)
-    write_partials<MIXED>(a, blk, e, w, ke, ef, wf, nt);
-}
-
-// ---- small systems: several lanes per atom ----
+// ---- the Verlet-list pair kernel: several lanes per atom ----
 //
-// Below kLanesMaxAtoms atoms one thread per atom leaves most of the chip idle
-// and every thread walks a dependent chain of ~80 gathers. LANES consecutive
-// lanes then share an atom: lane s takes entries s, s + LANES, ... of its row,
-// and the group's sums are combined by shuffles in a fixed butterfly (FP64:
+// LANES consecutive lanes share an atom: lane s takes entries s, s + LANES,
+// ... of its row, a few at a time (walk_lane64, walk_lane_fixed), and the
+// group's sums are combined by shuffles in a fixed butterfly (FP64:
 // deterministic; mixed: each lane's share becomes fixed point first and the
 // group adds integers). A group redoes its row exactly when any of its lanes
 // saw a band hit, which is the row's own condition.
 //
-// Above that size the mixed kernel keeps one lane per atom: its
-// octet-batched walk keeps eight 12-byte gathers in flight per thread. The
-// FP64 kernel gathers 32 bytes per partner and keeps its registers low
-// (about 80, six waves per SIMD) with kFp64Lanes lanes per atom, each taking
-// its entries kFp64Batch at a time (walk_lane64). At 1M atoms, measured per
-// launch: 8 lanes x 2 entries 212 us, 4 x 4 214 us, 8 x 4 229 us, 16 x 2
-// 296 us, and 4 lanes one entry at a time 264 us.
+// Up to kLanesMaxAtoms atoms the lane count fills the chip (16 while
+// 16 N <= 262,144, halved until it fits). Above it the split is the
+// measured best at 1M atoms (per launch; one lane per atom walking whole
+// octets of 8 entries with the next octet's indices in flight took 160 us
+// mixed, and one entry at a time per lane 264 us FP64):
+//   FP64 (32-byte partners):  8 lanes x 2 entries 212 us, 4 x 4 214 us,
+//                             8 x 4 229 us, 16 x 2 296 us;
+//   mixed (12-byte partners): 4 lanes x 4 entries 153 us, 4 x 8 152 us,
+//                             4 x 6 155 us, 4 x 2 171 us, 8 x 4 180 us,
+//                             2 x 8 187 us, 2 x 4 197 us, 16 x 2 304 us.
 constexpr long kLanesMaxAtoms = 32768;
 constexpr int kFp64Lanes = 8;
+constexpr int kMixLanes = 4;
 
 inline int lj_lanes(long n, bool mixed) {
-    if (n > kLanesMaxAtoms) return mixed ? 1 : kFp64Lanes;
+    if (n > kLanesMaxAtoms) return mixed ? kMixLanes : kFp64Lanes;
     int l = 16;
     while (l > 1 && static_cast<long>(l) * n > 262144) l >>= 1;
     return l;
@@ -778,6 +561,47 @@ __device__ __forceinline__ void walk_lane64(const ForceArgs& a, const double4& p
     }
 }
 
+// The fixed-point counterpart of walk_lane64 (mixed lanes kernel): BATCH
+// entries of the lane at a time, full batches unmasked, the last one masked
+// (its slots past the row end read the lane's first entry again and enter
+// as dead entries). Returns the lane's count of entries below the band's
+// upper bound (mix_fixed_kb); q.k counts those inside the cutoff.
+constexpr int kMixBatch = 4;
+
+template <bool LFS, bool CUBIC, int LANES, int BATCH, class EntryFn, class AtFn>
+__device__ __forceinline__ int walk_lane_fixed(const ForceArgs& a, const int4& qi, int sub, int cnt,
+                                               EntryFn entry, AtFn at, AccMix& q) {
+    constexpr int kStep = BATCH * LANES, kLast = (BATCH - 1) * LANES;
+    int kb = 0;
+    uint32_t e[BATCH];
+    auto indices = [&](int k0) {  // k0 < cnt
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int k = k0 + b * LANES;
+            e[b] = entry(k < cnt ? k : sub);
+        }
+    };
+    if (sub < cnt) indices(sub);
+    int k = sub;
+    for (; k + kLast < cnt; k += kStep) {
+        int4 p[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) p[b] = at(e[b]);
+        indices(k + kStep < cnt ? k + kStep : k);  // the next batch's (this one's after the last)
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) mix_fixed_kb<LFS, CUBIC, false>(a, qi, p[b], true, q, kb);
+    }
+    if (k < cnt) {  // the last, partial batch (its indices are in e)
+        int4 p[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) p[b] = at(e[b]);
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+            mix_fixed_kb<LFS, CUBIC, true>(a, qi, p[b], k + b * LANES < cnt, q, kb);
+    }
+    return kb;
+}
+
 template <int LANES, class T>
 __device__ __forceinline__ T group_sum(T v) {
     for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -786,8 +610,10 @@ __device__ __forceinline__ T group_sum(T v) {
 
 template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK, int LANES>
 __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
+    const long blk = xcd_block();
+    if (blk * kBlock >= static_cast<long>(LANES) * a.n) return;  // padding block of the XCD-aware grid
     if (a.sc->overflow) return;  // a truncated list was built: md_run restores and retries
-    const long gt = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long gt = blk * kBlock + threadIdx.x;
     const long t = gt / LANES;
     const int sub = static_cast<int>(gt & (LANES - 1));
     // a group is whole lanes of one wave (LANES divides 64); every lane of the
@@ -823,10 +649,9 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         if (QP) {
             const int4 qi = a.qpos[tt];
             const char* qb = reinterpret_cast<const char*>(a.qpos);
-            bool hit = false;
-            for (int k = sub; k < cnt; k += LANES)
-                mix_fixed<LFS, CUBIC>(a, qi, *reinterpret_cast<const int4*>(qb + entry(k)), q, hit);
-            if (group_sum<LANES>(hit ? 1 : 0) != 0) {
+            auto at = [&](uint32_t off) { return *reinterpret_cast<const int4*>(qb + off); };
+            const int kb = walk_lane_fixed<LFS, CUBIC, LANES, kMixBatch>(a, qi, sub, cnt, entry, at, q);
+            if (group_sum<LANES>(kb != q.k ? 1 : 0) != 0) {  // a band hit in the group's row
                 q = AccMix{};
                 exact(a.mq);
             }
@@ -858,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         e = w = 0.0;
         ef = wf = nt = 0;
     }
-    write_partials<MIXED>(a, blockIdx.x, e, w, ke, ef, wf, nt);
+    write_partials<MIXED>(a, blk, e, w, ke, ef, wf, nt);
 }
 
 
@@ -870,10 +695,10 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
 // whose entries are halo indices: every partner read is an LDS read instead
 // of a gather through the vector L1. The halo rows are contiguous slot
 // ranges (two where a row wraps in x), copied by one wave each with
-// consecutive lanes on consecutive slots. The rows, their order and the
-// arithmetic are those of k_lj (mixed) and k_lj_lanes (FP64), so forces,
-// energies and virials are bit for bit the same (FP64 E and W: the same up
-// to the grouping of the workgroup sums).
+// consecutive lanes on consecutive slots. The rows, their order, the lanes
+// per atom and the arithmetic are those of k_lj_lanes, so forces, energies
+// and virials are bit for bit the same (FP64 E and W: the same up to the
+// grouping of the workgroup sums).
 constexpr int kBrickMaxWaves = 8;
 
 __device__ __forceinline__ int wave_incl_sum(int x, int lane) {
@@ -986,12 +811,13 @@ __device__ __forceinline__ int own_row(const OwnRows& s, int nr, int k) {
     return lo;
 }
 
-// Mixed: the halo holds (X, Y, Z, slot) per atom, 16 bytes; one lane per
-// own atom walks its row as k_lj does (same entries in the same order, so
-// the same bits). Entries of a row's last octet past its end read halo atom
-// 0 and enter as outside the cutoff.
+// Mixed: the halo holds (X, Y, Z, slot) per atom, 16 bytes; kMixLanes lanes
+// per own atom walk its row as k_lj_lanes does (walk_lane_fixed, partners
+// from LDS), so the same entries in the same order per lane and the same
+// bits.
 template <bool LFS, bool CUBIC, bool KICK>
 __global__ __launch_bounds__(kBrickMaxWaves * kWave) void k_lj_brick(ForceArgs a) {
+    constexpr int LANES = kMixLanes;
     extern __shared__ int4 halo[];
     __shared__ OwnRows own;
     __shared__ HaloRows hr;
@@ -1012,83 +838,52 @@ __global__ __launch_bounds__(kBrickMaxWaves * kWave) void k_lj_brick(ForceArgs a
     const int nown = own.pre[nr];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const int sub = lane & (LANES - 1);
+    constexpr int kPer = kWave / LANES;  // own atoms per wave chunk
     const MixK& m = a.mq;
+    auto at = [&](uint32_t e) { return halo[e]; };
     double ke = 0.0;
     long long ef = 0, wf = 0, nt = 0;
-    for (int c = wv; c * kWave < nown; c += nw) {
-        const int k = c * kWave + lane;
-        if (k >= nown) break;
-        const int r = own_row(own, nr, k);
-        const int i = own.g0[r] + (k - own.pre[r]);
-        const int4 qi = halo[own.l0[r] + (k - own.pre[r])];
-        const int cnt = min(a.ncnt[i], a.cap);
-        AccMix q;
-        bool hit = false;
-        // octets with the next one's indices in flight, partners four at a
-        // time from LDS
+    for (int c = wv; c * kPer < nown; c += nw) {
+        const int k = c * kPer + lane / LANES;
+        const bool live = k < nown;
+        const int kk = live ? k : 0;
+        const int r = own_row(own, nr, kk);
+        const int i = own.g0[r] + (kk - own.pre[r]);
+        const int4 qi = halo[own.l0[r] + (kk - own.pre[r])];
+        const int cnt = live ? min(a.ncnt[i], a.cap) : 0;
         const uint32_t* row = a.nbr + oct_index(i, 0, a.cap);
-        const int noct = (cnt + 7) >> 3;
-        u32x4 lo = {0u, 0u, 0u, 0u}, hi = {0u, 0u, 0u, 0u};
-        if (noct > 0) load_octet(row, lo, hi);
-#pragma unroll 1
-        for (int g = 0; g + 1 < noct; ++g) {
-            u32x4 nlo, nhi;
-            load_octet(row + (g + 1) * 512, nlo, nhi);
-#pragma unroll 1
-            for (int h = 0; h < 2; ++h) {
-                const u32x4 j = h ? hi : lo;
-                const int4 p0 = halo[j.x], p1 = halo[j.y], p2 = halo[j.z], p3 = halo[j.w];
-                mix_fixed<LFS, CUBIC>(a, qi, p0, q, hit);
-                mix_fixed<LFS, CUBIC>(a, qi, p1, q, hit);
-                mix_fixed<LFS, CUBIC>(a, qi, p2, q, hit);
-                mix_fixed<LFS, CUBIC>(a, qi, p3, q, hit);
-            }
-            lo = nlo;
-            hi = nhi;
-        }
-        if (noct > 0) {  // the last octet, past-the-end entries masked
-#pragma unroll 1
-            for (int h = 0; h < 2; ++h) {
-                const u32x4 j = h ? hi : lo;
-                const int base = 8 * (noct - 1) + 4 * h;
-                const uint32_t e0 = base < cnt ? j.x : 0u;
-                const uint32_t e1 = base + 1 < cnt ? j.y : 0u;
-                const uint32_t e2 = base + 2 < cnt ? j.z : 0u;
-                const uint32_t e3 = base + 3 < cnt ? j.w : 0u;
-                const int4 p0 = halo[e0], p1 = halo[e1], p2 = halo[e2], p3 = halo[e3];
-                mix_fixed_masked<LFS, CUBIC>(a, qi, p0, base < cnt, q, hit);
-                mix_fixed_masked<LFS, CUBIC>(a, qi, p1, base + 1 < cnt, q, hit);
-                mix_fixed_masked<LFS, CUBIC>(a, qi, p2, base + 2 < cnt, q, hit);
-                mix_fixed_masked<LFS, CUBIC>(a, qi, p3, base + 3 < cnt, q, hit);
-            }
-        }
-        if (hit) {  // rare: a decision inside the band, the row again exactly
+        auto entry = [&](int j) { return row[(j >> 3) * 512 + (j & 7)]; };
+        AccMix q;
+        const int kb = walk_lane_fixed<LFS, CUBIC, LANES, kMixBatch>(a, qi, sub, cnt, entry, at, q);
+        if (group_sum<LANES>(kb != q.k ? 1 : 0) != 0) {  // rare: a band hit, the row again exactly
             q = AccMix{};
             const double4 pi = a.pos[i];
-#pragma unroll 1
-            for (int k2 = 0; k2 < cnt; ++k2) {
-                const uint32_t e = row[(k2 >> 3) * 512 + (k2 & 7)];
-                mix_exact<LFS>(a, m, pi, a.pos[halo[e].w], q);
-            }
+            for (int j = sub; j < cnt; j += LANES) mix_exact<LFS>(a, m, pi, a.pos[halo[entry(j)].w], q);
         }
         bool sat = false;
         const FixRow fr = mix_out<LFS>(m, q, sat);
-        if (sat) atomicOr(&a.sc->fix_saturated, 1);
-        const double fx = static_cast<double>(fr.fx) * kFixInv;
-        const double fy = static_cast<double>(fr.fy) * kFixInv;
-        const double fz = static_cast<double>(fr.fz) * kFixInv;
-        ef += fr.e;
-        wf += fr.w;
-        nt += q.k;
-        a.frc[i] = make_double4(fx, fy, fz, 0.0);
-        if (KICK) {
-            double4 v = a.vel[i];
-            const double ch = a.half_dt * (1.0 / v.w);
-            v.x = fma(ch, fx, v.x);
-            v.y = fma(ch, fy, v.y);
-            v.z = fma(ch, fz, v.z);
-            a.vel[i] = v;
-            ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+        if (live && sat) atomicOr(&a.sc->fix_saturated, 1);
+        const double fx = static_cast<double>(group_sum<LANES>(fr.fx)) * kFixInv;
+        const double fy = static_cast<double>(group_sum<LANES>(fr.fy)) * kFixInv;
+        const double fz = static_cast<double>(group_sum<LANES>(fr.fz)) * kFixInv;
+        const long long ge = group_sum<LANES>(fr.e);
+        const long long gw = group_sum<LANES>(fr.w);
+        const long long gn = group_sum<LANES>(static_cast<long long>(q.k));
+        if (live && sub == 0) {
+            ef += ge;
+            wf += gw;
+            nt += gn;
+            a.frc[i] = make_double4(fx, fy, fz, 0.0);
+            if (KICK) {
+                double4 v = a.vel[i];
+                const double ch = a.half_dt * (1.0 / v.w);
+                v.x = fma(ch, fx, v.x);
+                v.y = fma(ch, fy, v.y);
+                v.z = fma(ch, fz, v.z);
+                a.vel[i] = v;
+                ke += (0.5 * v.w) * norm2_ref(v.x, v.y, v.z);
+            }
         }
     }
     write_partials<true, kBrickMaxWaves>(a, b, 0.0, 0.0, ke, ef, wf, nt);
@@ -1189,29 +984,23 @@ __global__ __launch_bounds__(kFinish) void k_lj_finish(ForceArgs a, int nb) {
     finish_step(a, a.mixed != 0, nb);
 }
 
+// lj_lanes is 16 or 8 up to kLanesMaxAtoms atoms, kMixLanes or kFp64Lanes above
 template <bool LFS, bool MIXED, bool QP, bool CUBIC, bool KICK>
 void dispatch_lanes(const ForceArgs& a, hipStream_t s) {
-    const dim3 g(static_cast<unsigned>(lj_groups(a.n, MIXED))), b(kBlock);
+    const dim3 g(xcd_grid(static_cast<long>(lj_lanes(a.n, MIXED)) * a.n)), b(kBlock);
     switch (lj_lanes(a.n, MIXED)) {
         case 16: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 16>), g, b, 0, s, a); break;
         case 8: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 8>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 4>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 2>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_lj_lanes<LFS, MIXED, QP, CUBIC, KICK, 4>), g, b, 0, s, a); break;
     }
 }
 
 template <bool LFS, bool MIXED, bool QP, bool CUBIC>
-void dispatch_kick(const ForceArgs& a, dim3 g, hipStream_t s) {
-    if (lj_lanes(a.n, MIXED) > 1) {
-        if (a.kick)
-            dispatch_lanes<LFS, MIXED, QP, CUBIC, true>(a, s);
-        else
-            dispatch_lanes<LFS, MIXED, QP, CUBIC, false>(a, s);
-    } else if (a.kick) {
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, CUBIC, true>), g, dim3(kBlock), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((k_lj<LFS, MIXED, QP, CUBIC, false>), g, dim3(kBlock), 0, s, a);
-    }
+void dispatch_kick(const ForceArgs& a, hipStream_t s) {
+    if (a.kick)
+        dispatch_lanes<LFS, MIXED, QP, CUBIC, true>(a, s);
+    else
+        dispatch_lanes<LFS, MIXED, QP, CUBIC, false>(a, s);
 }
 
 // ---- direct (no list) evaluations: brute force and cells at the cutoff ----
@@ -1471,19 +1260,18 @@ void launch_lj_force(const ForceArgs& a, hipStream_t s) {
         else launch_brick<false>(a, s);
         return;
     }
-    const dim3 g(xcd_grid(a.n));
     const bool qp = a.qpos != nullptr;
     const bool cubic = a.quant.cubic != 0;
     if (a.lj.lfs) {
-        if (!a.mixed) dispatch_kick<true, false, false, false>(a, g, s);
-        else if (qp && cubic) dispatch_kick<true, true, true, true>(a, g, s);
-        else if (qp) dispatch_kick<true, true, true, false>(a, g, s);
-        else dispatch_kick<true, true, false, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<true, false, false, false>(a, s);
+        else if (qp && cubic) dispatch_kick<true, true, true, true>(a, s);
+        else if (qp) dispatch_kick<true, true, true, false>(a, s);
+        else dispatch_kick<true, true, false, false>(a, s);
     } else {
-        if (!a.mixed) dispatch_kick<false, false, false, false>(a, g, s);
-        else if (qp && cubic) dispatch_kick<false, true, true, true>(a, g, s);
-        else if (qp) dispatch_kick<false, true, true, false>(a, g, s);
-        else dispatch_kick<false, true, false, false>(a, g, s);
+        if (!a.mixed) dispatch_kick<false, false, false, false>(a, s);
+        else if (qp && cubic) dispatch_kick<false, true, true, true>(a, s);
+        else if (qp) dispatch_kick<false, true, true, false>(a, s);
+        else dispatch_kick<false, true, false, false>(a, s);
     }
 }
 

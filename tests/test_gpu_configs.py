@@ -9,7 +9,8 @@ inputs:
 * orthorhombic boxes (sides in the ratio 7 : 11 : 13 as the reference's
   tests/test_box.cpp:18 uses) above and below the one-lane threshold;
 * crystallites of 32,768 and 32,769 atoms, either side of the size where the
-  pair kernels and the list build change from several lanes per atom to one;
+  pair kernels change their lanes per atom and the list build from eight
+  lanes per slot to one;
 * the 864-atom NVE run (1000 velocity-Verlet steps) against the oracle;
 * all 256 replicas of the batched recipe, primed forces against the oracle;
 * SPC/E water at its configuration (4096 molecules, 12,288 sites, rc 9 A,
@@ -232,14 +233,15 @@ def test_orthorhombic_box(gpu_ok, orc, sites):
         ctx.close()
 
 
-def test_one_lane_threshold(gpu_ok, orc):
-    """A 32^3 simple-cubic crystallite (32,768 atoms) in a larger box runs the
-    pair kernels and the list build with several lanes per atom; with one
-    isolated extra atom (32,769) the mixed pair kernel and the build run one
-    lane per atom and the FP64 kernel four. Both match the oracle (FP64 to
-    1e-12, pair sets bit-exact), and the mixed forces of the two agree within
-    eight units of the 2^-32 fixed-point grid (each of the eight lanes' row
-    sums is rounded to the grid once, as is the one lane's)."""
+def test_lane_threshold(gpu_ok, orc):
+    """A 32^3 simple-cubic crystallite (32,768 atoms) in a larger box runs both
+    pair kernels with 8 lanes per atom and the list build with 8 lanes per
+    slot; with one isolated extra atom (32,769) the mixed pair kernel runs 4
+    lanes per atom, the FP64 kernel 8 and the build one thread per slot. Both
+    match the oracle (FP64 to 1e-12, pair sets bit-exact), and the mixed
+    forces of the two agree within eight units of the 2^-32 fixed-point grid
+    (each lane's share of a row is rounded to the grid once: at most 8 + 4
+    half units apart)."""
     from moleculardynamics_amd import native
 
     m = 32

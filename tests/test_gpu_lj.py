@@ -408,8 +408,8 @@ def test_lane_split_16_and_8_lanes(gpu_ok, orc):
     a mixed row becomes 64-bit fixed point (2^-32) before the lanes are added,
     so the two mixed results agree within half a unit per lane (16 units at
     most); the FP64 results agree to rounding, and both match the oracle. The
-    change to one lane per atom above 32,768 atoms is checked by
-    test_gpu_configs.py::test_one_lane_threshold."""
+    change of lanes per atom above 32,768 atoms is checked by
+    test_gpu_configs.py::test_lane_threshold."""
     from moleculardynamics_amd import native
     from oracle.pyoracle import Frame
 
