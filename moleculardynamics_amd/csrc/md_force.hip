@@ -323,11 +323,15 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, co
 // band. The LJ powers are FP32: s2 = sigma^2 / r2 (hardware reciprocal and one
 // Newton step), s6 = s2^3, t = 2 s6^2 - s6 (one rounding); c = t / r2. The
 // force term c d is one FP64 FMA per axis into the row's sum, and the row
-// also sums s6 and t in FP64 (E = 2 eps (T - S6), W = 24 eps T per row). FP32 values have 24-bit mantissas, so these FP64 row sums are
-// exact to about 2^-53 relative and their order (list order) is fixed. Each
-// row's force, energy and virial then become 64-bit fixed point (2^-32
-// units), and every combination of rows or lanes -- multi-lane groups, the
-// workgroup and global E and W -- is an integer sum, independent of order.
+// also sums s6 and t in FP64 (E = 2 eps (T - S6), W = 24 eps T per row).
+// FP32 values have 24-bit mantissas, so these FP64 sums are exact to about
+// 2^-53 relative, and their order (the lane's walk order) is fixed. Each
+// lane's share of a row (force, energy, virial) then becomes 64-bit fixed
+// point (2^-32 units), and every combination of lanes or rows -- the lanes
+// of a row, the workgroup and global E and W -- is an integer sum,
+// independent of order. (Summing s6 and t in FP32 over each batch of four
+// entries, one conversion per batch, cut the loop's instructions but made
+// the 1M-atom kernel 3 % slower.)
 struct AccMix {
     double fx = 0.0, fy = 0.0, fz = 0.0;  // sum of c d (scaled lengths)
     double s6 = 0.0, t = 0.0;             // sums of s^6 and t = 2 s^12 - s^6
@@ -452,19 +456,28 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // block_reduce, so the same bits), for the values the precision uses (FP64:
 // E, W, KE and the term count; mixed: the fixed-point E, W, the term count
 // and KE); the unused slots are written as 0.
-template <bool MIXED, int WAVES = kBlock / kWave>
+// STRIDE: only lanes that are multiples of STRIDE hold values (the first lane
+// of each group of a multi-lane kernel), so the shuffle steps below STRIDE
+// are skipped.
+template <int STRIDE, class T>
+__device__ __forceinline__ T wave_sum_strided(T v) {
+    for (int off = kWave / 2; off >= STRIDE; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+template <bool MIXED, int WAVES = kBlock / kWave, int STRIDE = 1>
 __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, double e, double w,
                                                double ke, long long ef, long long wf,
                                                long long nt) {
     __shared__ double xd[3][WAVES];
     __shared__ long long xl[3][WAVES];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const double v0 = MIXED ? 0.0 : wave_sum(e);
-    const double v1 = MIXED ? 0.0 : wave_sum(w);
-    const double v2 = wave_sum(ke);
-    const long long l0 = MIXED ? wave_sum_ll(ef) : 0;
-    const long long l1 = MIXED ? wave_sum_ll(wf) : 0;
-    const long long l2 = wave_sum_ll(nt);
+    const double v0 = MIXED ? 0.0 : wave_sum_strided<STRIDE>(e);
+    const double v1 = MIXED ? 0.0 : wave_sum_strided<STRIDE>(w);
+    const double v2 = wave_sum_strided<STRIDE>(ke);
+    const long long l0 = MIXED ? wave_sum_strided<STRIDE>(ef) : 0;
+    const long long l1 = MIXED ? wave_sum_strided<STRIDE>(wf) : 0;
+    const long long l2 = wave_sum_strided<STRIDE>(nt);
     if (lane == 0) {
         xd[0][wid] = v0; xd[1][wid] = v1; xd[2][wid] = v2;
         xl[0][wid] = l0; xl[1][wid] = l1; xl[2][wid] = l2;
@@ -711,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_lanes(ForceArgs a) {
         e = w = 0.0;
         ef = wf = nt = 0;
     }
-    write_partials<MIXED>(a, blk, e, w, ke, ef, wf, nt);
+    write_partials<MIXED, kBlock / kWave, LANES>(a, blk, e, w, ke, ef, wf, nt);
 }
 
 
@@ -916,7 +929,7 @@ __global__ __launch_bounds__(kBrickMaxWaves * kWave) void k_lj_brick(ForceArgs a
             }
         }
     }
-    write_partials<true, kBrickMaxWaves>(a, b, 0.0, 0.0, ke, ef, wf, nt);
+    write_partials<true, kBrickMaxWaves, LANES>(a, b, 0.0, 0.0, ke, ef, wf, nt);
 }
 
 // FP64: the halo holds each atom's FP64 position (double4 with the upload
@@ -981,7 +994,7 @@ __global__ __launch_bounds__(kBrick64MaxWaves * kWave) void k_lj_brick64(ForceAr
             }
         }
     }
-    write_partials<false, kBrick64MaxWaves>(a, b, e, w, ke, 0, 0, nt);
+    write_partials<false, kBrick64MaxWaves, LANES>(a, b, e, w, ke, 0, 0, nt);
 }
 
 // Translation of a brick list to slot entries (entry_of), same layout, for
