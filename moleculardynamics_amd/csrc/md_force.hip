@@ -142,7 +142,90 @@ struct ChainStep {
     double scale, conserved;
 };
 
+// Short chains (M <= 4, the usual 3 links) in registers: the same
+// operations in the same order as chain_half over ChainWork, with every
+// index a compile-time constant, so each link update is a register
+// dependence instead of LDS round trips. The chain runs on one thread of the
+// last-arriving workgroup, so its latency is on every step's critical path.
+
+template <int M>
+__device__ __forceinline__ ChainStep chain_step_reg(Chain* chain, double& ke, double delta, bool twice) {
+    double p[M], xi[M], iQ[M];
+    const double dof = chain->dof, T = chain->T;
+    const double sy0 = chain->sy[0], sy1 = chain->sy[1], sy2 = chain->sy[2];
+    auto load = [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        p[j] = chain->p[j];
+        xi[j] = chain->xi[j];
+        iQ[j] = chain->iQ[j];
+    };
+    unroll_up<M>(load);
+    // one MTK96 chain propagation of length delta (chain_half)
+    auto half = [&](double& k) {
+        double scale = 1.0;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const double d = (s == 0 ? sy0 : (s == 1 ? sy1 : sy2)) * delta;
+            const double d2 = 0.5 * d, d4 = 0.25 * d;
+            auto link = [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                double damp = 1.0;
+                if constexpr (j + 1 < M) damp = chain_exp(-d4 * p[j + 1] * iQ[j + 1]);
+                double g;
+                if constexpr (j == 0)
+                    g = 2.0 * k - dof * T;
+                else
+                    g = p[j - 1] * p[j - 1] * iQ[j - 1] - T;
+                p[j] = (p[j] * damp + d2 * g) * damp;
+            };
+            auto drift = [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                xi[j] += d * p[j] * iQ[j];
+            };
+            unroll_down<M>(link);
+            unroll_up<M>(drift);
+            const double f = chain_exp(-d * p[0] * iQ[0]);
+            scale *= f;
+            k *= f * f;
+            unroll_up<M>(link);
+        }
+        return scale;
+    };
+    ChainStep r;
+    r.scale = half(ke);
+    double acc = dof * T * xi[0];
+    auto cons_xi = [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr (j > 0) acc += T * xi[j];
+    };
+    auto cons_p = [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        acc += 0.5 * p[j] * p[j] * iQ[j];
+    };
+    unroll_up<M>(cons_xi);
+    unroll_up<M>(cons_p);
+    r.conserved = acc;
+    if (twice) {
+        double ke2 = ke;
+        r.scale *= half(ke2);
+    }
+    auto store = [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        chain->p[j] = p[j];
+        chain->xi[j] = xi[j];
+    };
+    unroll_up<M>(store);
+    return r;
+}
+
 __device__ ChainStep chain_step(Chain* chain, ChainWork& c, double& ke, double delta, bool twice) {
+    switch (chain->M) {
+        case 1: return chain_step_reg<1>(chain, ke, delta, twice);
+        case 2: return chain_step_reg<2>(chain, ke, delta, twice);
+        case 3: return chain_step_reg<3>(chain, ke, delta, twice);
+        case 4: return chain_step_reg<4>(chain, ke, delta, twice);
+        default: break;
+    }
     c.load(*chain);
     ChainStep r;
     r.scale = chain_half(c, ke, delta);
