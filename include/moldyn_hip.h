@@ -545,11 +545,17 @@ md_status md_mix_random_state(long n, double box_length, uint64_t seed, double t
 
 /* ---- measurement ---------------------------------------------------------- */
 
-/* When enabled, every pair-kernel launch is bracketed by HIP events on the
- * context's stream. md_timing_read returns the summed device time (ms) and
- * launch count since the last reset, for kernel id 0 = LJ pair kernel,
- * 1 = neighbour build, 2 = integrator kernels. */
+/* When enabled, every launch of the timed kernel groups is bracketed by HIP
+ * events on the context's stream. md_timing_read returns the summed device
+ * time (ms) and launch count since the last reset, for kernel id 0 = LJ pair
+ * kernel, 1 = neighbour build, 2 = integrator kernels. md_timing_enable times
+ * all three groups (on != 0) or none; md_timing_select times the groups whose
+ * bit (1 << kernel id) is set in mask. Each bracket adds two event records
+ * to the stream, a few microseconds of device time per step, and any timing
+ * keeps md_run from replaying HIP graphs. (Measurement only: the reference
+ * has no counterpart.) */
 md_status md_timing_enable(md_ctx* ctx, int on);
+md_status md_timing_select(md_ctx* ctx, int mask);
 md_status md_timing_read(md_ctx* ctx, int kernel_id, double* total_ms, long* launches);
 md_status md_timing_reset(md_ctx* ctx);
 /* Candidate-pair statistics of the last force evaluation: pairs evaluated

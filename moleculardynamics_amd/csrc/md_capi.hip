@@ -95,7 +95,7 @@ struct md_ctx {
     long snap_alloc = 0;
     uint64_t snap_step = 0;
 
-    bool timing = false;
+    int timing = 0;  // bit k: launches of kernel group k bracketed by events
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
     double ms[3] = {0, 0, 0};
     long launches[3] = {0, 0, 0};
@@ -171,13 +171,13 @@ md_status read_scalars(md_ctx* c, mdk::Scalars* out) {
 }
 
 void ev_begin(md_ctx* c, int id, hipEvent_t* a) {
-    if (!c->timing) return;
+    if (!((c->timing >> id) & 1)) return;
     (void)hipEventCreate(a);
     (void)hipEventRecord(*a, c->stream);
     (void)id;
 }
 void ev_end(md_ctx* c, int id, hipEvent_t a) {
-    if (!c->timing) return;
+    if (!((c->timing >> id) & 1)) return;
     hipEvent_t b;
     (void)hipEventCreate(&b);
     (void)hipEventRecord(b, c->stream);
@@ -1027,7 +1027,7 @@ static md_status run_steps(md_ctx* c, long nsteps, int precision, md_thermo* the
         mdk::launch_chain_first(c->chain, c->sc, 0.5 * c->integ.dt, c->stream);
     }
     hipGraphExec_t gexec = nullptr;
-    if (!c->timing && c->n <= kGraphMaxAtoms && nsteps > kGraphSegment) {
+    if (c->timing == 0 && c->n <= kGraphMaxAtoms && nsteps > kGraphSegment) {
         // Launch-bound runs replay a captured segment of kGraphSegment steps:
         // the step counter (BAOAB noise) and the thermo row come from the
         // device counter sc->step, advanced by each step's finish. Every
@@ -1256,7 +1256,13 @@ md_status md_nhc_state(md_ctx* c, double* conserved, double* xi, double* p_xi) {
 
 md_status md_timing_enable(md_ctx* c, int on) {
     if (c == nullptr) return MD_ERR_INVALID_ARGUMENT;
-    c->timing = on != 0;
+    c->timing = on != 0 ? 7 : 0;
+    return MD_OK;
+}
+
+md_status md_timing_select(md_ctx* c, int mask) {
+    if (c == nullptr || mask < 0 || mask > 7) return MD_ERR_INVALID_ARGUMENT;
+    c->timing = mask;
     return MD_OK;
 }
 

@@ -109,6 +109,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.md_run.argtypes = [vp, C.c_long, C.c_int, C.POINTER(Thermo)]
     lib.md_nhc_state.argtypes = [vp, _dp, _dp, _dp]
     lib.md_timing_enable.argtypes = [vp, C.c_int]
+    lib.md_timing_select.argtypes = [vp, C.c_int]
     lib.md_timing_read.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_long)]
     lib.md_timing_reset.argtypes = [vp]
     lib.md_last_pair_count.argtypes = [vp, C.POINTER(C.c_long)]
@@ -474,6 +475,13 @@ class Context:
     # ---- measurement -------------------------------------------------------
     def timing(self, on=True):
         self._chk(self.lib.md_timing_enable(self.h, int(on)))
+
+    # kernel groups of timing_read / timing_select
+    TIME_PAIR, TIME_BUILD, TIME_STEP = 1, 2, 4
+
+    def timing_select(self, mask):
+        """Times only the kernel groups in mask (TIME_PAIR | TIME_BUILD | TIME_STEP)."""
+        self._chk(self.lib.md_timing_select(self.h, int(mask)))
 
     def timing_read(self, kernel_id=0):
         ms, k = C.c_double(), C.c_long()

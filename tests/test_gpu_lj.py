@@ -259,6 +259,47 @@ def test_graph_segments_match_direct_launches(gpu_ok, orc, kind, prec):
             assert np.array_equal(a[key], b[key]), key
 
 
+def test_timing_select_brackets_only_the_chosen_kernels(gpu_ok, orc):
+    """md_timing_select times only the kernel groups in its mask: with the pair
+    kernel alone, every step's pair launch is counted and the build and
+    integrator groups stay at zero; the trajectory equals the one timed in
+    full (every group) bit for bit; masks outside 0..7 are refused."""
+    from moleculardynamics_amd import native
+
+    fr = orc.fcc_lattice(5, 0.8442, 0.9, 31)
+    n = fr.n
+    outs = []
+    for mask in (native.Context.TIME_PAIR, 7):
+        c = native.Context()
+        try:
+            c.upload(fr.L, _pos(fr), _vel(fr))
+            c.set_lj(1.0, 1.0, 2.5, False)
+            c.config_list(2.5, 0.3)
+            c.build_list()
+            c.forces(native.VERLET, native.MIXED)
+            c.set_integrator(kind=2, dt=0.005, temperature=0.9, tau=0.5, chain_length=3,
+                             dof=3 * n - 3)
+            c.timing_select(mask)
+            c.timing_reset()
+            th = c.run(60, native.MIXED)
+            counts = [c.timing_read(k)[1] for k in range(3)]
+            if mask == native.Context.TIME_PAIR:
+                assert counts == [60, 0, 0]
+            else:
+                assert counts[0] == 60 and counts[1] == 60 and counts[2] == 60
+            outs.append((c.download(), th))
+            for bad in (-1, 8):
+                with pytest.raises(native.MDError):
+                    c.timing_select(bad)
+        finally:
+            c.close()
+    (s0, t0), (s1, t1) = outs
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+    for key in t0:
+        assert np.array_equal(t0[key], t1[key]), key
+
+
 def test_nhc_short_trajectory_matches_oracle(ctx, orc):
     fr = orc.fcc_lattice(4, 0.8442, 0.722, 20260001)
     n = fr.n
