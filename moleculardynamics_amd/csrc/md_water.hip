@@ -38,6 +38,7 @@ constexpr int kSiteTile = 32;  // sites per LDS tile of the S(k) kernel
 constexpr int kSegLen = 8;     // k-vectors per column segment of the S(k) kernel
 constexpr int kSkSub = 8;      // lanes per segment of the S(k) kernel (sites split among them)
 constexpr int kWaterTpa = 16;  // lanes per site of the real-space kernel
+constexpr int kWaterBatch = 2;  // row entries per lane in flight in the real-space kernel
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -150,15 +151,13 @@ __global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
         const bool oi = (i % 3) == 0;
         const int cnt = min(a.ncnt[i], a.cap);
         const uint32_t* row = a.nbr + nbr_index(i, 0, a.cap);
-        for (int k = sub; k < cnt; k += kWaterTpa) {
-            const uint32_t jj = row[k * 64];
+        auto term = [&](uint32_t jj, const double4& pj) {
             const int j = static_cast<int>(jj & kIndexMask);
             const bool flip = (jj & kFlipBit) != 0;
-            const double4 pj = a.pos[j];
             double dx, dy, dz;
             pair_delta(a.box, pi, pj, flip, dx, dy, dz);
             const double r2 = norm2_ref(dx, dy, dz);
-            if (!(r2 < a.rc2)) continue;
+            if (!(r2 < a.rc2)) return;
             double w = 0.0;
             const double qq = pi.w * pj.w;
             if (MIXED) {
@@ -187,6 +186,32 @@ __global__ __launch_bounds__(kBlock) void k_wreal(WaterArgs a) {
             fx += c * dx;
             fy += c * dy;
             fz += c * dz;
+        };
+        // kWaterBatch entries of the lane at a time: their partner gathers
+        // together, then the next batch's indices, then the terms in entry
+        // order (the order one entry at a time adds them)
+        constexpr int kStep = kWaterBatch * kWaterTpa;
+        uint32_t e[kWaterBatch];
+        auto indices = [&](int k0) {  // k0 < cnt; slots past the row end reread entry sub
+#pragma unroll
+            for (int b = 0; b < kWaterBatch; ++b) {
+                const int k = k0 + b * kWaterTpa;
+                e[b] = row[(k < cnt ? k : sub) * 64];
+            }
+        };
+        if (sub < cnt) indices(sub);
+        for (int k = sub; k < cnt; k += kStep) {
+            double4 p[kWaterBatch];
+            uint32_t cur[kWaterBatch];  // this batch's entries (e takes the next batch's)
+#pragma unroll
+            for (int b = 0; b < kWaterBatch; ++b) {
+                cur[b] = e[b];
+                p[b] = a.pos[e[b] & kIndexMask];
+            }
+            indices(k + kStep < cnt ? k + kStep : k);
+#pragma unroll
+            for (int b = 0; b < kWaterBatch; ++b)
+                if (k + b * kWaterTpa < cnt) term(cur[b], p[b]);
         }
         // intramolecular correction: lanes 0 and 1 of the site take its two partners
         if (sub < 2) {
