@@ -602,6 +602,9 @@ __device__ __forceinline__ void write_partials(const ForceArgs& a, long b, doubl
 constexpr long kLanesMaxAtoms = 32768;
 constexpr int kFp64Lanes = 8;
 constexpr int kMixLanes = 4;
+// dispatch_lanes instantiates 16, 8 and 4 lanes per atom
+static_assert(kFp64Lanes == 4 || kFp64Lanes == 8 || kFp64Lanes == 16, "lanes per atom: 4, 8 or 16");
+static_assert(kMixLanes == 4 || kMixLanes == 8 || kMixLanes == 16, "lanes per atom: 4, 8 or 16");
 
 inline int lj_lanes(long n, bool mixed) {
     if (n > kLanesMaxAtoms) return mixed ? kMixLanes : kFp64Lanes;
