@@ -1,9 +1,10 @@
 // Lennard-Jones pair kernels.
 //
-// k_lj: one thread per atom over its row of the full Verlet list (each
-// unordered pair is evaluated from both atoms, so every thread owns its
-// atom's force outright: no atomics, a fixed per-thread summation order, and
-// a deterministic result). Pair arithmetic follows
+// k_lj_lanes: a few lanes per atom over its row of the full Verlet list (each
+// unordered pair is evaluated from both atoms, so every group of lanes owns
+// its atom's force outright: no atomics, a fixed per-lane summation order and
+// a fixed combination of the lanes, and a deterministic result). Pair
+// arithmetic follows
 // forcefield/lennard_jones.cpp:50-88:
 //   d = minimumImage(r_i - r_j) (i<j by upload index), r2 = |d|^2,
 //   skip unless r2 < rc2, s2 = sig^2/r2, s6 = s2^3,
@@ -15,17 +16,17 @@
 // difference from the reference's two divisions, far inside 1e-12).
 // MIXED: the partner's periodic fixed-point position (Quant) gives d as exact
 // integer differences and r2 in FP64 from them; the LJ powers and c = w/r2
-// are FP32; the row sums c d, s^6 and s^12 in FP64 (AccMix), and each row's
-// force, energy and virial become 64-bit fixed point (2^-32), so every sum
-// of rows or lanes is an exact integer addition. A row with a cutoff decision
-// too close to rc for the fixed-point r2 is evaluated again from the FP64
-// positions with the reference's test (mix_exact), so the pairs inside the
-// cutoff are always the FP64 kernel's. Without fixed-point positions (boxes
+// are FP32; each lane sums c d, s^6 and s^12 in FP64 (AccMix), and each
+// lane's share of a row's force, energy and virial becomes 64-bit fixed point
+// (2^-32), so every sum of lanes or rows is an exact integer addition. A row
+// with a cutoff decision too close to rc for the fixed-point r2 is evaluated
+// again from the FP64 positions with the reference's test (mix_exact), so the
+// pairs inside the cutoff are always the FP64 kernel's. Without fixed-point positions (boxes
 // longer than 128 rc) the mixed kernel reads the FP64 positions.
 //
 // E and W come out of a fixed reduction tree: wave shuffles, a 4-wave LDS
-// step, per-workgroup partials, and a one-workgroup finish kernel summing the
-// partials in index order. The finish kernel also records the step's thermo
+// step, per-workgroup partials, and a finish kernel summing the partials in a
+// fixed order. The finish kernel also records the step's thermo
 // row and advances the Nose-Hoover chain when one is attached.
 #include "md_common.hpp"
 #include "md_kernels.hpp"
