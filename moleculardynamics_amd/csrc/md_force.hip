@@ -728,9 +728,37 @@ __device__ __forceinline__ int walk_lane_fixed(const ForceArgs& a, const int4& q
     return kb;
 }
 
+// The sum over a group of LANES consecutive lanes (LANES = 4, 8 or 16, the
+// group aligned to LANES), left in every lane of the group. The partner
+// values move by DPP lane permutations on the vector ALU instead of LDS
+// permutes: swap neighbours, swap pairs (quad_perm), then pair each quad with
+// the other quad of its half-row (row_half_mirror) and each half-row with the
+// other half of its row (row_mirror). Each step adds two equal-length sums
+// that are the same in both partners, so every lane of the group ends with
+// the same bits. Every lane of the wave takes part.
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ long long dpp_move(long long v) {
+    const int lo = dpp_move<CTRL>(static_cast<int>(static_cast<unsigned long long>(v)));
+    const int hi = dpp_move<CTRL>(static_cast<int>(static_cast<unsigned long long>(v) >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                  static_cast<unsigned>(lo));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    return __longlong_as_double(dpp_move<CTRL>(static_cast<long long>(__double_as_longlong(v))));
+}
+
 template <int LANES, class T>
 __device__ __forceinline__ T group_sum(T v) {
-    for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    static_assert(LANES == 4 || LANES == 8 || LANES == 16, "groups of 4, 8 or 16 lanes");
+    v += dpp_move<0xB1>(v);                    // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);                    // quad_perm [2, 3, 0, 1]
+    if (LANES >= 8) v += dpp_move<0x141>(v);   // row_half_mirror
+    if (LANES >= 16) v += dpp_move<0x140>(v);  // row_mirror
     return v;
 }
 
