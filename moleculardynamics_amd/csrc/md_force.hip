@@ -413,9 +413,11 @@ __device__ __forceinline__ void pair64(const ForceArgs& a, const double4& pi, co
 // lane's share of a row (force, energy, virial) then becomes 64-bit fixed
 // point (2^-32 units), and every combination of lanes or rows -- the lanes
 // of a row, the workgroup and global E and W -- is an integer sum,
-// independent of order. (Summing s6 and t in FP32 over each batch of four
-// entries, one conversion per batch, cut the loop's instructions but made
-// the 1M-atom kernel 3 % slower.)
+// independent of order. Two cuts measured slower at 1M atoms and not kept:
+// summing s6 and t in FP32 over each batch of four entries, one conversion
+// per batch (fewer loop instructions, 3 % slower); and an FP64 butterfly over
+// the lanes' sums with only the row's E and W converted (74 fewer
+// instructions in the kernel, 3 % slower).
 struct AccMix {
     double fx = 0.0, fy = 0.0, fz = 0.0;  // sum of c d (scaled lengths)
     double s6 = 0.0, t = 0.0;             // sums of s^6 and t = 2 s^12 - s^6
